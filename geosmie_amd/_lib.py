@@ -1,0 +1,156 @@
+"""ctypes binding of libgeosmie_hip.so and device-buffer plumbing.
+
+The library is the only compute path: if it is missing, or no ROCm device is
+visible, every compute call raises (there is no CPU fallback).
+"""
+import ctypes
+import os
+import threading
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libgeosmie_hip.so")
+
+GM_OK = 0
+GM_MIE_SORT = 1
+GM_MIE_DN_CONVERGED = 2
+PREC = {"fp64": 0, "fp32": 1, "bf16x3": 2}
+
+_lib = None
+_lock = threading.Lock()
+
+c_i64 = ctypes.c_int64
+c_i32 = ctypes.c_int32
+c_u32 = ctypes.c_uint32
+c_dbl = ctypes.c_double
+c_vp = ctypes.c_void_p
+
+# name -> (restype, argtypes); every symbol declared in include/geosmie.h
+SIGNATURES = {
+    "gm_version": (ctypes.c_char_p, []),
+    "gm_last_error": (ctypes.c_char_p, []),
+    "gm_ctx_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(c_vp)]),
+    "gm_ctx_destroy": (ctypes.c_int, [c_vp]),
+    "gm_ctx_check": (ctypes.c_int, [c_vp, c_vp, ctypes.POINTER(c_i32)]),
+    "gm_ctx_set_timing": (ctypes.c_int, [c_vp, ctypes.c_int]),
+    "gm_ctx_kernel_time": (ctypes.c_int, [c_vp, ctypes.c_int, ctypes.POINTER(c_dbl),
+                                          ctypes.POINTER(c_i64)]),
+    "gm_nmax": (ctypes.c_int, [c_dbl]),
+    "gm_coef_offsets": (ctypes.c_int, [c_vp, c_i64, c_vp, c_vp, c_vp]),
+    "gm_mie_batch": (ctypes.c_int, [c_vp, c_i64, c_vp, c_vp, c_vp, c_vp, c_vp, c_dbl, c_vp,
+                                    c_vp, c_vp, c_vp, c_vp, c_u32, c_vp]),
+    "gm_angle_tables": (ctypes.c_int, [c_vp, ctypes.c_int, c_vp, ctypes.c_int, c_vp, c_vp,
+                                       c_vp]),
+    "gm_mie_s12": (ctypes.c_int, [c_vp, c_i64, c_vp, c_vp, c_vp, c_vp, ctypes.c_int, c_vp, c_vp,
+                                  ctypes.c_int, c_vp, c_vp, ctypes.c_int, c_vp]),
+    "gm_mie_props": (ctypes.c_int, [c_vp, c_i64, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp]),
+    "gm_s12_mueller": (ctypes.c_int, [c_vp, c_i64, c_vp, c_vp, c_vp, c_vp, ctypes.c_int, c_vp,
+                                      c_vp, ctypes.c_int, c_vp, ctypes.c_int, c_vp]),
+    "gm_integrate_psd": (ctypes.c_int, [c_vp, c_i64, ctypes.c_int, c_vp, c_vp, c_vp, c_vp,
+                                        c_dbl, c_vp, c_vp]),
+    "gm_pmatrix_normalize": (ctypes.c_int, [c_vp, c_i64, ctypes.c_int, c_vp, c_vp, c_vp]),
+    "gm_gsf_expand": (ctypes.c_int, [c_vp, c_i64, ctypes.c_int, c_vp, c_vp, ctypes.c_int,
+                                     ctypes.c_int, c_vp, c_vp]),
+}
+
+
+class GMError(RuntimeError):
+    pass
+
+
+def lib():
+    """Load the shared library once; raise if it is absent (no fallback)."""
+    global _lib
+    with _lock:
+        if _lib is None:
+            if not os.path.exists(LIB_PATH):
+                raise GMError("libgeosmie_hip.so not built (run python -m geosmie_amd.build): %s"
+                              % LIB_PATH)
+            L = ctypes.CDLL(LIB_PATH)
+            for name, (res, args) in SIGNATURES.items():
+                if not hasattr(L, name):
+                    continue
+                f = getattr(L, name)
+                f.restype = res
+                f.argtypes = args
+            _lib = L
+    return _lib
+
+
+def exported_symbols():
+    L = lib()
+    return [n for n in SIGNATURES if hasattr(L, n)]
+
+
+def check(rc, what=""):
+    if rc != GM_OK:
+        msg = lib().gm_last_error().decode(errors="replace")
+        if rc < 0:
+            raise ValueError(msg)
+        raise GMError("%s failed (status %d): %s" % (what, rc, msg))
+
+
+# ---------------------------------------------------------------------------
+# Device plumbing (torch-ROCm tensors as buffers, torch's current stream)
+# ---------------------------------------------------------------------------
+_torch = None
+
+
+def torch():
+    global _torch
+    if _torch is None:
+        import torch as t
+        if not t.cuda.is_available():
+            raise GMError("no ROCm GPU visible: geosmie_amd computes only on the GPU")
+        _torch = t
+    return _torch
+
+
+def device():
+    return torch().device("cuda", torch().cuda.current_device())
+
+
+def to_dev(a, dtype=np.float64):
+    """numpy (or scalar) -> contiguous device tensor."""
+    t = torch()
+    if isinstance(a, t.Tensor):
+        return a.to(device=device()).contiguous()
+    arr = np.ascontiguousarray(np.asarray(a, dtype=dtype))
+    return t.from_numpy(arr).to(device())
+
+
+def cplx_dev(a):
+    """complex array -> interleaved (re, im) float64 device tensor."""
+    arr = np.ascontiguousarray(np.asarray(a, dtype=np.complex128))
+    return to_dev(arr.view(np.float64))
+
+
+def ptr(t):
+    return None if t is None else ctypes.c_void_p(t.data_ptr())
+
+
+def stream():
+    return ctypes.c_void_p(torch().cuda.current_stream().cuda_stream)
+
+
+_ctx = {}
+
+
+def ctx():
+    """Per-(thread, device) gm_ctx handle."""
+    t = torch()
+    key = (threading.get_ident(), t.cuda.current_device())
+    h = _ctx.get(key)
+    if h is None:
+        h = c_vp()
+        check(lib().gm_ctx_create(key[1], ctypes.byref(h)), "gm_ctx_create")
+        _ctx[key] = h
+    return h
+
+
+def check_flags():
+    """Synchronise and return the context's device status flags."""
+    f = c_i32(0)
+    check(lib().gm_ctx_check(ctx(), stream(), ctypes.byref(f)), "gm_ctx_check")
+    return f.value

@@ -1,0 +1,262 @@
+// C ABI of libgeosmie_hip.so (declared in include/geosmie.h).
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <mutex>
+#include <vector>
+
+#include "../../include/geosmie.h"
+#include "gm_ctx.h"
+#include "gm_internal.h"
+
+static thread_local char g_err[512] = "";
+
+int gm_fail(int code, const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+  return code;
+}
+
+#define HIP_OK(expr)                                                               \
+  do {                                                                             \
+    hipError_t e_ = (expr);                                                        \
+    if (e_ != hipSuccess)                                                          \
+      return gm_fail(GM_ERUNTIME, "%s: %s", #expr, hipGetErrorString(e_));         \
+  } while (0)
+
+int gm_ctx::reserve(Buf& b, size_t bytes) {
+  if (b.bytes >= bytes) return GM_OK;
+  if (b.ptr) {
+    hipError_t e = hipFree(b.ptr);
+    b.ptr = nullptr;
+    b.bytes = 0;
+    if (e != hipSuccess) return gm_fail(GM_ERUNTIME, "hipFree: %s", hipGetErrorString(e));
+  }
+  size_t want = std::max<size_t>(bytes, 256);
+  hipError_t e = hipMalloc(&b.ptr, want);
+  if (e != hipSuccess) {
+    b.ptr = nullptr;
+    return gm_fail(GM_ENOMEM, "hipMalloc(%zu): %s", want, hipGetErrorString(e));
+  }
+  b.bytes = want;
+  return GM_OK;
+}
+
+gm_ctx::~gm_ctx() {
+  hipSetDevice(device);
+  for (Buf* b : all_bufs())
+    if (b->ptr) hipFree(b->ptr);
+  for (auto& p : pairs) {
+    hipEventDestroy(p.a);
+    hipEventDestroy(p.b);
+  }
+  for (auto e : pool) hipEventDestroy(e);
+}
+
+hipEvent_t gm_ctx::take_event() {
+  if (!pool.empty()) {
+    hipEvent_t e = pool.back();
+    pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+void gm_ctx::mark_begin(int kernel, hipStream_t s, size_t* slot) {
+  *slot = (size_t)-1;
+  if (!timing) return;
+  Pair p{kernel, take_event(), take_event()};
+  if (!p.a || !p.b) return;
+  hipEventRecord(p.a, s);
+  pairs.push_back(p);
+  *slot = pairs.size() - 1;
+}
+
+void gm_ctx::mark_end(size_t slot, hipStream_t s) {
+  if (slot == (size_t)-1 || slot >= pairs.size()) return;
+  hipEventRecord(pairs[slot].b, s);
+}
+
+extern "C" {
+
+const char* gm_version(void) { return "geosmie_amd 0.1 (gfx950)"; }
+const char* gm_last_error(void) { return g_err; }
+
+int gm_ctx_create(int device, gm_ctx** out) {
+  if (!out) return gm_fail(GM_EINVAL, "gm_ctx_create: out is NULL");
+  int ndev = 0;
+  HIP_OK(hipGetDeviceCount(&ndev));
+  if (device < 0 || device >= ndev)
+    return gm_fail(GM_EINVAL, "gm_ctx_create: device %d of %d", device, ndev);
+  HIP_OK(hipSetDevice(device));
+  gm_ctx* c = new gm_ctx();
+  c->device = device;
+  hipDeviceProp_t prop;
+  hipError_t e = hipGetDeviceProperties(&prop, device);
+  if (e != hipSuccess) {
+    delete c;
+    return gm_fail(GM_ERUNTIME, "hipGetDeviceProperties: %s", hipGetErrorString(e));
+  }
+  c->n_cu = prop.multiProcessorCount;
+  int rc = c->reserve(c->status, 64);
+  if (rc) {
+    delete c;
+    return rc;
+  }
+  e = hipMemset(c->status.ptr, 0, 64);
+  if (e != hipSuccess) {
+    delete c;
+    return gm_fail(GM_ERUNTIME, "hipMemset: %s", hipGetErrorString(e));
+  }
+  *out = c;
+  return GM_OK;
+}
+
+int gm_ctx_destroy(gm_ctx* ctx) {
+  if (!ctx) return GM_OK;
+  delete ctx;
+  return GM_OK;
+}
+
+int gm_ctx_check(gm_ctx* ctx, gm_stream stream, int32_t* flags_host) {
+  if (!ctx || !flags_host) return gm_fail(GM_EINVAL, "gm_ctx_check: NULL argument");
+  HIP_OK(hipSetDevice(ctx->device));
+  hipStream_t s = (hipStream_t)stream;
+  int32_t v = 0;
+  HIP_OK(hipMemcpyAsync(&v, ctx->status.ptr, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  HIP_OK(hipStreamSynchronize(s));
+  HIP_OK(hipMemsetAsync(ctx->status.ptr, 0, sizeof(int32_t), s));
+  HIP_OK(hipStreamSynchronize(s));
+  *flags_host = v;
+  return GM_OK;
+}
+
+int gm_ctx_set_timing(gm_ctx* ctx, int enable) {
+  if (!ctx) return gm_fail(GM_EINVAL, "gm_ctx_set_timing: ctx is NULL");
+  ctx->timing = enable != 0;
+  return GM_OK;
+}
+
+int gm_ctx_kernel_time(gm_ctx* ctx, int kernel, double* ms_total, int64_t* launches) {
+  if (!ctx || !ms_total || !launches || kernel < 0 || kernel >= gm::kKCount)
+    return gm_fail(GM_EINVAL, "gm_ctx_kernel_time: bad arguments");
+  HIP_OK(hipSetDevice(ctx->device));
+  double tot = 0.0;
+  int64_t cnt = 0;
+  std::vector<gm_ctx::Pair> keep;
+  for (auto& p : ctx->pairs) {
+    if (p.kernel != kernel) {
+      keep.push_back(p);
+      continue;
+    }
+    HIP_OK(hipEventSynchronize(p.b));
+    float ms = 0.f;
+    HIP_OK(hipEventElapsedTime(&ms, p.a, p.b));
+    tot += ms;
+    ++cnt;
+    ctx->pool.push_back(p.a);
+    ctx->pool.push_back(p.b);
+  }
+  ctx->pairs.swap(keep);
+  *ms_total = tot;
+  *launches = cnt;
+  return GM_OK;
+}
+
+int gm_nmax(double size) { return (int)rint(2.0 + size + 4.0 * pow(size, 1.0 / 3.0)); }
+
+int gm_coef_offsets(gm_ctx* ctx, int64_t n, const double* size, int64_t* off, gm_stream stream) {
+  if (!ctx || n < 0 || (n > 0 && (!size || !off)))
+    return gm_fail(GM_EINVAL, "gm_coef_offsets: bad arguments");
+  HIP_OK(hipSetDevice(ctx->device));
+  hipStream_t s = (hipStream_t)stream;
+  // counts into off[1..n], then an in-place inclusive scan
+  HIP_OK(hipMemsetAsync(off, 0, sizeof(int64_t), s));
+  if (n == 0) return GM_OK;
+  hipLaunchKernelGGL(gm::k_nmax_of, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, size,
+                     off + 1);
+  HIP_OK(hipGetLastError());
+  return gm::scan_inclusive_i64(ctx, off + 1, n, s);
+}
+
+int gm_mie_batch(gm_ctx* ctx, int64_t n, const double* x, const double* y, const double* eps,
+                 const double* mu, const double* eps2, double size_max, const int64_t* coef_off,
+                 double* an, double* bn, double* props, int32_t* nmax_out, uint32_t flags,
+                 gm_stream stream) {
+  if (!ctx) return gm_fail(GM_EINVAL, "gm_mie_batch: ctx is NULL");
+  if (n < 0) return gm_fail(GM_EINVAL, "gm_mie_batch: n < 0");
+  if (n == 0) return GM_OK;
+  if (!x || !eps) return gm_fail(GM_EINVAL, "Must specify x and either eps or m.");
+  if ((y != nullptr) != (eps2 != nullptr))
+    return gm_fail(GM_EINVAL, "Must specify both y and m2 for coated particles.");
+  if (coef_off && (!an || !bn)) return gm_fail(GM_EINVAL, "gm_mie_batch: an/bn NULL");
+  if (!(size_max >= 0.0) || !isfinite(size_max))
+    return gm_fail(GM_EINVAL, "gm_mie_batch: size_max must be finite and >= 0");
+  HIP_OK(hipSetDevice(ctx->device));
+  hipStream_t s = (hipStream_t)stream;
+  int rc;
+  const int nb = 2 * gm::kBuckets;
+  if ((rc = ctx->reserve(ctx->order, sizeof(int64_t) * n))) return rc;
+  if ((rc = ctx->reserve(ctx->key, sizeof(int32_t) * n))) return rc;
+  if ((rc = ctx->reserve(ctx->hist, sizeof(int64_t) * (2 * nb + 8)))) return rc;
+  int64_t* hist = (int64_t*)ctx->hist.ptr;
+  int64_t* cursor = hist + nb;
+  int64_t* range = cursor + nb;
+  int32_t* status = (int32_t*)ctx->status.ptr;
+  int32_t* max_nmax = status + 4;
+  int64_t* order = (int64_t*)ctx->order.ptr;
+  unsigned gblk = (unsigned)((n + 255) / 256);
+  if (flags & GM_MIE_SORT) {
+    hipLaunchKernelGGL(gm::k_zero_i64, dim3((nb + 255) / 256), dim3(256), 0, s, hist, nb);
+    hipLaunchKernelGGL(gm::k_plan_keys, dim3(gblk), dim3(256), 0, s, n, x, y, eps, mu, eps2,
+                       (int32_t*)ctx->key.ptr, nmax_out, hist, max_nmax, status);
+    hipLaunchKernelGGL(gm::k_plan_scan, dim3(1), dim3(256), 0, s, hist, cursor, range);
+    hipLaunchKernelGGL(gm::k_plan_scatter, dim3(gblk), dim3(256), 0, s, n,
+                       (const int32_t*)ctx->key.ptr, cursor, order);
+  } else {
+    // input order: all points go through the kernel of their class; the
+    // other class's kernel sees an empty range.  Mixed batches need sorting.
+    hipLaunchKernelGGL(gm::k_plan_identity, dim3(gblk), dim3(256), 0, s, n, y ? 1 : 0, order,
+                       range);
+    if (nmax_out) {
+      hipLaunchKernelGGL(gm::k_nmax_i32, dim3(gblk), dim3(256), 0, s, n, y ? y : x, nmax_out);
+    }
+  }
+  HIP_OK(hipGetLastError());
+
+  const int rows_cap = std::max(1, gm_nmax(size_max));
+  const int dn_extra = (flags & GM_MIE_DN_CONVERGED) ? -1 : 0;
+  const int row_doubles = y ? gm::kCoatedRowDoubles : gm::kHomogRowDoubles;
+  const size_t slot_bytes = (size_t)rows_cap * 64 * row_doubles * sizeof(double);
+  int64_t waves = (n + 63) / 64;
+  int64_t slots = std::min<int64_t>(waves, (int64_t)ctx->n_cu * ctx->waves_per_cu);
+  slots = std::min<int64_t>(slots, std::max<int64_t>(1, (int64_t)(ctx->scratch_budget / slot_bytes)));
+  if ((rc = ctx->reserve(ctx->scratch, slot_bytes * slots))) return rc;
+  double* scratch = (double*)ctx->scratch.ptr;
+  size_t slot;
+  ctx->mark_begin(gm::kKMieHomog, s, &slot);
+  hipLaunchKernelGGL(gm::k_mie_homog, dim3((unsigned)slots), dim3(64), 0, s, order, range, n, x,
+                     y, eps, mu, eps2, coef_off, an, bn, props, status, scratch, rows_cap,
+                     dn_extra);
+  ctx->mark_end(slot, s);
+  HIP_OK(hipGetLastError());
+  if (y) {
+    ctx->mark_begin(gm::kKMieCoated, s, &slot);
+    hipLaunchKernelGGL(gm::k_mie_coated, dim3((unsigned)slots), dim3(64), 0, s, order, range, n,
+                       x, y, eps, mu, eps2, coef_off, an, bn, props, status, scratch, rows_cap,
+                       dn_extra);
+    ctx->mark_end(slot, s);
+    HIP_OK(hipGetLastError());
+  }
+  return GM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,56 @@
+// Context object behind the opaque gm_ctx handle of include/geosmie.h.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include <initializer_list>
+#include <vector>
+
+#include "../../include/geosmie.h"
+
+int gm_fail(int code, const char* fmt, ...);
+
+struct Buf {
+  void* ptr = nullptr;
+  size_t bytes = 0;
+};
+
+struct gm_ctx {
+  int device = 0;
+  int n_cu = 256;
+  int waves_per_cu = 16;                         // persistent Mie wavefronts per CU
+  size_t scratch_budget = (size_t)3 << 30;       // bytes of recurrence scratch
+  Buf status;                                    // int32 flags + int32 max_nmax + scan scratch
+  Buf order, key, hist, scratch, scan_tmp;
+  Buf tab_pi, tab_tau, gemm_tmp;
+  std::vector<Buf*> all_bufs() {
+    return {&status, &order, &key, &hist, &scratch, &scan_tmp, &tab_pi, &tab_tau, &gemm_tmp};
+  }
+  int reserve(Buf& b, size_t bytes);
+  ~gm_ctx();
+
+  // Optional per-kernel timing: event pairs recorded on the launch stream.
+  bool timing = false;
+  struct Pair {
+    int kernel;
+    hipEvent_t a, b;
+  };
+  std::vector<Pair> pairs;
+  std::vector<hipEvent_t> pool;
+  hipEvent_t take_event();
+  void mark_begin(int kernel, hipStream_t s, size_t* slot);
+  void mark_end(size_t slot, hipStream_t s);
+};
+
+namespace gm {
+// kernel ids for gm_ctx_kernel_time
+enum KernelId { kKMieHomog = 0, kKMieCoated = 1, kKS12 = 2, kKPsd = 3, kKGsf = 4, kKMlp = 5,
+                kKCount = 6 };
+}  // namespace gm
+
+namespace gm {
+// In-place inclusive prefix sum of n int64 values (device) on stream s.
+int scan_inclusive_i64(gm_ctx* ctx, int64_t* data, int64_t n, hipStream_t s);
+__global__ void k_nmax_i32(int64_t n, const double* size, int32_t* out);
+}  // namespace gm

@@ -1,0 +1,36 @@
+// Shared declarations of the libgeosmie_hip kernels (not part of the C ABI).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace gm {
+
+// Counting-sort buckets for the recurrence-length sort: cost >> kBucketShift.
+constexpr int kBucketShift = 2;
+constexpr int kBuckets = 1024;
+
+__global__ void k_zero_i64(int64_t* p, int n);
+__global__ void k_plan_keys(int64_t n, const double* x, const double* y, const double* eps,
+                            const double* mu, const double* eps2, int32_t* key,
+                            int32_t* nmax_out, int64_t* hist, int32_t* max_nmax,
+                            int32_t* status);
+__global__ void k_plan_scan(const int64_t* hist, int64_t* cursor, int64_t* range);
+__global__ void k_plan_scatter(int64_t n, const int32_t* key, int64_t* cursor, int64_t* order);
+__global__ void k_plan_identity(int64_t n, int cls, int64_t* order, int64_t* range);
+__global__ void k_mie_homog(const int64_t* order, const int64_t* range, int64_t n_all,
+                            const double* x, const double* y, const double* eps, const double* mu,
+                            const double* eps2, const int64_t* coef_off, double* an, double* bn,
+                            double* props, int32_t* status, double* scratch, int rows_cap,
+                            int dn_extra);
+__global__ void k_mie_coated(const int64_t* order, const int64_t* range, int64_t n_all,
+                             const double* x, const double* y, const double* eps, const double* mu,
+                             const double* eps2, const int64_t* coef_off, double* an, double* bn,
+                             double* props, int32_t* status, double* scratch, int rows_cap,
+                             int dn_extra);
+__global__ void k_nmax_of(int64_t n, const double* size, int64_t* cnt);
+
+// Doubles of scratch per row per lane.
+constexpr int kHomogRowDoubles = 3;
+constexpr int kCoatedRowDoubles = 11;
+
+}  // namespace gm

@@ -1,0 +1,515 @@
+// Mie coefficient kernels for gfx950: homogeneous and coated spheres, FP64.
+//
+// One (size, m) point per lane.  Points are first classified (homogeneous /
+// coated, following the dispatch of
+// src/pymiecoated/pymiecoated/mie_coeffs.py:53-73) and counting-sorted by
+// recurrence length, longest first, so that the 64 lanes of a wavefront run
+// nearly equal trip counts and the hardware dispatcher hands out the long
+// wavefronts first.  The coefficient kernels are persistent: each wavefront
+// owns one scratch slot (reused for every chunk it processes, so the slot
+// stays L2/MALL resident) and walks the sorted list with a grid stride.
+//
+// Scratch layout per slot: plane[q][row n][lane] so every row access of a
+// wavefront is one coalesced 512 B (double) or 1 KiB (complex) transaction.
+//
+// Series lengths follow mie_coeffs.py:99,101 (homogeneous) and :203-205
+// (coated): nmax = round(2 + s + 4 s^(1/3)), nmx = round(max(nmax,|z|)+16)
+// with Python's half-to-even rounding (rint).  The log-derivative D_n is the
+// reference's downward recurrence from D_nmx = 0 (mie_coeffs.py:118-121),
+// reproduced exactly, including its start index.
+//
+// Riccati-Bessel functions psi_n = s j_n(s), chi_n = -s y_n(s) (which the
+// reference takes from scipy jv/yv, mie_coeffs.py:105-113, 220-227) are
+// computed here: chi_n by upward recurrence (dominant, stable); psi_n by
+// upward recurrence while n <= |s| (oscillatory region) and, above it, from
+// psi_n = psi_{n-1} / (D_n(s) + n/s) with D_n(s) from a separate downward
+// recurrence started 16 + 8 ceil(|s|^(1/3)) above nmax (converged), which is
+// stable where psi_n decays.
+#include "gm_complex.h"
+#include "gm_internal.h"
+
+namespace gm {
+
+// ---------------------------------------------------------------------------
+// Series-length rules
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int nmax_rule(double s) {
+  return (int)rint(2.0 + s + 4.0 * pow(s, 1.0 / 3.0));
+}
+__device__ __forceinline__ int nmx_rule(int nmax, double zabs) {
+  return (int)rint(fmax((double)nmax, zabs) + 16.0);
+}
+__device__ __forceinline__ int ratio_start(int nmax, double zabs) {
+  return nmax + 16 + 8 * (int)ceil(cbrt(zabs));
+}
+
+// Effective per-point parameters after the reference's dispatch.
+struct PointDesc {
+  int cls;       // 0 homogeneous, 1 coated
+  double size;   // x for homogeneous (after dispatch), y for coated
+  double xcore;  // coated core size
+  cd eps, mu, eps2;
+};
+
+__device__ __forceinline__ PointDesc describe(int64_t p, const double* __restrict__ x,
+                                              const double* __restrict__ y,
+                                              const double* __restrict__ eps,
+                                              const double* __restrict__ mu,
+                                              const double* __restrict__ eps2) {
+  PointDesc d;
+  d.eps = cd{eps[2 * p], eps[2 * p + 1]};
+  d.mu = mu ? cd{mu[2 * p], mu[2 * p + 1]} : cd{1.0, 0.0};
+  double xx = x[p];
+  if (!y) {
+    d.cls = 0;
+    d.size = xx;
+    d.xcore = xx;
+    d.eps2 = d.eps;
+    return d;
+  }
+  double yy = y[p];
+  cd e2 = cd{eps2[2 * p], eps2[2 * p + 1]};
+  d.eps2 = e2;
+  d.size = yy;
+  d.xcore = xx;
+  // mie_coeffs.py:66-71
+  if (xx == yy || (d.eps.re == e2.re && d.eps.im == e2.im)) {
+    d.cls = 0;
+  } else if (xx == 0.0) {
+    d.cls = 0;
+    d.eps = e2;
+  } else {
+    d.cls = 1;
+  }
+  return d;
+}
+
+// Work estimate used as the sort key (number of complex recurrence steps).
+__device__ __forceinline__ int point_cost(const PointDesc& d, int* nmax_out) {
+  int nmax = nmax_rule(d.size);
+  *nmax_out = nmax;
+  if (d.cls == 0) {
+    cd z = csqrt_(d.eps * d.mu) * d.size;
+    return nmx_rule(nmax, cabs_(z)) + 2 * nmax;
+  }
+  cd m1 = csqrt_(d.eps), m2 = csqrt_(d.eps2);
+  double mx = fmax(cabs_(m1) * d.size, cabs_(m2) * d.size);
+  return 3 * nmx_rule(nmax, mx) + 6 * nmax;
+}
+
+// ---------------------------------------------------------------------------
+// Plan: classify, n_max, counting sort by cost (descending) per class
+// ---------------------------------------------------------------------------
+__global__ void k_zero_i64(int64_t* __restrict__ p, int n) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
+// Block-local LDS histogram, then one global atomic per non-empty bucket.
+// status bit 2: a coated point with mu != 1 (mie_coeffs.py:57-59).
+__global__ __launch_bounds__(256) void k_plan_keys(
+    int64_t n, const double* __restrict__ x, const double* __restrict__ y,
+    const double* __restrict__ eps, const double* __restrict__ mu,
+    const double* __restrict__ eps2, int32_t* __restrict__ key, int32_t* __restrict__ nmax_out,
+    int64_t* __restrict__ hist, int32_t* __restrict__ max_nmax, int32_t* __restrict__ status) {
+  __shared__ int32_t lh[2 * kBuckets];
+  __shared__ int32_t lmax;
+  for (int i = threadIdx.x; i < 2 * kBuckets; i += blockDim.x) lh[i] = 0;
+  if (threadIdx.x == 0) lmax = 0;
+  __syncthreads();
+  int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < n) {
+    PointDesc d = describe(p, x, y, eps, mu, eps2);
+    int nmax;
+    int cost = point_cost(d, &nmax);
+    int b = min(cost >> kBucketShift, kBuckets - 1);
+    int k = d.cls * kBuckets + (kBuckets - 1 - b);  // descending cost inside a class
+    key[p] = k;
+    if (nmax_out) nmax_out[p] = nmax;
+    atomicAdd(&lh[k], 1);
+    atomicMax(&lmax, nmax);
+    if (y && mu && (d.mu.re != 1.0 || d.mu.im != 0.0)) atomicOr(status, 2);
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < 2 * kBuckets; i += blockDim.x)
+    if (lh[i]) atomicAdd((unsigned long long*)&hist[i], (unsigned long long)lh[i]);
+  if (threadIdx.x == 0) atomicMax(max_nmax, lmax);
+}
+
+// Exclusive scan of the 2*kBuckets histogram (one block of 256).  cursor[k] =
+// start of bucket k; range = {homog begin, homog end, coated begin, coated end}.
+__global__ __launch_bounds__(256) void k_plan_scan(const int64_t* __restrict__ hist,
+                                                   int64_t* __restrict__ cursor,
+                                                   int64_t* __restrict__ range) {
+  __shared__ int64_t part[256];
+  const int per = (2 * kBuckets) / 256;
+  int t = threadIdx.x;
+  int64_t s = 0;
+  for (int i = 0; i < per; ++i) s += hist[t * per + i];
+  part[t] = s;
+  __syncthreads();
+  if (t == 0) {
+    int64_t acc = 0;
+    for (int i = 0; i < 256; ++i) {
+      int64_t v = part[i];
+      part[i] = acc;
+      acc += v;
+    }
+  }
+  __syncthreads();
+  int64_t acc = part[t];
+  for (int i = 0; i < per; ++i) {
+    cursor[t * per + i] = acc;
+    acc += hist[t * per + i];
+  }
+  if (t == 0) {
+    int64_t nh = 0, nc = 0;
+    for (int i = 0; i < kBuckets; ++i) nh += hist[i];
+    for (int i = kBuckets; i < 2 * kBuckets; ++i) nc += hist[i];
+    range[0] = 0;
+    range[1] = nh;
+    range[2] = nh;
+    range[3] = nh + nc;
+  }
+}
+
+// Scatter with block-local ranks: one global atomic per (block, bucket).
+__global__ __launch_bounds__(256) void k_plan_scatter(int64_t n, const int32_t* __restrict__ key,
+                                                      int64_t* __restrict__ cursor,
+                                                      int64_t* __restrict__ order) {
+  __shared__ int32_t lcnt[2 * kBuckets];
+  __shared__ int64_t lbase[2 * kBuckets];
+  for (int i = threadIdx.x; i < 2 * kBuckets; i += blockDim.x) lcnt[i] = 0;
+  __syncthreads();
+  int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int k = -1, rank = 0;
+  if (p < n) {
+    k = key[p];
+    rank = atomicAdd(&lcnt[k], 1);
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < 2 * kBuckets; i += blockDim.x)
+    if (lcnt[i])
+      lbase[i] = (int64_t)atomicAdd((unsigned long long*)&cursor[i], (unsigned long long)lcnt[i]);
+  __syncthreads();
+  if (k >= 0) order[lbase[k] + rank] = p;
+}
+
+// Identity order (no sort): homogeneous-only or coated-only batches in input order.
+__global__ void k_plan_identity(int64_t n, int cls, int64_t* __restrict__ order,
+                                int64_t* __restrict__ range) {
+  int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < n) order[p] = p;
+  if (p == 0) {
+    range[0] = 0;
+    range[1] = cls == 0 ? n : 0;
+    range[2] = cls == 0 ? n : 0;
+    range[3] = n;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Efficiency accumulator, mie_props.py:28-70 (sequential FP64 sums over n)
+// ---------------------------------------------------------------------------
+struct Props {
+  double sext, ssca, asy1, asy2;
+  cd qb;
+  cd ap, bp;  // a_{n-1}, b_{n-1}
+  __device__ __forceinline__ void init() {
+    sext = ssca = asy1 = asy2 = 0.0;
+    qb = cd{0.0, 0.0};
+    ap = bp = cd{0.0, 0.0};
+  }
+  __device__ __forceinline__ void add(int n, cd a, cd b) {
+    double dn = (double)n;
+    double cn = 2.0 * dn + 1.0;
+    sext = fma(cn, a.re + b.re, sext);
+    ssca = fma(cn, norm2(a) + norm2(b), ssca);
+    double sg = (n & 1) ? -cn : cn;
+    qb = qb + (a - b) * sg;
+    if (n > 1) {
+      double dm = dn - 1.0;  // previous index
+      double c1 = dm * (dm + 2.0) / (dm + 1.0);
+      asy1 = fma(c1, ap.re * a.re + ap.im * a.im + bp.re * b.re + bp.im * b.im, asy1);
+    }
+    double c2 = cn / (dn * (dn + 1.0));
+    asy2 = fma(c2, a.re * b.re + a.im * b.im, asy2);
+    ap = a;
+    bp = b;
+  }
+  __device__ __forceinline__ void finish(double size, int64_t p, int64_t n_all,
+                                         double* __restrict__ out) {
+    double y2 = size * size;
+    double qext = 2.0 * sext / y2;
+    double qsca = 2.0 * ssca / y2;
+    double qbv = (qb.re * qb.re + qb.im * qb.im) / y2;
+    double asy = 4.0 / y2 * (asy1 + asy2) / qsca;
+    out[0 * n_all + p] = qext;
+    out[1 * n_all + p] = qsca;
+    out[2 * n_all + p] = qext - qsca;
+    out[3 * n_all + p] = qbv;
+    out[4 * n_all + p] = asy;
+    out[5 * n_all + p] = qbv / qsca;
+  }
+};
+
+// ---------------------------------------------------------------------------
+// Homogeneous sphere, mie_coeffs.py:83-130 / 132-180 + mie_props.py:28-70
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(64) void k_mie_homog(
+    const int64_t* __restrict__ order, const int64_t* __restrict__ range, int64_t n_all,
+    const double* __restrict__ x, const double* __restrict__ y, const double* __restrict__ eps,
+    const double* __restrict__ mu, const double* __restrict__ eps2,
+    const int64_t* __restrict__ coef_off, double* __restrict__ an, double* __restrict__ bn,
+    double* __restrict__ props, int32_t* __restrict__ status, double* __restrict__ scratch,
+    int rows_cap, int dn_extra) {
+  const int lane = threadIdx.x;
+  const int64_t begin = range[0], end = range[1];
+  // slot planes: D (complex) and psi ratio (real)
+  double* slot = scratch + (size_t)blockIdx.x * (size_t)rows_cap * 64 * 3;
+  cd* Dp = reinterpret_cast<cd*>(slot);             // [row][lane]
+  double* Rp = slot + (size_t)rows_cap * 64 * 2;    // [row][lane]
+
+  for (int64_t base = begin + (int64_t)blockIdx.x * 64; base < end;
+       base += (int64_t)gridDim.x * 64) {
+    int64_t idx = base + lane;
+    if (idx >= end) continue;
+    int64_t p = order[idx];
+    PointDesc d = describe(p, x, y, eps, mu, eps2);
+    const double s = d.size;
+    const int nmax = nmax_rule(s);
+    if (nmax > rows_cap) {
+      atomicOr(status, 1);
+      continue;
+    }
+    cd z = csqrt_(d.eps * d.mu) * s;
+    cd m = csqrt_(cdiv(d.eps, d.mu));
+    cd minv = cdiv(1.0, m);
+    const double zabs = cabs_(z);
+    int nmx = nmx_rule(nmax, zabs) + (dn_extra < 0 ? 16 + 10 * (int)ceil(cbrt(zabs)) : dn_extra);
+    cd zinv = cdiv(1.0, z);
+    // D_n(mx) downward from D_nmx = 0 (mie_coeffs.py:118-121)
+    {
+      cd dd = cd{0.0, 0.0};
+      for (int j = nmx - 1; j >= 1; --j) {
+        cd r = zinv * (double)(j + 1);
+        dd = r - crcp_fast(dd + r);
+        if (j <= nmax) Dp[(size_t)(j - 1) * 64 + lane] = dd;
+      }
+    }
+    const double xinv = 1.0 / s;
+    const int nsw = min((int)s, nmax);
+    // psi ratio psi_{n-1}/psi_n = D_n(x) + n/x above the oscillatory region
+    if (nsw < nmax) {
+      double dd = 0.0;
+      for (int n = ratio_start(nmax, s); n > nsw; --n) {
+        double r = (double)n * xinv;
+        double t = dd + r;
+        if (n <= nmax) Rp[(size_t)(n - 1) * 64 + lane] = t;
+        dd = r - 1.0 / t;
+      }
+    }
+    double sn, cs;
+    sincos(s, &sn, &cs);
+    double psi_pp = cs, psi_p = sn;   // psi_{-1}, psi_0
+    double chi_pp = -sn, chi_p = cs;  // chi_{-1}, chi_0
+    Props pr;
+    pr.init();
+    const bool store = coef_off != nullptr;
+    int64_t off = store ? coef_off[p] : 0;
+    for (int n = 1; n <= nmax; ++n) {
+      double f = (double)(2 * n - 1) * xinv;
+      double psi = (n <= nsw) ? fma(f, psi_p, -psi_pp) : psi_p / Rp[(size_t)(n - 1) * 64 + lane];
+      double chi = fma(f, chi_p, -chi_pp);
+      cd D = Dp[(size_t)(n - 1) * 64 + lane];
+      double nx = (double)n * xinv;
+      cd da = D * minv + nx;
+      cd db = D * m + nx;
+      cd xi = cd{psi, -chi};
+      cd xi1 = cd{psi_p, -chi_p};
+      cd a = cdiv(da * psi - psi_p, da * xi - xi1);
+      cd b = cdiv(db * psi - psi_p, db * xi - xi1);
+      pr.add(n, a, b);
+      if (store) {
+        an[2 * (off + n - 1)] = a.re;
+        an[2 * (off + n - 1) + 1] = a.im;
+        bn[2 * (off + n - 1)] = b.re;
+        bn[2 * (off + n - 1) + 1] = b.im;
+      }
+      psi_pp = psi_p;
+      psi_p = psi;
+      chi_pp = chi_p;
+      chi_p = chi;
+    }
+    if (props) pr.finish(s, p, n_all, props);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Coated sphere, mie_coeffs.py:183-251 + mie_props.py:28-70 (size = y)
+// ---------------------------------------------------------------------------
+// Complex Riccati-Bessel upward step with the oscillatory / ratio switch.
+struct RBc {
+  cd psi_pp, psi_p, chi_pp, chi_p;
+  int nsw;
+  __device__ __forceinline__ void init(cd z, int nmax) {
+    psi_pp = ccos_(z);
+    psi_p = csin_(z);
+    chi_pp = -psi_p;
+    chi_p = psi_pp;
+    nsw = min((int)cabs_(z), nmax);
+  }
+};
+
+__global__ __launch_bounds__(64) void k_mie_coated(
+    const int64_t* __restrict__ order, const int64_t* __restrict__ range, int64_t n_all,
+    const double* __restrict__ x, const double* __restrict__ y, const double* __restrict__ eps,
+    const double* __restrict__ mu, const double* __restrict__ eps2,
+    const int64_t* __restrict__ coef_off, double* __restrict__ an, double* __restrict__ bn,
+    double* __restrict__ props, int32_t* __restrict__ status, double* __restrict__ scratch,
+    int rows_cap, int dn_extra) {
+  const int lane = threadIdx.x;
+  const int64_t begin = range[2], end = range[3];
+  // 11 doubles per row: uu, vv, dnw, rv, rw (complex), ry (real)
+  double* slot = scratch + (size_t)blockIdx.x * (size_t)rows_cap * 64 * 11;
+  const size_t plane = (size_t)rows_cap * 64;
+  cd* UU = reinterpret_cast<cd*>(slot);
+  cd* VV = reinterpret_cast<cd*>(slot + 2 * plane);
+  cd* DW = reinterpret_cast<cd*>(slot + 4 * plane);
+  cd* RV = reinterpret_cast<cd*>(slot + 6 * plane);
+  cd* RW = reinterpret_cast<cd*>(slot + 8 * plane);
+  double* RY = slot + 10 * plane;
+
+  for (int64_t base = begin + (int64_t)blockIdx.x * 64; base < end;
+       base += (int64_t)gridDim.x * 64) {
+    int64_t idx = base + lane;
+    if (idx >= end) continue;
+    int64_t p = order[idx];
+    PointDesc d = describe(p, x, y, eps, mu, eps2);
+    const double yy = d.size, xx = d.xcore;
+    const int nmax = nmax_rule(yy);
+    if (nmax > rows_cap) {
+      atomicOr(status, 1);
+      continue;
+    }
+    cd m1 = csqrt_(d.eps), m2 = csqrt_(d.eps2);
+    cd m = cdiv(m2, m1);
+    cd minv = cdiv(m1, m2);
+    cd u = m1 * xx, v = m2 * xx, w = m2 * yy;
+    double mxv = fmax(cabs_(m1 * yy), cabs_(w));
+    int nmx = nmx_rule(nmax, mxv) + (dn_extra < 0 ? 16 + 10 * (int)ceil(cbrt(mxv)) : dn_extra);
+    // three D_n recurrences from D_nmx = 0 sharing one start (mie_coeffs.py:214-218)
+    {
+      cd ui = cdiv(1.0, u), vi = cdiv(1.0, v), wi = cdiv(1.0, w);
+      cd du = cd{0.0, 0.0}, dv = du, dw = du;
+      for (int j = nmx - 1; j >= 1; --j) {
+        double jj = (double)(j + 1);
+        cd ru = ui * jj, rv = vi * jj, rw = wi * jj;
+        du = ru - crcp_fast(du + ru);
+        dv = rv - crcp_fast(dv + rv);
+        dw = rw - crcp_fast(dw + rw);
+        if (j <= nmax) {
+          size_t o = (size_t)(j - 1) * 64 + lane;
+          UU[o] = m * du - dv;      // mie_coeffs.py:230
+          VV[o] = du * minv - dv;   // mie_coeffs.py:231
+          DW[o] = dw;
+        }
+      }
+    }
+    RBc bv, bw;
+    bv.init(v, nmax);
+    bw.init(w, nmax);
+    const int nsw_y = min((int)yy, nmax);
+    // converged psi ratios where psi decays
+    if (bv.nsw < nmax) {
+      cd vi = cdiv(1.0, v);
+      cd dd = cd{0.0, 0.0};
+      for (int n = ratio_start(nmax, cabs_(v)); n > bv.nsw; --n) {
+        cd r = vi * (double)n;
+        cd t = dd + r;
+        if (n <= nmax) RV[(size_t)(n - 1) * 64 + lane] = t;
+        dd = r - crcp_fast(t);
+      }
+    }
+    if (bw.nsw < nmax) {
+      cd wi = cdiv(1.0, w);
+      cd dd = cd{0.0, 0.0};
+      for (int n = ratio_start(nmax, cabs_(w)); n > bw.nsw; --n) {
+        cd r = wi * (double)n;
+        cd t = dd + r;
+        if (n <= nmax) RW[(size_t)(n - 1) * 64 + lane] = t;
+        dd = r - crcp_fast(t);
+      }
+    }
+    const double yinv = 1.0 / yy;
+    if (nsw_y < nmax) {
+      double dd = 0.0;
+      for (int n = ratio_start(nmax, yy); n > nsw_y; --n) {
+        double r = (double)n * yinv;
+        double t = dd + r;
+        if (n <= nmax) RY[(size_t)(n - 1) * 64 + lane] = t;
+        dd = r - 1.0 / t;
+      }
+    }
+    double sn, cs;
+    sincos(yy, &sn, &cs);
+    double py_pp = cs, py_p = sn, cy_pp = -sn, cy_p = cs;
+    cd vinv = cdiv(1.0, v), winv = cdiv(1.0, w);
+    cd m2inv = cdiv(1.0, m2);
+    Props pr;
+    pr.init();
+    const bool store = coef_off != nullptr;
+    int64_t off = store ? coef_off[p] : 0;
+    for (int n = 1; n <= nmax; ++n) {
+      size_t o = (size_t)(n - 1) * 64 + lane;
+      double fn = (double)(2 * n - 1);
+      // psi/chi at v
+      cd pv = (n <= bv.nsw) ? (vinv * fn) * bv.psi_p - bv.psi_pp : cdiv(bv.psi_p, RV[o]);
+      cd chv = (vinv * fn) * bv.chi_p - bv.chi_pp;
+      bv.psi_pp = bv.psi_p; bv.psi_p = pv; bv.chi_pp = bv.chi_p; bv.chi_p = chv;
+      // psi/chi at w
+      cd pw = (n <= bw.nsw) ? (winv * fn) * bw.psi_p - bw.psi_pp : cdiv(bw.psi_p, RW[o]);
+      cd chw = (winv * fn) * bw.chi_p - bw.chi_pp;
+      bw.psi_pp = bw.psi_p; bw.psi_p = pw; bw.chi_pp = bw.chi_p; bw.chi_p = chw;
+      // psi/chi at y (real)
+      double f = fn * yinv;
+      double py = (n <= nsw_y) ? fma(f, py_p, -py_pp) : py_p / RY[o];
+      double chy = fma(f, cy_p, -cy_pp);
+      // shell combination, mie_coeffs.py:230-249
+      cd uu = UU[o], vv = VV[o], dnw = DW[o];
+      cd fv = cdiv(pv, chv);
+      cd fvw = cdiv(fv, pw);
+      cd ku1 = uu * fvw;
+      cd kv1 = vv * fvw;
+      cd pt = pw - chw * fv;
+      cd prat = cdiv(cdiv(pw, pv), chv);
+      cd ku2 = uu * pt + prat;
+      cd kv2 = vv * pt + prat;
+      cd dns = cdiv(ku1, ku2) + dnw;
+      cd gns = cdiv(kv1, kv2) + dnw;
+      double nrat = (double)n * yinv;
+      cd a1 = dns * m2inv + nrat;
+      cd b1 = m2 * gns + nrat;
+      cd gsy = cd{py, -chy}, gs1y = cd{py_p, -cy_p};
+      cd a = cdiv(a1 * py - py_p, gsy * a1 - gs1y);
+      cd b = cdiv(b1 * py - py_p, gsy * b1 - gs1y);
+      pr.add(n, a, b);
+      if (store) {
+        an[2 * (off + n - 1)] = a.re;
+        an[2 * (off + n - 1) + 1] = a.im;
+        bn[2 * (off + n - 1)] = b.re;
+        bn[2 * (off + n - 1) + 1] = b.im;
+      }
+      py_pp = py_p; py_p = py; cy_pp = cy_p; cy_p = chy;
+    }
+    if (props) pr.finish(yy, p, n_all, props);
+  }
+}
+
+// n_max per point and its exclusive prefix (coefficient offsets), one block.
+__global__ void k_nmax_of(int64_t n, const double* __restrict__ size, int64_t* __restrict__ cnt) {
+  int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < n) cnt[p] = nmax_rule(size[p]);
+}
+
+}  // namespace gm

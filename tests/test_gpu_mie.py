@@ -1,0 +1,217 @@
+"""GPU parity of the Mie coefficient / efficiency / S12 kernels against the oracle.
+
+Tolerances (relative unless stated): Qext, Qsca, g 1e-9 (the target for Q and
+g); qb, qratio 1e-8 where the alternating sum cancels; S1/S2 1e-11 of the
+largest |S| on the grid.  Coated points with an absorbing shell inherit the
+reference's own ~1e-6 conditioning (see tests/test_oracle_mie.py).
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from oracle import mie as M
+
+pytestmark = pytest.mark.gpu
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLD = json.load(open(os.path.join(HERE, "golden", "test_mie_golden.json")))
+MP = np.load(os.path.join(HERE, "golden", "mp_vectors.npz"))
+
+
+def _rel(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        return np.where(b != 0, np.abs(a - b) / np.abs(b), np.abs(a - b))
+
+
+@pytest.mark.parametrize("case", GOLD["cases"], ids=[c["name"] for c in GOLD["cases"]])
+def test_mie_class_golden(gpu, case):
+    """test_mie.py:45-109 golden values through the drop-in Mie class."""
+    from geosmie_amd import Mie
+    if "eps" in case:
+        mie = Mie(eps=complex(*case["eps"]), mu=complex(*case["mu"]), x=case["x"])
+    elif "m2" in case:
+        mie = Mie(m=complex(*case["m"]), m2=complex(*case["m2"]), x=case["x"], y=case["y"])
+    else:
+        mie = Mie(m=complex(*case["m"]), x=case["x"])
+    got = [mie.qext(), mie.qsca(), mie.qabs(), mie.qb(), mie.asy(), mie.qratio()]
+    assert _rel(got, case["props"]).max() < 1e-12
+    s1, s2 = mie.S12(case["u"])
+    assert abs(s1 - complex(*case["s1"])) / abs(complex(*case["s1"])) < 1e-12
+    assert abs(s2 - complex(*case["s2"])) / abs(complex(*case["s2"])) < 1e-12
+
+
+def test_mie_class_keywords_and_errors(gpu):
+    """test_mie.py:112-156: keyword/setter equivalence and ValueError cases."""
+    from geosmie_amd import Mie
+    a = Mie(m=complex(1.5, 0.5), m2=complex(1.2, 0.2), x=1.5, y=5.0)
+    b = Mie()
+    b.m = complex(1.5, 0.5)
+    b.m2 = complex(1.2, 0.2)
+    b.x = 1.5
+    b.y = 5.0
+    for f1, f2 in zip((a.qext, a.qsca, a.qabs, a.qb, a.asy, a.qratio),
+                      (b.qext, b.qsca, b.qabs, b.qb, b.asy, b.qratio)):
+        assert f1() == f2()
+    mie = Mie()
+    with pytest.raises(ValueError):
+        mie.x = -1.0
+    mie.x = 1.0
+    with pytest.raises(ValueError):
+        mie.qext()
+    mie.m = complex(1.5, 0.5)
+    with pytest.raises(ValueError):
+        mie.y = 0.5
+    mie.y = 1.5
+    with pytest.raises(ValueError):
+        mie.qext()
+    mie.m2 = complex(1.2, 0.5)
+    with pytest.raises(ValueError):
+        mie.S12(-1.5)
+    mie.mu = complex(1.5, 0.6)
+    with pytest.raises(ValueError):
+        mie.qext()
+
+
+def test_batch_vs_arbitrary_precision_homog(gpu):
+    """All homogeneous mp vectors (x up to 2100) in one batch."""
+    from geosmie_amd.mie import mie_batch
+    m = MP["h_m"]
+    x = MP["h_x"]
+    r = mie_batch(x, m=m)
+    got = np.stack([r[k] for k in ("qext", "qsca", "qabs", "qb", "asy", "qratio")], 1)
+    ref = MP["h_props_ref"]
+    rel = _rel(got, ref)
+    assert rel[:, [0, 1]].max() < 1e-9, rel[:, [0, 1]].max()
+    assert np.abs(got[:, 4] - ref[:, 4]).max() < 1e-9
+    assert rel[:, [3, 5]].max() < 1e-8
+
+
+def test_batch_converged_start_vs_arbitrary_precision(gpu):
+    from geosmie_amd.mie import mie_batch
+    r = mie_batch(MP["h_x"], m=MP["h_m"], converged_dn=True)
+    got = np.stack([r[k] for k in ("qext", "qsca")], 1)
+    assert _rel(got, MP["h_props_conv"][:, :2]).max() < 1e-9
+
+
+def test_batch_vs_arbitrary_precision_coated(gpu):
+    from geosmie_amd.mie import mie_batch
+    r = mie_batch(MP["c_x"], m=MP["c_m1"], y=MP["c_y"], m2=MP["c_m2"])
+    got = np.stack([r[k] for k in ("qext", "qsca", "qabs", "qb", "asy", "qratio")], 1)
+    ref = MP["c_props_ref"]
+    absorbing = np.abs(MP["c_m2"].imag) >= 0.05
+    rel = _rel(got, ref)
+    assert rel[~absorbing][:, [0, 1]].max() < 1e-9
+    assert np.abs(got[~absorbing, 4] - ref[~absorbing, 4]).max() < 1e-9
+    assert rel[absorbing][:, [0, 1]].max() < 1e-6
+
+
+def _cfg1_sample(n_r=200, n_l=5):
+    r = np.logspace(np.log10(.005), np.log10(20), n_r)
+    lam = np.logspace(np.log10(.3), np.log10(2.5), n_l)
+    return (2 * np.pi * r[None, :] / lam[:, None]).ravel()
+
+
+def test_batch_homog_cfg1_sample_vs_oracle(gpu):
+    """Plain homogeneous FP64 Mie, m = 1.53+0.005i, on a subsample of the
+    2000-radius x 10-wavelength grid, against the numpy/scipy restatement."""
+    from geosmie_amd.mie import mie_batch
+    x = _cfg1_sample()
+    m = 1.53 + 0.005j
+    r = mie_batch(x, m=m, coeffs=True)
+    for i in range(0, x.size, 7):
+        an, bn, nmax = M.single_mie_coeff(m * m, 1.0, float(x[i]))
+        assert r["nmax"][i] == nmax
+        q = M.mie_props_raw(an, bn, nmax, float(x[i]))
+        got = [r[k][i] for k in ("qext", "qsca", "qabs", "qb", "asy", "qratio")]
+        rel = _rel(got, q)
+        assert rel[[0, 1]].max() < 1e-9, (x[i], rel)
+        assert abs(got[4] - q[4]) < 1e-9
+        assert rel[[3, 5]].max() < 1e-8, (x[i], rel)
+        scale = np.abs(an).max()
+        assert np.abs(r["an"][i] - an).max() < 1e-9 * scale
+        assert np.abs(r["bn"][i] - bn).max() < 1e-9 * scale
+
+
+def test_batch_coated_cfg2_sample_vs_oracle(gpu):
+    """Coated core 1.95+0.79i with real and water-mixed shells (cfg2 ranges)."""
+    from geosmie_amd.mie import mie_batch
+    rng = np.random.default_rng(20261202)
+    n = 300
+    y = 10 ** rng.uniform(np.log10(0.021), np.log10(209), n)
+    ratio = rng.choice(np.linspace(.3, .9, 7), n)
+    gf = rng.choice(np.linspace(1, 2.5, 12), n)
+    m2 = 1.33 + (1.45 - 1.33) / gf ** 3
+    m1 = 1.95 + 0.79j
+    r = mie_batch(ratio * y, m=m1, y=y, m2=m2)
+    for i in range(n):
+        an, bn, nmax = M.coated_mie_coeff(m1 * m1, m2[i] ** 2, ratio[i] * y[i], y[i])
+        q = M.mie_props_raw(an, bn, nmax, y[i])
+        got = [r[k][i] for k in ("qext", "qsca", "qabs", "qb", "asy", "qratio")]
+        rel = _rel(got, q)
+        assert rel[[0, 1]].max() < 1e-9, (y[i], ratio[i], rel)
+        assert abs(got[4] - q[4]) < 1e-9
+
+
+def test_batch_mixed_dispatch(gpu):
+    """mie_coeffs.py:66-71 dispatch inside one batch: x == y, eps == eps2, x == 0."""
+    from geosmie_amd.mie import mie_batch
+    m1, m2 = 1.7 + 0.3j, 1.4 + 0.01j
+    x = np.array([3.0, 8.0, 0.0, 3.0])
+    y = np.array([8.0, 8.0, 8.0, 8.0])
+    mm1 = np.array([m1, m1, m1, m2])
+    r = mie_batch(x, m=mm1, y=y, m2=np.full(4, m2))
+    a_c = M.mie_coeffs(m1 * m1, 3.0, y=8.0, eps2=m2 * m2)
+    a_xy = M.single_mie_coeff(m1 * m1, 1.0, 8.0)
+    a_x0 = M.single_mie_coeff(m2 * m2, 1.0, 8.0)
+    for i, ref in enumerate([a_c, a_xy, a_x0, a_x0]):
+        q = M.mie_props_raw(*ref, 8.0)
+        assert _rel(r["qext"][i], q[0]) < 1e-10
+        assert _rel(r["qsca"][i], q[1]) < 1e-10
+
+
+def test_coated_magnetic_raises(gpu):
+    from geosmie_amd.mie import mie_batch
+    with pytest.raises(ValueError):
+        mie_batch([1.0], m=1.5, y=[2.0], m2=1.2, mu=[1.5 + 0.6j])
+
+
+def test_sorted_equals_unsorted(gpu):
+    from geosmie_amd.mie import mie_batch
+    x = _cfg1_sample(100, 4)[::-1].copy()
+    a = mie_batch(x, m=1.53 + 0.005j, sort=True)
+    b = mie_batch(x, m=1.53 + 0.005j, sort=False)
+    for k in ("qext", "qsca", "asy", "qb"):
+        assert np.array_equal(a[k], b[k])
+
+
+def test_s12_vs_oracle(gpu):
+    """S1/S2 (FP64 MFMA contraction) on the 181-angle grid against numpy dots."""
+    from geosmie_amd.mie import mie_batch
+    from geosmie_amd.s12 import s12_batch
+    x = _cfg1_sample(60, 3)
+    m = 1.53 + 0.005j
+    cost = np.cos(np.radians(np.linspace(0, 180, 181)))
+    r = mie_batch(x, m=m, coeffs=True)
+    s1, s2 = s12_batch(r["an"], r["bn"], cost=cost)
+    nm = int(r["nmax"].max())
+    P, T = M.pt_table(cost, nm)
+    for i in range(x.size):
+        e1, e2 = M.s12_from_pt(r["an"][i], r["bn"][i], P, T)
+        scale = max(np.abs(e1).max(), np.abs(e2).max())
+        assert np.abs(s1[i] - e1).max() < 1e-12 * scale, (i, x[i])
+        assert np.abs(s2[i] - e2).max() < 1e-12 * scale, (i, x[i])
+
+
+def test_mie_props_and_s12_pt(gpu):
+    """mie_props / mie_S12_pt drop-ins on given coefficients."""
+    from geosmie_amd import MieCoeffs, mie_props
+    from geosmie_amd.mie import mie_S12_pt
+    c = MieCoeffs({"eps": (1.5 + 0.5j) ** 2, "x": 2.5})
+    p = mie_props(c, 2.5)
+    ref = GOLD["cases"][0]["props"]
+    assert abs(p["qext"] - ref[0]) / ref[0] < 1e-12
+    pin, tin = M.mie_pt(-0.6, c.nmax)
+    s1, s2 = mie_S12_pt(c, pin, tin)
+    assert abs(s1 - complex(*GOLD["cases"][0]["s1"])) < 1e-12

@@ -1,0 +1,50 @@
+"""The C-ABI library loads and exports every entry point include/geosmie.h declares
+(no compute calls: this runs without a GPU)."""
+import ctypes
+import os
+import re
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    src = open(os.path.join(ROOT, "include", "geosmie.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(gm_[a-z0-9_]+)\s*\(", src)))
+
+
+def test_header_declares_entry_points():
+    names = declared_functions()
+    for must in ["gm_ctx_create", "gm_mie_batch", "gm_mie_s12", "gm_angle_tables"]:
+        assert must in names
+
+
+def test_library_exports_every_declared_symbol():
+    from geosmie_amd import _lib
+    L = ctypes.CDLL(_lib.LIB_PATH)
+    missing = [n for n in declared_functions() if not hasattr(L, n)]
+    assert not missing, missing
+    # and the ctypes layer binds each of them
+    unbound = [n for n in declared_functions() if n not in _lib.SIGNATURES]
+    assert not unbound, unbound
+
+
+def test_host_only_entry_points():
+    from geosmie_amd import _lib
+    L = _lib.lib()
+    assert L.gm_version().startswith(b"geosmie_amd")
+    # n_max rule with half-to-even rounding (mie_coeffs.py:99)
+    from oracle.mie import nmax_of
+    for s in [0.01, 0.5, 1.0, 2.5, 10.0, 123.456, 2100.0]:
+        assert L.gm_nmax(s) == nmax_of(s)
+
+
+def test_no_gpu_raises_loudly():
+    import torch
+    if torch.cuda.is_available():
+        return
+    import pytest
+    from geosmie_amd import _lib
+    from geosmie_amd.mie import mie_batch
+    with pytest.raises(_lib.GMError):
+        mie_batch([1.0], m=1.5)
