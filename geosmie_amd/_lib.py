@@ -47,11 +47,15 @@ SIGNATURES = {
     "gm_mie_props": (ctypes.c_int, [c_vp, c_i64, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp]),
     "gm_s12_mueller": (ctypes.c_int, [c_vp, c_i64, c_vp, c_vp, c_vp, c_vp, ctypes.c_int, c_vp,
                                       c_vp, ctypes.c_int, c_vp, ctypes.c_int, c_vp]),
-    "gm_integrate_psd": (ctypes.c_int, [c_vp, c_i64, ctypes.c_int, c_vp, c_vp, c_vp, c_vp,
-                                        c_dbl, c_vp, c_vp]),
-    "gm_pmatrix_normalize": (ctypes.c_int, [c_vp, c_i64, ctypes.c_int, c_vp, c_vp, c_vp]),
+    "gm_psd_contract": (ctypes.c_int, [c_vp, ctypes.c_int, c_i64, ctypes.c_int, c_vp, c_vp,
+                                       c_vp, c_vp]),
+    "gm_integrate_psd": (ctypes.c_int, [c_vp, c_i64, ctypes.c_int, c_vp, c_vp, c_vp, c_vp, c_vp,
+                                        c_dbl, c_dbl, c_dbl, c_vp, c_vp]),
+    "gm_pmatrix_normalize": (ctypes.c_int, [c_vp, c_i64, ctypes.c_int, c_vp, c_vp, c_vp, c_vp]),
     "gm_gsf_expand": (ctypes.c_int, [c_vp, c_i64, ctypes.c_int, c_vp, c_vp, ctypes.c_int,
                                      ctypes.c_int, c_vp, c_vp]),
+    "gm_legendre_eval": (ctypes.c_int, [c_vp, c_i64, c_vp, ctypes.c_int, ctypes.c_int, c_vp,
+                                        c_vp, c_vp]),
 }
 
 

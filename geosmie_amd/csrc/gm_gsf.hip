@@ -1,0 +1,276 @@
+// Generalised-spherical-function expansion of phase matrices on gfx950.
+//
+// Replaces the per-entry process boundary of the reference (convertncdf.py
+// writes a text file and runs spher_expan.x, convertncdf.py:173-189) with one
+// batched device call.  spher_expan.f computes, for NG Gauss nodes x_i
+// (GAUSS, :520-579) and F linearly interpolated in angle to acos(x_i)
+// (LINTERPOL, :593-623; one_calc :157-168):
+//   A_t[l] = sum_i w_i F_t(acos x_i) P^t_l(x_i)     (SPHER_EXPAN :313-334)
+// with P^00 for F11, F44; P^22 for F22+F33; P^2-2 for F22-F33; P^02 for F12,
+// F34 (GENER :363-407), then alpha/beta by the (l + 1/2) factors (:335-347)
+// and normalisation by 1/alpha1(0) (:95-103).
+//
+// Interpolation is linear in F, so it folds into per-angle tables
+//   G_t[a][l] = sum_i w_i P^t_l(x_i) c_{i,a}
+// (c_{i,a} = interpolation weight of input angle a at node i), built once on
+// the host per (NG, angle grid) and cached in the context.  Each entry is
+// then a contraction of its 6 x n_ang input against the four tables.
+#include <math.h>
+#include <string.h>
+
+#include <vector>
+
+#include "gm_complex.h"
+#include "gm_ctx.h"
+
+namespace gm {
+
+// Gauss-Legendre nodes (ascending) and weights, Newton iteration with the
+// starting values of spher_expan.f:529-559.
+static void gauss_nodes(int n, std::vector<double>& z, std::vector<double>& w) {
+  z.assign(n, 0.0);
+  w.assign(n, 0.0);
+  const int ind = n % 2, k = n / 2 + ind;
+  const double f = (double)n;
+  double x = 0.0;
+  for (int i = 1; i <= k; ++i) {
+    const int m = n + 1 - i;
+    if (i == 1) x = 1.0 - 2.0 / ((f + 1.0) * f);
+    if (i == 2) x = (z[n - 1] - 1.0) * 4.0 + z[n - 1];
+    if (i == 3) x = (z[n - 2] - z[n - 1]) * 1.6 + z[n - 2];
+    if (i > 3) x = (z[m] - z[m + 1]) * 3.0 + z[m + 2];
+    if (i == k && ind == 1) x = 0.0;
+    int niter = 0;
+    double check = 1e-16, pa = 0.0, pb = 0.0, pc = 0.0;
+    for (;;) {
+      pb = 1.0;
+      if (++niter > 100) check *= 10.0;
+      pc = x;
+      double dj = 1.0;
+      for (int j = 2; j <= n; ++j) {
+        dj += 1.0;
+        pa = pb;
+        pb = pc;
+        pc = x * pb + (x * pb - pa) * (dj - 1.0) / dj;
+      }
+      pa = 1.0 / ((pb - x * pc) * f);
+      pb = pa * pc * (1.0 - x * x);
+      x = x - pb;
+      if (!(fabs(pb) > check * fabs(x))) break;
+    }
+    z[m - 1] = x;
+    w[m - 1] = 2.0 * pa * pa * (1.0 - x * x);
+    if (i == k && ind == 1) continue;
+    z[i - 1] = -z[m - 1];
+    w[i - 1] = w[m - 1];
+  }
+}
+
+// P^00, P^22, P^2-2, P^02 for l = 0..L-1 at u (GENER), into p[4][L].
+static void gener(double u, int L, double* p1, double* p2, double* p3, double* p4) {
+  for (int l = 0; l < L; ++l) p1[l] = p2[l] = p3[l] = p4[l] = 0.0;
+  const double dup = 1.0 + u, dum = 1.0 - u, du = u * u, d6 = 0.25 * sqrt(6.0);
+  p1[0] = 1.0;
+  if (L > 1) p1[1] = u;
+  if (L > 2) {
+    p1[2] = 0.5 * (3.0 * du - 1.0);
+    p2[2] = 0.25 * dup * dup;
+    p3[2] = 0.25 * dum * dum;
+    p4[2] = d6 * (du - 1.0);
+  }
+  for (int L1 = 3; L1 < L; ++L1) {
+    const int l = L1 - 1;
+    const double c1 = 1.0 / (l + 1), c2 = (double)(2 * l + 1);
+    const double c3 = 1.0 / sqrt((double)((l + 1) * (l + 1) - 4)), c4 = sqrt((double)(l * l - 4));
+    const double c5 = 1.0 / ((double)l * (double)((l + 1) * (l + 1) - 4));
+    const double c6 = (double)(2 * l + 1) * (double)(l * (l + 1));
+    const double c7 = (double)((2 * l + 1) * 4), c8 = (double)(l + 1) * (double)(l * l - 4);
+    const double cu1 = c2 * u, cu2 = c6 * u, dl = (double)l;
+    p1[L1] = c1 * (cu1 * p1[L1 - 1] - dl * p1[L1 - 2]);
+    p2[L1] = c5 * ((cu2 - c7) * p2[L1 - 1] - c8 * p2[L1 - 2]);
+    p3[L1] = c5 * ((cu2 + c7) * p3[L1 - 1] - c8 * p3[L1 - 2]);
+    p4[L1] = c3 * (cu1 * p4[L1 - 1] - c4 * p4[L1 - 2]);
+  }
+}
+
+// Host construction of G[4][n_ang][ng] (layout t, angle, l).
+static void build_tables(int ng, int n_ang, const double* ang_deg, std::vector<double>& G) {
+  const double d2r = acos(-1.0) / 180.0;
+  std::vector<double> xx(n_ang);
+  for (int a = 0; a < n_ang; ++a) xx[a] = ang_deg[a] * d2r;
+  std::vector<double> z, w;
+  gauss_nodes(ng, z, w);
+  G.assign((size_t)4 * n_ang * ng, 0.0);
+  std::vector<double> p(4 * (size_t)ng);
+  for (int i = 0; i < ng; ++i) {
+    const double x = acos(z[i]);
+    // LINTERPOL (IEX = 1): interval (lo, hi) and weight t of hi
+    int lo, hi;
+    if (x < xx[0]) {
+      lo = 0;
+      hi = 1;
+    } else if (x > xx[n_ang - 1]) {
+      lo = n_ang - 2;
+      hi = n_ang - 1;
+    } else {
+      int I = 1;
+      while (I < n_ang && !(xx[I] > x)) ++I;
+      if (I >= n_ang) I = n_ang - 1;  // x == last grid angle: weight 1 on it
+      lo = I - 1;
+      hi = I;
+    }
+    const double t = (x - xx[lo]) / (xx[hi] - xx[lo]);
+    gener(z[i], ng, &p[0], &p[ng], &p[2 * (size_t)ng], &p[3 * (size_t)ng]);
+    for (int tt = 0; tt < 4; ++tt) {
+      double* Gt = &G[(size_t)tt * n_ang * ng];
+      const double* pt = &p[(size_t)tt * ng];
+      for (int l = 0; l < ng; ++l) {
+        const double v = w[i] * pt[l];
+        Gt[(size_t)lo * ng + l] += v * (1.0 - t);
+        Gt[(size_t)hi * ng + l] += v * t;
+      }
+    }
+  }
+}
+
+constexpr int kGsfEntries = 4;   // entries per workgroup (G loads amortised)
+
+// One workgroup per kGsfEntries entries, threads over l.  F[e][6][n_ang] in
+// table order p11, p12, p33, p34, p22, p44; out[e][6][ng] in the same order
+// (pmom(bin, wavelength, rh, p, m), convertncdf.py:76, 381-394).
+__global__ __launch_bounds__(128) void k_gsf_expand(int64_t n_ent, int n_ang, int ng,
+                                                    const double* __restrict__ G,
+                                                    const double* __restrict__ F,
+                                                    double* __restrict__ out, int quantize) {
+  const int64_t e0 = (int64_t)blockIdx.x * kGsfEntries;
+  const size_t plane = (size_t)n_ang * ng;
+  for (int l = threadIdx.x; l < ng; l += blockDim.x) {
+    double acc[kGsfEntries][6];
+    for (int q = 0; q < kGsfEntries; ++q)
+      for (int f = 0; f < 6; ++f) acc[q][f] = 0.0;
+    for (int a = 0; a < n_ang; ++a) {
+      const double g1 = G[(size_t)a * ng + l], g2 = G[plane + (size_t)a * ng + l];
+      const double g3 = G[2 * plane + (size_t)a * ng + l], g4 = G[3 * plane + (size_t)a * ng + l];
+#pragma unroll
+      for (int q = 0; q < kGsfEntries; ++q) {
+        const int64_t e = e0 + q;
+        if (e >= n_ent) break;
+        const double* Fe = F + (size_t)e * 6 * n_ang;
+        const double f11 = Fe[a], f12 = Fe[n_ang + a], f33 = Fe[2 * n_ang + a];
+        const double f34 = Fe[3 * n_ang + a], f22 = Fe[4 * n_ang + a], f44 = Fe[5 * n_ang + a];
+        acc[q][0] = fma(f11, g1, acc[q][0]);          // alpha1 <- F11 P00
+        acc[q][1] = fma(f22 + f33, g2, acc[q][1]);    // A2 <- (F22+F33) P22
+        acc[q][2] = fma(f22 - f33, g3, acc[q][2]);    // A3 <- (F22-F33) P2-2
+        acc[q][3] = fma(f44, g1, acc[q][3]);          // alpha4 <- F44 P00
+        acc[q][4] = fma(f12, g4, acc[q][4]);          // beta1 <- F12 P02
+        acc[q][5] = fma(f34, g4, acc[q][5]);          // beta2 <- F34 P02
+      }
+    }
+    const double cl = (double)l + 0.5;
+    for (int q = 0; q < kGsfEntries; ++q) {
+      const int64_t e = e0 + q;
+      if (e >= n_ent) break;
+      double* o = out + (size_t)e * 6 * ng;
+      const double a2 = acc[q][1] * cl * 0.5, a3 = acc[q][2] * cl * 0.5;
+      o[0 * ng + l] = acc[q][0] * cl;   // p11 <- alpha1
+      o[1 * ng + l] = acc[q][4] * cl;   // p12 <- beta1
+      o[2 * ng + l] = a2 - a3;          // p33 <- alpha3
+      o[3 * ng + l] = acc[q][5] * cl;   // p34 <- beta2
+      o[4 * ng + l] = a2 + a3;          // p22 <- alpha2
+      o[5 * ng + l] = acc[q][3] * cl;   // p44 <- alpha4
+    }
+  }
+  __syncthreads();
+  // normalise by 1/alpha1(0) (spher_expan.f:95-103); optional 10-decimal rounding
+  for (int q = 0; q < kGsfEntries; ++q) {
+    const int64_t e = e0 + q;
+    if (e >= n_ent) break;
+    double* o = out + (size_t)e * 6 * ng;
+    const double cnorm = 1.0 / o[0];
+    __syncthreads();
+    for (int idx = threadIdx.x; idx < 6 * ng; idx += blockDim.x) {
+      double v = o[idx] * cnorm;
+      if (quantize) v = rint(v * 1e10) * 1e-10;
+      o[idx] = v;
+    }
+  }
+}
+
+// Legendre series sum_l c[l] P_l(x) (convertncdf.py:32-43), one thread per x.
+__global__ void k_legendre_eval(int64_t n_x, const double* __restrict__ x, int n_series, int L,
+                                const double* __restrict__ coef, double* __restrict__ out) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int s = blockIdx.y;
+  if (i >= n_x || s >= n_series) return;
+  const double u = x[i];
+  const double* c = coef + (size_t)s * L;
+  double pm = 1.0, p = u;
+  double acc = c[0];
+  if (L > 1) acc += c[1] * u;
+  for (int m = 2; m < L; ++m) {
+    double pn = 1.0 / m * ((2.0 * m - 1.0) * u * p - (m - 1.0) * pm);
+    acc += c[m] * pn;
+    pm = p;
+    p = pn;
+  }
+  out[(size_t)s * n_x + i] = acc;
+}
+
+}  // namespace gm
+
+extern "C" {
+
+int gm_gsf_expand(gm_ctx* ctx, int64_t n_entries, int n_ang, const double* ang_deg_host,
+                  const double* F, int ng, int quantize, double* out, gm_stream stream) {
+  if (!ctx || n_entries < 0 || n_ang < 2 || ng < 3 || !ang_deg_host)
+    return gm_fail(-1, "gm_gsf_expand: bad arguments");
+  if (n_entries == 0) return 0;
+  if (!F || !out) return gm_fail(-1, "gm_gsf_expand: NULL array");
+  for (int a = 1; a < n_ang; ++a)
+    if (!(ang_deg_host[a] > ang_deg_host[a - 1]))
+      return gm_fail(-1, "gm_gsf_expand: angles must increase");
+  hipError_t e = hipSetDevice(ctx->device);
+  if (e != hipSuccess) return gm_fail(1, "hipSetDevice: %s", hipGetErrorString(e));
+  hipStream_t s = (hipStream_t)stream;
+  double key = 0.0;
+  for (int a = 0; a < n_ang; ++a) key += ang_deg_host[a] * (1.0 + 1e-3 * a);
+  if (ctx->gsf_ng != ng || ctx->gsf_nang != n_ang || ctx->gsf_key != key) {
+    std::vector<double> G;
+    gm::build_tables(ng, n_ang, ang_deg_host, G);
+    int rc = ctx->reserve(ctx->gsf_tab, G.size() * sizeof(double));
+    if (rc) return rc;
+    e = hipMemcpyAsync(ctx->gsf_tab.ptr, G.data(), G.size() * sizeof(double),
+                       hipMemcpyHostToDevice, s);
+    if (e != hipSuccess) return gm_fail(1, "gsf tables: %s", hipGetErrorString(e));
+    e = hipStreamSynchronize(s);  // G is a host temporary
+    if (e != hipSuccess) return gm_fail(1, "gsf tables: %s", hipGetErrorString(e));
+    ctx->gsf_ng = ng;
+    ctx->gsf_nang = n_ang;
+    ctx->gsf_key = key;
+  }
+  size_t slot;
+  ctx->mark_begin(gm::kKGsf, s, &slot);
+  unsigned nblk = (unsigned)((n_entries + gm::kGsfEntries - 1) / gm::kGsfEntries);
+  hipLaunchKernelGGL(gm::k_gsf_expand, dim3(nblk), dim3(128), 0, s, n_entries, n_ang, ng,
+                     (const double*)ctx->gsf_tab.ptr, F, out, quantize);
+  ctx->mark_end(slot, s);
+  e = hipGetLastError();
+  if (e != hipSuccess) return gm_fail(1, "k_gsf_expand: %s", hipGetErrorString(e));
+  return 0;
+}
+
+int gm_legendre_eval(gm_ctx* ctx, int64_t n_x, const double* x, int n_series, int L,
+                     const double* coef, double* out, gm_stream stream) {
+  if (!ctx || n_x < 0 || n_series < 0 || L < 1) return gm_fail(-1, "gm_legendre_eval: bad arguments");
+  if (n_x == 0 || n_series == 0) return 0;
+  if (!x || !coef || !out) return gm_fail(-1, "gm_legendre_eval: NULL array");
+  hipError_t e = hipSetDevice(ctx->device);
+  if (e != hipSuccess) return gm_fail(1, "hipSetDevice: %s", hipGetErrorString(e));
+  hipLaunchKernelGGL(gm::k_legendre_eval, dim3((unsigned)((n_x + 127) / 128), n_series), dim3(128),
+                     0, (hipStream_t)stream, n_x, x, n_series, L, coef, out);
+  e = hipGetLastError();
+  if (e != hipSuccess) return gm_fail(1, "k_legendre_eval: %s", hipGetErrorString(e));
+  return 0;
+}
+
+}  // extern "C"

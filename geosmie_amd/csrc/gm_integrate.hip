@@ -1,0 +1,290 @@
+// Size-distribution integration (runoptics integratePSD) and phase-matrix
+// normalisation on gfx950.
+//
+// Every integral of src/geosmie/dointegration.py:1064-1209 is a weighted sum
+// over the size grid, so a batch of PSDs against per-size quantities is one
+// contraction  out[k][c] = sum_p w[k][p] v[p][c]:
+//   - v = Mueller planes [p][4 * n_ang] for the phase-matrix integrals
+//     (:1164-1166);
+//   - v = 11 per-size features (k_scalar_features) for the efficiency, g,
+//     cross-section and radius integrals (:1102-1190), finished by
+//     k_scalar_finish with the reference's weighting rules.
+// Large PSD batches (LUT sweeps) use FP64 MFMA 16x16x4 tiles; one or a few
+// PSDs (a runoptics slab) use a split-K VALU kernel streaming v at HBM rate.
+// Both reduce in a fixed order (bitwise reproducible).
+#include <algorithm>
+
+#include "gm_complex.h"
+#include "gm_ctx.h"
+
+typedef double double4_ __attribute__((ext_vector_type(4)));
+
+namespace gm {
+
+constexpr int kSmallPsd = 8;     // VALU path up to this many PSDs
+constexpr int kSplitChunk = 512; // points per split-K chunk
+
+// out_part[split][k][c] = sum_{p in chunk} w[k][p] v[p][c]   (k < n_psd <= kSmallPsd)
+__global__ __launch_bounds__(256) void k_contract_small(int n_psd, int64_t n_pts, int n_cols,
+                                                        const double* __restrict__ w,
+                                                        const double* __restrict__ v,
+                                                        double* __restrict__ part) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t p0 = (int64_t)blockIdx.y * kSplitChunk;
+  const int64_t p1 = min(p0 + (int64_t)kSplitChunk, n_pts);
+  if (c >= n_cols) return;
+  double acc[kSmallPsd];
+#pragma unroll
+  for (int k = 0; k < kSmallPsd; ++k) acc[k] = 0.0;
+  for (int64_t p = p0; p < p1; ++p) {
+    double x = v[(size_t)p * n_cols + c];
+#pragma unroll
+    for (int k = 0; k < kSmallPsd; ++k)
+      if (k < n_psd) acc[k] = fma(w[(size_t)k * n_pts + p], x, acc[k]);
+  }
+  for (int k = 0; k < n_psd; ++k)
+    part[((size_t)blockIdx.y * n_psd + k) * n_cols + c] = acc[k];
+}
+
+__global__ void k_reduce_splits(int n_split, int64_t n_out, const double* __restrict__ part,
+                                double* __restrict__ out) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_out) return;
+  double s = 0.0;
+  for (int q = 0; q < n_split; ++q) s += part[(size_t)q * n_out + i];
+  out[i] = s;
+}
+
+// Batched PSDs on FP64 MFMA: a wavefront owns 16 PSDs x 64 columns.
+// A = w (k rows, p = K), B = v (p = K, c columns); D[g + 4r][j] in lane j + 16g.
+__global__ __launch_bounds__(256) void k_contract_mfma(int n_psd, int64_t n_pts, int n_cols,
+                                                       const double* __restrict__ w,
+                                                       const double* __restrict__ v,
+                                                       double* __restrict__ out) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int k0 = blockIdx.y * 16;
+  const int c0 = (blockIdx.x * 4 + wave) * 64;
+  if (c0 >= n_cols) return;
+  const int i = lane & 15, kq = lane >> 4;
+  const int ka = k0 + i;
+  double4_ acc[4];
+  for (int t = 0; t < 4; ++t) acc[t] = (double4_){0.0, 0.0, 0.0, 0.0};
+  for (int64_t p0 = 0; p0 < n_pts; p0 += 4) {
+    const int64_t p = p0 + kq;
+    const bool pin = p < n_pts;
+    double a = (pin && ka < n_psd) ? w[(size_t)ka * n_pts + p] : 0.0;
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      const int c = c0 + t * 16 + i;
+      double b = (pin && c < n_cols) ? v[(size_t)p * n_cols + c] : 0.0;
+      acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[t], 0, 0, 0);
+    }
+  }
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    const int c = c0 + t * 16 + i;
+    if (c >= n_cols) continue;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      int k = k0 + kq + 4 * r;
+      if (k < n_psd) out[(size_t)k * n_cols + c] = acc[t][r];
+    }
+  }
+}
+
+// Per-size features for integratePSD (r = x lam / 2 pi; props planes
+// qext, qsca, qabs, qb, asy, qratio):
+//   0 1   1 r^2   2 r^3   3 r^4   4 qext r^2   5 qsca r^2   6 qabs r^2
+//   7 qb r^2   8 g qsca r^2   9 csca qsca r^2   10 cext qsca r^2
+// (csca = qsca pi r^2, cext = qext pi r^2, dointegration.py:1219-1220)
+constexpr int kNFeat = 11;
+__global__ void k_scalar_features(int64_t n, const double* __restrict__ xx, double lam,
+                                  const double* __restrict__ props, double* __restrict__ v) {
+  int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const double pi = 3.14159265358979323846;
+  double r = xx[p] * lam / (2.0 * pi);
+  double r2 = r * r;
+  double qext = props[p], qsca = props[n + p], qabs = props[2 * n + p], qb = props[3 * n + p];
+  double g = props[4 * n + p];
+  double* o = v + (size_t)p * kNFeat;
+  o[0] = 1.0;
+  o[1] = r2;
+  o[2] = r2 * r;
+  o[3] = r2 * r2;
+  o[4] = qext * r2;
+  o[5] = qsca * r2;
+  o[6] = qabs * r2;
+  o[7] = qb * r2;
+  o[8] = g * qsca * r2;
+  o[9] = (qsca * pi * r2) * qsca * r2;
+  o[10] = (qext * pi * r2) * qsca * r2;
+}
+
+// Finish integratePSD for n_modes PSDs (sums[k][feature]) and mix them with
+// fracs (dointegration.py:1094-1200, mixing 'dumix').  One thread, modes in
+// order.  out (kNOut): num, area, volume, mass, rEff, rMass, qext, qsca, qabs,
+// qb, g, csca, cext, bsca, bext, bbck.
+constexpr int kNOut = 16;
+__global__ void k_scalar_finish(int n_modes, const double* __restrict__ sums,
+                                const double* __restrict__ fracs,
+                                const double* __restrict__ reff0, double rhop0, double rhop,
+                                double* __restrict__ out) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const double pi = 3.14159265358979323846;
+  double acc[kNOut];
+  for (int q = 0; q < kNOut; ++q) acc[q] = 0.0;
+  for (int k = 0; k < n_modes; ++k) {
+    const double* s = sums + (size_t)k * kNFeat;
+    double num = s[0], S2 = s[1], S3 = s[2], S4 = s[3];
+    double th[kNOut];
+    th[0] = num;
+    th[1] = pi * S2 / num;                    // area (:1107)
+    th[2] = 4.0 / 3.0 * pi * S3 / num;        // volume (:1108)
+    th[3] = th[2] * rhop;                     // mass (:1109)
+    th[4] = S3 / S2;                          // rEff (:1114)
+    double reff_mass = S4 / S3;               // (:1122)
+    th[5] = 4.0 / 3.0 * pi * rhop * reff_mass * reff_mass * reff_mass;   // rMass
+    double rMass0 = 4.0 / 3.0 * pi * rhop0 * reff0[k] * reff0[k] * reff0[k];
+    // efficiencies, area weighted then normalised by the weight sum (:1188-1190)
+    th[6] = s[4] / S2;                        // qext
+    th[7] = s[5] / S2;                        // qsca
+    th[8] = s[6] / S2;                        // qabs
+    // qb from the backed-out P11 at 180 deg (:1133-1145, 1168-1169)
+    th[9] = (4.0 * pi * s[7] / s[5]) * th[7] / (4.0 * pi);
+    // g and every key after it carry the extra Qsca weight (:1111-1112, 1154-1157)
+    th[10] = s[8] / s[5];                     // g
+    th[11] = s[9] / s[5];                     // csca
+    th[12] = s[10] / s[5];                    // cext
+    double mc = 1.0 / rhop / th[4] * th[5] / rMass0;   // (:1193)
+    th[13] = 0.75 * th[7] * mc;               // bsca
+    th[14] = 0.75 * th[6] * mc;               // bext
+    th[15] = 0.75 * th[9] * mc / (4.0 * pi);  // bbck
+    for (int q = 0; q < kNOut; ++q) acc[q] += fracs[k] * th[q];
+  }
+  for (int q = 0; q < kNOut; ++q) out[q] = acc[q];
+}
+
+// Phase-matrix normalisation, dointegration.py:978-988, per entry:
+// p11n = 2 p11 / trapz(p11 sin(theta), theta); other elements * p11n / p11.
+// pmat[e][4][n_ang] (s11, s12, s33, s34) in place; pback[e][6] in the table's
+// element order p11, p12, p33, p34, p22, p44 at the last angle.
+__global__ __launch_bounds__(256) void k_pmat_normalize(int64_t n_ent, int n_ang,
+                                                        const double* __restrict__ theta,
+                                                        double* __restrict__ pmat,
+                                                        double* __restrict__ pback) {
+  __shared__ double red[256];
+  const int64_t e = blockIdx.x;
+  if (e >= n_ent) return;
+  double* P = pmat + (size_t)e * 4 * n_ang;
+  double s = 0.0;
+  for (int j = threadIdx.x; j < n_ang - 1; j += blockDim.x) {
+    double f0 = P[j] * sin(theta[j]), f1 = P[j + 1] * sin(theta[j + 1]);
+    s += (theta[j + 1] - theta[j]) * (f0 + f1) * 0.5;
+  }
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if (threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+    __syncthreads();
+  }
+  const double integ = red[0];
+  for (int j = threadIdx.x; j < n_ang; j += blockDim.x) {
+    double p11 = P[j];
+    double p11n = 2.0 * p11 / integ;
+    double f = p11n / p11;
+    P[n_ang + j] *= f;
+    P[2 * n_ang + j] *= f;
+    P[3 * n_ang + j] *= f;
+    P[j] = p11n;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0 && pback) {
+    int j = n_ang - 1;
+    double* b = pback + (size_t)e * 6;
+    b[0] = P[j];
+    b[1] = P[n_ang + j];
+    b[2] = P[2 * n_ang + j];
+    b[3] = P[3 * n_ang + j];
+    b[4] = P[j];
+    b[5] = P[2 * n_ang + j];
+  }
+}
+
+}  // namespace gm
+
+extern "C" {
+
+int gm_psd_contract(gm_ctx* ctx, int n_psd, int64_t n_pts, int n_cols, const double* w,
+                    const double* v, double* out, gm_stream stream) {
+  if (!ctx || n_psd < 0 || n_pts < 0 || n_cols < 0)
+    return gm_fail(-1, "gm_psd_contract: bad arguments");
+  if (n_psd == 0 || n_cols == 0) return 0;
+  if (!w || !v || !out) return gm_fail(-1, "gm_psd_contract: NULL array");
+  hipError_t e = hipSetDevice(ctx->device);
+  if (e != hipSuccess) return gm_fail(1, "hipSetDevice: %s", hipGetErrorString(e));
+  hipStream_t s = (hipStream_t)stream;
+  size_t slot;
+  ctx->mark_begin(gm::kKPsd, s, &slot);
+  if (n_psd <= gm::kSmallPsd) {
+    int nsplit = (int)std::max<int64_t>(1, (n_pts + gm::kSplitChunk - 1) / gm::kSplitChunk);
+    size_t bytes = sizeof(double) * (size_t)nsplit * n_psd * n_cols;
+    int rc = ctx->reserve(ctx->gemm_tmp, bytes);
+    if (rc) return rc;
+    double* part = (double*)ctx->gemm_tmp.ptr;
+    hipLaunchKernelGGL(gm::k_contract_small, dim3((n_cols + 255) / 256, nsplit), dim3(256), 0, s,
+                       n_psd, n_pts, n_cols, w, v, part);
+    int64_t n_out = (int64_t)n_psd * n_cols;
+    hipLaunchKernelGGL(gm::k_reduce_splits, dim3((unsigned)((n_out + 255) / 256)), dim3(256), 0,
+                       s, nsplit, n_out, part, out);
+  } else {
+    dim3 grid((n_cols + 255) / 256, (n_psd + 15) / 16);
+    hipLaunchKernelGGL(gm::k_contract_mfma, grid, dim3(256), 0, s, n_psd, n_pts, n_cols, w, v,
+                       out);
+  }
+  ctx->mark_end(slot, s);
+  e = hipGetLastError();
+  if (e != hipSuccess) return gm_fail(1, "gm_psd_contract: %s", hipGetErrorString(e));
+  return 0;
+}
+
+int gm_integrate_psd(gm_ctx* ctx, int64_t n_pts, int n_modes, const double* xx, const double* props,
+                     const double* psd, const double* fracs, const double* reff0, double lam,
+                     double rhop0, double rhop, double* out, gm_stream stream) {
+  if (!ctx || n_pts <= 0 || n_modes <= 0 || !xx || !props || !psd || !fracs || !reff0 || !out)
+    return gm_fail(-1, "gm_integrate_psd: bad arguments");
+  hipError_t e = hipSetDevice(ctx->device);
+  if (e != hipSuccess) return gm_fail(1, "hipSetDevice: %s", hipGetErrorString(e));
+  hipStream_t s = (hipStream_t)stream;
+  size_t feat_bytes = sizeof(double) * (size_t)n_pts * gm::kNFeat;
+  size_t sum_bytes = sizeof(double) * (size_t)n_modes * gm::kNFeat;
+  int rc = ctx->reserve(ctx->feat_tmp, feat_bytes + sum_bytes);
+  if (rc) return rc;
+  double* v = (double*)ctx->feat_tmp.ptr;
+  double* sums = v + (size_t)n_pts * gm::kNFeat;
+  hipLaunchKernelGGL(gm::k_scalar_features, dim3((unsigned)((n_pts + 255) / 256)), dim3(256), 0, s,
+                     n_pts, xx, lam, props, v);
+  rc = gm_psd_contract(ctx, n_modes, n_pts, gm::kNFeat, psd, v, sums, stream);
+  if (rc) return rc;
+  hipLaunchKernelGGL(gm::k_scalar_finish, dim3(1), dim3(64), 0, s, n_modes, sums, fracs, reff0,
+                     rhop0, rhop, out);
+  e = hipGetLastError();
+  if (e != hipSuccess) return gm_fail(1, "gm_integrate_psd: %s", hipGetErrorString(e));
+  return 0;
+}
+
+int gm_pmatrix_normalize(gm_ctx* ctx, int64_t n_entries, int n_ang, const double* theta_rad,
+                         double* pmat, double* pback, gm_stream stream) {
+  if (!ctx || n_entries < 0 || n_ang < 2 || !theta_rad || !pmat)
+    return gm_fail(-1, "gm_pmatrix_normalize: bad arguments");
+  if (n_entries == 0) return 0;
+  hipError_t e = hipSetDevice(ctx->device);
+  if (e != hipSuccess) return gm_fail(1, "hipSetDevice: %s", hipGetErrorString(e));
+  hipLaunchKernelGGL(gm::k_pmat_normalize, dim3((unsigned)n_entries), dim3(256), 0,
+                     (hipStream_t)stream, n_entries, n_ang, theta_rad, pmat, pback);
+  e = hipGetLastError();
+  if (e != hipSuccess) return gm_fail(1, "gm_pmatrix_normalize: %s", hipGetErrorString(e));
+  return 0;
+}
+
+}  // extern "C"

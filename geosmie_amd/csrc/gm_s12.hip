@@ -9,8 +9,9 @@
 //
 // FP64 path: v_mfma_f64_16x16x4_f64.  A wavefront owns 8 points (16 rows =
 // re/im interleaved) x 64 angles (four 16-wide sub-tiles) and walks K in steps
-// of 4.  Operand layouts of the 16x16x4 instruction: A[i][k] from lane
-// i + 16k, B[k][j] from lane j + 16k, D[4g + r][j] in lane j + 16g, element r.
+// of 4.  Operand layouts of v_mfma_f64_16x16x4_f64 (measured on gfx950):
+// A[i][k] from lane i + 16k, B[k][j] from lane j + 16k, D[g + 4r][j] in lane
+// j + 16g, element r  (the f32/bf16 16x16 variants use D[4g + r][j] instead).
 // Epilogue modes: write S1, S2 (complex), or the four independent Mueller
 // elements s11, s12, s33, s34 (dointegration.py:1044-1050).
 #include "gm_complex.h"
@@ -34,12 +35,14 @@ __global__ __launch_bounds__(256) void k_s12_f64(
   if (j0 >= n_ang) return;  // whole wavefront past the last angle
   const int i = lane & 15;   // A row / B column / D column
   const int kq = lane >> 4;  // k within the 4-step, D row group
-  // this lane's A row: point p0 + i/2, component i&1
-  const int64_t pa = p0 + (i >> 1);
+  // The f64 16x16x4 result row of (lane group g, element r) is g + 4r.  A row
+  // i = g + 4r is assigned to point p0 + 2g + (r >> 1), component r & 1, so
+  // lane group g ends up holding (re, im) of points p0+2g and p0+2g+1.
+  const int64_t pa = p0 + 2 * (i & 3) + (i >> 3);
   const bool pa_ok = pa < n;
   const int nm_a = pa_ok ? nmax[pa] : 0;
   const int64_t oa = pa_ok ? off[pa] : 0;
-  const int comp = i & 1;
+  const int comp = (i >> 2) & 1;
   // tile K = max nmax of the 8 points
   int kmax = 0;
   for (int q = 0; q < 8; ++q) {
@@ -74,7 +77,7 @@ __global__ __launch_bounds__(256) void k_s12_f64(
       accm[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(va, pp - tt, accm[t], 0, 0, 0);
     }
   }
-  // epilogue: lane holds rows 4*kq .. 4*kq+3 = points p0+2kq (re,im), p0+2kq+1 (re,im)
+  // epilogue: elements (0,1) = point p0+2kq (re,im), (2,3) = point p0+2kq+1 (re,im)
 #pragma unroll
   for (int t = 0; t < 4; ++t) {
     const int j = j0 + t * 16 + i;
@@ -96,11 +99,11 @@ __global__ __launch_bounds__(256) void k_s12_f64(
       } else {
         double a1 = norm2(s1), a2 = norm2(s2);
         cd c = s1 * conj(s2);
-        size_t plane = (size_t)n * n_ang;
-        out0[o] = 0.5 * (a1 + a2);          // s11 (= s22)
-        out0[plane + o] = 0.5 * (a2 - a1);  // s12
-        out0[2 * plane + o] = c.re;         // s33 (= s44)
-        out0[3 * plane + o] = c.im;         // s34 = -Im(conj(S1) S2)
+        size_t q = (size_t)p * 4 * n_ang + j;  // layout [point][element][angle]
+        out0[q] = 0.5 * (a1 + a2);              // s11 (= s22)
+        out0[q + n_ang] = 0.5 * (a2 - a1);      // s12
+        out0[q + 2 * n_ang] = c.re;             // s33 (= s44)
+        out0[q + 3 * n_ang] = c.im;             // s34 = -Im(conj(S1) S2)
       }
     }
   }

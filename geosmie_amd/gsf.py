@@ -1,0 +1,66 @@
+"""GSF expansion and Legendre evaluation on the GPU.
+
+Drop-in for the rungsf / convertncdf 'pygeos' path (src/gsf/convertncdf.py:
+170-189, 319-401), which runs the Fortran program spher_expan.x once per
+(bin, wavelength, rh) through a text file.  Here all entries of a table go
+to the device in one call.
+"""
+import numpy as np
+
+from . import _lib
+from ._lib import check, ptr, stream
+
+NSPHER = 129        # src/gsf/params.h:14
+TABLE_P = [11, 12, 33, 34, 22, 44]   # element order of the runoptics table p axis
+
+
+def pmom_batch(ang_deg, pmat, ng=NSPHER, quantize=True):
+    """pmom[n, 6, ng] from phase matrices pmat[n, 6, n_ang] (table order p11, p12,
+    p33, p34, p22, p44), as convertncdf.py:369-394 fills pmom(bin, wavelength,
+    rh, p, m)."""
+    t = _lib.torch()
+    ang = np.ascontiguousarray(ang_deg, dtype=np.float64)
+    pmat = np.ascontiguousarray(pmat, dtype=np.float64)
+    if pmat.ndim == 2:
+        pmat = pmat[None]
+    n = pmat.shape[0]
+    d_f = _lib.to_dev(pmat)
+    d_o = t.empty(n * 6 * ng, dtype=t.float64, device=_lib.device())
+    check(_lib.lib().gm_gsf_expand(_lib.ctx(), n, ang.size, ang.ctypes.data_as(_lib.c_vp),
+                                   ptr(d_f), int(ng), int(bool(quantize)), ptr(d_o), stream()),
+          "gm_gsf_expand")
+    return d_o.cpu().numpy().reshape(n, 6, ng)
+
+
+def spher_expan(ang_deg, f11, f22, f33, f44, f12, f34, ng=NSPHER, quantize=True):
+    """One phase matrix in the program's column order (spher_expan.f:218-220);
+    returns rows alpha1, alpha2, alpha3, alpha4, beta1, beta2 [6, ng] as in
+    '<file>.expan_coeff' (spher_expan.f:93-107)."""
+    pm = np.stack([f11, f12, f33, f34, f22, f44])
+    out = pmom_batch(ang_deg, pm, ng=ng, quantize=quantize)[0]
+    return np.stack([out[0], out[4], out[2], out[5], out[1], out[3]])
+
+
+def hbleg(x, pmoms, leg=None):
+    """sum_l pmoms[l] P_l(x) (convertncdf.py:32-43).  `leg` (the reference's
+    scratch array) is accepted and ignored."""
+    t = _lib.torch()
+    x = np.ascontiguousarray(np.atleast_1d(x), dtype=np.float64)
+    c = np.ascontiguousarray(np.atleast_2d(pmoms), dtype=np.float64)
+    d_o = t.empty(c.shape[0] * x.size, dtype=t.float64, device=_lib.device())
+    check(_lib.lib().gm_legendre_eval(_lib.ctx(), x.size, ptr(_lib.to_dev(x)), c.shape[0],
+                                      c.shape[1], ptr(_lib.to_dev(c)), ptr(d_o), stream()),
+          "gm_legendre_eval")
+    out = d_o.cpu().numpy().reshape(c.shape[0], x.size)
+    return out[0] if np.ndim(pmoms) == 1 else out
+
+
+def add_pmom(table, ng=NSPHER, quantize=True):
+    """Fill table['pmom'][bin, wavelength, rh, p, m] from the p11..p44 variables
+    of a runoptics table dict (the pygeos path of convertncdf.processFileRaw)."""
+    p = np.stack([table[k] for k in ("p11", "p12", "p33", "p34", "p22", "p44")], axis=-2)
+    shp = p.shape[:-2]
+    flat = p.reshape(-1, 6, p.shape[-1])
+    pm = pmom_batch(table["ang"], flat, ng=ng, quantize=quantize)
+    table["pmom"] = pm.reshape(shp + (6, ng))
+    return table

@@ -138,13 +138,60 @@ int gm_mie_s12(gm_ctx* ctx, int64_t n, const int64_t* coef_off, const int32_t* n
                const double* tau_t, int nmax_cap, double* s1, double* s2, int precision,
                gm_stream stream);
 
-/* Fused S1/S2 -> Mueller elements: mueller[4][n][n_ang] = s11, s12, s33, s34
- * (s22 = s11, s44 = s33).  Replaces the S12 loop plus calculateScatVals
- * (src/geosmie/dointegration.py:1044-1050, 1223-1235). */
+/* Fused S1/S2 -> Mueller elements: mueller[n][4][n_ang] = s11, s12, s33, s34
+ * per point (s22 = s11, s44 = s33).  Replaces the S12 loop plus
+ * calculateScatVals (src/geosmie/dointegration.py:1044-1050, 1223-1235). */
 int gm_s12_mueller(gm_ctx* ctx, int64_t n, const int64_t* coef_off, const int32_t* nmax,
                    const double* an, const double* bn, int n_ang, const double* pi_t,
                    const double* tau_t, int nmax_cap, double* mueller, int precision,
                    gm_stream stream);
+
+/* Weighted sums over the size grid: out[k][c] = sum_p w[k][p] * v[p][c] for
+ * n_psd weight rows w [n_psd][n_pts] and per-size columns v [n_pts][n_cols].
+ * The integration of every phase-matrix element in integratePSD
+ * (src/geosmie/dointegration.py:1164-1166; v = gm_s12_mueller output viewed
+ * as [n_pts][4*n_ang]) and the LUT sweep's batched PSDs.  FP64; MFMA tiles
+ * when n_psd > 8, split-K otherwise; fixed reduction order. */
+int gm_psd_contract(gm_ctx* ctx, int n_psd, int64_t n_pts, int n_cols, const double* w,
+                    const double* v, double* out, gm_stream stream);
+
+/* integratePSD scalars (src/geosmie/dointegration.py:1064-1209, mixing
+ * 'dumix', including its Qsca weighting of g, csca, cext from the aliased
+ * thisweight, :1111-1112 and :1154-1157, and the qb back-out, :1133-1169).
+ *   xx [n_pts] size parameters; props [6][n_pts] from gm_mie_batch;
+ *   psd [n_modes][n_pts] number weights; fracs, reff0 [n_modes];
+ *   lam wavelength (same unit as the radii); rhop0, rhop densities.
+ * out [16]: num, area, volume, mass, rEff, rMass, qext, qsca, qabs, qb, g,
+ *   csca, cext, bsca, bext, bbck (mode-summed with fracs, :1199-1200). */
+int gm_integrate_psd(gm_ctx* ctx, int64_t n_pts, int n_modes, const double* xx, const double* props,
+                     const double* psd, const double* fracs, const double* reff0, double lam,
+                     double rhop0, double rhop, double* out, gm_stream stream);
+
+/* Phase-matrix normalisation (dointegration.py:978-988) in place on
+ * pmat [n_entries][4][n_ang] (s11, s12, s33, s34): P11 -> 2 P11 / trapz(P11
+ * sin(theta), theta), other elements scaled by the same factor; pback
+ * [n_entries][6] (or NULL) = values at the last angle in the order p11, p12,
+ * p33, p34, p22, p44.  theta_rad [n_ang] device. */
+int gm_pmatrix_normalize(gm_ctx* ctx, int64_t n_entries, int n_ang, const double* theta_rad,
+                         double* pmat, double* pback, gm_stream stream);
+
+/* Generalised-spherical-function expansion of n_entries phase matrices.
+ * Replaces spher_expan.x (src/gsf/spher_expan.f) as driven per (bin,
+ * wavelength, rh) by convertncdf.py:170-189, 369-394.
+ *   ang_deg_host [n_ang] increasing scattering angles in degrees (host);
+ *   F [n_entries][6][n_ang] elements in table order p11, p12, p33, p34, p22, p44;
+ *   ng  number of Gauss nodes = number of coefficients (NSPHER = 129 in the
+ *       reference, params.h:14);
+ *   quantize  nonzero: round to 10 decimals like the F17.10 text output;
+ *   out [n_entries][6][ng] normalised coefficients, same element order
+ *       (alpha1, beta1, alpha3, beta2, alpha2, alpha4), i.e. pmom[p][m]. */
+int gm_gsf_expand(gm_ctx* ctx, int64_t n_entries, int n_ang, const double* ang_deg_host,
+                  const double* F, int ng, int quantize, double* out, gm_stream stream);
+
+/* out[s][i] = sum_l coef[s][l] P_l(x[i]) for n_series series of L terms
+ * (hbleg, src/gsf/convertncdf.py:32-43). */
+int gm_legendre_eval(gm_ctx* ctx, int64_t n_x, const double* x, int n_series, int L,
+                     const double* coef, double* out, gm_stream stream);
 
 #ifdef __cplusplus
 }

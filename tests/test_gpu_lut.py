@@ -1,0 +1,120 @@
+"""GPU parity of the runoptics chain (rawMie, integratePSD, normalisation, PSD
+contraction) against the oracle and the reference's own integratePSD vectors.
+
+Tolerances: integrated P elements and efficiencies 1e-12 relative when fed
+identical per-size inputs (pure FP64 reductions, order differs); the full
+device chain (Mie + S12 + integration) 1e-9 on Q/g and 1e-9 of max |P| on P.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from oracle import lut as OL, mie as M
+
+pytestmark = pytest.mark.gpu
+HERE = os.path.dirname(os.path.abspath(__file__))
+LV = np.load(os.path.join(HERE, "golden", "lut_vectors.npz"))
+
+
+def _raw():
+    raw = {k: LV["scatvals"][i] for i, k in enumerate(OL.SCATKEYS)}
+    for k in OL.SCALARKEYS:
+        raw[k] = LV["raw_" + k]
+    return raw
+
+
+def test_integrate_psd_vs_reference_vectors(gpu):
+    """Drop-in integratePSD on the reference's own inputs/outputs (two modes)."""
+    from geosmie_amd.lut import integratePSD
+    raw = _raw()
+    out = integratePSD(LV["xx"], [raw, raw], list(LV["psd"]), list(LV["fracs"]), float(LV["lam"]),
+                       list(LV["reff0"]), float(LV["rhop0"]), float(LV["rhop"]))
+    for k in ["num", "area", "volume", "mass", "rEff", "rMass", "qext", "qsca", "qabs", "qb",
+              "g", "csca", "cext", "bsca", "bext", "bbck"] + OL.SCATKEYS:
+        ref = LV["out_" + k]
+        assert np.allclose(out[k], ref, rtol=1e-12, atol=0), (k, out[k], ref)
+
+
+def test_rawmie_vs_oracle(gpu):
+    from geosmie_amd.lut import MultipleMie, rawMie, scatkeys, scalarkeys
+    xx = LV["xx"][::3]
+    ang = OL.angle_grid()
+    cost = np.cos(np.radians(ang))
+    mm = MultipleMie(xx, None, cost)
+    mm.preCalculate()
+    lam, m = 0.55, 1.50 + 0.01j
+    got = rawMie(mm, scatkeys, scalarkeys, lam, m.real, m.imag, None, cost)
+    ref = OL.raw_mie(xx, lam, M.size_range(xx, cost, m.real, m.imag))
+    for k in OL.SCATKEYS:
+        scale = np.abs(ref[k]).max(axis=1, keepdims=True)
+        assert np.all(np.abs(got[k] - ref[k]) <= 1e-11 * scale), k
+    for k in OL.SCALARKEYS:
+        assert np.allclose(got[k], ref[k], rtol=1e-9, atol=0), k
+
+
+def test_size_range_structure(gpu):
+    from geosmie_amd.lut import MultipleMie
+    xx = np.array([0.3, 2.0, 9.0])
+    cost = np.cos(np.radians([0., 45., 90., 180.]))
+    mm = MultipleMie(xx, None, cost)
+    mm.preCalculate()
+    got = mm.calculateS12SizeRange(1.45, 0.002)
+    ref = M.size_range(xx, cost, 1.45, 0.002)
+    assert len(got["s12"]) == 3 and len(got["s12"][0]) == 4
+    for i in range(3):
+        for j in range(4):
+            s1, s2 = got["s12"][i][j]
+            assert abs(s1 - ref["s1"][i, j]) <= 1e-12 * np.abs(ref["s1"][i]).max()
+            assert abs(s2 - ref["s2"][i, j]) <= 1e-12 * np.abs(ref["s2"][i]).max()
+    assert np.allclose(got["qext"], ref["qext"], rtol=1e-12)
+
+
+def test_post_integration_vs_oracle(gpu):
+    from geosmie_amd.lut import post_integration
+    raw = _raw()
+    one = OL.integrate_psd(LV["xx"], [raw], [LV["psd"][0]], [1.0], float(LV["lam"]),
+                           [LV["reff0"][0]], 1700., 1500.)
+    ref = OL.post_integration(one, LV["ang"])
+    got = post_integration(one, LV["ang"])
+    for k in ["p11", "p12", "p22", "p33", "p34", "p44", "pback"]:
+        assert np.allclose(got[k], ref[k], rtol=1e-12, atol=1e-15), k
+    assert abs(got["ssa"] - ref["ssa"]) < 1e-15
+    assert abs(got["lidar_ratio"] - ref["lidar_ratio"]) < 1e-12 * ref["lidar_ratio"]
+
+
+def test_lut_entries_end_to_end(gpu):
+    """Device chain Mie -> Mueller -> PSD contraction -> scalars -> normalisation."""
+    from geosmie_amd.lut import lut_entries
+    xx, dr = OL.getXArrCarma(0.05, 60.0, 30)
+    lam, m = 0.65, 1.53 + 0.005j
+    ang = OL.angle_grid()
+    psd = []
+    reff0 = []
+    for rm, s in [(0.1, 1.6), (0.5, 2.0), (1.2, 1.8)]:
+        p, r0 = OL.lognorm_psd(rm, s, xx, dr, lam, 50.0, 0.001)
+        psd.append(p)
+        reff0.append(r0)
+    got = lut_entries(xx, lam, m, np.array(psd), ang, reff0=reff0, rhop0=1700., rhop=1700.)
+    raw = OL.raw_mie(xx, lam, M.size_range(xx, np.cos(np.radians(ang)), m.real, m.imag))
+    for r in range(3):
+        one = OL.integrate_psd(xx, [raw], [psd[r]], [1.0], lam, [reff0[r]], 1700., 1700.)
+        ref = OL.post_integration(one, ang)
+        for k in ["qext", "qsca", "qabs", "g", "qb", "bext", "bsca"]:
+            assert abs(got[k][r] - ref[k]) <= 1e-9 * abs(ref[k]), (r, k)
+        for i, k in enumerate(["p11", "p12", "p33", "p34"]):
+            scale = np.abs(ref["p11"]).max()
+            assert np.abs(got["pmat"][r, i] - ref[k]).max() <= 1e-9 * scale, (r, k)
+        assert np.allclose(got["pback"][r], ref["pback"], rtol=1e-8, atol=1e-12)
+
+
+def test_psd_contract_mfma_and_small(gpu):
+    """Batched-PSD MFMA path (n_psd > 8) and split-K VALU path vs numpy."""
+    from geosmie_amd.lut import _contract
+    rng = np.random.default_rng(11)
+    v = rng.normal(size=(1337, 71))
+    for n_psd in (1, 3, 8, 9, 40):
+        w = rng.uniform(size=(n_psd, 1337))
+        got = _contract(w, v)
+        ref = w @ v
+        assert np.allclose(got, ref, rtol=1e-12, atol=1e-12 * np.abs(ref).max()), n_psd

@@ -135,8 +135,17 @@ def test_batch_homog_cfg1_sample_vs_oracle(gpu):
 
 
 def test_batch_coated_cfg2_sample_vs_oracle(gpu):
-    """Coated core 1.95+0.79i with real and water-mixed shells (cfg2 ranges)."""
+    """Coated core 1.95+0.79i with real and water-mixed shells (cfg2 ranges).
+
+    Against the C restatement (the reference's formulas; psi/chi by
+    recurrence, within 1e-10 of the exact coefficients) at 1e-9, and against
+    the scipy-based restatement within the reference's own error: scipy's
+    complex-argument jv/yv inside the shell combination (mie_coeffs.py:223-244)
+    leave up to ~5e-9 relative error at y ~ 170 even for real shells (the
+    exact mpmath value confirms the GPU side there; see the coated mp test).
+    """
     from geosmie_amd.mie import mie_batch
+    from oracle import cmie
     rng = np.random.default_rng(20261202)
     n = 300
     y = 10 ** rng.uniform(np.log10(0.021), np.log10(209), n)
@@ -145,13 +154,16 @@ def test_batch_coated_cfg2_sample_vs_oracle(gpu):
     m2 = 1.33 + (1.45 - 1.33) / gf ** 3
     m1 = 1.95 + 0.79j
     r = mie_batch(ratio * y, m=m1, y=y, m2=m2)
-    for i in range(n):
+    got = np.stack([r[k] for k in ("qext", "qsca", "qabs", "qb", "asy", "qratio")])
+    c = cmie.batch(ratio * y, m1, y, m2)
+    rel = _rel(got, c)
+    assert rel[[0, 1]].max() < 1e-9, rel[[0, 1]].max()
+    assert np.abs(got[4] - c[4]).max() < 1e-9
+    for i in range(0, n, 3):
         an, bn, nmax = M.coated_mie_coeff(m1 * m1, m2[i] ** 2, ratio[i] * y[i], y[i])
         q = M.mie_props_raw(an, bn, nmax, y[i])
-        got = [r[k][i] for k in ("qext", "qsca", "qabs", "qb", "asy", "qratio")]
-        rel = _rel(got, q)
-        assert rel[[0, 1]].max() < 1e-9, (y[i], ratio[i], rel)
-        assert abs(got[4] - q[4]) < 1e-9
+        rel = _rel(got[:, i], q)
+        assert rel[[0, 1]].max() < 2e-8, (y[i], ratio[i], rel)
 
 
 def test_batch_mixed_dispatch(gpu):
