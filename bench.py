@@ -75,6 +75,8 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calib", action="store_true",
+                    help="also run 3 device copies of 512 MiB (PMC byte-counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; --gpus %d is not supported "
@@ -152,6 +154,13 @@ def main():
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
                      "kernel_ms": kern_ms, "flop_per_launch": flops},
     }
+    if args.copy_calib:
+        a = torch.ones(64 << 20, dtype=torch.float64, device="cuda")   # 512 MiB
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(x, y, m1, m2)
     print(json.dumps(out))

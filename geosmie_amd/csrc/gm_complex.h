@@ -52,6 +52,49 @@ __device__ __forceinline__ cd crcp_fast(cd t) {
   return cd{t.re * inv, -t.im * inv};
 }
 
+// Reciprocals carried as an unevaluated sum hi + lo (double-double).
+//
+// Recurrences multiply by 1/z, 1/m, 1/x at every step.  A reciprocal rounded
+// once is a fixed relative error of up to an ulp, the same at every step --
+// equivalent to perturbing z (or m) by an ulp, which near a Mie resonance
+// moves Q by ~1e-9.  The reference divides at every step instead, whose
+// roundings are independent.  a * (hi + lo) rounded once restores that.
+struct ddr {
+  double hi, lo;
+};
+struct cdd {
+  cd hi, lo;
+};
+__device__ __forceinline__ ddr rcp_dd(double x) {
+  double hi = 1.0 / x;
+  double e = fma(-x, hi, 1.0);  // exact residual 1 - x hi
+  return ddr{hi, e * hi};
+}
+__device__ __forceinline__ double mul_dd(double a, ddr b) { return fma(a, b.hi, a * b.lo); }
+__device__ __forceinline__ cd mul_dd(cd a, ddr b) {
+  return cd{fma(a.re, b.hi, a.re * b.lo), fma(a.im, b.hi, a.im * b.lo)};
+}
+// a / z for complex z: hi = 1/z (Smith), lo = hi * (1 - z hi) with the
+// residual formed from FMA-exact products.
+__device__ __forceinline__ cdd crcp_dd(cd z) {
+  cd hi = cdiv(cd{1.0, 0.0}, z);
+  // residual r = 1 - z*hi, each product's rounding error recovered by fma
+  double p1 = z.re * hi.re, e1 = fma(z.re, hi.re, -p1);
+  double p2 = z.im * hi.im, e2 = fma(z.im, hi.im, -p2);
+  double q1 = z.re * hi.im, f1 = fma(z.re, hi.im, -q1);
+  double q2 = z.im * hi.re, f2 = fma(z.im, hi.re, -q2);
+  double rr = ((1.0 - p1) + p2) - e1 + e2;
+  double ri = -(q1 + q2) - f1 - f2;
+  cd r = cd{rr, ri};
+  return cdd{hi, hi * r};
+}
+// a * (hi + lo) with a real scalar a (e.g. (j+1)/z)
+__device__ __forceinline__ cd mul_dd(double a, cdd b) {
+  return cd{fma(a, b.hi.re, a * b.lo.re), fma(a, b.hi.im, a * b.lo.im)};
+}
+// a * (hi + lo) with complex a
+__device__ __forceinline__ cd mul_dd(cd a, cdd b) { return a * b.hi + a * b.lo; }
+
 // Principal square root, the branch numpy's complex sqrt returns.
 __device__ __forceinline__ cd csqrt_(cd z) {
   double a = z.re, b = z.im;

@@ -284,26 +284,26 @@ __global__ __launch_bounds__(64) void k_mie_homog(
     }
     cd z = csqrt_(d.eps * d.mu) * s;
     cd m = csqrt_(cdiv(d.eps, d.mu));
-    cd minv = cdiv(1.0, m);
+    cdd minv = crcp_dd(m);
     const double zabs = cabs_(z);
     int nmx = nmx_rule(nmax, zabs) + (dn_extra < 0 ? 16 + 10 * (int)ceil(cbrt(zabs)) : dn_extra);
-    cd zinv = cdiv(1.0, z);
+    cdd zinv = crcp_dd(z);
     // D_n(mx) downward from D_nmx = 0 (mie_coeffs.py:118-121)
     {
       cd dd = cd{0.0, 0.0};
       for (int j = nmx - 1; j >= 1; --j) {
-        cd r = zinv * (double)(j + 1);
+        cd r = mul_dd((double)(j + 1), zinv);
         dd = r - crcp_fast(dd + r);
         if (j <= nmax) Dp[(size_t)(j - 1) * 64 + lane] = dd;
       }
     }
-    const double xinv = 1.0 / s;
+    const ddr xinv = rcp_dd(s);
     const int nsw = min((int)s, nmax);
     // psi ratio psi_{n-1}/psi_n = D_n(x) + n/x above the oscillatory region
     if (nsw < nmax) {
       double dd = 0.0;
       for (int n = ratio_start(nmax, s); n > nsw; --n) {
-        double r = (double)n * xinv;
+        double r = mul_dd((double)n, xinv);
         double t = dd + r;
         if (n <= nmax) Rp[(size_t)(n - 1) * 64 + lane] = t;
         dd = r - 1.0 / t;
@@ -318,12 +318,12 @@ __global__ __launch_bounds__(64) void k_mie_homog(
     const bool store = coef_off != nullptr;
     int64_t off = store ? coef_off[p] : 0;
     for (int n = 1; n <= nmax; ++n) {
-      double f = (double)(2 * n - 1) * xinv;
+      double f = mul_dd((double)(2 * n - 1), xinv);
       double psi = (n <= nsw) ? fma(f, psi_p, -psi_pp) : psi_p / Rp[(size_t)(n - 1) * 64 + lane];
       double chi = fma(f, chi_p, -chi_pp);
       cd D = Dp[(size_t)(n - 1) * 64 + lane];
-      double nx = (double)n * xinv;
-      cd da = D * minv + nx;
+      double nx = mul_dd((double)n, xinv);
+      cd da = mul_dd(D, minv) + nx;
       cd db = D * m + nx;
       cd xi = cd{psi, -chi};
       cd xi1 = cd{psi_p, -chi_p};
@@ -394,24 +394,24 @@ __global__ __launch_bounds__(64) void k_mie_coated(
     }
     cd m1 = csqrt_(d.eps), m2 = csqrt_(d.eps2);
     cd m = cdiv(m2, m1);
-    cd minv = cdiv(m1, m2);
+    cdd minv = crcp_dd(m);
     cd u = m1 * xx, v = m2 * xx, w = m2 * yy;
     double mxv = fmax(cabs_(m1 * yy), cabs_(w));
     int nmx = nmx_rule(nmax, mxv) + (dn_extra < 0 ? 16 + 10 * (int)ceil(cbrt(mxv)) : dn_extra);
     // three D_n recurrences from D_nmx = 0 sharing one start (mie_coeffs.py:214-218)
     {
-      cd ui = cdiv(1.0, u), vi = cdiv(1.0, v), wi = cdiv(1.0, w);
+      cdd ui = crcp_dd(u), vi = crcp_dd(v), wi = crcp_dd(w);
       cd du = cd{0.0, 0.0}, dv = du, dw = du;
       for (int j = nmx - 1; j >= 1; --j) {
         double jj = (double)(j + 1);
-        cd ru = ui * jj, rv = vi * jj, rw = wi * jj;
+        cd ru = mul_dd(jj, ui), rv = mul_dd(jj, vi), rw = mul_dd(jj, wi);
         du = ru - crcp_fast(du + ru);
         dv = rv - crcp_fast(dv + rv);
         dw = rw - crcp_fast(dw + rw);
         if (j <= nmax) {
           size_t o = (size_t)(j - 1) * 64 + lane;
           UU[o] = m * du - dv;      // mie_coeffs.py:230
-          VV[o] = du * minv - dv;   // mie_coeffs.py:231
+          VV[o] = mul_dd(du, minv) - dv;   // mie_coeffs.py:231
           DW[o] = dw;
         }
       }
@@ -422,30 +422,30 @@ __global__ __launch_bounds__(64) void k_mie_coated(
     const int nsw_y = min((int)yy, nmax);
     // converged psi ratios where psi decays
     if (bv.nsw < nmax) {
-      cd vi = cdiv(1.0, v);
+      cdd vi = crcp_dd(v);
       cd dd = cd{0.0, 0.0};
       for (int n = ratio_start(nmax, cabs_(v)); n > bv.nsw; --n) {
-        cd r = vi * (double)n;
+        cd r = mul_dd((double)n, vi);
         cd t = dd + r;
         if (n <= nmax) RV[(size_t)(n - 1) * 64 + lane] = t;
         dd = r - crcp_fast(t);
       }
     }
     if (bw.nsw < nmax) {
-      cd wi = cdiv(1.0, w);
+      cdd wi = crcp_dd(w);
       cd dd = cd{0.0, 0.0};
       for (int n = ratio_start(nmax, cabs_(w)); n > bw.nsw; --n) {
-        cd r = wi * (double)n;
+        cd r = mul_dd((double)n, wi);
         cd t = dd + r;
         if (n <= nmax) RW[(size_t)(n - 1) * 64 + lane] = t;
         dd = r - crcp_fast(t);
       }
     }
-    const double yinv = 1.0 / yy;
+    const ddr yinv = rcp_dd(yy);
     if (nsw_y < nmax) {
       double dd = 0.0;
       for (int n = ratio_start(nmax, yy); n > nsw_y; --n) {
-        double r = (double)n * yinv;
+        double r = mul_dd((double)n, yinv);
         double t = dd + r;
         if (n <= nmax) RY[(size_t)(n - 1) * 64 + lane] = t;
         dd = r - 1.0 / t;
@@ -454,8 +454,8 @@ __global__ __launch_bounds__(64) void k_mie_coated(
     double sn, cs;
     sincos(yy, &sn, &cs);
     double py_pp = cs, py_p = sn, cy_pp = -sn, cy_p = cs;
-    cd vinv = cdiv(1.0, v), winv = cdiv(1.0, w);
-    cd m2inv = cdiv(1.0, m2);
+    cdd vinv = crcp_dd(v), winv = crcp_dd(w);
+    cdd m2inv = crcp_dd(m2);
     Props pr;
     pr.init();
     const bool store = coef_off != nullptr;
@@ -464,15 +464,16 @@ __global__ __launch_bounds__(64) void k_mie_coated(
       size_t o = (size_t)(n - 1) * 64 + lane;
       double fn = (double)(2 * n - 1);
       // psi/chi at v
-      cd pv = (n <= bv.nsw) ? (vinv * fn) * bv.psi_p - bv.psi_pp : cdiv(bv.psi_p, RV[o]);
-      cd chv = (vinv * fn) * bv.chi_p - bv.chi_pp;
+      const cd fv_ = mul_dd(fn, vinv), fw_ = mul_dd(fn, winv);
+      cd pv = (n <= bv.nsw) ? fv_ * bv.psi_p - bv.psi_pp : cdiv(bv.psi_p, RV[o]);
+      cd chv = fv_ * bv.chi_p - bv.chi_pp;
       bv.psi_pp = bv.psi_p; bv.psi_p = pv; bv.chi_pp = bv.chi_p; bv.chi_p = chv;
       // psi/chi at w
-      cd pw = (n <= bw.nsw) ? (winv * fn) * bw.psi_p - bw.psi_pp : cdiv(bw.psi_p, RW[o]);
-      cd chw = (winv * fn) * bw.chi_p - bw.chi_pp;
+      cd pw = (n <= bw.nsw) ? fw_ * bw.psi_p - bw.psi_pp : cdiv(bw.psi_p, RW[o]);
+      cd chw = fw_ * bw.chi_p - bw.chi_pp;
       bw.psi_pp = bw.psi_p; bw.psi_p = pw; bw.chi_pp = bw.chi_p; bw.chi_p = chw;
       // psi/chi at y (real)
-      double f = fn * yinv;
+      double f = mul_dd(fn, yinv);
       double py = (n <= nsw_y) ? fma(f, py_p, -py_pp) : py_p / RY[o];
       double chy = fma(f, cy_p, -cy_pp);
       // shell combination, mie_coeffs.py:230-249
@@ -487,8 +488,8 @@ __global__ __launch_bounds__(64) void k_mie_coated(
       cd kv2 = vv * pt + prat;
       cd dns = cdiv(ku1, ku2) + dnw;
       cd gns = cdiv(kv1, kv2) + dnw;
-      double nrat = (double)n * yinv;
-      cd a1 = dns * m2inv + nrat;
+      double nrat = mul_dd((double)n, yinv);
+      cd a1 = mul_dd(dns, m2inv) + nrat;
       cd b1 = m2 * gns + nrat;
       cd gsy = cd{py, -chy}, gs1y = cd{py_p, -cy_p};
       cd a = cdiv(a1 * py - py_p, gsy * a1 - gs1y);

@@ -48,9 +48,9 @@ def hbleg(x, pmoms, leg=None):
     x = np.ascontiguousarray(np.atleast_1d(x), dtype=np.float64)
     c = np.ascontiguousarray(np.atleast_2d(pmoms), dtype=np.float64)
     d_o = t.empty(c.shape[0] * x.size, dtype=t.float64, device=_lib.device())
-    check(_lib.lib().gm_legendre_eval(_lib.ctx(), x.size, ptr(_lib.to_dev(x)), c.shape[0],
-                                      c.shape[1], ptr(_lib.to_dev(c)), ptr(d_o), stream()),
-          "gm_legendre_eval")
+    d_x, d_c = _lib.to_dev(x), _lib.to_dev(c)   # keep alive until the kernel has read them
+    check(_lib.lib().gm_legendre_eval(_lib.ctx(), x.size, ptr(d_x), c.shape[0], c.shape[1],
+                                      ptr(d_c), ptr(d_o), stream()), "gm_legendre_eval")
     out = d_o.cpu().numpy().reshape(c.shape[0], x.size)
     return out[0] if np.ndim(pmoms) == 1 else out
 

@@ -226,11 +226,13 @@ def _integrate_scalars(xxarr, props, psd, fracs, lam, reff0, rhop0, rhop):
     t = _lib.torch()
     psd = np.ascontiguousarray(np.atleast_2d(psd), dtype=np.float64)
     d_out = t.empty(len(SCALAR_OUT), dtype=t.float64, device=_lib.device())
+    # device copies held in locals until the kernels have consumed them
+    d_x, d_p, d_w = _lib.to_dev(xxarr), _lib.to_dev(props), _lib.to_dev(psd)
+    d_f = _lib.to_dev(np.asarray(fracs, np.float64))
+    d_r = _lib.to_dev(np.asarray(reff0, np.float64))
     check(_lib.lib().gm_integrate_psd(
-        _lib.ctx(), psd.shape[1], psd.shape[0], ptr(_lib.to_dev(xxarr)), ptr(_lib.to_dev(props)),
-        ptr(_lib.to_dev(psd)), ptr(_lib.to_dev(np.asarray(fracs, np.float64))),
-        ptr(_lib.to_dev(np.asarray(reff0, np.float64))), float(lam), float(rhop0), float(rhop),
-        ptr(d_out), stream()), "gm_integrate_psd")
+        _lib.ctx(), psd.shape[1], psd.shape[0], ptr(d_x), ptr(d_p), ptr(d_w), ptr(d_f), ptr(d_r),
+        float(lam), float(rhop0), float(rhop), ptr(d_out), stream()), "gm_integrate_psd")
     return dict(zip(SCALAR_OUT, d_out.cpu().numpy().tolist()))
 
 

@@ -157,7 +157,9 @@ def test_batch_coated_cfg2_sample_vs_oracle(gpu):
     got = np.stack([r[k] for k in ("qext", "qsca", "qabs", "qb", "asy", "qratio")])
     c = cmie.batch(ratio * y, m1, y, m2)
     rel = _rel(got, c)
-    assert rel[[0, 1]].max() < 1e-9, rel[[0, 1]].max()
+    worst = int(rel[[0, 1]].max(0).argmax())
+    assert rel[[0, 1]].max() < 1e-9, (rel[[0, 1]].max(), worst, y[worst], ratio[worst], m2[worst],
+                                      got[:, worst].tolist(), c[:, worst].tolist())
     assert np.abs(got[4] - c[4]).max() < 1e-9
     for i in range(0, n, 3):
         an, bn, nmax = M.coated_mie_coeff(m1 * m1, m2[i] ** 2, ratio[i] * y[i], y[i])
