@@ -9,7 +9,7 @@
 #include <mutex>
 #include <vector>
 
-#include "../../include/geosmie.h"
+#include "geosmie.h"
 #include "gm_ctx.h"
 #include "gm_internal.h"
 
@@ -203,10 +203,10 @@ int gm_mie_batch(gm_ctx* ctx, int64_t n, const double* x, const double* y, const
   HIP_OK(hipSetDevice(ctx->device));
   hipStream_t s = (hipStream_t)stream;
   int rc;
-  const int nb = 2 * gm::kBuckets;
+  const int nb = gm::kClasses * gm::kBuckets;
   if ((rc = ctx->reserve(ctx->order, sizeof(int64_t) * n))) return rc;
   if ((rc = ctx->reserve(ctx->key, sizeof(int32_t) * n))) return rc;
-  if ((rc = ctx->reserve(ctx->hist, sizeof(int64_t) * (2 * nb + 8)))) return rc;
+  if ((rc = ctx->reserve(ctx->hist, sizeof(int64_t) * (2 * nb + 2 * gm::kClasses)))) return rc;
   int64_t* hist = (int64_t*)ctx->hist.ptr;
   int64_t* cursor = hist + nb;
   int64_t* range = cursor + nb;
@@ -222,10 +222,8 @@ int gm_mie_batch(gm_ctx* ctx, int64_t n, const double* x, const double* y, const
     hipLaunchKernelGGL(gm::k_plan_scatter, dim3(gblk), dim3(256), 0, s, n,
                        (const int32_t*)ctx->key.ptr, cursor, order);
   } else {
-    // input order: all points go through the kernel of their class; the
-    // other class's kernel sees an empty range.  Mixed batches need sorting.
-    hipLaunchKernelGGL(gm::k_plan_identity, dim3(gblk), dim3(256), 0, s, n, y ? 1 : 0, order,
-                       range);
+    // input order: each class kernel walks every point and keeps its own
+    hipLaunchKernelGGL(gm::k_plan_identity, dim3(gblk), dim3(256), 0, s, n, order, range);
     if (nmax_out) {
       hipLaunchKernelGGL(gm::k_nmax_i32, dim3(gblk), dim3(256), 0, s, n, y ? y : x, nmax_out);
     }
@@ -250,9 +248,15 @@ int gm_mie_batch(gm_ctx* ctx, int64_t n, const double* x, const double* y, const
   HIP_OK(hipGetLastError());
   if (y) {
     ctx->mark_begin(gm::kKMieCoated, s, &slot);
-    hipLaunchKernelGGL(gm::k_mie_coated, dim3((unsigned)slots), dim3(64), 0, s, order, range, n,
-                       x, y, eps, mu, eps2, coef_off, an, bn, props, status, scratch, rows_cap,
-                       dn_extra);
+    hipLaunchKernelGGL(gm::k_mie_coated_stable, dim3((unsigned)slots), dim3(64), 0, s, order,
+                       range, n, x, y, eps, mu, eps2, coef_off, an, bn, props, status, scratch,
+                       rows_cap, dn_extra);
+    ctx->mark_end(slot, s);
+    HIP_OK(hipGetLastError());
+    ctx->mark_begin(gm::kKMieCoatedLit, s, &slot);
+    hipLaunchKernelGGL(gm::k_mie_coated_literal, dim3((unsigned)slots), dim3(64), 0, s, order,
+                       range, n, x, y, eps, mu, eps2, coef_off, an, bn, props, status, scratch,
+                       rows_cap, dn_extra);
     ctx->mark_end(slot, s);
     HIP_OK(hipGetLastError());
   }

@@ -7,7 +7,7 @@
 #include <initializer_list>
 #include <vector>
 
-#include "../../include/geosmie.h"
+#include "geosmie.h"
 
 int gm_fail(int code, const char* fmt, ...);
 
@@ -50,7 +50,7 @@ struct gm_ctx {
 namespace gm {
 // kernel ids for gm_ctx_kernel_time
 enum KernelId { kKMieHomog = 0, kKMieCoated = 1, kKS12 = 2, kKPsd = 3, kKGsf = 4, kKMlp = 5,
-                kKCount = 6 };
+                kKMieCoatedLit = 6, kKCount = 7 };
 }  // namespace gm
 
 namespace gm {

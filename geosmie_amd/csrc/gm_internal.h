@@ -8,6 +8,9 @@ namespace gm {
 // Counting-sort buckets for the recurrence-length sort: cost >> kBucketShift.
 constexpr int kBucketShift = 2;
 constexpr int kBuckets = 1024;
+// point classes: 0 homogeneous, 1 coated (real / weakly absorbing shell),
+// 2 coated (absorbing shell)
+constexpr int kClasses = 3;
 
 __global__ void k_zero_i64(int64_t* p, int n);
 __global__ void k_plan_keys(int64_t n, const double* x, const double* y, const double* eps,
@@ -16,17 +19,22 @@ __global__ void k_plan_keys(int64_t n, const double* x, const double* y, const d
                             int32_t* status);
 __global__ void k_plan_scan(const int64_t* hist, int64_t* cursor, int64_t* range);
 __global__ void k_plan_scatter(int64_t n, const int32_t* key, int64_t* cursor, int64_t* order);
-__global__ void k_plan_identity(int64_t n, int cls, int64_t* order, int64_t* range);
+__global__ void k_plan_identity(int64_t n, int64_t* order, int64_t* range);
 __global__ void k_mie_homog(const int64_t* order, const int64_t* range, int64_t n_all,
                             const double* x, const double* y, const double* eps, const double* mu,
                             const double* eps2, const int64_t* coef_off, double* an, double* bn,
                             double* props, int32_t* status, double* scratch, int rows_cap,
                             int dn_extra);
-__global__ void k_mie_coated(const int64_t* order, const int64_t* range, int64_t n_all,
-                             const double* x, const double* y, const double* eps, const double* mu,
-                             const double* eps2, const int64_t* coef_off, double* an, double* bn,
-                             double* props, int32_t* status, double* scratch, int rows_cap,
-                             int dn_extra);
+__global__ void k_mie_coated_stable(const int64_t* order, const int64_t* range, int64_t n_all,
+                                    const double* x, const double* y, const double* eps,
+                                    const double* mu, const double* eps2, const int64_t* coef_off,
+                                    double* an, double* bn, double* props, int32_t* status,
+                                    double* scratch, int rows_cap, int dn_extra);
+__global__ void k_mie_coated_literal(const int64_t* order, const int64_t* range, int64_t n_all,
+                                     const double* x, const double* y, const double* eps,
+                                     const double* mu, const double* eps2, const int64_t* coef_off,
+                                     double* an, double* bn, double* props, int32_t* status,
+                                     double* scratch, int rows_cap, int dn_extra);
 __global__ void k_nmax_of(int64_t n, const double* size, int64_t* cnt);
 
 // Doubles of scratch per row per lane.

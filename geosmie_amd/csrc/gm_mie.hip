@@ -1,7 +1,8 @@
 // Mie coefficient kernels for gfx950: homogeneous and coated spheres, FP64.
 //
-// One (size, m) point per lane.  Points are first classified (homogeneous /
-// coated, following the dispatch of
+// One (size, m) point per lane.  Points are first classified (homogeneous,
+// coated with a real or weakly absorbing shell, coated with an absorbing
+// shell; the homogeneous/coated split follows the dispatch of
 // src/pymiecoated/pymiecoated/mie_coeffs.py:53-73) and counting-sorted by
 // recurrence length, longest first, so that the 64 lanes of a wavefront run
 // nearly equal trip counts and the hardware dispatcher hands out the long
@@ -42,10 +43,16 @@ __device__ __forceinline__ int nmx_rule(int nmax, double zabs) {
 __device__ __forceinline__ int ratio_start(int nmax, double zabs) {
   return nmax + 16 + 8 * (int)ceil(cbrt(zabs));
 }
+// Whether a downward D_n recurrence started at nmx has converged (relative
+// error < ~1e-17) by the time it reaches n <= |z|: the error decays like
+// exp(-(4/3) zeta^(3/2)) across the Airy region, zeta = (nmx - |z|) / (|z|/2)^(1/3).
+__device__ __forceinline__ bool converged_start(int nmx, double zabs) {
+  return (double)nmx - zabs >= 8.0 * cbrt(zabs);
+}
 
 // Effective per-point parameters after the reference's dispatch.
 struct PointDesc {
-  int cls;       // 0 homogeneous, 1 coated
+  int cls;       // 0 homogeneous, 1 coated (|Im m2| y <= 1), 2 coated (absorbing shell)
   double size;   // x for homogeneous (after dispatch), y for coated
   double xcore;  // coated core size
   cd eps, mu, eps2;
@@ -79,7 +86,12 @@ __device__ __forceinline__ PointDesc describe(int64_t p, const double* __restric
     d.cls = 0;
     d.eps = e2;
   } else {
-    d.cls = 1;
+    // The cancellation-free shell combination (k_mie_coated_stable) suits real
+    // and weakly absorbing shells; strongly absorbing shells (no real zeros of
+    // psi_n(m2 y), but e^{Im}-sized psi/chi products) use the reference's
+    // ratio form (k_mie_coated_literal).
+    cd m2 = csqrt_(e2);
+    d.cls = (fabs(m2.im) * yy <= 1.0) ? 1 : 2;
   }
   return d;
 }
@@ -94,6 +106,7 @@ __device__ __forceinline__ int point_cost(const PointDesc& d, int* nmax_out) {
   }
   cd m1 = csqrt_(d.eps), m2 = csqrt_(d.eps2);
   double mx = fmax(cabs_(m1) * d.size, cabs_(m2) * d.size);
+  if (d.cls == 1) return nmx_rule(nmax, mx) + 4 * nmax;
   return 3 * nmx_rule(nmax, mx) + 6 * nmax;
 }
 
@@ -112,9 +125,9 @@ __global__ __launch_bounds__(256) void k_plan_keys(
     const double* __restrict__ eps, const double* __restrict__ mu,
     const double* __restrict__ eps2, int32_t* __restrict__ key, int32_t* __restrict__ nmax_out,
     int64_t* __restrict__ hist, int32_t* __restrict__ max_nmax, int32_t* __restrict__ status) {
-  __shared__ int32_t lh[2 * kBuckets];
+  __shared__ int32_t lh[kClasses * kBuckets];
   __shared__ int32_t lmax;
-  for (int i = threadIdx.x; i < 2 * kBuckets; i += blockDim.x) lh[i] = 0;
+  for (int i = threadIdx.x; i < kClasses * kBuckets; i += blockDim.x) lh[i] = 0;
   if (threadIdx.x == 0) lmax = 0;
   __syncthreads();
   int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -131,18 +144,18 @@ __global__ __launch_bounds__(256) void k_plan_keys(
     if (y && mu && (d.mu.re != 1.0 || d.mu.im != 0.0)) atomicOr(status, 2);
   }
   __syncthreads();
-  for (int i = threadIdx.x; i < 2 * kBuckets; i += blockDim.x)
+  for (int i = threadIdx.x; i < kClasses * kBuckets; i += blockDim.x)
     if (lh[i]) atomicAdd((unsigned long long*)&hist[i], (unsigned long long)lh[i]);
   if (threadIdx.x == 0) atomicMax(max_nmax, lmax);
 }
 
-// Exclusive scan of the 2*kBuckets histogram (one block of 256).  cursor[k] =
-// start of bucket k; range = {homog begin, homog end, coated begin, coated end}.
+// Exclusive scan of the kClasses*kBuckets histogram (one block of 256).
+// cursor[k] = start of bucket k; range[2c], range[2c+1] = [begin, end) of class c.
 __global__ __launch_bounds__(256) void k_plan_scan(const int64_t* __restrict__ hist,
                                                    int64_t* __restrict__ cursor,
                                                    int64_t* __restrict__ range) {
   __shared__ int64_t part[256];
-  const int per = (2 * kBuckets) / 256;
+  const int per = (kClasses * kBuckets) / 256;
   int t = threadIdx.x;
   int64_t s = 0;
   for (int i = 0; i < per; ++i) s += hist[t * per + i];
@@ -163,13 +176,12 @@ __global__ __launch_bounds__(256) void k_plan_scan(const int64_t* __restrict__ h
     acc += hist[t * per + i];
   }
   if (t == 0) {
-    int64_t nh = 0, nc = 0;
-    for (int i = 0; i < kBuckets; ++i) nh += hist[i];
-    for (int i = kBuckets; i < 2 * kBuckets; ++i) nc += hist[i];
-    range[0] = 0;
-    range[1] = nh;
-    range[2] = nh;
-    range[3] = nh + nc;
+    int64_t acc2 = 0;
+    for (int c = 0; c < kClasses; ++c) {
+      range[2 * c] = acc2;
+      for (int i = c * kBuckets; i < (c + 1) * kBuckets; ++i) acc2 += hist[i];
+      range[2 * c + 1] = acc2;
+    }
   }
 }
 
@@ -177,9 +189,9 @@ __global__ __launch_bounds__(256) void k_plan_scan(const int64_t* __restrict__ h
 __global__ __launch_bounds__(256) void k_plan_scatter(int64_t n, const int32_t* __restrict__ key,
                                                       int64_t* __restrict__ cursor,
                                                       int64_t* __restrict__ order) {
-  __shared__ int32_t lcnt[2 * kBuckets];
-  __shared__ int64_t lbase[2 * kBuckets];
-  for (int i = threadIdx.x; i < 2 * kBuckets; i += blockDim.x) lcnt[i] = 0;
+  __shared__ int32_t lcnt[kClasses * kBuckets];
+  __shared__ int64_t lbase[kClasses * kBuckets];
+  for (int i = threadIdx.x; i < kClasses * kBuckets; i += blockDim.x) lcnt[i] = 0;
   __syncthreads();
   int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   int k = -1, rank = 0;
@@ -188,24 +200,24 @@ __global__ __launch_bounds__(256) void k_plan_scatter(int64_t n, const int32_t* 
     rank = atomicAdd(&lcnt[k], 1);
   }
   __syncthreads();
-  for (int i = threadIdx.x; i < 2 * kBuckets; i += blockDim.x)
+  for (int i = threadIdx.x; i < kClasses * kBuckets; i += blockDim.x)
     if (lcnt[i])
       lbase[i] = (int64_t)atomicAdd((unsigned long long*)&cursor[i], (unsigned long long)lcnt[i]);
   __syncthreads();
   if (k >= 0) order[lbase[k] + rank] = p;
 }
 
-// Identity order (no sort): homogeneous-only or coated-only batches in input order.
-__global__ void k_plan_identity(int64_t n, int cls, int64_t* __restrict__ order,
+// Identity order (no sort): every class kernel walks all points and keeps
+// those of its own class.
+__global__ void k_plan_identity(int64_t n, int64_t* __restrict__ order,
                                 int64_t* __restrict__ range) {
   int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (p < n) order[p] = p;
-  if (p == 0) {
-    range[0] = 0;
-    range[1] = cls == 0 ? n : 0;
-    range[2] = cls == 0 ? n : 0;
-    range[3] = n;
-  }
+  if (p == 0)
+    for (int c = 0; c < kClasses; ++c) {
+      range[2 * c] = 0;
+      range[2 * c + 1] = n;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -229,10 +241,10 @@ struct Props {
     qb = qb + (a - b) * sg;
     if (n > 1) {
       double dm = dn - 1.0;  // previous index
-      double c1 = dm * (dm + 2.0) / (dm + 1.0);
+      double c1 = dm * (dm + 2.0) * rcp_nr(dm + 1.0);
       asy1 = fma(c1, ap.re * a.re + ap.im * a.im + bp.re * b.re + bp.im * b.im, asy1);
     }
-    double c2 = cn / (dn * (dn + 1.0));
+    double c2 = cn * rcp_nr(dn * (dn + 1.0));
     asy2 = fma(c2, a.re * b.re + a.im * b.im, asy2);
     ap = a;
     bp = b;
@@ -276,6 +288,7 @@ __global__ __launch_bounds__(64) void k_mie_homog(
     if (idx >= end) continue;
     int64_t p = order[idx];
     PointDesc d = describe(p, x, y, eps, mu, eps2);
+    if (d.cls != 0) continue;
     const double s = d.size;
     const int nmax = nmax_rule(s);
     if (nmax > rows_cap) {
@@ -306,7 +319,7 @@ __global__ __launch_bounds__(64) void k_mie_homog(
         double r = mul_dd((double)n, xinv);
         double t = dd + r;
         if (n <= nmax) Rp[(size_t)(n - 1) * 64 + lane] = t;
-        dd = r - 1.0 / t;
+        dd = r - rcp_nr(t);
       }
     }
     double sn, cs;
@@ -317,11 +330,20 @@ __global__ __launch_bounds__(64) void k_mie_homog(
     pr.init();
     const bool store = coef_off != nullptr;
     int64_t off = store ? coef_off[p] : 0;
+    // one-row-ahead prefetch of the scratch rows (hides their latency
+    // behind the previous row's arithmetic)
+    cd D_nx = Dp[lane];
+    double R_nx = Rp[lane];
     for (int n = 1; n <= nmax; ++n) {
+      const cd D = D_nx;
+      const double R = R_nx;
+      if (n < nmax) {
+        D_nx = Dp[(size_t)n * 64 + lane];
+        R_nx = Rp[(size_t)n * 64 + lane];
+      }
       double f = mul_dd((double)(2 * n - 1), xinv);
-      double psi = (n <= nsw) ? fma(f, psi_p, -psi_pp) : psi_p / Rp[(size_t)(n - 1) * 64 + lane];
+      double psi = (n <= nsw) ? fma(f, psi_p, -psi_pp) : psi_p * rcp_nr(R);
       double chi = fma(f, chi_p, -chi_pp);
-      cd D = Dp[(size_t)(n - 1) * 64 + lane];
       double nx = mul_dd((double)n, xinv);
       cd da = mul_dd(D, minv) + nx;
       cd db = D * m + nx;
@@ -346,7 +368,191 @@ __global__ __launch_bounds__(64) void k_mie_homog(
 }
 
 // ---------------------------------------------------------------------------
-// Coated sphere, mie_coeffs.py:183-251 + mie_props.py:28-70 (size = y)
+// Recurrence helpers (one lane, rows [n-1][lane] of a scratch plane)
+// ---------------------------------------------------------------------------
+// D_1..D_nmax of z from D_nstart = 0 (mie_coeffs.py:118-121); subtract=true
+// stores plane[row] - D instead (difference of two starts).
+__device__ __forceinline__ void dn_chain(cdd zi, int nstart, int nmax, cd* __restrict__ plane,
+                                         int lane, bool subtract) {
+  cd dd = cd{0.0, 0.0};
+  for (int j = nstart - 1; j >= 1; --j) {
+    cd r = mul_dd((double)(j + 1), zi);
+    dd = r - crcp_fast(dd + r);
+    if (j <= nmax) {
+      const size_t o = (size_t)(j - 1) * 64 + lane;
+      plane[o] = subtract ? plane[o] - dd : dd;
+    }
+  }
+}
+// psi_{n-1}(z)/psi_n(z) = D_n(z) + n/z for nsw < n <= nmax, converged start.
+__device__ __forceinline__ void ratio_chain(cd z, int nsw, int nmax, cd* __restrict__ plane,
+                                            int lane) {
+  const cdd zi = crcp_dd(z);
+  cd dd = cd{0.0, 0.0};
+  for (int n = ratio_start(nmax, cabs_(z)); n > nsw; --n) {
+    cd r = mul_dd((double)n, zi);
+    cd t = dd + r;
+    if (n <= nmax) plane[(size_t)(n - 1) * 64 + lane] = t;
+    dd = r - crcp_fast(t);
+  }
+}
+__device__ __forceinline__ void ratio_chain(double z, int nsw, int nmax, double* __restrict__ plane,
+                                            int lane) {
+  const ddr zi = rcp_dd(z);
+  double dd = 0.0;
+  for (int n = ratio_start(nmax, z); n > nsw; --n) {
+    double r = mul_dd((double)n, zi);
+    double t = dd + r;
+    if (n <= nmax) plane[(size_t)(n - 1) * 64 + lane] = t;
+    dd = r - rcp_nr(t);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Coated sphere, real or weakly absorbing shell (|Im m2| y <= 1)
+// ---------------------------------------------------------------------------
+// The coefficients of mie_coeffs.py:183-251, with the shell combination
+// (:230-244) rearranged so that nothing is divided by psi_n(m2 y) and no
+// D_n(m2 x) is multiplied by psi_n(m2 x).  The literal formula does both: near
+// a zero of either function its FP64 evaluation (the reference's included)
+// loses up to -2 log10|psi| digits, up to 1e-3 in Q.  With psi'_n =
+// psi_{n-1} - (n/z) psi_n, chi' likewise and psi' chi - psi chi' = 1:
+//   uu psi_v = m D_u psi_v - psi'_v - dD_v psi_v
+//   dns      = [psi'_w + uu psi_v (psi'_w chi_v - chi'_w psi_v)]
+//            / [psi_w  + uu psi_v (psi_w  chi_v - chi_w  psi_v)] + dD_w
+// (vv and gns alike), identical in exact arithmetic to the reference's
+// dns1 + dnw.  dD = D_ref - D_exact corrects for the reference's start index
+// (D_ref from nmx = round(max(nmax, |m1 y|, |w|) + 16)); it vanishes unless
+// that start is within a few Airy widths of |v| or |w|, and is only then
+// computed (two extra recurrences).  D_n(m2 x) and D_n(m2 y) are otherwise
+// not needed at all: pass 1 runs only the D_n(m1 x) recurrence.
+__global__ __launch_bounds__(64) void k_mie_coated_stable(
+    const int64_t* __restrict__ order, const int64_t* __restrict__ range, int64_t n_all,
+    const double* __restrict__ x, const double* __restrict__ y, const double* __restrict__ eps,
+    const double* __restrict__ mu, const double* __restrict__ eps2,
+    const int64_t* __restrict__ coef_off, double* __restrict__ an, double* __restrict__ bn,
+    double* __restrict__ props, int32_t* __restrict__ status, double* __restrict__ scratch,
+    int rows_cap, int dn_extra) {
+  const int lane = threadIdx.x;
+  const int64_t begin = range[2], end = range[3];
+  // 11 doubles per row: DU, RV, RW, DDV, DDW (complex), RY (real)
+  double* slot = scratch + (size_t)blockIdx.x * (size_t)rows_cap * 64 * 11;
+  const size_t plane = (size_t)rows_cap * 64;
+  cd* DU = reinterpret_cast<cd*>(slot);
+  cd* RV = reinterpret_cast<cd*>(slot + 2 * plane);
+  cd* RW = reinterpret_cast<cd*>(slot + 4 * plane);
+  cd* DDV = reinterpret_cast<cd*>(slot + 6 * plane);
+  cd* DDW = reinterpret_cast<cd*>(slot + 8 * plane);
+  double* RY = slot + 10 * plane;
+
+  for (int64_t base = begin + (int64_t)blockIdx.x * 64; base < end;
+       base += (int64_t)gridDim.x * 64) {
+    const int64_t idx = base + lane;
+    if (idx >= end) continue;
+    const int64_t p = order[idx];
+    PointDesc d = describe(p, x, y, eps, mu, eps2);
+    if (d.cls != 1) continue;
+    const double yy = d.size, xx = d.xcore;
+    const int nmax = nmax_rule(yy);
+    if (nmax > rows_cap) {
+      atomicOr(status, 1);
+      continue;
+    }
+    const cd m1 = csqrt_(d.eps), m2 = csqrt_(d.eps2);
+    const cd m = cdiv(m2, m1);
+    const cd u = m1 * xx, v = m2 * xx, w = m2 * yy;
+    const double mxv = fmax(cabs_(m1 * yy), cabs_(w));
+    const int nmx =
+        nmx_rule(nmax, mxv) + (dn_extra < 0 ? 16 + 10 * (int)ceil(cbrt(mxv)) : dn_extra);
+    const double av = cabs_(v), aw = cabs_(w);
+    // ---- pass 1: downward recurrences
+    dn_chain(crcp_dd(u), nmx, nmax, DU, lane, false);
+    const bool ddv_on = !converged_start(nmx, av), ddw_on = !converged_start(nmx, aw);
+    if (ddv_on) {
+      const cdd vi = crcp_dd(v);
+      dn_chain(vi, nmx, nmax, DDV, lane, false);
+      dn_chain(vi, nmx + 16 + 8 * (int)ceil(cbrt(av)), nmax, DDV, lane, true);
+    }
+    if (ddw_on) {
+      const cdd wi = crcp_dd(w);
+      dn_chain(wi, nmx, nmax, DDW, lane, false);
+      dn_chain(wi, nmx + 16 + 8 * (int)ceil(cbrt(aw)), nmax, DDW, lane, true);
+    }
+    const int nsw_v = min((int)av, nmax), nsw_w = min((int)aw, nmax), nsw_y = min((int)yy, nmax);
+    if (nsw_v < nmax) ratio_chain(v, nsw_v, nmax, RV, lane);
+    if (nsw_w < nmax) ratio_chain(w, nsw_w, nmax, RW, lane);
+    if (nsw_y < nmax) ratio_chain(yy, nsw_y, nmax, RY, lane);
+    // ---- pass 2: upward Riccati-Bessel recurrences and the combination
+    const cdd vinv = crcp_dd(v), winv = crcp_dd(w), minv = crcp_dd(m), m2inv = crcp_dd(m2);
+    const ddr yinv = rcp_dd(yy);
+    cd pv_pp = ccos_(v), pv_p = csin_(v);   // psi_{-1}(v), psi_0(v)
+    cd cv_pp = -pv_p, cv_p = pv_pp;         // chi_{-1}(v), chi_0(v)
+    cd pw_pp = ccos_(w), pw_p = csin_(w);
+    cd cw_pp = -pw_p, cw_p = pw_pp;
+    double sn, cs;
+    sincos(yy, &sn, &cs);
+    double py_pp = cs, py_p = sn, cy_pp = -sn, cy_p = cs;
+    Props pr;
+    pr.init();
+    const bool store = coef_off != nullptr;
+    const int64_t off = store ? coef_off[p] : 0;
+    cd du_nx = DU[lane];
+    for (int n = 1; n <= nmax; ++n) {
+      const size_t o = (size_t)(n - 1) * 64 + lane;
+      const cd du = du_nx;
+      if (n < nmax) du_nx = DU[o + 64];
+      const double fn = (double)(2 * n - 1), dn = (double)n;
+      const cd fv = mul_dd(fn, vinv), fw = mul_dd(fn, winv);
+      const cd pv = (n <= nsw_v) ? fv * pv_p - pv_pp : cdiv(pv_p, RV[o]);
+      const cd chv = fv * cv_p - cv_pp;
+      const cd pw = (n <= nsw_w) ? fw * pw_p - pw_pp : cdiv(pw_p, RW[o]);
+      const cd chw = fw * cw_p - cw_pp;
+      const double f = mul_dd(fn, yinv);
+      const double py = (n <= nsw_y) ? fma(f, py_p, -py_pp) : py_p * rcp_nr(RY[o]);
+      const double chy = fma(f, cy_p, -cy_pp);
+      // derivatives psi'_n = psi_{n-1} - (n/z) psi_n
+      const cd dpv = pv_p - mul_dd(dn, vinv) * pv;
+      const cd nw = mul_dd(dn, winv);
+      const cd dpw = pw_p - nw * pw, dcw = cw_p - nw * chw;
+      const cd t = du * pv;
+      cd U = m * t - dpv, V = mul_dd(t, minv) - dpv;
+      if (ddv_on) {
+        const cd c = DDV[o] * pv;
+        U = U - c;
+        V = V - c;
+      }
+      const cd A = pw * chv - chw * pv, B = dpw * chv - dcw * pv;
+      cd dns = cdiv(dpw + U * B, pw + U * A);
+      cd gns = cdiv(dpw + V * B, pw + V * A);
+      if (ddw_on) {
+        const cd c = DDW[o];
+        dns = dns + c;
+        gns = gns + c;
+      }
+      const double nrat = mul_dd(dn, yinv);
+      const cd a1 = mul_dd(dns, m2inv) + nrat;
+      const cd b1 = m2 * gns + nrat;
+      const cd gsy = cd{py, -chy}, gs1y = cd{py_p, -cy_p};
+      const cd a = cdiv(a1 * py - py_p, gsy * a1 - gs1y);
+      const cd b = cdiv(b1 * py - py_p, gsy * b1 - gs1y);
+      pr.add(n, a, b);
+      if (store) {
+        an[2 * (off + n - 1)] = a.re;
+        an[2 * (off + n - 1) + 1] = a.im;
+        bn[2 * (off + n - 1)] = b.re;
+        bn[2 * (off + n - 1) + 1] = b.im;
+      }
+      pv_pp = pv_p; pv_p = pv; cv_pp = cv_p; cv_p = chv;
+      pw_pp = pw_p; pw_p = pw; cw_pp = cw_p; cw_p = chw;
+      py_pp = py_p; py_p = py; cy_pp = cy_p; cy_p = chy;
+    }
+    if (props) pr.finish(yy, p, n_all, props);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Coated sphere, mie_coeffs.py:183-251 + mie_props.py:28-70 (size = y), literal
+// form for absorbing shells (|Im m2| y > 1)
 // ---------------------------------------------------------------------------
 // Complex Riccati-Bessel upward step with the oscillatory / ratio switch.
 struct RBc {
@@ -361,7 +567,7 @@ struct RBc {
   }
 };
 
-__global__ __launch_bounds__(64) void k_mie_coated(
+__global__ __launch_bounds__(64) void k_mie_coated_literal(
     const int64_t* __restrict__ order, const int64_t* __restrict__ range, int64_t n_all,
     const double* __restrict__ x, const double* __restrict__ y, const double* __restrict__ eps,
     const double* __restrict__ mu, const double* __restrict__ eps2,
@@ -369,7 +575,7 @@ __global__ __launch_bounds__(64) void k_mie_coated(
     double* __restrict__ props, int32_t* __restrict__ status, double* __restrict__ scratch,
     int rows_cap, int dn_extra) {
   const int lane = threadIdx.x;
-  const int64_t begin = range[2], end = range[3];
+  const int64_t begin = range[4], end = range[5];
   // 11 doubles per row: uu, vv, dnw, rv, rw (complex), ry (real)
   double* slot = scratch + (size_t)blockIdx.x * (size_t)rows_cap * 64 * 11;
   const size_t plane = (size_t)rows_cap * 64;
@@ -386,6 +592,7 @@ __global__ __launch_bounds__(64) void k_mie_coated(
     if (idx >= end) continue;
     int64_t p = order[idx];
     PointDesc d = describe(p, x, y, eps, mu, eps2);
+    if (d.cls != 2) continue;
     const double yy = d.size, xx = d.xcore;
     const int nmax = nmax_rule(yy);
     if (nmax > rows_cap) {
@@ -448,7 +655,7 @@ __global__ __launch_bounds__(64) void k_mie_coated(
         double r = mul_dd((double)n, yinv);
         double t = dd + r;
         if (n <= nmax) RY[(size_t)(n - 1) * 64 + lane] = t;
-        dd = r - 1.0 / t;
+        dd = r - rcp_nr(t);
       }
     }
     double sn, cs;
@@ -460,24 +667,36 @@ __global__ __launch_bounds__(64) void k_mie_coated(
     pr.init();
     const bool store = coef_off != nullptr;
     int64_t off = store ? coef_off[p] : 0;
+    // one-row-ahead prefetch of the scratch rows
+    cd uu_nx = UU[lane], vv_nx = VV[lane], dw_nx = DW[lane], rv_nx = RV[lane], rw_nx = RW[lane];
+    double ry_nx = RY[lane];
     for (int n = 1; n <= nmax; ++n) {
-      size_t o = (size_t)(n - 1) * 64 + lane;
+      const cd uu = uu_nx, vv = vv_nx, dnw = dw_nx, rvn = rv_nx, rwn = rw_nx;
+      const double ryn = ry_nx;
+      if (n < nmax) {
+        const size_t o = (size_t)n * 64 + lane;
+        uu_nx = UU[o];
+        vv_nx = VV[o];
+        dw_nx = DW[o];
+        rv_nx = RV[o];
+        rw_nx = RW[o];
+        ry_nx = RY[o];
+      }
       double fn = (double)(2 * n - 1);
       // psi/chi at v
       const cd fv_ = mul_dd(fn, vinv), fw_ = mul_dd(fn, winv);
-      cd pv = (n <= bv.nsw) ? fv_ * bv.psi_p - bv.psi_pp : cdiv(bv.psi_p, RV[o]);
+      cd pv = (n <= bv.nsw) ? fv_ * bv.psi_p - bv.psi_pp : cdiv(bv.psi_p, rvn);
       cd chv = fv_ * bv.chi_p - bv.chi_pp;
       bv.psi_pp = bv.psi_p; bv.psi_p = pv; bv.chi_pp = bv.chi_p; bv.chi_p = chv;
       // psi/chi at w
-      cd pw = (n <= bw.nsw) ? fw_ * bw.psi_p - bw.psi_pp : cdiv(bw.psi_p, RW[o]);
+      cd pw = (n <= bw.nsw) ? fw_ * bw.psi_p - bw.psi_pp : cdiv(bw.psi_p, rwn);
       cd chw = fw_ * bw.chi_p - bw.chi_pp;
       bw.psi_pp = bw.psi_p; bw.psi_p = pw; bw.chi_pp = bw.chi_p; bw.chi_p = chw;
       // psi/chi at y (real)
       double f = mul_dd(fn, yinv);
-      double py = (n <= nsw_y) ? fma(f, py_p, -py_pp) : py_p / RY[o];
+      double py = (n <= nsw_y) ? fma(f, py_p, -py_pp) : py_p * rcp_nr(ryn);
       double chy = fma(f, cy_p, -cy_pp);
       // shell combination, mie_coeffs.py:230-249
-      cd uu = UU[o], vv = VV[o], dnw = DW[o];
       cd fv = cdiv(pv, chv);
       cd fvw = cdiv(fv, pw);
       cd ku1 = uu * fvw;

@@ -176,12 +176,87 @@ static int coated(double x, double y, cplx m1, cplx m2, cplx* an, cplx* bn, cplx
 }
 
 /*
+ * Coated sphere, the same coefficients rearranged to avoid the reference's
+ * divisions by psi_n(m2 y) (ku1 = uu fv / pw, mie_coeffs.py:234) and its
+ * products D_n(m2 x) psi_n(m2 x) near zeros of psi_n(m2 x), where any FP64
+ * evaluation of the literal formula loses up to ~|log10 psi| digits.
+ * With psi' = psi_{n-1} - (n/z) psi_n, chi' likewise, and the Wronskian
+ * psi' chi - psi chi' = 1:
+ *   uu psi_v = m D_u psi_v - psi'_v - dD_v psi_v,   (dD = D_ref - D_exact)
+ *   dns = [psi'_w + uu psi_v (psi'_w chi_v - chi'_w psi_v)]
+ *       / [psi_w + uu psi_v (psi_w chi_v - chi_w psi_v)] + dD_w
+ * which equals dns1 + dnw of mie_coeffs.py:230-244 exactly.  dD_v, dD_w are
+ * zero unless the reference's start index is too close to |v| or |w| for
+ * D_n to have converged (then both recurrences are run and differenced).
+ */
+static int converged_start(int nmx, double zabs) { return (double)nmx - zabs >= 8.0 * cbrt(zabs); }
+
+static int coated_stable(double x, double y, cplx m1, cplx m2, cplx* an, cplx* bn, cplx* work) {
+  cplx m = m2 / m1, u = m1 * x, v = m2 * x, w = m2 * y;
+  int nmax = nmax_rule(y);
+  double mx = fmax(cabs(m1 * y), cabs(w));
+  int nmx = nmx_rule(nmax, mx);
+  cplx* dnu = work;
+  cplx* ddv = dnu + nmax;
+  cplx* ddw = ddv + nmax;
+  cplx* pv = ddw + nmax;
+  cplx* chv = pv + nmax;
+  cplx* pw = chv + nmax + 1;
+  cplx* chw = pw + nmax;
+  cplx* py = chw + nmax + 1;
+  cplx* chy = py + nmax;
+  cplx* t1 = chy + nmax + 1;
+  cplx* t2 = t1 + nmax;
+  cplx* tmp = t2 + nmax;
+  dn_down(u, nmx, nmax, dnu);
+  cplx zs[2] = {v, w};
+  cplx* dd[2] = {ddv, ddw};
+  for (int q = 0; q < 2; ++q) {
+    double za = cabs(zs[q]);
+    if (converged_start(nmx, za)) {
+      for (int i = 0; i < nmax; ++i) dd[q][i] = 0.0;
+    } else {
+      dn_down(zs[q], nmx, nmax, t1);
+      dn_down(zs[q], nmx + 16 + 8 * (int)ceil(cbrt(za)), nmax, t2);
+      for (int i = 0; i < nmax; ++i) dd[q][i] = t1[i] - t2[i];
+    }
+  }
+  riccati(v, nmax, pv, chv, tmp);
+  riccati(w, nmax, pw, chw, tmp);
+  riccati(y, nmax, py, chy, tmp);
+  cplx pv0 = csin(v), pw0 = csin(w);
+  double sy = sin(y);
+  for (int n = 1; n <= nmax; ++n) {
+    int i = n - 1;
+    cplx PV = pv[i], CHV = chv[n], PW = pw[i], CHW = chw[n];
+    cplx pv1 = (n == 1) ? pv0 : pv[i - 1];
+    cplx pw1 = (n == 1) ? pw0 : pw[i - 1];
+    cplx dpv = pv1 - (double)n / v * PV;
+    cplx dpw = pw1 - (double)n / w * PW;
+    cplx dcw = chw[n - 1] - (double)n / w * CHW;
+    cplx U = m * dnu[i] * PV - dpv - ddv[i] * PV;
+    cplx V = cdivs(dnu[i], m) * PV - dpv - ddv[i] * PV;
+    cplx A = PW * CHV - CHW * PV, B = dpw * CHV - dcw * PV;
+    cplx dns = cdivs(dpw + U * B, PW + U * A) + ddw[i];
+    cplx gns = cdivs(dpw + V * B, PW + V * A) + ddw[i];
+    double nrat = n / y;
+    cplx a1 = cdivs(dns, m2) + nrat, b1 = m2 * gns + nrat;
+    double p = creal(py[i]), p1 = (n == 1) ? sy : creal(py[i - 1]);
+    double c = creal(chy[n]), c1 = creal(chy[n - 1]);
+    cplx gsy = CMPLX(p, -c), gs1y = CMPLX(p1, -c1);
+    an[i] = cdivs(p * a1 - p1, gsy * a1 - gs1y);
+    bn[i] = cdivs(p * b1 - p1, gsy * b1 - gs1y);
+  }
+  return nmax;
+}
+
+/*
  * Batch entry: props[6*n] planes (qext, qsca, qabs, qb, asy, qratio).
  * x[n], m[2n] interleaved; y == NULL -> homogeneous, else m2[2n] shell.
  * threads <= 0: OpenMP default.  Returns 0.
  */
 int cmie_batch(int64_t n, const double* x, const double* y, const double* m, const double* m2,
-               double* props, int threads) {
+               double* props, int threads, int stable) {
 #ifdef _OPENMP
   if (threads > 0) omp_set_num_threads(threads);
 #endif
@@ -213,6 +288,8 @@ int cmie_batch(int64_t n, const double* x, const double* y, const double* m, con
         int k;
         if (x[i] == y[i] || mi == mo) k = homog(y[i], mi, an, bn, work);
         else if (x[i] == 0.0) k = homog(y[i], mo, an, bn, work);
+        else if (stable && fabs(cimag(mo)) * y[i] <= 1.0)
+          k = coated_stable(x[i], y[i], mi, mo, an, bn, work);
         else k = coated(x[i], y[i], mi, mo, an, bn, work);
         props_of(an, bn, k, y[i], out);
       }

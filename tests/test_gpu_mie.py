@@ -137,12 +137,11 @@ def test_batch_homog_cfg1_sample_vs_oracle(gpu):
 def test_batch_coated_cfg2_sample_vs_oracle(gpu):
     """Coated core 1.95+0.79i with real and water-mixed shells (cfg2 ranges).
 
-    Against the C restatement (the reference's formulas; psi/chi by
-    recurrence, within 1e-10 of the exact coefficients) at 1e-9, and against
-    the scipy-based restatement within the reference's own error: scipy's
-    complex-argument jv/yv inside the shell combination (mie_coeffs.py:223-244)
-    leave up to ~5e-9 relative error at y ~ 170 even for real shells (the
-    exact mpmath value confirms the GPU side there; see the coated mp test).
+    Against the C restatement in its cancellation-free form (within 1e-11 of
+    the exact coefficients of the reference's formula) at 1e-9, and against
+    the scipy-based literal restatement within the reference's own FP64
+    error, which reaches ~5e-9 here (and up to 1e-3 near zeros of
+    psi_n(m2 x), psi_n(m2 y): see test_coated_full_workload_vs_oracle).
     """
     from geosmie_amd.mie import mie_batch
     from oracle import cmie
@@ -155,7 +154,7 @@ def test_batch_coated_cfg2_sample_vs_oracle(gpu):
     m1 = 1.95 + 0.79j
     r = mie_batch(ratio * y, m=m1, y=y, m2=m2)
     got = np.stack([r[k] for k in ("qext", "qsca", "qabs", "qb", "asy", "qratio")])
-    c = cmie.batch(ratio * y, m1, y, m2)
+    c = cmie.batch(ratio * y, m1, y, m2, stable=True)
     rel = _rel(got, c)
     worst = int(rel[[0, 1]].max(0).argmax())
     assert rel[[0, 1]].max() < 1e-9, (rel[[0, 1]].max(), worst, y[worst], ratio[worst], m2[worst],
@@ -166,6 +165,40 @@ def test_batch_coated_cfg2_sample_vs_oracle(gpu):
         q = M.mie_props_raw(an, bn, nmax, y[i])
         rel = _rel(got[:, i], q)
         assert rel[[0, 1]].max() < 2e-8, (y[i], ratio[i], rel)
+
+
+def test_coated_full_workload_vs_oracle(gpu):
+    """All 8.4 M points of the coated benchmark batch against the C restatement
+    (cancellation-free form, 16 host threads): Qext, Qsca within 1e-9, g within
+    1e-9 absolute.  The reference's literal formula evaluated in FP64 (the
+    same restatement with stable=False) is also reported: it differs from the
+    exact coefficients by more than 1e-9 at ~0.1% of these points, near zeros
+    of psi_n(m2 y) or psi_n(m2 x), where the literal form divides by them."""
+    import bench
+    from oracle import cmie
+    from geosmie_amd import _lib
+    import torch
+    x, y, m1, m2 = bench.coated_workload()
+    n = x.size
+    d_x, d_y = _lib.to_dev(x), _lib.to_dev(y)
+    d_eps = _lib.cplx_dev(np.full(n, m1 * m1))
+    d_eps2 = _lib.cplx_dev(m2 * m2)
+    d_props = torch.empty(6 * n, dtype=torch.float64, device="cuda")
+    _lib.check(_lib.lib().gm_mie_batch(_lib.ctx(), n, _lib.ptr(d_x), _lib.ptr(d_y),
+                                       _lib.ptr(d_eps), None, _lib.ptr(d_eps2), float(y.max()),
+                                       None, None, None, _lib.ptr(d_props), None,
+                                       _lib.GM_MIE_SORT, _lib.stream()))
+    assert _lib.check_flags() == 0
+    got = d_props.view(6, n).cpu().numpy()
+    c = cmie.batch(x, m1, y, m2, threads=16, stable=True)
+    rel = np.abs(got[:2] - c[:2]) / np.abs(c[:2])
+    worst = int(rel.max(0).argmax())
+    assert rel.max() < 1e-9, (rel.max(), worst, x[worst], y[worst], m2[worst])
+    assert np.abs(got[4] - c[4]).max() < 1e-9
+    assert np.all(np.isfinite(got))
+    lit = cmie.batch(x[::7], m1, y[::7], m2[::7], threads=16, stable=False)
+    frac = np.mean(np.abs(lit[1] - c[1, ::7]) / c[1, ::7] > 1e-9)
+    print("literal-formula FP64 points off by > 1e-9: %.4f%%" % (100 * frac))
 
 
 def test_batch_mixed_dispatch(gpu):
