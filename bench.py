@@ -49,6 +49,37 @@ def coated_flops(x, y, m1, m2):
     return float(np.sum(30 * nmx + 270 * nmax)), float(nmax.sum()), float(nmx.sum())
 
 
+def homog_lut_workload():
+    """2000 radii log-spaced 0.005-20 um x 10 wavelengths 0.3-2.5 um, m = 1.53+0.005i,
+    181 angles 0-180 deg, lognormal number PSD r_mode 0.5 um, sigma_g 2.0 with
+    weights n(r) dr (dr from getDR) normalised per wavelength."""
+    from geosmie_amd.lut import getDR, getLogNormPSD
+    r = np.logspace(np.log10(0.005), np.log10(20.0), 2000)
+    lam = np.logspace(np.log10(0.3), np.log10(2.5), 10)
+    ang = np.linspace(0.0, 180.0, 181)
+    x = (2 * np.pi * r[None, :] / lam[:, None]).ravel()           # [lambda][r]
+    dr = getDR(r)
+    w = np.zeros((lam.size, x.size))
+    for i, L in enumerate(lam):
+        xx = 2 * np.pi * r / L
+        p = getLogNormPSD(0.5, 2.0, xx, L, 1e9, 0.0) * (2 * np.pi / L) * dr
+        w[i, i * r.size:(i + 1) * r.size] = p / p.sum()
+    return x, lam, ang, w, 1.53 + 0.005j
+
+
+def largex_workload():
+    """5000 radii log-spaced 0.1-100 um x 8 wavelengths 0.3-1.0 um (x up to ~2100),
+    m_r ~ U(1.33, 1.60), m_i = 10^U(-6,-2) per point (rng 20261202), 1801 angles."""
+    rng = np.random.default_rng(20261202)
+    r = np.logspace(-1, 2, 5000)
+    lam = np.linspace(0.3, 1.0, 8)
+    x = (2 * np.pi * r[None, :] / lam[:, None]).ravel()
+    n = x.size
+    m = rng.uniform(1.33, 1.60, n) + 1j * 10 ** rng.uniform(-6, -2, n)
+    ang = np.linspace(0.0, 180.0, 1801)
+    return x, m, ang
+
+
 def cpu_baseline(x, y, m1, m2, target_s=12.0):
     from oracle import cmie
     cores = len(os.sched_getaffinity(0))
@@ -69,6 +100,151 @@ def cpu_baseline(x, y, m1, m2, target_s=12.0):
                       "C OpenMP restatement oracle/cmie.c" % (stride, n, dt)}
 
 
+class _Pipeline(object):
+    """Device-resident runoptics chain for a homogeneous batch: coefficients,
+    angle tables, fused S1/S2 -> Mueller on MFMA, PSD contraction, integratePSD
+    scalars, P normalisation (and optionally the GSF expansion)."""
+
+    def __init__(self, x, m, ang, w, lam_of_row, ranges, precision=0, gsf_ng=0):
+        import torch
+        from geosmie_amd import _lib
+        self.t, self.L = torch, _lib
+        self.n = x.size
+        self.x = x
+        self.size_max = float(x.max())
+        self.nmax_cap = int(np.rint(2 + self.size_max + 4 * self.size_max ** (1. / 3.)))
+        self.n_ang = ang.size
+        self.rows = w.shape[0]
+        self.lam = lam_of_row
+        self.ranges = ranges          # per row: contiguous [start, start + count) of its points
+        self.precision = precision
+        self.gsf_ng = gsf_ng
+        self.ang = np.ascontiguousarray(ang, dtype=np.float64)
+        dev = "cuda"
+        self.d_x = _lib.to_dev(x)
+        self.d_eps = _lib.cplx_dev(np.broadcast_to(np.asarray(m, np.complex128) ** 2, (self.n,)))
+        self.d_cost = _lib.to_dev(np.cos(np.radians(ang)))
+        self.d_theta = _lib.to_dev(np.radians(ang))
+        self.d_w = _lib.to_dev(w)
+        nm = np.rint(2 + x + 4 * x ** (1. / 3.)).astype(np.int64)
+        total = int(nm.sum())
+        self.sum_nmax = total
+        self.d_off = torch.empty(self.n + 1, dtype=torch.int64, device=dev)
+        self.d_an = torch.empty(2 * total, dtype=torch.float64, device=dev)
+        self.d_bn = torch.empty_like(self.d_an)
+        self.d_props = torch.empty(6 * self.n, dtype=torch.float64, device=dev)
+        self.d_nmax = torch.empty(self.n, dtype=torch.int32, device=dev)
+        self.d_pi = torch.empty(self.nmax_cap * self.n_ang, dtype=torch.float64, device=dev)
+        self.d_tau = torch.empty_like(self.d_pi)
+        self.d_mu = torch.empty(self.n * 4 * self.n_ang, dtype=torch.float64, device=dev)
+        self.d_pm = torch.empty(self.rows * 4 * self.n_ang, dtype=torch.float64, device=dev)
+        self.d_back = torch.empty(self.rows * 6, dtype=torch.float64, device=dev)
+        self.d_sc = torch.empty(self.rows * 16, dtype=torch.float64, device=dev)
+        self.d_one = _lib.to_dev(np.array([1.0]))
+        rr = x[None, :] * np.asarray(lam_of_row)[:, None] / (2 * np.pi)
+        reff0 = (w * rr ** 4).sum(1) / (w * rr ** 3).sum(1)
+        self.d_reff0 = _lib.to_dev(reff0)
+        if gsf_ng:
+            self.d_F = torch.empty(self.rows * 6 * self.n_ang, dtype=torch.float64, device=dev)
+            self.d_pmom = torch.empty(self.rows * 6 * gsf_ng, dtype=torch.float64, device=dev)
+
+    def step(self):
+        L, lib, h, s, P = self.L, self.L.lib(), self.L.ctx(), self.L.stream(), self.L.ptr
+        chk = L.check
+        chk(lib.gm_coef_offsets(h, self.n, P(self.d_x), P(self.d_off), s))
+        chk(lib.gm_mie_batch(h, self.n, P(self.d_x), None, P(self.d_eps), None, None,
+                             self.size_max, P(self.d_off), P(self.d_an), P(self.d_bn),
+                             P(self.d_props), P(self.d_nmax), L.GM_MIE_SORT, s))
+        chk(lib.gm_angle_tables(h, self.n_ang, P(self.d_cost), self.nmax_cap, P(self.d_pi),
+                                P(self.d_tau), s))
+        chk(lib.gm_s12_mueller(h, self.n, P(self.d_off), P(self.d_nmax), P(self.d_an),
+                               P(self.d_bn), self.n_ang, P(self.d_pi), P(self.d_tau),
+                               self.nmax_cap, P(self.d_mu), self.precision, s))
+        nc = 4 * self.n_ang
+        for r, (a, c) in enumerate(self.ranges):
+            chk(lib.gm_psd_contract(h, 1, c, nc, P(self.d_w[r, a:a + c]), P(self.d_mu[a * nc:]),
+                                    P(self.d_pm[r * nc:]), s))
+        for r in range(self.rows):
+            chk(lib.gm_integrate_psd(h, self.n, 1, P(self.d_x), P(self.d_props), P(self.d_w[r]),
+                                     P(self.d_one), P(self.d_reff0[r:r + 1]), float(self.lam[r]),
+                                     1000.0, 1000.0, P(self.d_sc[16 * r:16 * r + 16]), s))
+        chk(lib.gm_pmatrix_normalize(h, self.rows, self.n_ang, P(self.d_theta), P(self.d_pm),
+                                     P(self.d_back), s))
+        if self.gsf_ng:
+            pm = self.d_pm.view(self.rows, 4, self.n_ang)
+            F = self.d_F.view(self.rows, 6, self.n_ang)
+            F[:, 0] = pm[:, 0]
+            F[:, 1] = pm[:, 1]
+            F[:, 2] = pm[:, 2]
+            F[:, 3] = pm[:, 3]
+            F[:, 4] = pm[:, 0]
+            F[:, 5] = pm[:, 2]
+            chk(lib.gm_gsf_expand(h, self.rows, self.n_ang,
+                                  self.ang.ctypes.data_as(ctypes.c_void_p), P(self.d_F),
+                                  self.gsf_ng, 0, P(self.d_pmom), s))
+
+
+def time_pipeline(pipe, steps, warmup):
+    import torch
+    from geosmie_amd import _lib
+    L, h = _lib.lib(), _lib.ctx()
+    for _ in range(warmup):
+        pipe.step()
+    torch.cuda.synchronize()
+    L.gm_ctx_set_timing(h, 1)
+    ms = ctypes.c_double(0)
+    cnt = ctypes.c_int64(0)
+    for k in range(7):
+        L.gm_ctx_kernel_time(h, k, ctypes.byref(ms), ctypes.byref(cnt))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        pipe.step()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    kt = {}
+    for k, name in [(0, "k_mie_homog"), (2, "k_s12_f64"), (3, "psd_contract"), (4, "k_gsf_expand")]:
+        L.gm_ctx_kernel_time(h, k, ctypes.byref(ms), ctypes.byref(cnt))
+        if cnt.value:
+            kt[name] = ms.value / cnt.value
+    L.gm_ctx_set_timing(h, 0)
+    return (t1 - t0) * 1e3 / steps, kt
+
+
+def secondary(steps=5, warmup=1):
+    out = []
+    x, lam, ang, w, m = homog_lut_workload()
+    pipe = _Pipeline(x, m, ang, w, lam, [(i * 2000, 2000) for i in range(10)])
+    ms, kt = time_pipeline(pipe, steps, warmup)
+    s12_flop = 16.0 * pipe.sum_nmax * ang.size
+    out.append({"workload": "plain homogeneous FP64 Mie, 2000 radii 0.005-20 um x 10 wavelengths "
+                            "0.3-2.5 um, m=1.53+0.005i, 181 angles, lognormal r_mode 0.5 um "
+                            "sigma_g 2.0 integrated per wavelength (Qext/Qsca/g, P11/P12/P33/P34)",
+                "points": int(x.size), "ms_per_step": ms, "value": x.size / (ms * 1e-3),
+                "unit": "points/s", "kernel_ms": kt,
+                "s12_tflops": s12_flop / (kt.get("k_s12_f64", ms) * 1e-3) / 1e12})
+    del pipe
+    x, m, ang = largex_workload()
+    lam_row = np.repeat(np.linspace(0.3, 1.0, 8), 1)
+    from geosmie_amd.lut import getLogNormPSD
+    r = np.logspace(-1, 2, 5000)
+    w = np.zeros((8, x.size))
+    for i, L_ in enumerate(lam_row):
+        p = getLogNormPSD(5.0, 2.0, 2 * np.pi * r / L_, L_, 1e9, 0.0) * np.gradient(r)
+        w[i, i * 5000:(i + 1) * 5000] = p / p.sum()
+    pipe = _Pipeline(x, m, ang, w, lam_row, [(i * 5000, 5000) for i in range(8)], precision=0,
+                     gsf_ng=300)
+    ms, kt = time_pipeline(pipe, max(2, steps // 2), warmup)
+    s12_flop = 16.0 * pipe.sum_nmax * ang.size
+    out.append({"workload": "large-x homogeneous Mie, 5000 radii 0.1-100 um x 8 wavelengths "
+                            "0.3-1.0 um (x <= 2094), m_r U(1.33,1.60), m_i 10^U(-6,-2), 1801 "
+                            "angles, S1/S2 FP64 on MFMA, lognormal integration, 300 GSF moments",
+                "points": int(x.size), "ms_per_step": ms, "value": x.size / (ms * 1e-3),
+                "unit": "points/s", "kernel_ms": kt, "s12_flop": s12_flop,
+                "s12_tflops": s12_flop / (kt.get("k_s12_f64", ms) * 1e-3) / 1e12})
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -77,6 +253,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calib", action="store_true",
                     help="also run 3 device copies of 512 MiB (PMC byte-counter calibration)")
+    ap.add_argument("--no-secondary", action="store_true",
+                    help="skip the homogeneous-LUT and large-x secondary measurements")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; --gpus %d is not supported "
@@ -161,6 +339,9 @@ def main():
             b.copy_(a)
         torch.cuda.synchronize()
         del a, b
+    if not args.no_secondary:
+        del d_x, d_y, d_eps, d_eps2, d_props
+        out["secondary"] = secondary()
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(x, y, m1, m2)
     print(json.dumps(out))

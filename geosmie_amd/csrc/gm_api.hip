@@ -214,12 +214,13 @@ int gm_mie_batch(gm_ctx* ctx, int64_t n, const double* x, const double* y, const
   int32_t* max_nmax = status + 4;
   int64_t* order = (int64_t*)ctx->order.ptr;
   unsigned gblk = (unsigned)((n + 255) / 256);
+  const unsigned pblk = (unsigned)((n + 256 * gm::kPlanPer - 1) / (256 * gm::kPlanPer));
   if (flags & GM_MIE_SORT) {
     hipLaunchKernelGGL(gm::k_zero_i64, dim3((nb + 255) / 256), dim3(256), 0, s, hist, nb);
-    hipLaunchKernelGGL(gm::k_plan_keys, dim3(gblk), dim3(256), 0, s, n, x, y, eps, mu, eps2,
+    hipLaunchKernelGGL(gm::k_plan_keys, dim3(pblk), dim3(256), 0, s, n, x, y, eps, mu, eps2,
                        (int32_t*)ctx->key.ptr, nmax_out, hist, max_nmax, status);
     hipLaunchKernelGGL(gm::k_plan_scan, dim3(1), dim3(256), 0, s, hist, cursor, range);
-    hipLaunchKernelGGL(gm::k_plan_scatter, dim3(gblk), dim3(256), 0, s, n,
+    hipLaunchKernelGGL(gm::k_plan_scatter, dim3(pblk), dim3(256), 0, s, n,
                        (const int32_t*)ctx->key.ptr, cursor, order);
   } else {
     // input order: each class kernel walks every point and keeps its own

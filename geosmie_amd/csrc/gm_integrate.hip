@@ -57,21 +57,27 @@ __global__ void k_reduce_splits(int n_split, int64_t n_out, const double* __rest
 
 // Batched PSDs on FP64 MFMA: a wavefront owns 16 PSDs x 64 columns.
 // A = w (k rows, p = K), B = v (p = K, c columns); D[g + 4r][j] in lane j + 16g.
+// Split-K: blockIdx.z covers points [z * kSplitChunkMfma, ...); partial sums
+// to part[z][k][c], reduced in order by k_reduce_splits.
+constexpr int kSplitChunkMfma = 2048;
 __global__ __launch_bounds__(256) void k_contract_mfma(int n_psd, int64_t n_pts, int n_cols,
                                                        const double* __restrict__ w,
                                                        const double* __restrict__ v,
-                                                       double* __restrict__ out) {
+                                                       double* __restrict__ part) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int k0 = blockIdx.y * 16;
   const int c0 = (blockIdx.x * 4 + wave) * 64;
   if (c0 >= n_cols) return;
   const int i = lane & 15, kq = lane >> 4;
   const int ka = k0 + i;
+  const int64_t pbeg = (int64_t)blockIdx.z * kSplitChunkMfma;
+  const int64_t pend = min(pbeg + (int64_t)kSplitChunkMfma, n_pts);
+  double* out = part + (size_t)blockIdx.z * n_psd * n_cols;
   double4_ acc[4];
   for (int t = 0; t < 4; ++t) acc[t] = (double4_){0.0, 0.0, 0.0, 0.0};
-  for (int64_t p0 = 0; p0 < n_pts; p0 += 4) {
+  for (int64_t p0 = pbeg; p0 < pend; p0 += 4) {
     const int64_t p = p0 + kq;
-    const bool pin = p < n_pts;
+    const bool pin = p < pend;
     double a = (pin && ka < n_psd) ? w[(size_t)ka * n_pts + p] : 0.0;
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
@@ -238,9 +244,18 @@ int gm_psd_contract(gm_ctx* ctx, int n_psd, int64_t n_pts, int n_cols, const dou
     hipLaunchKernelGGL(gm::k_reduce_splits, dim3((unsigned)((n_out + 255) / 256)), dim3(256), 0,
                        s, nsplit, n_out, part, out);
   } else {
-    dim3 grid((n_cols + 255) / 256, (n_psd + 15) / 16);
+    int nsplit =
+        (int)std::max<int64_t>(1, (n_pts + gm::kSplitChunkMfma - 1) / gm::kSplitChunkMfma);
+    size_t bytes = sizeof(double) * (size_t)nsplit * n_psd * n_cols;
+    int rc = ctx->reserve(ctx->gemm_tmp, bytes);
+    if (rc) return rc;
+    double* part = (double*)ctx->gemm_tmp.ptr;
+    dim3 grid((n_cols + 255) / 256, (n_psd + 15) / 16, nsplit);
     hipLaunchKernelGGL(gm::k_contract_mfma, grid, dim3(256), 0, s, n_psd, n_pts, n_cols, w, v,
-                       out);
+                       part);
+    int64_t n_out = (int64_t)n_psd * n_cols;
+    hipLaunchKernelGGL(gm::k_reduce_splits, dim3((unsigned)((n_out + 255) / 256)), dim3(256), 0,
+                       s, nsplit, n_out, part, out);
   }
   ctx->mark_end(slot, s);
   e = hipGetLastError();

@@ -11,6 +11,8 @@ constexpr int kBuckets = 1024;
 // point classes: 0 homogeneous, 1 coated (real / weakly absorbing shell),
 // 2 coated (absorbing shell)
 constexpr int kClasses = 3;
+// points per thread in the plan kernels
+constexpr int kPlanPer = 16;
 
 __global__ void k_zero_i64(int64_t* p, int n);
 __global__ void k_plan_keys(int64_t n, const double* x, const double* y, const double* eps,

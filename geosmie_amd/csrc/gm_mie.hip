@@ -119,7 +119,8 @@ __global__ void k_zero_i64(int64_t* __restrict__ p, int n) {
 }
 
 // Block-local LDS histogram, then one global atomic per non-empty bucket.
-// status bit 2: a coated point with mu != 1 (mie_coeffs.py:57-59).
+// Each block covers kPlanPer * 256 points so the histogram's zeroing and flush
+// are amortised.  status bit 2: a coated point with mu != 1 (mie_coeffs.py:57-59).
 __global__ __launch_bounds__(256) void k_plan_keys(
     int64_t n, const double* __restrict__ x, const double* __restrict__ y,
     const double* __restrict__ eps, const double* __restrict__ mu,
@@ -130,8 +131,12 @@ __global__ __launch_bounds__(256) void k_plan_keys(
   for (int i = threadIdx.x; i < kClasses * kBuckets; i += blockDim.x) lh[i] = 0;
   if (threadIdx.x == 0) lmax = 0;
   __syncthreads();
-  int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (p < n) {
+  const int64_t p0 = (int64_t)blockIdx.x * (256 * kPlanPer);
+  int tmax = 0;
+  bool bad_mu = false;
+  for (int q = 0; q < kPlanPer; ++q) {
+    const int64_t p = p0 + q * 256 + threadIdx.x;
+    if (p >= n) break;
     PointDesc d = describe(p, x, y, eps, mu, eps2);
     int nmax;
     int cost = point_cost(d, &nmax);
@@ -140,9 +145,11 @@ __global__ __launch_bounds__(256) void k_plan_keys(
     key[p] = k;
     if (nmax_out) nmax_out[p] = nmax;
     atomicAdd(&lh[k], 1);
-    atomicMax(&lmax, nmax);
-    if (y && mu && (d.mu.re != 1.0 || d.mu.im != 0.0)) atomicOr(status, 2);
+    tmax = max(tmax, nmax);
+    bad_mu |= (y && mu && (d.mu.re != 1.0 || d.mu.im != 0.0));
   }
+  atomicMax(&lmax, tmax);
+  if (bad_mu) atomicOr(status, 2);
   __syncthreads();
   for (int i = threadIdx.x; i < kClasses * kBuckets; i += blockDim.x)
     if (lh[i]) atomicAdd((unsigned long long*)&hist[i], (unsigned long long)lh[i]);
@@ -185,7 +192,8 @@ __global__ __launch_bounds__(256) void k_plan_scan(const int64_t* __restrict__ h
   }
 }
 
-// Scatter with block-local ranks: one global atomic per (block, bucket).
+// Scatter with block-local ranks: one global atomic per (block, bucket),
+// kPlanPer * 256 points per block.
 __global__ __launch_bounds__(256) void k_plan_scatter(int64_t n, const int32_t* __restrict__ key,
                                                       int64_t* __restrict__ cursor,
                                                       int64_t* __restrict__ order) {
@@ -193,18 +201,26 @@ __global__ __launch_bounds__(256) void k_plan_scatter(int64_t n, const int32_t* 
   __shared__ int64_t lbase[kClasses * kBuckets];
   for (int i = threadIdx.x; i < kClasses * kBuckets; i += blockDim.x) lcnt[i] = 0;
   __syncthreads();
-  int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  int k = -1, rank = 0;
-  if (p < n) {
-    k = key[p];
-    rank = atomicAdd(&lcnt[k], 1);
+  const int64_t p0 = (int64_t)blockIdx.x * (256 * kPlanPer);
+  int kk[kPlanPer], rank[kPlanPer];
+#pragma unroll
+  for (int q = 0; q < kPlanPer; ++q) {
+    const int64_t p = p0 + q * 256 + threadIdx.x;
+    kk[q] = -1;
+    rank[q] = 0;
+    if (p < n) {
+      kk[q] = key[p];
+      rank[q] = atomicAdd(&lcnt[kk[q]], 1);
+    }
   }
   __syncthreads();
   for (int i = threadIdx.x; i < kClasses * kBuckets; i += blockDim.x)
     if (lcnt[i])
       lbase[i] = (int64_t)atomicAdd((unsigned long long*)&cursor[i], (unsigned long long)lcnt[i]);
   __syncthreads();
-  if (k >= 0) order[lbase[k] + rank] = p;
+#pragma unroll
+  for (int q = 0; q < kPlanPer; ++q)
+    if (kk[q] >= 0) order[lbase[kk[q]] + rank[q]] = p0 + q * 256 + threadIdx.x;
 }
 
 // Identity order (no sort): every class kernel walks all points and keeps

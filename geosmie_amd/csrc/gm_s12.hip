@@ -126,8 +126,11 @@ int gm_mie_s12(gm_ctx* ctx, int64_t n, const int64_t* coef_off, const int32_t* n
   hipError_t e = hipSetDevice(ctx->device);
   if (e != hipSuccess) return gm_fail(1, "hipSetDevice: %s", hipGetErrorString(e));
   dim3 grid((unsigned)((n + 7) / 8), (unsigned)((n_ang + 255) / 256));
+  size_t slot;
+  ctx->mark_begin(gm::kKS12, (hipStream_t)stream, &slot);
   hipLaunchKernelGGL(gm::k_s12_f64, grid, dim3(256), 0, (hipStream_t)stream, n, coef_off, nmax,
                      an, bn, n_ang, pi_t, tau_t, nmax_cap, (int)gm::kOutS12, s1, s2);
+  ctx->mark_end(slot, (hipStream_t)stream);
   e = hipGetLastError();
   if (e != hipSuccess) return gm_fail(1, "k_s12_f64: %s", hipGetErrorString(e));
   return 0;
@@ -145,9 +148,12 @@ int gm_s12_mueller(gm_ctx* ctx, int64_t n, const int64_t* coef_off, const int32_
   hipError_t e = hipSetDevice(ctx->device);
   if (e != hipSuccess) return gm_fail(1, "hipSetDevice: %s", hipGetErrorString(e));
   dim3 grid((unsigned)((n + 7) / 8), (unsigned)((n_ang + 255) / 256));
+  size_t slot;
+  ctx->mark_begin(gm::kKS12, (hipStream_t)stream, &slot);
   hipLaunchKernelGGL(gm::k_s12_f64, grid, dim3(256), 0, (hipStream_t)stream, n, coef_off, nmax,
                      an, bn, n_ang, pi_t, tau_t, nmax_cap, (int)gm::kOutMueller, mueller,
                      (double*)nullptr);
+  ctx->mark_end(slot, (hipStream_t)stream);
   e = hipGetLastError();
   if (e != hipSuccess) return gm_fail(1, "k_s12_f64: %s", hipGetErrorString(e));
   return 0;
