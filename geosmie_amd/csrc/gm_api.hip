@@ -33,7 +33,9 @@ int gm_fail(int code, const char* fmt, ...) {
 int gm_ctx::reserve(Buf& b, size_t bytes) {
   if (b.bytes >= bytes) return GM_OK;
   if (b.ptr) {
-    hipError_t e = hipFree(b.ptr);
+    // Work queued earlier on any stream may still read the old buffer.
+    hipError_t e = hipDeviceSynchronize();
+    if (e == hipSuccess) e = hipFree(b.ptr);
     b.ptr = nullptr;
     b.bytes = 0;
     if (e != hipSuccess) return gm_fail(GM_ERUNTIME, "hipFree: %s", hipGetErrorString(e));
@@ -106,6 +108,16 @@ int gm_ctx_create(int device, gm_ctx** out) {
     return gm_fail(GM_ERUNTIME, "hipGetDeviceProperties: %s", hipGetErrorString(e));
   }
   c->n_cu = prop.multiProcessorCount;
+  // resident one-wavefront blocks per CU of each persistent Mie kernel
+  int occ = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, gm::k_mie_homog, 64, 0) == hipSuccess)
+    c->occ_homog = occ;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, gm::k_mie_coated_stable, 64, 0) ==
+      hipSuccess)
+    c->occ_stable = occ;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, gm::k_mie_coated_literal, 64, 0) ==
+      hipSuccess)
+    c->occ_literal = occ;
   int rc = c->reserve(c->status, 64);
   if (rc) {
     delete c;
@@ -233,29 +245,38 @@ int gm_mie_batch(gm_ctx* ctx, int64_t n, const double* x, const double* y, const
 
   const int rows_cap = std::max(1, gm_nmax(size_max));
   const int dn_extra = (flags & GM_MIE_DN_CONVERGED) ? -1 : 0;
-  const int row_doubles = y ? gm::kCoatedRowDoubles : gm::kHomogRowDoubles;
-  const size_t slot_bytes = (size_t)rows_cap * 64 * row_doubles * sizeof(double);
-  int64_t waves = (n + 63) / 64;
-  int64_t slots = std::min<int64_t>(waves, (int64_t)ctx->n_cu * ctx->waves_per_cu);
-  slots = std::min<int64_t>(slots, std::max<int64_t>(1, (int64_t)(ctx->scratch_budget / slot_bytes)));
-  if ((rc = ctx->reserve(ctx->scratch, slot_bytes * slots))) return rc;
+  // Persistent grids: one wavefront per resident slot (occupancy measured per
+  // kernel at context creation), each owning a reused scratch slot.
+  const int64_t waves = (n + 63) / 64;
+  auto grid_for = [&](int occ, int row_doubles) {
+    const size_t slot_bytes = (size_t)rows_cap * 64 * row_doubles * sizeof(double);
+    int64_t g = std::min<int64_t>(waves, (int64_t)ctx->n_cu * std::max(occ, 1));
+    g = std::min<int64_t>(g, std::max<int64_t>(1, (int64_t)(ctx->scratch_budget / slot_bytes)));
+    return std::make_pair(g, (size_t)g * slot_bytes);
+  };
+  auto gh = grid_for(ctx->occ_homog, gm::kHomogRowDoubles);
+  auto gs = grid_for(ctx->occ_stable, gm::kCoatedRowDoubles);
+  auto gl = grid_for(ctx->occ_literal, gm::kCoatedRowDoubles);
+  size_t need = gh.second;
+  if (y) need = std::max(need, std::max(gs.second, gl.second));
+  if ((rc = ctx->reserve(ctx->scratch, need))) return rc;
   double* scratch = (double*)ctx->scratch.ptr;
   size_t slot;
   ctx->mark_begin(gm::kKMieHomog, s, &slot);
-  hipLaunchKernelGGL(gm::k_mie_homog, dim3((unsigned)slots), dim3(64), 0, s, order, range, n, x,
-                     y, eps, mu, eps2, coef_off, an, bn, props, status, scratch, rows_cap,
+  hipLaunchKernelGGL(gm::k_mie_homog, dim3((unsigned)gh.first), dim3(64), 0, s, order, range, n,
+                     x, y, eps, mu, eps2, coef_off, an, bn, props, status, scratch, rows_cap,
                      dn_extra);
   ctx->mark_end(slot, s);
   HIP_OK(hipGetLastError());
   if (y) {
     ctx->mark_begin(gm::kKMieCoated, s, &slot);
-    hipLaunchKernelGGL(gm::k_mie_coated_stable, dim3((unsigned)slots), dim3(64), 0, s, order,
+    hipLaunchKernelGGL(gm::k_mie_coated_stable, dim3((unsigned)gs.first), dim3(64), 0, s, order,
                        range, n, x, y, eps, mu, eps2, coef_off, an, bn, props, status, scratch,
                        rows_cap, dn_extra);
     ctx->mark_end(slot, s);
     HIP_OK(hipGetLastError());
     ctx->mark_begin(gm::kKMieCoatedLit, s, &slot);
-    hipLaunchKernelGGL(gm::k_mie_coated_literal, dim3((unsigned)slots), dim3(64), 0, s, order,
+    hipLaunchKernelGGL(gm::k_mie_coated_literal, dim3((unsigned)gl.first), dim3(64), 0, s, order,
                        range, n, x, y, eps, mu, eps2, coef_off, an, bn, props, status, scratch,
                        rows_cap, dn_extra);
     ctx->mark_end(slot, s);

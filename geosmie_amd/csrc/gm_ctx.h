@@ -19,7 +19,7 @@ struct Buf {
 struct gm_ctx {
   int device = 0;
   int n_cu = 256;
-  int waves_per_cu = 16;                         // persistent Mie wavefronts per CU
+  int occ_homog = 8, occ_stable = 8, occ_literal = 4;  // resident Mie wavefronts per CU
   size_t scratch_budget = (size_t)3 << 30;       // bytes of recurrence scratch
   Buf status;                                    // int32 flags + int32 max_nmax + scan scratch
   Buf order, key, hist, scratch, scan_tmp;

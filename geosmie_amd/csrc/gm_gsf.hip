@@ -133,66 +133,68 @@ static void build_tables(int ng, int n_ang, const double* ang_deg, std::vector<d
   }
 }
 
-constexpr int kGsfEntries = 4;   // entries per workgroup (G loads amortised)
+constexpr int kGsfWaves = 4;   // wavefronts splitting one entry's angle sum
 
-// One workgroup per kGsfEntries entries, threads over l.  F[e][6][n_ang] in
-// table order p11, p12, p33, p34, p22, p44; out[e][6][ng] in the same order
-// (pmom(bin, wavelength, rh, p, m), convertncdf.py:76, 381-394).
-__global__ __launch_bounds__(128) void k_gsf_expand(int64_t n_ent, int n_ang, int ng,
+// Block (entry e, 64 coefficients l0..l0+63); wavefront w sums angles
+// a = w, w + 4, ...; the four partial sums are added in fixed order in LDS.
+// F[e][6][n_ang] in table order p11, p12, p33, p34, p22, p44; raw[e][6][ng]
+// receives the unnormalised coefficients in the same order (pmom(bin,
+// wavelength, rh, p, m), convertncdf.py:76, 381-394).
+__global__ __launch_bounds__(256) void k_gsf_expand(int64_t n_ent, int n_ang, int ng,
                                                     const double* __restrict__ G,
                                                     const double* __restrict__ F,
-                                                    double* __restrict__ out, int quantize) {
-  const int64_t e0 = (int64_t)blockIdx.x * kGsfEntries;
+                                                    double* __restrict__ raw) {
+  __shared__ double part[kGsfWaves][6][64];
+  const int64_t e = blockIdx.x;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int l = blockIdx.y * 64 + lane;
   const size_t plane = (size_t)n_ang * ng;
-  for (int l = threadIdx.x; l < ng; l += blockDim.x) {
-    double acc[kGsfEntries][6];
-    for (int q = 0; q < kGsfEntries; ++q)
-      for (int f = 0; f < 6; ++f) acc[q][f] = 0.0;
-    for (int a = 0; a < n_ang; ++a) {
+  const double* Fe = F + (size_t)e * 6 * n_ang;
+  double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  if (l < ng) {
+    for (int a = wv; a < n_ang; a += kGsfWaves) {
       const double g1 = G[(size_t)a * ng + l], g2 = G[plane + (size_t)a * ng + l];
       const double g3 = G[2 * plane + (size_t)a * ng + l], g4 = G[3 * plane + (size_t)a * ng + l];
-#pragma unroll
-      for (int q = 0; q < kGsfEntries; ++q) {
-        const int64_t e = e0 + q;
-        if (e >= n_ent) break;
-        const double* Fe = F + (size_t)e * 6 * n_ang;
-        const double f11 = Fe[a], f12 = Fe[n_ang + a], f33 = Fe[2 * n_ang + a];
-        const double f34 = Fe[3 * n_ang + a], f22 = Fe[4 * n_ang + a], f44 = Fe[5 * n_ang + a];
-        acc[q][0] = fma(f11, g1, acc[q][0]);          // alpha1 <- F11 P00
-        acc[q][1] = fma(f22 + f33, g2, acc[q][1]);    // A2 <- (F22+F33) P22
-        acc[q][2] = fma(f22 - f33, g3, acc[q][2]);    // A3 <- (F22-F33) P2-2
-        acc[q][3] = fma(f44, g1, acc[q][3]);          // alpha4 <- F44 P00
-        acc[q][4] = fma(f12, g4, acc[q][4]);          // beta1 <- F12 P02
-        acc[q][5] = fma(f34, g4, acc[q][5]);          // beta2 <- F34 P02
-      }
-    }
-    const double cl = (double)l + 0.5;
-    for (int q = 0; q < kGsfEntries; ++q) {
-      const int64_t e = e0 + q;
-      if (e >= n_ent) break;
-      double* o = out + (size_t)e * 6 * ng;
-      const double a2 = acc[q][1] * cl * 0.5, a3 = acc[q][2] * cl * 0.5;
-      o[0 * ng + l] = acc[q][0] * cl;   // p11 <- alpha1
-      o[1 * ng + l] = acc[q][4] * cl;   // p12 <- beta1
-      o[2 * ng + l] = a2 - a3;          // p33 <- alpha3
-      o[3 * ng + l] = acc[q][5] * cl;   // p34 <- beta2
-      o[4 * ng + l] = a2 + a3;          // p22 <- alpha2
-      o[5 * ng + l] = acc[q][3] * cl;   // p44 <- alpha4
+      const double f11 = Fe[a], f12 = Fe[n_ang + a], f33 = Fe[2 * n_ang + a];
+      const double f34 = Fe[3 * n_ang + a], f22 = Fe[4 * n_ang + a], f44 = Fe[5 * n_ang + a];
+      acc[0] = fma(f11, g1, acc[0]);          // alpha1 <- F11 P00
+      acc[1] = fma(f22 + f33, g2, acc[1]);    // A2 <- (F22+F33) P22
+      acc[2] = fma(f22 - f33, g3, acc[2]);    // A3 <- (F22-F33) P2-2
+      acc[3] = fma(f44, g1, acc[3]);          // alpha4 <- F44 P00
+      acc[4] = fma(f12, g4, acc[4]);          // beta1 <- F12 P02
+      acc[5] = fma(f34, g4, acc[5]);          // beta2 <- F34 P02
     }
   }
+#pragma unroll
+  for (int f = 0; f < 6; ++f) part[wv][f][lane] = acc[f];
   __syncthreads();
-  // normalise by 1/alpha1(0) (spher_expan.f:95-103); optional 10-decimal rounding
-  for (int q = 0; q < kGsfEntries; ++q) {
-    const int64_t e = e0 + q;
-    if (e >= n_ent) break;
-    double* o = out + (size_t)e * 6 * ng;
-    const double cnorm = 1.0 / o[0];
-    __syncthreads();
-    for (int idx = threadIdx.x; idx < 6 * ng; idx += blockDim.x) {
-      double v = o[idx] * cnorm;
-      if (quantize) v = rint(v * 1e10) * 1e-10;
-      o[idx] = v;
-    }
+  if (wv != 0 || l >= ng) return;
+#pragma unroll
+  for (int f = 0; f < 6; ++f)
+    acc[f] = ((part[0][f][lane] + part[1][f][lane]) + part[2][f][lane]) + part[3][f][lane];
+  const double cl = (double)l + 0.5;
+  double* o = raw + (size_t)e * 6 * ng;
+  const double a2 = acc[1] * cl * 0.5, a3 = acc[2] * cl * 0.5;
+  o[0 * ng + l] = acc[0] * cl;   // p11 <- alpha1
+  o[1 * ng + l] = acc[4] * cl;   // p12 <- beta1
+  o[2 * ng + l] = a2 - a3;       // p33 <- alpha3
+  o[3 * ng + l] = acc[5] * cl;   // p34 <- beta2
+  o[4 * ng + l] = a2 + a3;       // p22 <- alpha2
+  o[5 * ng + l] = acc[3] * cl;   // p44 <- alpha4
+}
+
+// Normalise by 1/alpha1(0) (spher_expan.f:95-103); optional 10-decimal rounding
+// (the F17.10 output, :104-105).  One block per entry.
+__global__ __launch_bounds__(256) void k_gsf_normalize(int ng, const double* __restrict__ raw,
+                                                       double* __restrict__ out, int quantize) {
+  const int64_t e = blockIdx.x;
+  const double* r = raw + (size_t)e * 6 * ng;
+  double* o = out + (size_t)e * 6 * ng;
+  const double cnorm = 1.0 / r[0];
+  for (int i = threadIdx.x; i < 6 * ng; i += blockDim.x) {
+    double v = r[i] * cnorm;
+    if (quantize) v = rint(v * 1e10) * 1e-10;
+    o[i] = v;
   }
 }
 
@@ -248,11 +250,17 @@ int gm_gsf_expand(gm_ctx* ctx, int64_t n_entries, int n_ang, const double* ang_d
     ctx->gsf_nang = n_ang;
     ctx->gsf_key = key;
   }
+  size_t raw_bytes = sizeof(double) * (size_t)n_entries * 6 * ng;
+  int rc = ctx->reserve(ctx->gemm_tmp, raw_bytes);
+  if (rc) return rc;
+  double* raw = (double*)ctx->gemm_tmp.ptr;
   size_t slot;
   ctx->mark_begin(gm::kKGsf, s, &slot);
-  unsigned nblk = (unsigned)((n_entries + gm::kGsfEntries - 1) / gm::kGsfEntries);
-  hipLaunchKernelGGL(gm::k_gsf_expand, dim3(nblk), dim3(128), 0, s, n_entries, n_ang, ng,
-                     (const double*)ctx->gsf_tab.ptr, F, out, quantize);
+  hipLaunchKernelGGL(gm::k_gsf_expand, dim3((unsigned)n_entries, (unsigned)((ng + 63) / 64)),
+                     dim3(64 * gm::kGsfWaves), 0, s, n_entries, n_ang, ng,
+                     (const double*)ctx->gsf_tab.ptr, F, raw);
+  hipLaunchKernelGGL(gm::k_gsf_normalize, dim3((unsigned)n_entries), dim3(256), 0, s, ng, raw, out,
+                     quantize);
   ctx->mark_end(slot, s);
   e = hipGetLastError();
   if (e != hipSuccess) return gm_fail(1, "k_gsf_expand: %s", hipGetErrorString(e));

@@ -22,7 +22,7 @@ typedef double double4_ __attribute__((ext_vector_type(4)));
 namespace gm {
 
 constexpr int kSmallPsd = 8;     // VALU path up to this many PSDs
-constexpr int kSplitChunk = 512; // points per split-K chunk
+constexpr int kSplitChunk = 128; // points per split-K chunk
 
 // out_part[split][k][c] = sum_{p in chunk} w[k][p] v[p][c]   (k < n_psd <= kSmallPsd)
 __global__ __launch_bounds__(256) void k_contract_small(int n_psd, int64_t n_pts, int n_cols,
