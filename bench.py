@@ -203,7 +203,7 @@ def time_pipeline(pipe, steps, warmup):
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     kt = {}
-    for k, name in [(0, "k_mie_homog"), (2, "k_s12_f64"), (3, "psd_contract"), (4, "k_gsf_expand")]:
+    for k, name in [(0, "k_mie_homog"), (2, "s12"), (3, "psd_contract"), (4, "k_gsf_expand")]:
         L.gm_ctx_kernel_time(h, k, ctypes.byref(ms), ctypes.byref(cnt))
         if cnt.value:
             kt[name] = ms.value / cnt.value
@@ -222,7 +222,7 @@ def secondary(steps=5, warmup=1):
                             "sigma_g 2.0 integrated per wavelength (Qext/Qsca/g, P11/P12/P33/P34)",
                 "points": int(x.size), "ms_per_step": ms, "value": x.size / (ms * 1e-3),
                 "unit": "points/s", "kernel_ms": kt,
-                "s12_tflops": s12_flop / (kt.get("k_s12_f64", ms) * 1e-3) / 1e12})
+                "s12_tflops": s12_flop / (kt.get("s12", ms) * 1e-3) / 1e12})
     del pipe
     x, m, ang = largex_workload()
     lam_row = np.repeat(np.linspace(0.3, 1.0, 8), 1)
@@ -234,14 +234,49 @@ def secondary(steps=5, warmup=1):
         w[i, i * 5000:(i + 1) * 5000] = p / p.sum()
     pipe = _Pipeline(x, m, ang, w, lam_row, [(i * 5000, 5000) for i in range(8)], precision=0,
                      gsf_ng=300)
-    ms, kt = time_pipeline(pipe, max(2, steps // 2), warmup)
     s12_flop = 16.0 * pipe.sum_nmax * ang.size
+    modes = []
+    ref = None
+    for prec, name in [(0, "fp64"), (1, "fp32"), (2, "bf16x3")]:
+        pipe.precision = prec
+        ms, kt = time_pipeline(pipe, max(2, steps // 2), warmup)
+        res = {"pm": pipe.d_pm.view(8, 4, ang.size).cpu().numpy(),
+               "back": pipe.d_back.view(8, 6).cpu().numpy(),
+               "pmom": pipe.d_pmom.view(8, 6, 300).cpu().numpy(),
+               "sc": pipe.d_sc.view(8, 16).cpu().numpy()}
+        ent = {"precision": name, "ms_per_step": ms, "value": x.size / (ms * 1e-3),
+               "kernel_ms": kt,
+               "s12_tflops": s12_flop / (kt.get("s12", ms) * 1e-3) / 1e12}
+        if ref is None:
+            ref = res
+        else:
+            p11r = ref["pm"][:, 0]
+            scale = np.abs(p11r).max(axis=1, keepdims=True)
+            ent["err_vs_fp64"] = {
+                "P11_max_rel": float(np.max(np.abs(res["pm"][:, 0] - p11r) / np.abs(p11r))),
+                "P11_median_rel": float(np.median(np.abs(res["pm"][:, 0] - p11r) / np.abs(p11r))),
+                "P12_P33_P34_max_abs_over_maxP11": float(
+                    np.max(np.abs(res["pm"][:, 1:] - ref["pm"][:, 1:]) / scale[:, None])),
+                # moments: error over the largest moment of the same entry and element
+                "alpha1_moments_max_rel": float(np.max(
+                    np.abs(res["pmom"][:, 0] - ref["pmom"][:, 0]) /
+                    np.abs(ref["pmom"][:, 0]).max(axis=1, keepdims=True))),
+                "all_moments_max_rel": float(np.max(
+                    np.abs(res["pmom"] - ref["pmom"]) /
+                    np.abs(ref["pmom"]).max(axis=2, keepdims=True))),
+                "Q_g_max_rel": float(np.max(np.abs(res["sc"] - ref["sc"]) /
+                                            np.maximum(np.abs(ref["sc"]), 1e-300))),
+            }
+        modes.append(ent)
+    fp64 = modes[0]
     out.append({"workload": "large-x homogeneous Mie, 5000 radii 0.1-100 um x 8 wavelengths "
                             "0.3-1.0 um (x <= 2094), m_r U(1.33,1.60), m_i 10^U(-6,-2), 1801 "
-                            "angles, S1/S2 FP64 on MFMA, lognormal integration, 300 GSF moments",
-                "points": int(x.size), "ms_per_step": ms, "value": x.size / (ms * 1e-3),
-                "unit": "points/s", "kernel_ms": kt, "s12_flop": s12_flop,
-                "s12_tflops": s12_flop / (kt.get("k_s12_f64", ms) * 1e-3) / 1e12})
+                            "angles, S1/S2 on MFMA (FP64; FP32 and split-BF16 below), lognormal "
+                            "integration (r_mode 5 um, sigma_g 2.0), 300 GSF moments",
+                "points": int(x.size), "ms_per_step": fp64["ms_per_step"],
+                "value": fp64["value"], "unit": "points/s", "kernel_ms": fp64["kernel_ms"],
+                "s12_flop": s12_flop, "s12_tflops": fp64["s12_tflops"],
+                "precision_modes": modes[1:]})
     return out
 
 
@@ -249,7 +284,7 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calib", action="store_true",
                     help="also run 3 device copies of 512 MiB (PMC byte-counter calibration)")

@@ -23,13 +23,13 @@ struct gm_ctx {
   size_t scratch_budget = (size_t)3 << 30;       // bytes of recurrence scratch
   Buf status;                                    // int32 flags + int32 max_nmax + scan scratch
   Buf order, key, hist, scratch, scan_tmp;
-  Buf tab_pi, tab_tau, gemm_tmp, feat_tmp, gsf_tab;
+  Buf tab_pi, tab_tau, gemm_tmp, feat_tmp, gsf_tab, tab_lp;
   // cached GSF tables: key = (ng, n_ang, checksum of the angle grid)
   int gsf_ng = 0, gsf_nang = 0;
   double gsf_key = 0.0;
   std::vector<Buf*> all_bufs() {
     return {&status, &order, &key, &hist, &scratch, &scan_tmp, &tab_pi, &tab_tau, &gemm_tmp,
-            &feat_tmp, &gsf_tab};
+            &feat_tmp, &gsf_tab, &tab_lp};
   }
   int reserve(Buf& b, size_t bytes);
   ~gm_ctx();

@@ -14,14 +14,51 @@
 // j + 16g, element r  (the f32/bf16 16x16 variants use D[4g + r][j] instead).
 // Epilogue modes: write S1, S2 (complex), or the four independent Mueller
 // elements s11, s12, s33, s34 (dointegration.py:1044-1050).
+//
+// Reduced-precision paths (the coefficients and the Mie series sums stay FP64):
+//   FP32    v_mfma_f32_16x16x4_f32 on FP32 copies of U, V, P, M.
+//   BF16X3  v_mfma_f32_16x16x32_bf16 on split operands x = x_hi + x_lo (both
+//           BF16, x_lo = bf16(x - x_hi)); U P ~ U_hi P_hi + U_hi P_lo + U_lo P_hi
+//           (the lo*lo term, ~2^-18 relative, is dropped), FP32 accumulation.
+// Both read P, M from a per-call table (k_s12_prep_tables) padded to whole
+// K-steps and 64-angle tiles, so the inner loops carry no bounds tests on it.
+// f32 / bf16 16x16 results land as D[4g + r][j] in lane j + 16g, element r;
+// A row i is then point p0 + (i >> 1), component i & 1, which gives the same
+// per-lane element order as the FP64 kernel: (0,1) = point p0 + 2g (re, im),
+// (2,3) = point p0 + 2g + 1.
 #include "gm_complex.h"
 #include "gm_ctx.h"
 
 typedef double double4_ __attribute__((ext_vector_type(4)));
+typedef float float4_ __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8_ __attribute__((ext_vector_type(8)));
 
 namespace gm {
 
 enum { kOutS12 = 0, kOutMueller = 1 };
+
+// S1 = (sp + sm) / 2, S2 = (sp - sm) / 2 for point p, angle j; then either the
+// amplitudes or the Mueller elements (dointegration.py:1045-1050).
+__device__ __forceinline__ void s12_store(int mode, int64_t p, int j, int n_ang, cd sp, cd sm,
+                                          double* __restrict__ out0, double* __restrict__ out1) {
+  cd s1 = (sp + sm) * 0.5;
+  cd s2 = (sp - sm) * 0.5;
+  size_t o = (size_t)p * n_ang + j;
+  if (mode == kOutS12) {
+    out0[2 * o] = s1.re;
+    out0[2 * o + 1] = s1.im;
+    out1[2 * o] = s2.re;
+    out1[2 * o + 1] = s2.im;
+  } else {
+    double a1 = norm2(s1), a2 = norm2(s2);
+    cd c = s1 * conj(s2);
+    size_t q = (size_t)p * 4 * n_ang + j;  // layout [point][element][angle]
+    out0[q] = 0.5 * (a1 + a2);              // s11 (= s22)
+    out0[q + n_ang] = 0.5 * (a2 - a1);      // s12
+    out0[q + 2 * n_ang] = c.re;             // s33 (= s44)
+    out0[q + 3 * n_ang] = c.im;             // s34 = -Im(conj(S1) S2)
+  }
+}
 
 __global__ __launch_bounds__(256) void k_s12_f64(
     int64_t n, const int64_t* __restrict__ off, const int32_t* __restrict__ nmax,
@@ -86,30 +123,250 @@ __global__ __launch_bounds__(256) void k_s12_f64(
     for (int h = 0; h < 2; ++h) {
       const int64_t p = p0 + 2 * kq + h;
       if (p >= n) continue;
-      cd sp = cd{accp[t][2 * h], accp[t][2 * h + 1]};
-      cd sm = cd{accm[t][2 * h], accm[t][2 * h + 1]};
-      cd s1 = (sp + sm) * 0.5;
-      cd s2 = (sp - sm) * 0.5;
-      size_t o = (size_t)p * n_ang + j;
-      if (mode == kOutS12) {
-        out0[2 * o] = s1.re;
-        out0[2 * o + 1] = s1.im;
-        out1[2 * o] = s2.re;
-        out1[2 * o + 1] = s2.im;
-      } else {
-        double a1 = norm2(s1), a2 = norm2(s2);
-        cd c = s1 * conj(s2);
-        size_t q = (size_t)p * 4 * n_ang + j;  // layout [point][element][angle]
-        out0[q] = 0.5 * (a1 + a2);              // s11 (= s22)
-        out0[q + n_ang] = 0.5 * (a2 - a1);      // s12
-        out0[q + 2 * n_ang] = c.re;             // s33 (= s44)
-        out0[q + 3 * n_ang] = c.im;             // s34 = -Im(conj(S1) S2)
+      s12_store(mode, p, j, n_ang, cd{accp[t][2 * h], accp[t][2 * h + 1]},
+                cd{accm[t][2 * h], accm[t][2 * h + 1]}, out0, out1);
+    }
+  }
+}
+
+// ---- reduced-precision paths -------------------------------------------------
+
+constexpr int kLpK = 32;     // K padding of the reduced-precision tables
+constexpr int kLpAng = 64;   // angle padding
+
+__device__ __forceinline__ unsigned short bf16_rne(float f) {
+  unsigned u = __float_as_uint(f);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (unsigned short)(u >> 16);
+}
+__device__ __forceinline__ float bf16_val(unsigned short h) {
+  return __uint_as_float((unsigned)h << 16);
+}
+// x = hi + lo, both BF16 (hi = bf16(x), lo = bf16(x - hi) with the residual in FP64).
+__device__ __forceinline__ void bf16_split(double x, unsigned short& hi, unsigned short& lo) {
+  hi = bf16_rne((float)x);
+  lo = bf16_rne((float)(x - (double)bf16_val(hi)));
+}
+
+// P = pi + tau and M = pi - tau from the FP64 tables [k][angle] (k < nmax_cap).
+//   FP32 : t32[2][kpad][apad]            (plane 0 P, plane 1 M)
+//   BF16 : t16[4][kpad / 8][apad][8]     (planes P_hi, P_lo, M_hi, M_lo)
+// Zero outside k < nmax_cap, j < n_ang.
+__global__ void k_s12_prep_tables(int n_ang, int nmax_cap, int kpad, int apad,
+                                  const double* __restrict__ pi_t,
+                                  const double* __restrict__ tau_t, int precision,
+                                  float* __restrict__ t32, unsigned short* __restrict__ t16) {
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (int64_t)kpad * apad) return;
+  const int k = (int)(idx / apad), j = (int)(idx % apad);
+  double P = 0.0, M = 0.0;
+  if (k < nmax_cap && j < n_ang) {
+    const double pv = pi_t[(size_t)k * n_ang + j], tv = tau_t[(size_t)k * n_ang + j];
+    P = pv + tv;
+    M = pv - tv;
+  }
+  if (precision == GM_PREC_FP32) {
+    t32[(size_t)k * apad + j] = (float)P;
+    t32[((size_t)kpad + k) * apad + j] = (float)M;
+  } else {
+    const size_t plane = (size_t)kpad * apad;
+    const size_t o = ((size_t)(k >> 3) * apad + j) * 8 + (k & 7);
+    unsigned short h, l;
+    bf16_split(P, h, l);
+    t16[o] = h;
+    t16[plane + o] = l;
+    bf16_split(M, h, l);
+    t16[2 * plane + o] = h;
+    t16[3 * plane + o] = l;
+  }
+}
+
+// K extent of the tile of 8 points starting at p0, rounded up to a whole step.
+__device__ __forceinline__ int tile_kmax(int64_t n, int64_t p0, const int32_t* __restrict__ nmax,
+                                         int nmax_cap, int step) {
+  int kmax = 0;
+  for (int q = 0; q < 8; ++q) {
+    int64_t pq = p0 + q;
+    if (pq < n) kmax = max(kmax, nmax[pq]);
+  }
+  kmax = min(kmax, nmax_cap);
+  return (kmax + step - 1) / step * step;
+}
+
+__global__ __launch_bounds__(256) void k_s12_f32(
+    int64_t n, const int64_t* __restrict__ off, const int32_t* __restrict__ nmax,
+    const double* __restrict__ an, const double* __restrict__ bn, int n_ang, int apad, int kpad,
+    const float* __restrict__ t32, int nmax_cap, int mode, double* __restrict__ out0,
+    double* __restrict__ out1) {
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int64_t p0 = (int64_t)blockIdx.x * 8;
+  const int j0 = (blockIdx.y * 4 + wave) * 64;
+  if (j0 >= n_ang) return;
+  const int i = lane & 15, kq = lane >> 4;
+  const int64_t pa = p0 + (i >> 1);
+  const int comp = i & 1;
+  const bool pa_ok = pa < n;
+  const int nm_a = pa_ok ? nmax[pa] : 0;
+  const int64_t oa = pa_ok ? off[pa] : 0;
+  const int kmax = tile_kmax(n, p0, nmax, nmax_cap, 4);
+  const float* Pt = t32;
+  const float* Mt = t32 + (size_t)kpad * apad;
+  float4_ accp[4], accm[4];
+  for (int t = 0; t < 4; ++t) {
+    accp[t] = (float4_){0.f, 0.f, 0.f, 0.f};
+    accm[t] = (float4_){0.f, 0.f, 0.f, 0.f};
+  }
+  for (int k0 = 0; k0 < kmax; k0 += 4) {
+    const int kk = k0 + kq;
+    float ua = 0.f, va = 0.f;
+    if (kk < nm_a) {
+      const double ar = an[2 * (oa + kk) + comp], br = bn[2 * (oa + kk) + comp];
+      ua = (float)(ar + br);
+      va = (float)(ar - br);
+    }
+    const size_t row = (size_t)kk * apad + j0 + i;
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      accp[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(ua, Pt[row + 16 * t], accp[t], 0, 0, 0);
+      accm[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(va, Mt[row + 16 * t], accm[t], 0, 0, 0);
+    }
+  }
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    const int j = j0 + t * 16 + i;
+    if (j >= n_ang) continue;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int64_t p = p0 + 2 * kq + h;
+      if (p >= n) continue;
+      s12_store(mode, p, j, n_ang, cd{(double)accp[t][2 * h], (double)accp[t][2 * h + 1]},
+                cd{(double)accm[t][2 * h], (double)accm[t][2 * h + 1]}, out0, out1);
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void k_s12_bf16x3(
+    int64_t n, const int64_t* __restrict__ off, const int32_t* __restrict__ nmax,
+    const double* __restrict__ an, const double* __restrict__ bn, int n_ang, int apad, int kpad,
+    const unsigned short* __restrict__ t16, int nmax_cap, int mode, double* __restrict__ out0,
+    double* __restrict__ out1) {
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int64_t p0 = (int64_t)blockIdx.x * 8;
+  const int j0 = (blockIdx.y * 4 + wave) * 64;
+  if (j0 >= n_ang) return;
+  const int i = lane & 15, kq = lane >> 4;
+  const int64_t pa = p0 + (i >> 1);
+  const int comp = i & 1;
+  const bool pa_ok = pa < n;
+  const int nm_a = pa_ok ? nmax[pa] : 0;
+  const int64_t oa = pa_ok ? off[pa] : 0;
+  const int kmax = tile_kmax(n, p0, nmax, nmax_cap, kLpK);
+  const size_t plane = (size_t)kpad * apad;
+  const uint4* tab = (const uint4*)t16;   // 8 BF16 (one K-octet) per uint4
+  const size_t plane4 = plane / 8;
+  float4_ accp[4], accm[4];
+  for (int t = 0; t < 4; ++t) {
+    accp[t] = (float4_){0.f, 0.f, 0.f, 0.f};
+    accm[t] = (float4_){0.f, 0.f, 0.f, 0.f};
+  }
+  for (int k0 = 0; k0 < kmax; k0 += kLpK) {
+    const int kb = k0 + 8 * kq;   // this lane's K-octet
+    union { bf16x8_ v; unsigned short s[8]; } uh, ul, vh, vl;
+#pragma unroll
+    for (int t = 0; t < 8; ++t) {
+      const int kk = kb + t;
+      double u = 0.0, v = 0.0;
+      if (kk < nm_a) {
+        const double ar = an[2 * (oa + kk) + comp], br = bn[2 * (oa + kk) + comp];
+        u = ar + br;
+        v = ar - br;
       }
+      bf16_split(u, uh.s[t], ul.s[t]);
+      bf16_split(v, vh.s[t], vl.s[t]);
+    }
+    const size_t o4 = (size_t)(kb >> 3) * apad + j0 + i;
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      union { uint4 q; bf16x8_ v; } ph, pl, mh, ml;
+      ph.q = tab[o4 + 16 * t];
+      pl.q = tab[plane4 + o4 + 16 * t];
+      mh.q = tab[2 * plane4 + o4 + 16 * t];
+      ml.q = tab[3 * plane4 + o4 + 16 * t];
+      accp[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(uh.v, ph.v, accp[t], 0, 0, 0);
+      accp[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(uh.v, pl.v, accp[t], 0, 0, 0);
+      accp[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ul.v, ph.v, accp[t], 0, 0, 0);
+      accm[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(vh.v, mh.v, accm[t], 0, 0, 0);
+      accm[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(vh.v, ml.v, accm[t], 0, 0, 0);
+      accm[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(vl.v, mh.v, accm[t], 0, 0, 0);
+    }
+  }
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    const int j = j0 + t * 16 + i;
+    if (j >= n_ang) continue;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int64_t p = p0 + 2 * kq + h;
+      if (p >= n) continue;
+      s12_store(mode, p, j, n_ang, cd{(double)accp[t][2 * h], (double)accp[t][2 * h + 1]},
+                cd{(double)accm[t][2 * h], (double)accm[t][2 * h + 1]}, out0, out1);
     }
   }
 }
 
 }  // namespace gm
+
+namespace {
+
+// Launch the S1/S2 contraction at the requested precision.  Tables for the
+// reduced-precision paths are rebuilt on every call (one pass over pi, tau:
+// small next to the contraction itself).
+int launch_s12(gm_ctx* ctx, const char* who, int64_t n, const int64_t* off, const int32_t* nmax,
+               const double* an, const double* bn, int n_ang, const double* pi_t,
+               const double* tau_t, int nmax_cap, int mode, double* out0, double* out1,
+               int precision, hipStream_t s) {
+  if (precision != GM_PREC_FP64 && precision != GM_PREC_FP32 && precision != GM_PREC_BF16X3)
+    return gm_fail(GM_EINVAL, "%s: unknown precision %d", who, precision);
+  if (nmax_cap < 1) return gm_fail(GM_EINVAL, "%s: nmax_cap must be >= 1", who);
+  hipError_t e = hipSetDevice(ctx->device);
+  if (e != hipSuccess) return gm_fail(GM_ERUNTIME, "hipSetDevice: %s", hipGetErrorString(e));
+  dim3 grid((unsigned)((n + 7) / 8), (unsigned)((n_ang + 255) / 256));
+  size_t slot;
+  if (precision == GM_PREC_FP64) {
+    ctx->mark_begin(gm::kKS12, s, &slot);
+    hipLaunchKernelGGL(gm::k_s12_f64, grid, dim3(256), 0, s, n, off, nmax, an, bn, n_ang, pi_t,
+                       tau_t, nmax_cap, mode, out0, out1);
+    ctx->mark_end(slot, s);
+  } else {
+    const int kpad = (nmax_cap + gm::kLpK - 1) / gm::kLpK * gm::kLpK;
+    // whole 256-angle blocks: every wavefront with j0 < n_ang reads 64 padded columns
+    const int apad = (n_ang + 255) / 256 * 256;
+    const size_t elems = (size_t)kpad * apad;
+    const size_t bytes = precision == GM_PREC_FP32 ? 2 * elems * sizeof(float)
+                                                   : 4 * elems * sizeof(unsigned short);
+    int rc = ctx->reserve(ctx->tab_lp, bytes);
+    if (rc) return rc;
+    float* t32 = (float*)ctx->tab_lp.ptr;
+    unsigned short* t16 = (unsigned short*)ctx->tab_lp.ptr;
+    hipLaunchKernelGGL(gm::k_s12_prep_tables, dim3((unsigned)((elems + 255) / 256)), dim3(256), 0,
+                       s, n_ang, nmax_cap, kpad, apad, pi_t, tau_t, precision, t32, t16);
+    ctx->mark_begin(gm::kKS12, s, &slot);
+    if (precision == GM_PREC_FP32)
+      hipLaunchKernelGGL(gm::k_s12_f32, grid, dim3(256), 0, s, n, off, nmax, an, bn, n_ang, apad,
+                         kpad, (const float*)t32, nmax_cap, mode, out0, out1);
+    else
+      hipLaunchKernelGGL(gm::k_s12_bf16x3, grid, dim3(256), 0, s, n, off, nmax, an, bn, n_ang,
+                         apad, kpad, (const unsigned short*)t16, nmax_cap, mode, out0, out1);
+    ctx->mark_end(slot, s);
+  }
+  e = hipGetLastError();
+  if (e != hipSuccess) return gm_fail(GM_ERUNTIME, "%s: %s", who, hipGetErrorString(e));
+  return GM_OK;
+}
+
+}  // namespace
 
 extern "C" {
 
@@ -122,18 +379,8 @@ int gm_mie_s12(gm_ctx* ctx, int64_t n, const int64_t* coef_off, const int32_t* n
   if (n == 0 || n_ang == 0) return 0;
   if (!coef_off || !nmax || !an || !bn || !pi_t || !tau_t || !s1 || !s2)
     return gm_fail(-1, "gm_mie_s12: NULL array");
-  if (precision != 0) return gm_fail(-1, "gm_mie_s12: precision %d not available", precision);
-  hipError_t e = hipSetDevice(ctx->device);
-  if (e != hipSuccess) return gm_fail(1, "hipSetDevice: %s", hipGetErrorString(e));
-  dim3 grid((unsigned)((n + 7) / 8), (unsigned)((n_ang + 255) / 256));
-  size_t slot;
-  ctx->mark_begin(gm::kKS12, (hipStream_t)stream, &slot);
-  hipLaunchKernelGGL(gm::k_s12_f64, grid, dim3(256), 0, (hipStream_t)stream, n, coef_off, nmax,
-                     an, bn, n_ang, pi_t, tau_t, nmax_cap, (int)gm::kOutS12, s1, s2);
-  ctx->mark_end(slot, (hipStream_t)stream);
-  e = hipGetLastError();
-  if (e != hipSuccess) return gm_fail(1, "k_s12_f64: %s", hipGetErrorString(e));
-  return 0;
+  return launch_s12(ctx, "gm_mie_s12", n, coef_off, nmax, an, bn, n_ang, pi_t, tau_t, nmax_cap,
+                    (int)gm::kOutS12, s1, s2, precision, (hipStream_t)stream);
 }
 
 int gm_s12_mueller(gm_ctx* ctx, int64_t n, const int64_t* coef_off, const int32_t* nmax,
@@ -144,19 +391,9 @@ int gm_s12_mueller(gm_ctx* ctx, int64_t n, const int64_t* coef_off, const int32_
   if (n == 0 || n_ang == 0) return 0;
   if (!coef_off || !nmax || !an || !bn || !pi_t || !tau_t || !mueller)
     return gm_fail(-1, "gm_s12_mueller: NULL array");
-  if (precision != 0) return gm_fail(-1, "gm_s12_mueller: precision %d not available", precision);
-  hipError_t e = hipSetDevice(ctx->device);
-  if (e != hipSuccess) return gm_fail(1, "hipSetDevice: %s", hipGetErrorString(e));
-  dim3 grid((unsigned)((n + 7) / 8), (unsigned)((n_ang + 255) / 256));
-  size_t slot;
-  ctx->mark_begin(gm::kKS12, (hipStream_t)stream, &slot);
-  hipLaunchKernelGGL(gm::k_s12_f64, grid, dim3(256), 0, (hipStream_t)stream, n, coef_off, nmax,
-                     an, bn, n_ang, pi_t, tau_t, nmax_cap, (int)gm::kOutMueller, mueller,
-                     (double*)nullptr);
-  ctx->mark_end(slot, (hipStream_t)stream);
-  e = hipGetLastError();
-  if (e != hipSuccess) return gm_fail(1, "k_s12_f64: %s", hipGetErrorString(e));
-  return 0;
+  return launch_s12(ctx, "gm_s12_mueller", n, coef_off, nmax, an, bn, n_ang, pi_t, tau_t,
+                    nmax_cap, (int)gm::kOutMueller, mueller, (double*)nullptr, precision,
+                    (hipStream_t)stream);
 }
 
 }  // extern "C"

@@ -11,6 +11,11 @@ from ._lib import check, ptr, stream
 PREC = {"fp64": 0, "fp32": 1, "bf16x3": 2}
 
 
+def _prec(precision):
+    """gm_precision code of a name in PREC (integers pass through to the library)."""
+    return PREC[precision] if isinstance(precision, str) else int(precision)
+
+
 def _coef_dev(an_list, bn_list):
     """Pack ragged coefficient lists -> (d_off, d_nmax, d_an, d_bn)."""
     t = _lib.torch()
@@ -55,7 +60,7 @@ def s12_batch(an_list, bn_list, cost=None, pi_tab=None, tau_tab=None, precision=
     d_s2 = t.empty_like(d_s1)
     check(_lib.lib().gm_mie_s12(_lib.ctx(), n, ptr(d_off), ptr(d_nmax), ptr(d_an), ptr(d_bn),
                                 int(n_ang), ptr(pi_tab), ptr(tau_tab), int(pi_tab.shape[0]),
-                                ptr(d_s1), ptr(d_s2), PREC[precision], stream()), "gm_mie_s12")
+                                ptr(d_s1), ptr(d_s2), _prec(precision), stream()), "gm_mie_s12")
     s1 = d_s1.cpu().numpy().view(np.complex128).reshape(n, n_ang)
     s2 = d_s2.cpu().numpy().view(np.complex128).reshape(n, n_ang)
     return s1, s2
