@@ -26,6 +26,23 @@ METRIC = "(r,λ,m) Mie points/s + bulk LUT entries/s vs plain GPU at ≤1e-3 rel
 PEAK_FP64_TFLOPS = 78.6  # MI355X FP64 (vector = matrix on gfx950)
 
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles",
+                            "k_mie_coated_stable__coated8.4M__pmc_fetch_write.txt")
+
+
+def measured_traffic():
+    """HBM bytes per launch of the coated kernel from the committed PMC summary
+    (FETCH_SIZE and WRITE_SIZE passes, copy-calibrated), or None."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return float(line.split(":")[1])
+    except OSError:
+        pass
+    return None
+
+
 def coated_workload():
     r = np.logspace(np.log10(0.01), np.log10(10.0), 5000)
     lam = np.linspace(0.3, 3.0, 20)
@@ -364,8 +381,10 @@ def main():
         "roofline": {"bound": "mfma", "pipe": "FP64 VALU (gfx950 FP64 vector peak = FP64 "
                                               "matrix peak = 78.6 TFLOP/s)",
                      "kernel": "k_mie_coated", "achieved": achieved, "peak": PEAK_FP64_TFLOPS,
-                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
-                     "kernel_ms": kern_ms, "flop_per_launch": flops},
+                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS,
+                     "traffic": measured_traffic(), "kernel_ms": kern_ms,
+                     "flop_per_launch": flops,
+                     "algorithmic_bytes_per_launch": 104.0 * n},
     }
     if args.copy_calib:
         a = torch.ones(64 << 20, dtype=torch.float64, device="cuda")   # 512 MiB

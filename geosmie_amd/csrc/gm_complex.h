@@ -1,9 +1,12 @@
 // FP64 complex arithmetic for the gfx950 Mie kernels.
 //
 // Plain structs (no std::complex) so the compiler keeps every value in VGPRs
-// and fuses multiply-adds.  Division uses Smith's method, the same algorithm
-// numpy uses for complex128 division, so quotients agree with the reference's
-// numpy/numba arithmetic to a few ulps and cannot overflow through |b|^2.
+// and fuses multiply-adds.  Division is branch-free: the divisor is scaled by
+// a power of two (exponent only, exact) so |b|^2 cannot overflow or
+// underflow, the reciprocal comes from v_rcp_f64 refined by two Newton steps,
+// and the scale is undone; quotients are within a few ulps of numpy's
+// complex128 division.  Recurrences that need more use the double-double
+// reciprocals further down.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <math.h>
@@ -82,7 +85,7 @@ __device__ __forceinline__ double mul_dd(double a, ddr b) { return fma(a, b.hi, 
 __device__ __forceinline__ cd mul_dd(cd a, ddr b) {
   return cd{fma(a.re, b.hi, a.re * b.lo), fma(a.im, b.hi, a.im * b.lo)};
 }
-// a / z for complex z: hi = 1/z (Smith), lo = hi * (1 - z hi) with the
+// a / z for complex z: hi = 1/z (cdiv), lo = hi * (1 - z hi) with the
 // residual formed from FMA-exact products.
 __device__ __forceinline__ cdd crcp_dd(cd z) {
   cd hi = cdiv(cd{1.0, 0.0}, z);
