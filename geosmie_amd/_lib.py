@@ -56,6 +56,27 @@ SIGNATURES = {
                                      ctypes.c_int, c_vp, c_vp]),
     "gm_legendre_eval": (ctypes.c_int, [c_vp, c_i64, c_vp, ctypes.c_int, ctypes.c_int, c_vp,
                                         c_vp, c_vp]),
+    "gm_s12_p11": (ctypes.c_int, [c_vp, c_i64, c_vp, c_vp, c_vp, c_vp, ctypes.c_int, c_vp, c_vp,
+                                  ctypes.c_int, c_vp, ctypes.c_int, c_vp]),
+    "gm_lut_points": (ctypes.c_int, [c_vp, c_i64, ctypes.c_int, c_vp, c_vp, c_vp, c_vp, c_dbl,
+                                     c_dbl, c_vp, c_vp, c_vp]),
+    "gm_lut_lognormal": (ctypes.c_int, [c_vp, c_i64, ctypes.c_int, c_vp, c_vp, c_vp, c_vp, c_vp,
+                                        c_vp]),
+    "gm_lut_bulk": (ctypes.c_int, [c_vp, c_i64, ctypes.c_int, ctypes.c_int, ctypes.c_int, c_vp,
+                                   ctypes.c_int, c_vp, c_vp, c_vp, c_vp, c_vp, ctypes.c_int,
+                                   ctypes.c_int, c_vp, c_vp]),
+    "gm_mlp_param_count": (ctypes.c_int64, [ctypes.c_int, c_vp]),
+    "gm_mlp_forward": (ctypes.c_int, [c_vp, ctypes.c_int, c_vp, c_i64, c_vp, c_vp, c_vp, c_vp]),
+    "gm_mlp_grad": (ctypes.c_int, [c_vp, ctypes.c_int, c_vp, c_i64, c_vp, c_vp, c_vp, c_vp, c_vp,
+                                   c_vp, c_vp]),
+    "gm_mlp_adam": (ctypes.c_int, [c_vp, c_i64, c_vp, c_vp, c_vp, c_vp, ctypes.c_float,
+                                   ctypes.c_float, ctypes.c_float, ctypes.c_float, c_i64, c_vp]),
+    "gm_surrogate_prep": (ctypes.c_int, [c_vp, c_i64, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp,
+                                         c_vp, c_vp, c_vp]),
+    "gm_surrogate_post": (ctypes.c_int, [c_vp, c_i64, ctypes.c_int, c_vp, c_vp, c_vp, c_vp,
+                                         c_vp]),
+    "gm_surrogate_targets": (ctypes.c_int, [c_vp, c_i64, ctypes.c_int, c_vp, c_vp, c_vp, c_vp,
+                                            c_vp]),
 }
 
 

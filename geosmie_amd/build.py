@@ -12,7 +12,8 @@ LIB = os.path.join(HERE, "libgeosmie_hip.so")
 INCLUDE = os.path.join(os.path.dirname(HERE), "include")
 
 SOURCES = ["gm_api.hip", "gm_util.hip", "gm_mie.hip", "gm_angles.hip", "gm_s12.hip",
-           "gm_integrate.hip", "gm_gsf.hip", "gm_mlp.hip"]
+           "gm_integrate.hip", "gm_gsf.hip", "gm_lut.hip", "gm_mlp.hip",
+           "gm_surrogate.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 

@@ -23,13 +23,13 @@ struct gm_ctx {
   size_t scratch_budget = (size_t)3 << 30;       // bytes of recurrence scratch
   Buf status;                                    // int32 flags + int32 max_nmax + scan scratch
   Buf order, key, hist, scratch, scan_tmp;
-  Buf tab_pi, tab_tau, gemm_tmp, feat_tmp, gsf_tab, tab_lp;
+  Buf tab_pi, tab_tau, gemm_tmp, feat_tmp, gsf_tab, tab_lp, lut_tmp, mlp_ws;
   // cached GSF tables: key = (ng, n_ang, checksum of the angle grid)
   int gsf_ng = 0, gsf_nang = 0;
   double gsf_key = 0.0;
   std::vector<Buf*> all_bufs() {
     return {&status, &order, &key, &hist, &scratch, &scan_tmp, &tab_pi, &tab_tau, &gemm_tmp,
-            &feat_tmp, &gsf_tab, &tab_lp};
+            &feat_tmp, &gsf_tab, &tab_lp, &lut_tmp, &mlp_ws};
   }
   int reserve(Buf& b, size_t bytes);
   ~gm_ctx();
@@ -50,11 +50,14 @@ struct gm_ctx {
 namespace gm {
 // kernel ids for gm_ctx_kernel_time
 enum KernelId { kKMieHomog = 0, kKMieCoated = 1, kKS12 = 2, kKPsd = 3, kKGsf = 4, kKMlp = 5,
-                kKMieCoatedLit = 6, kKCount = 7 };
+                kKMieCoatedLit = 6, kKLut = 7, kKCount = 8 };
 }  // namespace gm
 
 namespace gm {
 // In-place inclusive prefix sum of n int64 values (device) on stream s.
 int scan_inclusive_i64(gm_ctx* ctx, int64_t* data, int64_t n, hipStream_t s);
+// Make ctx->gsf_tab hold the GSF tables G[4][n_ang][ng] (P00, P22, P2-2, P02
+// with Gauss weights and the angle interpolation folded in) for this grid.
+int gsf_tables(gm_ctx* ctx, int ng, int n_ang, const double* ang_deg_host, hipStream_t s);
 __global__ void k_nmax_i32(int64_t n, const double* size, int32_t* out);
 }  // namespace gm

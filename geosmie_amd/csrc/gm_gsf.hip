@@ -218,6 +218,26 @@ __global__ void k_legendre_eval(int64_t n_x, const double* __restrict__ x, int n
   out[(size_t)s * n_x + i] = acc;
 }
 
+// Make ctx->gsf_tab hold G[4][n_ang][ng] for this (ng, angle grid); rebuilt
+// only when either changes.
+int gsf_tables(gm_ctx* ctx, int ng, int n_ang, const double* ang_deg_host, hipStream_t s) {
+  double key = 0.0;
+  for (int a = 0; a < n_ang; ++a) key += ang_deg_host[a] * (1.0 + 1e-3 * a);
+  if (ctx->gsf_ng == ng && ctx->gsf_nang == n_ang && ctx->gsf_key == key) return GM_OK;
+  std::vector<double> G;
+  build_tables(ng, n_ang, ang_deg_host, G);
+  int rc = ctx->reserve(ctx->gsf_tab, G.size() * sizeof(double));
+  if (rc) return rc;
+  hipError_t e = hipMemcpyAsync(ctx->gsf_tab.ptr, G.data(), G.size() * sizeof(double),
+                                hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);  // G is a host temporary
+  if (e != hipSuccess) return gm_fail(GM_ERUNTIME, "gsf tables: %s", hipGetErrorString(e));
+  ctx->gsf_ng = ng;
+  ctx->gsf_nang = n_ang;
+  ctx->gsf_key = key;
+  return GM_OK;
+}
+
 }  // namespace gm
 
 extern "C" {
@@ -234,24 +254,10 @@ int gm_gsf_expand(gm_ctx* ctx, int64_t n_entries, int n_ang, const double* ang_d
   hipError_t e = hipSetDevice(ctx->device);
   if (e != hipSuccess) return gm_fail(1, "hipSetDevice: %s", hipGetErrorString(e));
   hipStream_t s = (hipStream_t)stream;
-  double key = 0.0;
-  for (int a = 0; a < n_ang; ++a) key += ang_deg_host[a] * (1.0 + 1e-3 * a);
-  if (ctx->gsf_ng != ng || ctx->gsf_nang != n_ang || ctx->gsf_key != key) {
-    std::vector<double> G;
-    gm::build_tables(ng, n_ang, ang_deg_host, G);
-    int rc = ctx->reserve(ctx->gsf_tab, G.size() * sizeof(double));
-    if (rc) return rc;
-    e = hipMemcpyAsync(ctx->gsf_tab.ptr, G.data(), G.size() * sizeof(double),
-                       hipMemcpyHostToDevice, s);
-    if (e != hipSuccess) return gm_fail(1, "gsf tables: %s", hipGetErrorString(e));
-    e = hipStreamSynchronize(s);  // G is a host temporary
-    if (e != hipSuccess) return gm_fail(1, "gsf tables: %s", hipGetErrorString(e));
-    ctx->gsf_ng = ng;
-    ctx->gsf_nang = n_ang;
-    ctx->gsf_key = key;
-  }
+  int rc = gm::gsf_tables(ctx, ng, n_ang, ang_deg_host, s);
+  if (rc) return rc;
   size_t raw_bytes = sizeof(double) * (size_t)n_entries * 6 * ng;
-  int rc = ctx->reserve(ctx->gemm_tmp, raw_bytes);
+  rc = ctx->reserve(ctx->gemm_tmp, raw_bytes);
   if (rc) return rc;
   double* raw = (double*)ctx->gemm_tmp.ptr;
   size_t slot;

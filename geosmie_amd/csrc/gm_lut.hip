@@ -1,0 +1,234 @@
+// LUT sweep on gfx950: bulk optics of lognormal size distributions for many
+// (refractive index, wavelength, humidity) slabs at once.
+//
+// A slab is one (m_dry, lambda, gf) combination over the dry radius grid
+// r_dry[n_r] -- one iteration of runoptics' (lambda, rh) loop
+// (src/geosmie/dointegration.py:787-889).  Points are slab-major (point
+// s * n_r + p).  Humidity growth follows the reference's 'simple' rule
+// (dointegration.py:497-507): wet radius gf * r_dry and volume mixing with
+// water  m = m_w + (m_dry - m_w) * (1/gf)^3.
+//
+// For every (slab s, PSD k) entry, with number weights w[k][p] on the dry
+// grid (growth moves every particle of a bin together, so the weights carry
+// over), integratePSD's area weighting ('dumix', dointegration.py:1102-1190):
+//   Qext = sum w qext r^2 / sum w r^2,  Qsca likewise,  SSA = Qsca / Qext (:961),
+//   g    = sum w g qsca r^2 / sum w qsca r^2   (the Qsca weight, :1111-1112),
+// and the first n_mom Legendre moments of P11 as spher_expan.f forms alpha1
+// (SPHER_EXPAN :313-347 with LINTERPOL and the Gauss nodes of GAUSS),
+// normalised by alpha1_0 (CNORM, :95-103).  P11's own normalisation
+// (dointegration.py:978-985) cancels in that ratio and is skipped.
+// Output rows are float32: Qext, SSA, g, alpha1_0 .. alpha1_{n_mom-1}.
+#include <math.h>
+
+#include "gm_complex.h"
+#include "gm_ctx.h"
+
+namespace gm {
+
+constexpr int kLutPsdChunk = 16;  // PSDs per thread (accumulators in VGPRs)
+constexpr int kLutStage = 64;     // radii per LDS stage of the weights
+constexpr int kLutCols = 128;     // columns (threads) per block
+constexpr int kLutFeat = 4;       // r^2, qext r^2, qsca r^2, g qsca r^2
+constexpr int kLutMaxMom = 64;
+
+__global__ void k_lut_points(int64_t n_slab, int n_r, const double* __restrict__ r_dry,
+                             const double* __restrict__ lam, const double* __restrict__ m_dry,
+                             const double* __restrict__ gf, double mw_re, double mw_im,
+                             double* __restrict__ x, double* __restrict__ eps) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_slab * n_r) return;
+  const int64_t s = i / n_r;
+  const int p = (int)(i - s * n_r);
+  const double g = gf[s];
+  const double rrat = 1.0 / g;
+  const double r3 = rrat * rrat * rrat;
+  const double mr = mw_re + (m_dry[2 * s] - mw_re) * r3;
+  const double mi = mw_im + (m_dry[2 * s + 1] - mw_im) * r3;
+  x[i] = 2.0 * 3.14159265358979323846 * g * r_dry[p] / lam[s];
+  eps[2 * i] = mr * mr - mi * mi;
+  eps[2 * i + 1] = 2.0 * mr * mi;
+}
+
+// sums[s][k][c], c < n_ang: sum_p w[k][p] P11[s][p][c];  c = n_ang + f: the
+// four area-weighted features.  Block = (slab, 128 columns, 16 PSDs); the
+// weights of 64 radii x 16 PSDs are staged in LDS and read as broadcasts.
+__global__ __launch_bounds__(kLutCols) void k_lut_contract(
+    int64_t n_slab, int n_r, int n_ang, int n_psd, const double* __restrict__ w,
+    int64_t w_slab_stride, const double* __restrict__ x, const double* __restrict__ lam,
+    const double* __restrict__ props, const double* __restrict__ p11, double* __restrict__ sums) {
+  __shared__ double ws[kLutPsdChunk][kLutStage];
+  const int ncol = n_ang + kLutFeat;
+  const int ncb = (ncol + kLutCols - 1) / kLutCols;
+  const int64_t s = blockIdx.x / ncb;
+  const int c = (int)(blockIdx.x - s * ncb) * kLutCols + threadIdx.x;
+  const int k0 = blockIdx.y * kLutPsdChunk;
+  const int kn = min(kLutPsdChunk, n_psd - k0);
+  const int64_t n_all = n_slab * n_r;
+  const double* wb = w + s * w_slab_stride + (size_t)k0 * n_r;
+  const double rscale = lam[s] / (2.0 * 3.14159265358979323846);
+  double acc[kLutPsdChunk];
+#pragma unroll
+  for (int k = 0; k < kLutPsdChunk; ++k) acc[k] = 0.0;
+  for (int pb = 0; pb < n_r; pb += kLutStage) {
+    const int pn = min(kLutStage, n_r - pb);
+    __syncthreads();
+    for (int t = threadIdx.x; t < kLutPsdChunk * kLutStage; t += kLutCols) {
+      const int kk = t / kLutStage, pp = t - kk * kLutStage;
+      ws[kk][pp] = (kk < kn && pp < pn) ? wb[(size_t)kk * n_r + pb + pp] : 0.0;
+    }
+    __syncthreads();
+    if (c < ncol) {
+      for (int pp = 0; pp < pn; ++pp) {
+        const int64_t i = s * n_r + pb + pp;
+        double v;
+        if (c < n_ang) {
+          v = p11[i * n_ang + c];
+        } else {
+          const double r = x[i] * rscale, r2 = r * r;
+          const int f = c - n_ang;
+          const double qext = props[i], qsca = props[n_all + i], g = props[4 * n_all + i];
+          v = f == 0 ? r2 : (f == 1 ? qext * r2 : (f == 2 ? qsca * r2 : g * qsca * r2));
+        }
+#pragma unroll
+        for (int k = 0; k < kLutPsdChunk; ++k) acc[k] = fma(ws[k][pp], v, acc[k]);
+      }
+    }
+  }
+  if (c >= ncol) return;
+  for (int k = 0; k < kn; ++k) sums[((size_t)s * n_psd + k0 + k) * ncol + c] = acc[k];
+}
+
+// One wavefront per entry: lane l forms alpha1_l from the GSF table G1[a][l].
+__global__ __launch_bounds__(64) void k_lut_finish(int n_ang, int ng, int n_mom,
+                                                   const double* __restrict__ G1,
+                                                   const double* __restrict__ sums,
+                                                   float* __restrict__ out) {
+  const int64_t e = blockIdx.x;
+  const int l = threadIdx.x;
+  const int ncol = n_ang + kLutFeat;
+  const double* S = sums + e * ncol;
+  double acc = 0.0;
+  if (l < n_mom)
+    for (int a = 0; a < n_ang; ++a) acc = fma(G1[(size_t)a * ng + l], S[a], acc);
+  const double alpha = acc * ((double)l + 0.5);
+  const double a0 = __shfl(alpha, 0);
+  float* o = out + e * (3 + n_mom);
+  if (l < n_mom) o[3 + l] = (float)(alpha / a0);
+  if (l == 0) {
+    const double sr2 = S[n_ang], sext = S[n_ang + 1], ssca = S[n_ang + 2], sg = S[n_ang + 3];
+    o[0] = (float)(sext / sr2);
+    o[1] = (float)(ssca / sext);
+    o[2] = (float)(sg / ssca);
+  }
+}
+
+// w[k][p] = n(r_p; r_mode_k, sigma_k) dr_p normalised to sum 1 over p: the
+// lognormal number density of particleparams.py:113-127 times the bin widths,
+// as calculatePSD builds a mode (dointegration.py:569-577).  One block per row.
+__global__ __launch_bounds__(256) void k_lut_lognormal(int n_r, const double* __restrict__ r,
+                                                       const double* __restrict__ dr,
+                                                       const double* __restrict__ rmode,
+                                                       const double* __restrict__ sigma,
+                                                       double* __restrict__ w) {
+  __shared__ double red[256];
+  const int64_t k = blockIdx.x;
+  const double ls = log(sigma[k]), rm = rmode[k];
+  const double c = 1.0 / (sqrt(2.0 * 3.14159265358979323846) * ls);
+  double* wk = w + k * n_r;
+  double s = 0.0;
+  for (int p = threadIdx.x; p < n_r; p += 256) {
+    const double t = log(r[p] / rm);
+    const double v = exp(-t * t / (2.0 * ls * ls)) * c / r[p] * dr[p];
+    wk[p] = v;
+    s += v;
+  }
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if (threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+    __syncthreads();
+  }
+  const double inv = 1.0 / red[0];
+  for (int p = threadIdx.x; p < n_r; p += 256) wk[p] *= inv;
+}
+
+}  // namespace gm
+
+extern "C" {
+
+int gm_lut_lognormal(gm_ctx* ctx, int64_t n_psd, int n_r, const double* r, const double* dr,
+                     const double* rmode, const double* sigma, double* w, gm_stream stream) {
+  if (!ctx || n_psd < 0 || n_r < 1) return gm_fail(GM_EINVAL, "gm_lut_lognormal: bad arguments");
+  if (n_psd == 0) return GM_OK;
+  if (!r || !dr || !rmode || !sigma || !w) return gm_fail(GM_EINVAL, "gm_lut_lognormal: NULL array");
+  if (n_psd > 0x7fffffffLL) return gm_fail(GM_EINVAL, "gm_lut_lognormal: too many rows");
+  hipError_t e = hipSetDevice(ctx->device);
+  if (e != hipSuccess) return gm_fail(GM_ERUNTIME, "hipSetDevice: %s", hipGetErrorString(e));
+  hipLaunchKernelGGL(gm::k_lut_lognormal, dim3((unsigned)n_psd), dim3(256), 0, (hipStream_t)stream,
+                     n_r, r, dr, rmode, sigma, w);
+  e = hipGetLastError();
+  if (e != hipSuccess) return gm_fail(GM_ERUNTIME, "k_lut_lognormal: %s", hipGetErrorString(e));
+  return GM_OK;
+}
+
+// See include/geosmie.h.
+int gm_lut_points(gm_ctx* ctx, int64_t n_slab, int n_r, const double* r_dry, const double* lam,
+                  const double* m_dry, const double* gf, double mw_re, double mw_im, double* x,
+                  double* eps, gm_stream stream) {
+  if (!ctx || n_slab < 0 || n_r < 0) return gm_fail(GM_EINVAL, "gm_lut_points: bad arguments");
+  if (n_slab == 0 || n_r == 0) return GM_OK;
+  if (!r_dry || !lam || !m_dry || !gf || !x || !eps)
+    return gm_fail(GM_EINVAL, "gm_lut_points: NULL array");
+  hipError_t e = hipSetDevice(ctx->device);
+  if (e != hipSuccess) return gm_fail(GM_ERUNTIME, "hipSetDevice: %s", hipGetErrorString(e));
+  const int64_t n = n_slab * n_r;
+  hipLaunchKernelGGL(gm::k_lut_points, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                     (hipStream_t)stream, n_slab, n_r, r_dry, lam, m_dry, gf, mw_re, mw_im, x,
+                     eps);
+  e = hipGetLastError();
+  if (e != hipSuccess) return gm_fail(GM_ERUNTIME, "k_lut_points: %s", hipGetErrorString(e));
+  return GM_OK;
+}
+
+int gm_lut_bulk(gm_ctx* ctx, int64_t n_slab, int n_r, int n_ang, int n_psd, const double* w,
+                int w_per_slab, const double* x, const double* lam, const double* props,
+                const double* p11, const double* ang_deg_host, int ng, int n_mom, float* out,
+                gm_stream stream) {
+  if (!ctx || n_slab < 0 || n_r < 1 || n_ang < 2 || n_psd < 1 || !ang_deg_host)
+    return gm_fail(GM_EINVAL, "gm_lut_bulk: bad arguments");
+  if (n_mom < 1 || n_mom > gm::kLutMaxMom || n_mom > ng)
+    return gm_fail(GM_EINVAL, "gm_lut_bulk: n_mom must be in [1, min(ng, %d)]", gm::kLutMaxMom);
+  if (n_slab == 0) return GM_OK;
+  if (!w || !x || !lam || !props || !p11 || !out)
+    return gm_fail(GM_EINVAL, "gm_lut_bulk: NULL array");
+  for (int a = 1; a < n_ang; ++a)
+    if (!(ang_deg_host[a] > ang_deg_host[a - 1]))
+      return gm_fail(GM_EINVAL, "gm_lut_bulk: angles must increase");
+  const int64_t ncb = (n_ang + gm::kLutFeat + gm::kLutCols - 1) / gm::kLutCols;
+  const int64_t n_ent = n_slab * n_psd;
+  if (n_slab * ncb > 0x7fffffffLL || n_ent > 0x7fffffffLL)
+    return gm_fail(GM_EINVAL, "gm_lut_bulk: too many slabs for one call");
+  hipError_t e = hipSetDevice(ctx->device);
+  if (e != hipSuccess) return gm_fail(GM_ERUNTIME, "hipSetDevice: %s", hipGetErrorString(e));
+  hipStream_t s = (hipStream_t)stream;
+  int rc = gm::gsf_tables(ctx, ng, n_ang, ang_deg_host, s);
+  if (rc) return rc;
+  rc = ctx->reserve(ctx->lut_tmp, sizeof(double) * (size_t)n_ent * (n_ang + gm::kLutFeat));
+  if (rc) return rc;
+  double* sums = (double*)ctx->lut_tmp.ptr;
+  size_t slot;
+  ctx->mark_begin(gm::kKLut, s, &slot);
+  hipLaunchKernelGGL(gm::k_lut_contract,
+                     dim3((unsigned)(n_slab * ncb),
+                          (unsigned)((n_psd + gm::kLutPsdChunk - 1) / gm::kLutPsdChunk)),
+                     dim3(gm::kLutCols), 0, s, n_slab, n_r, n_ang, n_psd, w,
+                     w_per_slab ? (int64_t)n_psd * n_r : (int64_t)0, x, lam, props, p11, sums);
+  hipLaunchKernelGGL(gm::k_lut_finish, dim3((unsigned)n_ent), dim3(64), 0, s, n_ang, ng, n_mom,
+                     (const double*)ctx->gsf_tab.ptr, sums, out);
+  ctx->mark_end(slot, s);
+  e = hipGetLastError();
+  if (e != hipSuccess) return gm_fail(GM_ERUNTIME, "gm_lut_bulk: %s", hipGetErrorString(e));
+  return GM_OK;
+}
+
+}  // extern "C"

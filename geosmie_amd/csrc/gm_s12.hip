@@ -35,7 +35,7 @@ typedef __bf16 bf16x8_ __attribute__((ext_vector_type(8)));
 
 namespace gm {
 
-enum { kOutS12 = 0, kOutMueller = 1 };
+enum { kOutS12 = 0, kOutMueller = 1, kOutP11 = 2 };
 
 // S1 = (sp + sm) / 2, S2 = (sp - sm) / 2 for point p, angle j; then either the
 // amplitudes or the Mueller elements (dointegration.py:1045-1050).
@@ -49,6 +49,8 @@ __device__ __forceinline__ void s12_store(int mode, int64_t p, int j, int n_ang,
     out0[2 * o + 1] = s1.im;
     out1[2 * o] = s2.re;
     out1[2 * o + 1] = s2.im;
+  } else if (mode == kOutP11) {
+    out0[o] = 0.5 * (norm2(s1) + norm2(s2));   // s11 only, layout [point][angle]
   } else {
     double a1 = norm2(s1), a2 = norm2(s2);
     cd c = s1 * conj(s2);
@@ -394,6 +396,17 @@ int gm_s12_mueller(gm_ctx* ctx, int64_t n, const int64_t* coef_off, const int32_
   return launch_s12(ctx, "gm_s12_mueller", n, coef_off, nmax, an, bn, n_ang, pi_t, tau_t,
                     nmax_cap, (int)gm::kOutMueller, mueller, (double*)nullptr, precision,
                     (hipStream_t)stream);
+}
+
+int gm_s12_p11(gm_ctx* ctx, int64_t n, const int64_t* coef_off, const int32_t* nmax,
+               const double* an, const double* bn, int n_ang, const double* pi_t,
+               const double* tau_t, int nmax_cap, double* p11, int precision, gm_stream stream) {
+  if (!ctx || n < 0 || n_ang < 0) return gm_fail(-1, "gm_s12_p11: bad arguments");
+  if (n == 0 || n_ang == 0) return 0;
+  if (!coef_off || !nmax || !an || !bn || !pi_t || !tau_t || !p11)
+    return gm_fail(-1, "gm_s12_p11: NULL array");
+  return launch_s12(ctx, "gm_s12_p11", n, coef_off, nmax, an, bn, n_ang, pi_t, tau_t, nmax_cap,
+                    (int)gm::kOutP11, p11, (double*)nullptr, precision, (hipStream_t)stream);
 }
 
 }  // extern "C"

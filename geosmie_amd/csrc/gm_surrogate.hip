@@ -1,0 +1,157 @@
+// Query handling of the LUT surrogate: feature transform, hull/trust gate and
+// output transform.  No counterpart in the reference.
+//
+// A query is q = (ln r_eff, sigma_g, m_r, ln m_i, lambda, f_w): r_eff [um] the
+// wet effective radius of a lognormal number distribution,
+// r_eff = gf r_mode exp(2.5 ln^2 sigma_g); f_w = 1 - gf^-3 the water volume
+// fraction of the humidity growth (dointegration.py:497-507); m_r + i m_i the
+// dry refractive index; lambda [um].
+//   network features  (ln r_eff, sigma_g, m_r, ln m_i, ln lambda, f_w), standardised;
+//   gate coordinates  (ln r_mode, sigma_g, m_r, ln m_i, ln lambda, f_w) with the
+//                     dry mode radius r_mode = r_eff / (gf exp(2.5 ln^2 sigma_g)).
+// flags[i] = 0: served by the network; 1: outside the training box [lo, hi];
+// 2: inside, but its cell of the uniform n_cells grid over the box is not
+// trusted (validation error above the bound).
+// Table rows and network targets (before standardisation) correspond as
+//   t0 = ln Qext,  t1 = ln(1 - SSA + d),  t2 = g,  t_{3+l} = alpha1_l / (2l + 1)
+// with d = 1e-4: the co-albedo is learnt on a log scale down to d, below which
+// SSA is within 1e-4 of 1 and its relative error is bounded by d anyway.
+#include <math.h>
+#include <stdint.h>
+
+#include "gm_ctx.h"
+
+namespace gm {
+
+constexpr int kSurrIn = 6;
+constexpr float kSsaFloor = 1e-4f;
+
+struct Cells {
+  int n[kSurrIn];
+};
+
+__global__ void k_surr_prep(int64_t n, const float* __restrict__ q, const float* __restrict__ fmean,
+                            const float* __restrict__ fstd, const float* __restrict__ lo,
+                            const float* __restrict__ hi, Cells cells,
+                            const uint8_t* __restrict__ cell_ok, float* __restrict__ x,
+                            int32_t* __restrict__ flags) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float* qi = q + i * kSurrIn;
+  float f[kSurrIn] = {qi[0], qi[1], qi[2], qi[3], logf(qi[4]), qi[5]};
+#pragma unroll
+  for (int j = 0; j < kSurrIn; ++j) x[i * kSurrIn + j] = (f[j] - fmean[j]) / fstd[j];
+  const float ls = logf(qi[1]);
+  const float gf = 1.0f / cbrtf(1.0f - qi[5]);
+  float gc[kSurrIn] = {qi[0] - logf(gf) - 2.5f * ls * ls, qi[1], qi[2], qi[3], f[4], qi[5]};
+  int flag = 0;
+#pragma unroll
+  for (int j = 0; j < kSurrIn; ++j)
+    if (!(gc[j] >= lo[j] && gc[j] <= hi[j])) flag = 1;   // NaN lands here too
+  if (flag == 0) {
+    int64_t cell = 0;
+#pragma unroll
+    for (int j = 0; j < kSurrIn; ++j) {
+      const float span = hi[j] - lo[j];
+      int c = span > 0.0f ? (int)((gc[j] - lo[j]) / span * (float)cells.n[j]) : 0;
+      c = min(max(c, 0), cells.n[j] - 1);
+      cell = cell * cells.n[j] + c;
+    }
+    if (!cell_ok[cell]) flag = 2;
+  }
+  flags[i] = flag;
+}
+
+__global__ void k_surr_post(int64_t n, int n_out, const float* __restrict__ y,
+                            const float* __restrict__ tmean, const float* __restrict__ tstd,
+                            float* __restrict__ rows) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n * n_out) return;
+  const int j = (int)(e % n_out);
+  const float t = y[e] * tstd[j] + tmean[j];
+  float v;
+  if (j == 0) v = expf(t);
+  else if (j == 1) v = 1.0f - (expf(t) - kSsaFloor);
+  else if (j == 2) v = t;
+  else v = (float)(2 * (j - 3) + 1) * t;
+  rows[e] = v;
+}
+
+__global__ void k_surr_targets(int64_t n, int n_out, const float* __restrict__ rows,
+                               const float* __restrict__ tmean, const float* __restrict__ tstd,
+                               float* __restrict__ y) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n * n_out) return;
+  const int j = (int)(e % n_out);
+  const float v = rows[e];
+  float t;
+  if (j == 0) t = logf(v);
+  else if (j == 1) t = logf(fmaxf(1.0f - v, 0.0f) + kSsaFloor);
+  else if (j == 2) t = v;
+  else t = v / (float)(2 * (j - 3) + 1);
+  y[e] = (t - tmean[j]) / tstd[j];
+}
+
+}  // namespace gm
+
+extern "C" {
+
+// See include/geosmie.h.
+int gm_surrogate_prep(gm_ctx* ctx, int64_t n, const float* q, const float* feat_mean,
+                      const float* feat_std, const float* box_lo, const float* box_hi,
+                      const int* n_cells_host, const uint8_t* cell_ok, float* x, int32_t* flags,
+                      gm_stream stream) {
+  if (!ctx || n < 0 || !n_cells_host) return gm_fail(GM_EINVAL, "gm_surrogate_prep: bad arguments");
+  gm::Cells cells;
+  int64_t total = 1;
+  for (int j = 0; j < gm::kSurrIn; ++j) {
+    if (n_cells_host[j] < 1) return gm_fail(GM_EINVAL, "gm_surrogate_prep: n_cells must be >= 1");
+    cells.n[j] = n_cells_host[j];
+    total *= n_cells_host[j];
+  }
+  if (total > ((int64_t)1 << 31)) return gm_fail(GM_EINVAL, "gm_surrogate_prep: too many cells");
+  if (n == 0) return GM_OK;
+  if (!q || !feat_mean || !feat_std || !box_lo || !box_hi || !cell_ok || !x || !flags)
+    return gm_fail(GM_EINVAL, "gm_surrogate_prep: NULL array");
+  hipError_t e = hipSetDevice(ctx->device);
+  if (e != hipSuccess) return gm_fail(GM_ERUNTIME, "hipSetDevice: %s", hipGetErrorString(e));
+  hipLaunchKernelGGL(gm::k_surr_prep, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                     (hipStream_t)stream, n, q, feat_mean, feat_std, box_lo, box_hi, cells,
+                     cell_ok, x, flags);
+  e = hipGetLastError();
+  if (e != hipSuccess) return gm_fail(GM_ERUNTIME, "k_surr_prep: %s", hipGetErrorString(e));
+  return GM_OK;
+}
+
+int gm_surrogate_post(gm_ctx* ctx, int64_t n, int n_out, const float* y, const float* tgt_mean,
+                      const float* tgt_std, float* rows, gm_stream stream) {
+  if (!ctx || n < 0 || n_out < 3) return gm_fail(GM_EINVAL, "gm_surrogate_post: bad arguments");
+  if (n == 0) return GM_OK;
+  if (!y || !tgt_mean || !tgt_std || !rows) return gm_fail(GM_EINVAL, "gm_surrogate_post: NULL array");
+  hipError_t e = hipSetDevice(ctx->device);
+  if (e != hipSuccess) return gm_fail(GM_ERUNTIME, "hipSetDevice: %s", hipGetErrorString(e));
+  const int64_t tot = n * n_out;
+  hipLaunchKernelGGL(gm::k_surr_post, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0,
+                     (hipStream_t)stream, n, n_out, y, tgt_mean, tgt_std, rows);
+  e = hipGetLastError();
+  if (e != hipSuccess) return gm_fail(GM_ERUNTIME, "k_surr_post: %s", hipGetErrorString(e));
+  return GM_OK;
+}
+
+int gm_surrogate_targets(gm_ctx* ctx, int64_t n, int n_out, const float* rows,
+                         const float* tgt_mean, const float* tgt_std, float* y, gm_stream stream) {
+  if (!ctx || n < 0 || n_out < 3) return gm_fail(GM_EINVAL, "gm_surrogate_targets: bad arguments");
+  if (n == 0) return GM_OK;
+  if (!rows || !tgt_mean || !tgt_std || !y)
+    return gm_fail(GM_EINVAL, "gm_surrogate_targets: NULL array");
+  hipError_t e = hipSetDevice(ctx->device);
+  if (e != hipSuccess) return gm_fail(GM_ERUNTIME, "hipSetDevice: %s", hipGetErrorString(e));
+  const int64_t tot = n * n_out;
+  hipLaunchKernelGGL(gm::k_surr_targets, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0,
+                     (hipStream_t)stream, n, n_out, rows, tgt_mean, tgt_std, y);
+  e = hipGetLastError();
+  if (e != hipSuccess) return gm_fail(GM_ERUNTIME, "k_surr_targets: %s", hipGetErrorString(e));
+  return GM_OK;
+}
+
+}  // extern "C"

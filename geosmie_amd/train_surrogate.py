@@ -1,0 +1,46 @@
+"""Train the LUT surrogate on the GPU and save it.
+
+    python -m geosmie_amd.train_surrogate --out geosmie_amd/data/surrogate_lut.npz
+
+Builds the training tables with the GPU LUT sweep (geosmie_amd.lut_sweep) on
+the full (m_r, m_i, lambda, RH, PSD) grid (or a strided subset), trains the
+6-256-256-256-256-35 network with the HIP forward/backward/Adam kernels, fits
+the trust gate on random validation slabs, and writes the model (parameters,
+standardisation, gate) as an .npz file.  Progress goes to stdout.
+"""
+import argparse
+import json
+import time
+
+import numpy as np
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--out", required=True)
+    ap.add_argument("--steps", type=int, default=100000)
+    ap.add_argument("--batch", type=int, default=16384)
+    ap.add_argument("--lr", type=float, default=2e-3)
+    ap.add_argument("--lr-final", type=float, default=1e-5)
+    ap.add_argument("--stride", type=int, nargs=4, default=(1, 1, 1, 1),
+                    help="subsampling of the m_r, m_i, lambda, gf axes")
+    ap.add_argument("--val-slabs", type=int, default=20000)
+    ap.add_argument("--cw-head", type=float, default=10.0)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--precision", default="fp32",
+                    help="S1/S2 precision of the training tables (Q and g are FP64 either way)")
+    args = ap.parse_args(argv)
+    from geosmie_amd import surrogate as S
+    t0 = time.time()
+    sur, rep = S.train(train_stride=tuple(args.stride), n_val_slabs=args.val_slabs,
+                       steps=args.steps, batch=args.batch, lr=args.lr, lr_final=args.lr_final,
+                       cw_head=args.cw_head, seed=args.seed, precision=args.precision,
+                       log=lambda s: print(s, flush=True))
+    rep["total_s"] = time.time() - t0
+    sur.info = rep
+    sur.save(args.out)
+    print(json.dumps(rep), flush=True)
+
+
+if __name__ == "__main__":
+    main()

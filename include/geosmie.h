@@ -73,7 +73,8 @@ int gm_ctx_check(gm_ctx* ctx, gm_stream stream, int32_t* flags_host);
  * pair on their launch stream around each main kernel.  gm_ctx_kernel_time
  * synchronises those events and returns the summed milliseconds and launch
  * count of `kernel` (0 homogeneous Mie, 1 coated Mie, 2 S1/S2 contraction,
- * 3 PSD contraction, 4 GSF expansion, 5 MLP), then forgets them. */
+ * 3 PSD contraction, 4 GSF expansion, 5 MLP, 6 coated Mie with absorbing
+ * shells, 7 LUT bulk reduction), then forgets them. */
 int gm_ctx_set_timing(gm_ctx* ctx, int enable);
 int gm_ctx_kernel_time(gm_ctx* ctx, int kernel, double* ms_total, int64_t* launches);
 
@@ -146,6 +147,12 @@ int gm_s12_mueller(gm_ctx* ctx, int64_t n, const int64_t* coef_off, const int32_
                    const double* tau_t, int nmax_cap, double* mueller, int precision,
                    gm_stream stream);
 
+/* P11 only: p11[n][n_ang] = (|S1|^2 + |S2|^2) / 2 (s11 of calculateScatVals,
+ * dointegration.py:1045), for the LUT sweep, which needs no other element. */
+int gm_s12_p11(gm_ctx* ctx, int64_t n, const int64_t* coef_off, const int32_t* nmax,
+               const double* an, const double* bn, int n_ang, const double* pi_t,
+               const double* tau_t, int nmax_cap, double* p11, int precision, gm_stream stream);
+
 /* Weighted sums over the size grid: out[k][c] = sum_p w[k][p] * v[p][c] for
  * n_psd weight rows w [n_psd][n_pts] and per-size columns v [n_pts][n_cols].
  * The integration of every phase-matrix element in integratePSD
@@ -192,6 +199,97 @@ int gm_gsf_expand(gm_ctx* ctx, int64_t n_entries, int n_ang, const double* ang_d
  * (hbleg, src/gsf/convertncdf.py:32-43). */
 int gm_legendre_eval(gm_ctx* ctx, int64_t n_x, const double* x, int n_series, int L,
                      const double* coef, double* out, gm_stream stream);
+
+/* ---- LUT sweep --------------------------------------------------------------
+ * A slab is one (m_dry, wavelength, growth factor) combination over a dry
+ * radius grid: one iteration of runoptics' (lambda, rh) loop
+ * (src/geosmie/dointegration.py:787-889).  Points are slab-major. */
+
+/* Points of n_slab slabs over r_dry[n_r]: x[s*n_r + p] = 2 pi gf[s] r_dry[p] / lam[s]
+ * and eps = m^2 with the volume mixing of dointegration.py:497-507,
+ * m = m_w + (m_dry[s] - m_w) (1/gf[s])^3.  m_dry complex [2*n_slab]; lam,
+ * gf [n_slab]; x [n_slab*n_r]; eps complex [2*n_slab*n_r].  Replaces the
+ * per-(lambda, rh) setup of runoptics (dointegration.py:813-832). */
+int gm_lut_points(gm_ctx* ctx, int64_t n_slab, int n_r, const double* r_dry, const double* lam,
+                  const double* m_dry, const double* gf, double mw_re, double mw_im, double* x,
+                  double* eps, gm_stream stream);
+
+/* Lognormal number weights on a radius grid, one row per distribution:
+ * w[k][p] = n(r_p; rmode[k], sigma[k]) dr[p] / sum_p(...) with the density of
+ * src/geosmie/particleparams.py:113-127 and the bin widths dr (getDR,
+ * dointegration.py:93-101), normalised as calculatePSD does (:569-577). */
+int gm_lut_lognormal(gm_ctx* ctx, int64_t n_psd, int n_r, const double* r, const double* dr,
+                     const double* rmode, const double* sigma, double* w, gm_stream stream);
+
+/* Bulk optics of n_psd size distributions per slab, float32 rows
+ * out[s][k][3 + n_mom] = Qext, SSA, g, alpha1_0 .. alpha1_{n_mom-1}:
+ * integratePSD's area-weighted Qext/Qsca and Qsca-weighted g
+ * (dointegration.py:1102-1190, 961) and the first n_mom Legendre moments of
+ * P11 as spher_expan.f forms alpha1 with ng Gauss nodes (SPHER_EXPAN
+ * :313-347), normalised by alpha1_0 (:95-103).
+ *   w: number weights [n_psd][n_r] shared by all slabs (w_per_slab = 0) or
+ *      [n_slab][n_psd][n_r] (w_per_slab = 1);
+ *   x [n_slab*n_r], lam [n_slab], props [6][n_slab*n_r] (gm_mie_batch),
+ *   p11 [n_slab*n_r][n_ang] (gm_s12_p11); ang_deg_host [n_ang] increasing;
+ *   1 <= n_mom <= min(ng, 64). */
+int gm_lut_bulk(gm_ctx* ctx, int64_t n_slab, int n_r, int n_ang, int n_psd, const double* w,
+                int w_per_slab, const double* x, const double* lam, const double* props,
+                const double* p11, const double* ang_deg_host, int ng, int n_mom, float* out,
+                gm_stream stream);
+
+/* ---- MLP surrogate (no reference counterpart) --------------------------------
+ * A fully connected network dims[0] -> dims[1] -> ... -> dims[n_layers] with
+ * SiLU after every layer but the last.  params: one FP32 vector, per layer
+ * W [out][in] row-major then b [out] (gm_mlp_param_count entries).  The
+ * model's weights are BF16 (every product uses the round-to-nearest-even BF16
+ * of the FP32 master value); activations are FP32, fed to the matrix cores as
+ * a two-term BF16 split; accumulation is FP32.  dims is a host array of
+ * n_layers + 1 widths (each 1..4096, n_layers <= 16).  x [n][dims[0]],
+ * y / target [n][dims[n_layers]], row-major FP32, device. */
+int64_t gm_mlp_param_count(int n_layers, const int* dims);
+
+/* y = net(x). */
+int gm_mlp_forward(gm_ctx* ctx, int n_layers, const int* dims, int64_t n, const float* params,
+                   const float* x, float* y, gm_stream stream);
+
+/* Loss sum_j out_weight[j] * mean_b (y[b][j] - target[b][j])^2 into loss[0]
+ * (device float) and its gradient with respect to params into grads. */
+int gm_mlp_grad(gm_ctx* ctx, int n_layers, const int* dims, int64_t n, const float* params,
+                const float* x, const float* target, const float* out_weight, float* grads,
+                float* loss, gm_stream stream);
+
+/* One Adam step (bias-corrected, step >= 1) on params with moments m, v. */
+int gm_mlp_adam(gm_ctx* ctx, int64_t n_params, float* params, const float* grads, float* m,
+                float* v, float lr, float beta1, float beta2, float eps, int64_t step,
+                gm_stream stream);
+
+/* Surrogate queries (no reference counterpart).  q [n][6] = (ln r_eff,
+ * sigma_g, m_r, ln m_i, lambda, f_w): wet effective radius [um] of a lognormal
+ * number distribution (r_eff = gf r_mode exp(2.5 ln^2 sigma_g)), its geometric
+ * width, the dry refractive index, the wavelength [um] and the water volume
+ * fraction f_w = 1 - gf^-3 of the humidity growth (dointegration.py:497-507).
+ * x [n][6]: network features (ln r_eff, sigma_g, m_r, ln m_i, ln lambda, f_w)
+ * standardised with feat_mean/feat_std.  flags [n]: 0 = served by the network,
+ * 1 = outside the box [box_lo, box_hi] of the gate coordinates (ln r_mode,
+ * sigma_g, m_r, ln m_i, ln lambda, f_w), 2 = inside but in a cell of the
+ * uniform n_cells_host[6] grid whose cell_ok byte is 0.  All arrays but
+ * n_cells_host are device arrays. */
+int gm_surrogate_prep(gm_ctx* ctx, int64_t n, const float* q, const float* feat_mean,
+                      const float* feat_std, const float* box_lo, const float* box_hi,
+                      const int* n_cells_host, const uint8_t* cell_ok, float* x, int32_t* flags,
+                      gm_stream stream);
+
+/* Network outputs y [n][n_out] (standardised) -> table rows [n][n_out]
+ * (Qext, SSA, g, alpha1_0..): Qext = exp(t0), SSA = 1 - (exp(t1) - 1e-4),
+ * g = t2, alpha1_l = (2l + 1) t_{3+l}, with t_j = y_j tgt_std[j] + tgt_mean[j]. */
+int gm_surrogate_post(gm_ctx* ctx, int64_t n, int n_out, const float* y, const float* tgt_mean,
+                      const float* tgt_std, float* rows, gm_stream stream);
+
+/* Inverse of gm_surrogate_post: table rows [n][n_out] -> standardised
+ * training targets y [n][n_out] (t0 = ln Qext, t1 = ln(1 - SSA + 1e-4),
+ * t2 = g, t_{3+l} = alpha1_l / (2l + 1); y_j = (t_j - tgt_mean[j]) / tgt_std[j]). */
+int gm_surrogate_targets(gm_ctx* ctx, int64_t n, int n_out, const float* rows,
+                         const float* tgt_mean, const float* tgt_std, float* y, gm_stream stream);
 
 #ifdef __cplusplus
 }
