@@ -117,101 +117,19 @@ def cpu_baseline(x, y, m1, m2, target_s=12.0):
                       "C OpenMP restatement oracle/cmie.c" % (stride, n, dt)}
 
 
-class _Pipeline(object):
-    """Device-resident runoptics chain for a homogeneous batch: coefficients,
-    angle tables, fused S1/S2 -> Mueller on MFMA, PSD contraction, integratePSD
-    scalars, P normalisation (and optionally the GSF expansion)."""
-
-    def __init__(self, x, m, ang, w, lam_of_row, ranges, precision=0, gsf_ng=0):
-        import torch
-        from geosmie_amd import _lib
-        self.t, self.L = torch, _lib
-        self.n = x.size
-        self.x = x
-        self.size_max = float(x.max())
-        self.nmax_cap = int(np.rint(2 + self.size_max + 4 * self.size_max ** (1. / 3.)))
-        self.n_ang = ang.size
-        self.rows = w.shape[0]
-        self.lam = lam_of_row
-        self.ranges = ranges          # per row: contiguous [start, start + count) of its points
-        self.precision = precision
-        self.gsf_ng = gsf_ng
-        self.ang = np.ascontiguousarray(ang, dtype=np.float64)
-        dev = "cuda"
-        self.d_x = _lib.to_dev(x)
-        self.d_eps = _lib.cplx_dev(np.broadcast_to(np.asarray(m, np.complex128) ** 2, (self.n,)))
-        self.d_cost = _lib.to_dev(np.cos(np.radians(ang)))
-        self.d_theta = _lib.to_dev(np.radians(ang))
-        self.d_w = _lib.to_dev(w)
-        nm = np.rint(2 + x + 4 * x ** (1. / 3.)).astype(np.int64)
-        total = int(nm.sum())
-        self.sum_nmax = total
-        self.d_off = torch.empty(self.n + 1, dtype=torch.int64, device=dev)
-        self.d_an = torch.empty(2 * total, dtype=torch.float64, device=dev)
-        self.d_bn = torch.empty_like(self.d_an)
-        self.d_props = torch.empty(6 * self.n, dtype=torch.float64, device=dev)
-        self.d_nmax = torch.empty(self.n, dtype=torch.int32, device=dev)
-        self.d_pi = torch.empty(self.nmax_cap * self.n_ang, dtype=torch.float64, device=dev)
-        self.d_tau = torch.empty_like(self.d_pi)
-        self.d_mu = torch.empty(self.n * 4 * self.n_ang, dtype=torch.float64, device=dev)
-        self.d_pm = torch.empty(self.rows * 4 * self.n_ang, dtype=torch.float64, device=dev)
-        self.d_back = torch.empty(self.rows * 6, dtype=torch.float64, device=dev)
-        self.d_sc = torch.empty(self.rows * 16, dtype=torch.float64, device=dev)
-        self.d_one = _lib.to_dev(np.array([1.0]))
-        rr = x[None, :] * np.asarray(lam_of_row)[:, None] / (2 * np.pi)
-        reff0 = (w * rr ** 4).sum(1) / (w * rr ** 3).sum(1)
-        self.d_reff0 = _lib.to_dev(reff0)
-        if gsf_ng:
-            self.d_F = torch.empty(self.rows * 6 * self.n_ang, dtype=torch.float64, device=dev)
-            self.d_pmom = torch.empty(self.rows * 6 * gsf_ng, dtype=torch.float64, device=dev)
-
-    def step(self):
-        L, lib, h, s, P = self.L, self.L.lib(), self.L.ctx(), self.L.stream(), self.L.ptr
-        chk = L.check
-        chk(lib.gm_coef_offsets(h, self.n, P(self.d_x), P(self.d_off), s))
-        chk(lib.gm_mie_batch(h, self.n, P(self.d_x), None, P(self.d_eps), None, None,
-                             self.size_max, P(self.d_off), P(self.d_an), P(self.d_bn),
-                             P(self.d_props), P(self.d_nmax), L.GM_MIE_SORT, s))
-        chk(lib.gm_angle_tables(h, self.n_ang, P(self.d_cost), self.nmax_cap, P(self.d_pi),
-                                P(self.d_tau), s))
-        chk(lib.gm_s12_mueller(h, self.n, P(self.d_off), P(self.d_nmax), P(self.d_an),
-                               P(self.d_bn), self.n_ang, P(self.d_pi), P(self.d_tau),
-                               self.nmax_cap, P(self.d_mu), self.precision, s))
-        nc = 4 * self.n_ang
-        for r, (a, c) in enumerate(self.ranges):
-            chk(lib.gm_psd_contract(h, 1, c, nc, P(self.d_w[r, a:a + c]), P(self.d_mu[a * nc:]),
-                                    P(self.d_pm[r * nc:]), s))
-        for r in range(self.rows):
-            chk(lib.gm_integrate_psd(h, self.n, 1, P(self.d_x), P(self.d_props), P(self.d_w[r]),
-                                     P(self.d_one), P(self.d_reff0[r:r + 1]), float(self.lam[r]),
-                                     1000.0, 1000.0, P(self.d_sc[16 * r:16 * r + 16]), s))
-        chk(lib.gm_pmatrix_normalize(h, self.rows, self.n_ang, P(self.d_theta), P(self.d_pm),
-                                     P(self.d_back), s))
-        if self.gsf_ng:
-            pm = self.d_pm.view(self.rows, 4, self.n_ang)
-            F = self.d_F.view(self.rows, 6, self.n_ang)
-            F[:, 0] = pm[:, 0]
-            F[:, 1] = pm[:, 1]
-            F[:, 2] = pm[:, 2]
-            F[:, 3] = pm[:, 3]
-            F[:, 4] = pm[:, 0]
-            F[:, 5] = pm[:, 2]
-            chk(lib.gm_gsf_expand(h, self.rows, self.n_ang,
-                                  self.ang.ctypes.data_as(ctypes.c_void_p), P(self.d_F),
-                                  self.gsf_ng, 0, P(self.d_pmom), s))
-
-
 def time_pipeline(pipe, steps, warmup):
+    """Eager ms/step with per-kernel HIP-event times, then ms/step replaying
+    the step captured as a HIP graph."""
     import torch
     from geosmie_amd import _lib
-    L, h = _lib.lib(), _lib.ctx()
+    L, h = _lib.lib(), pipe.ctx
     for _ in range(warmup):
         pipe.step()
     torch.cuda.synchronize()
     L.gm_ctx_set_timing(h, 1)
     ms = ctypes.c_double(0)
     cnt = ctypes.c_int64(0)
-    for k in range(7):
+    for k in range(8):
         L.gm_ctx_kernel_time(h, k, ctypes.byref(ms), ctypes.byref(cnt))
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -225,19 +143,32 @@ def time_pipeline(pipe, steps, warmup):
         if cnt.value:
             kt[name] = ms.value / cnt.value
     L.gm_ctx_set_timing(h, 0)
-    return (t1 - t0) * 1e3 / steps, kt
+    pipe.capture()
+    pipe.replay()
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    for _ in range(steps):
+        pipe.replay()
+    torch.cuda.synchronize()
+    t3 = time.perf_counter()
+    return (t1 - t0) * 1e3 / steps, kt, (t3 - t2) * 1e3 / steps
 
 
 def secondary(steps=5, warmup=1):
+    from geosmie_amd import autotune
+    from geosmie_amd.pipeline import RunopticsChain
     out = []
     x, lam, ang, w, m = homog_lut_workload()
-    pipe = _Pipeline(x, m, ang, w, lam, [(i * 2000, 2000) for i in range(10)])
-    ms, kt = time_pipeline(pipe, steps, warmup)
+    pipe = RunopticsChain(x, m, ang, w, lam, [(i * 2000, 2000) for i in range(10)])
+    tuned = autotune.tune(pipe.ctx, pipe.step, knobs=("s12_waves", "psd_chunk"), reps=3)
+    ms_eager, kt, ms = time_pipeline(pipe, steps, warmup)
     s12_flop = 16.0 * pipe.sum_nmax * ang.size
     out.append({"workload": "plain homogeneous FP64 Mie, 2000 radii 0.005-20 um x 10 wavelengths "
                             "0.3-2.5 um, m=1.53+0.005i, 181 angles, lognormal r_mode 0.5 um "
                             "sigma_g 2.0 integrated per wavelength (Qext/Qsca/g, P11/P12/P33/P34)",
                 "points": int(x.size), "ms_per_step": ms, "value": x.size / (ms * 1e-3),
+                "mode": "HIP graph replay", "ms_per_step_eager": ms_eager,
+                "autotune": {k: v["best"] for k, v in tuned.items()},
                 "unit": "points/s", "kernel_ms": kt,
                 "s12_tflops": s12_flop / (kt.get("s12", ms) * 1e-3) / 1e12})
     del pipe
@@ -249,20 +180,21 @@ def secondary(steps=5, warmup=1):
     for i, L_ in enumerate(lam_row):
         p = getLogNormPSD(5.0, 2.0, 2 * np.pi * r / L_, L_, 1e9, 0.0) * np.gradient(r)
         w[i, i * 5000:(i + 1) * 5000] = p / p.sum()
-    pipe = _Pipeline(x, m, ang, w, lam_row, [(i * 5000, 5000) for i in range(8)], precision=0,
-                     gsf_ng=300)
+    pipe = RunopticsChain(x, m, ang, w, lam_row, [(i * 5000, 5000) for i in range(8)],
+                          precision=0, gsf_ng=300)
+    tuned3 = autotune.tune(pipe.ctx, pipe.step, knobs=("s12_waves",), reps=2)
     s12_flop = 16.0 * pipe.sum_nmax * ang.size
     modes = []
     ref = None
     for prec, name in [(0, "fp64"), (1, "fp32"), (2, "bf16x3")]:
         pipe.precision = prec
-        ms, kt = time_pipeline(pipe, max(2, steps // 2), warmup)
+        ms_eager, kt, ms = time_pipeline(pipe, max(2, steps // 2), warmup)
         res = {"pm": pipe.d_pm.view(8, 4, ang.size).cpu().numpy(),
                "back": pipe.d_back.view(8, 6).cpu().numpy(),
                "pmom": pipe.d_pmom.view(8, 6, 300).cpu().numpy(),
                "sc": pipe.d_sc.view(8, 16).cpu().numpy()}
         ent = {"precision": name, "ms_per_step": ms, "value": x.size / (ms * 1e-3),
-               "kernel_ms": kt,
+               "mode": "HIP graph replay", "ms_per_step_eager": ms_eager, "kernel_ms": kt,
                "s12_tflops": s12_flop / (kt.get("s12", ms) * 1e-3) / 1e12}
         if ref is None:
             ref = res
@@ -293,7 +225,91 @@ def secondary(steps=5, warmup=1):
                 "points": int(x.size), "ms_per_step": fp64["ms_per_step"],
                 "value": fp64["value"], "unit": "points/s", "kernel_ms": fp64["kernel_ms"],
                 "s12_flop": s12_flop, "s12_tflops": fp64["s12_tflops"],
+                "autotune": {k: v["best"] for k, v in tuned3.items()},
                 "precision_modes": modes[1:]})
+    return out
+
+
+SURROGATE_FILE = os.path.join(ROOT, "geosmie_amd", "data", "surrogate_lut.npz")
+
+
+def lut_surrogate(n_queries=1_000_000, n_sample=4096, seed=20261202):
+    """Bulk LUT entries/s: plain GPU Mie (grid sweep and per-query) against the
+    trained surrogate with its plain-Mie fallback, on queries drawn uniformly
+    in the sweep's ranges (ln r_mode, sigma_g, m_r, ln m_i, ln lambda, f_w)."""
+    import torch
+    from geosmie_amd.lut_sweep import LutSweep, full_grid
+    g = full_grid()
+    out = {"workload": "LUT sweep grid: m_r 1.30-1.80 (26), m_i 1e-8-1 (30), 3000 radii "
+                       "0.005-50 um, 50 wavelengths 0.2-40 um, 36 RH growth factors, 40 "
+                       "lognormal PSDs; rows Qext, SSA, g, alpha1_0..31 (float32)",
+           "unit": "bulk LUT entries/s"}
+    sw = LutSweep(g["r"], g["rmode"], g["sigma"], precision="fp64")
+    sub = (g["mr"][::13], g["mi"][::10], g["lam"], g["gf"][::5])
+    sw.sweep(*[a[:1] for a in sub])                # warm: tables, workspaces
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    tab = sw.sweep(*sub)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    n_ent = tab.numel() // tab.shape[-1]
+    out["plain_grid"] = {"entries": int(n_ent), "s": dt, "value": n_ent / dt,
+                         "sample": "m_r[::13] x m_i[::10] x all 50 wavelengths x gf[::5] "
+                                   "(2400 slabs of the 1.4M-slab grid), FP64",
+                         "full_grid_s_est": dt * 1_404_000 / (n_ent / 40)}
+    del tab
+    if not os.path.exists(SURROGATE_FILE):
+        out["surrogate"] = None
+        return out
+    from geosmie_amd.surrogate import Surrogate, make_queries, rel_errors
+    sur = Surrogate.load(SURROGATE_FILE)
+    r = np.random.default_rng(seed)
+    n = int(n_queries)
+    gf = (1 - r.uniform(0, 1 - g["gf"].max() ** -3.0, n)) ** (-1.0 / 3.0)
+    q = make_queries(np.exp(r.uniform(np.log(g["rmode"].min()), np.log(g["rmode"].max()), n)),
+                     r.uniform(g["sigma"].min(), g["sigma"].max(), n),
+                     r.uniform(g["mr"].min(), g["mr"].max(), n),
+                     np.exp(r.uniform(np.log(g["mi"].min()), np.log(g["mi"].max()), n)),
+                     np.exp(r.uniform(np.log(g["lam"].min()), np.log(g["lam"].max()), n)), gf)
+    from geosmie_amd import _lib
+    qd = _lib.to_dev(q, np.float32)
+    sur.predict(qd[:1024])
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    rows, flags = sur.predict(qd)
+    torch.cuda.synchronize()
+    t_net = time.perf_counter() - t0
+    f = flags.cpu().numpy()
+    n_fb = int(np.sum(f != 0))
+    # plain GPU Mie on a random sample: per-query rate, fallback cost and the error judge
+    pick = r.choice(n, size=min(n_sample, n), replace=False)
+    sur.solve_plain(q[pick[:64]])
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    plain = sur.solve_plain(q[pick])
+    torch.cuda.synchronize()
+    t_plain = time.perf_counter() - t0
+    per_query = t_plain / pick.size
+    served = f[pick] == 0
+    net = rows[torch.from_numpy(pick).to(rows.device)].cpu().numpy()
+    err = rel_errors(net, plain.cpu().numpy())
+    t_total = t_net + n_fb * per_query
+    out["surrogate"] = {
+        "queries": n, "served_by_network": n - n_fb, "fallback": n_fb,
+        "network_s": t_net, "network_entries_per_s": n / t_net,
+        "plain_per_query_entries_per_s": 1.0 / per_query,
+        "value": n / t_total,
+        "value_note": "network time for all queries + fallback count x measured plain-Mie "
+                      "time per query (plain timed on %d random queries)" % pick.size,
+        "speedup_vs_plain_per_query": (n / t_total) * per_query,
+        "speedup_vs_plain_grid": (n / t_total) / out["plain_grid"]["value"],
+        "served_err_vs_plain": None if not served.any() else {
+            "qext_max": float(err[served, 0].max()), "ssa_max": float(err[served, 1].max()),
+            "g_max": float(err[served, 2].max()),
+            "frac_within_1e-3": float(np.mean(err[served].max(1) <= 1e-3))},
+        "all_err_median_network_only": [float(v) for v in np.median(err, axis=0)],
+        "training": {k: float(v) for k, v in sur.info.items()},
+    }
     return out
 
 
@@ -337,6 +353,8 @@ def main():
                              None, None, None, ptr(d_props), None, _lib.GM_MIE_SORT, s),
               "gm_mie_batch")
 
+    from geosmie_amd import autotune
+    tuned = autotune.tune(h, step, knobs=("mie_grid_mult",), reps=2, warmup=1)
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
@@ -385,6 +403,8 @@ def main():
                      "traffic": measured_traffic(), "kernel_ms": kern_ms,
                      "flop_per_launch": flops,
                      "algorithmic_bytes_per_launch": 104.0 * n},
+        "autotune": {k: {"best": v["best"], "ms": {str(a): b for a, b in v["ms"].items()}}
+                     for k, v in tuned.items()},
     }
     if args.copy_calib:
         a = torch.ones(64 << 20, dtype=torch.float64, device="cuda")   # 512 MiB
@@ -396,6 +416,7 @@ def main():
     if not args.no_secondary:
         del d_x, d_y, d_eps, d_eps2, d_props
         out["secondary"] = secondary()
+        out["lut"] = lut_surrogate()
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(x, y, m1, m2)
     print(json.dumps(out))

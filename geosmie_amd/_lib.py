@@ -36,6 +36,8 @@ SIGNATURES = {
     "gm_ctx_set_timing": (ctypes.c_int, [c_vp, ctypes.c_int]),
     "gm_ctx_kernel_time": (ctypes.c_int, [c_vp, ctypes.c_int, ctypes.POINTER(c_dbl),
                                           ctypes.POINTER(c_i64)]),
+    "gm_ctx_tune": (ctypes.c_int, [c_vp, ctypes.c_int, ctypes.c_int]),
+    "gm_ctx_get_tune": (ctypes.c_int, [c_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]),
     "gm_nmax": (ctypes.c_int, [c_dbl]),
     "gm_coef_offsets": (ctypes.c_int, [c_vp, c_i64, c_vp, c_vp, c_vp]),
     "gm_mie_batch": (ctypes.c_int, [c_vp, c_i64, c_vp, c_vp, c_vp, c_vp, c_vp, c_dbl, c_vp,
@@ -172,6 +174,32 @@ def ctx():
         check(lib().gm_ctx_create(key[1], ctypes.byref(h)), "gm_ctx_create")
         _ctx[key] = h
     return h
+
+
+class PrivateCtx(object):
+    """A gm_ctx owned by one object (destroyed with it); usable wherever a
+    context handle is passed.  Work captured into a graph must use such a
+    context: a shared one could grow (reallocate) a workspace the graph uses."""
+
+    def __init__(self):
+        t = torch()
+        self._h = c_vp()
+        check(lib().gm_ctx_create(t.cuda.current_device(), ctypes.byref(self._h)), "gm_ctx_create")
+        self._as_parameter_ = self._h
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value and _lib is not None:
+            try:
+                torch().cuda.synchronize()
+                _lib.gm_ctx_destroy(h)
+            except Exception:
+                pass
+            self._h = None
+
+
+def private_ctx():
+    return PrivateCtx()
 
 
 def check_flags():

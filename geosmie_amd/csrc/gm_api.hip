@@ -151,6 +151,44 @@ int gm_ctx_check(gm_ctx* ctx, gm_stream stream, int32_t* flags_host) {
   return GM_OK;
 }
 
+int gm_ctx_tune(gm_ctx* ctx, int param, int value) {
+  if (!ctx) return gm_fail(GM_EINVAL, "gm_ctx_tune: ctx is NULL");
+  switch (param) {
+    case GM_TUNE_MIE_GRID_MULT:
+      if (value < 1 || value > 16) return gm_fail(GM_EINVAL, "gm_ctx_tune: grid multiple %d", value);
+      ctx->tune_mie_grid_mult = value;
+      return GM_OK;
+    case GM_TUNE_S12_WAVES:
+      if (value != 1 && value != 2 && value != 4)
+        return gm_fail(GM_EINVAL, "gm_ctx_tune: S1/S2 wavefronts per block must be 1, 2 or 4");
+      ctx->tune_s12_waves = value;
+      return GM_OK;
+    case GM_TUNE_PSD_CHUNK:
+      if (value < 16 || value > (1 << 20))
+        return gm_fail(GM_EINVAL, "gm_ctx_tune: split-K chunk %d out of [16, 2^20]", value);
+      ctx->tune_psd_chunk = value;
+      return GM_OK;
+    case GM_TUNE_MLP_WEIGHT_TERMS:
+      if (value != 1 && value != 2)
+        return gm_fail(GM_EINVAL, "gm_ctx_tune: MLP weight terms must be 1 or 2");
+      ctx->tune_mlp_wterms = value;
+      return GM_OK;
+    default:
+      return gm_fail(GM_EINVAL, "gm_ctx_tune: unknown parameter %d", param);
+  }
+}
+
+int gm_ctx_get_tune(gm_ctx* ctx, int param, int* value) {
+  if (!ctx || !value) return gm_fail(GM_EINVAL, "gm_ctx_get_tune: NULL argument");
+  switch (param) {
+    case GM_TUNE_MIE_GRID_MULT: *value = ctx->tune_mie_grid_mult; return GM_OK;
+    case GM_TUNE_S12_WAVES: *value = ctx->tune_s12_waves; return GM_OK;
+    case GM_TUNE_PSD_CHUNK: *value = ctx->tune_psd_chunk; return GM_OK;
+    case GM_TUNE_MLP_WEIGHT_TERMS: *value = ctx->tune_mlp_wterms; return GM_OK;
+    default: return gm_fail(GM_EINVAL, "gm_ctx_get_tune: unknown parameter %d", param);
+  }
+}
+
 int gm_ctx_set_timing(gm_ctx* ctx, int enable) {
   if (!ctx) return gm_fail(GM_EINVAL, "gm_ctx_set_timing: ctx is NULL");
   ctx->timing = enable != 0;
@@ -190,8 +228,9 @@ int gm_coef_offsets(gm_ctx* ctx, int64_t n, const double* size, int64_t* off, gm
     return gm_fail(GM_EINVAL, "gm_coef_offsets: bad arguments");
   HIP_OK(hipSetDevice(ctx->device));
   hipStream_t s = (hipStream_t)stream;
-  // counts into off[1..n], then an in-place inclusive scan
-  HIP_OK(hipMemsetAsync(off, 0, sizeof(int64_t), s));
+  // counts into off[1..n], then an in-place inclusive scan; off[0] is cleared
+  // by a kernel (no memset node when the caller captures the stream)
+  hipLaunchKernelGGL(gm::k_zero_i64, dim3(1), dim3(64), 0, s, off, 1);
   if (n == 0) return GM_OK;
   hipLaunchKernelGGL(gm::k_nmax_of, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, size,
                      off + 1);
@@ -250,7 +289,8 @@ int gm_mie_batch(gm_ctx* ctx, int64_t n, const double* x, const double* y, const
   const int64_t waves = (n + 63) / 64;
   auto grid_for = [&](int occ, int row_doubles) {
     const size_t slot_bytes = (size_t)rows_cap * 64 * row_doubles * sizeof(double);
-    int64_t g = std::min<int64_t>(waves, (int64_t)ctx->n_cu * std::max(occ, 1));
+    int64_t g = std::min<int64_t>(
+        waves, (int64_t)ctx->n_cu * std::max(occ, 1) * std::max(ctx->tune_mie_grid_mult, 1));
     g = std::min<int64_t>(g, std::max<int64_t>(1, (int64_t)(ctx->scratch_budget / slot_bytes)));
     return std::make_pair(g, (size_t)g * slot_bytes);
   };

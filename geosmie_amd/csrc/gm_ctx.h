@@ -21,6 +21,11 @@ struct gm_ctx {
   int n_cu = 256;
   int occ_homog = 8, occ_stable = 8, occ_literal = 4;  // resident Mie wavefronts per CU
   size_t scratch_budget = (size_t)3 << 30;       // bytes of recurrence scratch
+  // launch shapes chosen by gm_ctx_tune (an autotuner's knobs)
+  int tune_mie_grid_mult = 1;   // persistent Mie grid = multiple of resident wavefronts
+  int tune_s12_waves = 4;       // wavefronts (64 angles each) per S1/S2 block
+  int tune_psd_chunk = 128;     // points per split-K chunk of the few-PSD contraction
+  int tune_mlp_wterms = 1;      // BF16 terms per MLP weight (1 = BF16 weights, 2 = hi + lo)
   Buf status;                                    // int32 flags + int32 max_nmax + scan scratch
   Buf order, key, hist, scratch, scan_tmp;
   Buf tab_pi, tab_tau, gemm_tmp, feat_tmp, gsf_tab, tab_lp, lut_tmp, mlp_ws;

@@ -22,16 +22,16 @@ typedef double double4_ __attribute__((ext_vector_type(4)));
 namespace gm {
 
 constexpr int kSmallPsd = 8;     // VALU path up to this many PSDs
-constexpr int kSplitChunk = 128; // points per split-K chunk
 
 // out_part[split][k][c] = sum_{p in chunk} w[k][p] v[p][c]   (k < n_psd <= kSmallPsd)
 __global__ __launch_bounds__(256) void k_contract_small(int n_psd, int64_t n_pts, int n_cols,
+                                                        int chunk,
                                                         const double* __restrict__ w,
                                                         const double* __restrict__ v,
                                                         double* __restrict__ part) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  const int64_t p0 = (int64_t)blockIdx.y * kSplitChunk;
-  const int64_t p1 = min(p0 + (int64_t)kSplitChunk, n_pts);
+  const int64_t p0 = (int64_t)blockIdx.y * chunk;
+  const int64_t p1 = min(p0 + (int64_t)chunk, n_pts);
   if (c >= n_cols) return;
   double acc[kSmallPsd];
 #pragma unroll
@@ -233,13 +233,14 @@ int gm_psd_contract(gm_ctx* ctx, int n_psd, int64_t n_pts, int n_cols, const dou
   size_t slot;
   ctx->mark_begin(gm::kKPsd, s, &slot);
   if (n_psd <= gm::kSmallPsd) {
-    int nsplit = (int)std::max<int64_t>(1, (n_pts + gm::kSplitChunk - 1) / gm::kSplitChunk);
+    const int chunk = ctx->tune_psd_chunk;
+    int nsplit = (int)std::max<int64_t>(1, (n_pts + chunk - 1) / chunk);
     size_t bytes = sizeof(double) * (size_t)nsplit * n_psd * n_cols;
     int rc = ctx->reserve(ctx->gemm_tmp, bytes);
     if (rc) return rc;
     double* part = (double*)ctx->gemm_tmp.ptr;
     hipLaunchKernelGGL(gm::k_contract_small, dim3((n_cols + 255) / 256, nsplit), dim3(256), 0, s,
-                       n_psd, n_pts, n_cols, w, v, part);
+                       n_psd, n_pts, n_cols, chunk, w, v, part);
     int64_t n_out = (int64_t)n_psd * n_cols;
     hipLaunchKernelGGL(gm::k_reduce_splits, dim3((unsigned)((n_out + 255) / 256)), dim3(256), 0,
                        s, nsplit, n_out, part, out);

@@ -9,7 +9,8 @@
 // (a = a_hi + a_lo), so a layer computes  a_hi W + a_lo W  with FP32
 // accumulation (v_mfma_f32_16x16x32_bf16) -- the activation rounding is
 // ~2^-17 instead of BF16's 2^-9.  Weight gradients use a three-term split of
-// both operands.
+// both operands.  A context may instead carry the weights as two BF16 terms
+// (GM_TUNE_MLP_WEIGHT_TERMS = 2: W = W_hi + W_lo, three products per layer).
 //
 // GEMM kernel: C[m][n] = sum_k A(m, k) B(k, n), operands given by strides so
 // one kernel serves the forward (X W^T), the activation gradient (dZ W) and
@@ -50,6 +51,28 @@ __device__ __forceinline__ float dsilu(float z) {
   return s * (1.0f + z * (1.0f - s));
 }
 
+// 8 consecutive floats at p (16-byte aligned) -> BF16 hi (and lo) planes, one
+// 16-byte LDS store each; zeros when !ok.
+template <int S>
+__device__ __forceinline__ void stage8(const float* __restrict__ p, bool ok,
+                                       unsigned short* __restrict__ dh,
+                                       unsigned short* __restrict__ dl) {
+  float4 a = make_float4(0.f, 0.f, 0.f, 0.f), b = a;
+  if (ok) {
+    a = *(const float4*)p;
+    b = *(const float4*)(p + 4);
+  }
+  const float v[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+  union { uint4 q; unsigned short s[8]; } h, l;
+#pragma unroll
+  for (int e = 0; e < 8; ++e) {
+    h.s[e] = f2bf(v[e]);
+    if (S == 2) l.s[e] = f2bf(v[e] - bf2f(h.s[e]));
+  }
+  *(uint4*)dh = h.q;
+  if (S == 2) *(uint4*)dl = l.q;
+}
+
 struct GemmArgs {
   int M, N, K;
   const float* A;
@@ -85,8 +108,16 @@ __global__ __launch_bounds__(256) void k_mlp_gemm(GemmArgs g) {
     for (int b = 0; b < 2; ++b) acc[a][b] = (float4_){0.f, 0.f, 0.f, 0.f};
 
   for (int k0 = kbeg; k0 < kend; k0 += kBK) {
-    // ---- stage A (64 x 32) and B (32 x 64) as BF16 planes [row][k]
-    {
+    // ---- stage A (64 x 32) and B (32 x 64) as BF16 planes [row][k].  Where
+    // the operand is K-contiguous, 16-byte aligned and the K-step is whole,
+    // a thread moves 8 consecutive k of one row with two 16-byte loads and
+    // one 16-byte LDS store per plane; otherwise element by element.
+    const bool full_k = k0 + kBK <= kend;
+    if (full_k && g.sak == 1 && (g.sam & 3) == 0 && (((uintptr_t)g.A) & 15) == 0) {
+      const int mm = t >> 2, kc = (t & 3) * 8;
+      stage8<SA>(g.A + (int64_t)(m0 + mm) * g.sam + k0 + kc, m0 + mm < g.M, &sA[0][mm][kc],
+                 &sA[SA - 1][mm][kc]);
+    } else {
       int r, c, dr, dc;   // element e at (r + e*dr, c + e*dc) of the tile
       if (g.sak == 1) { r = t >> 2; c = (t & 3) * 8; dr = 0; dc = 1; }
       else            { c = t >> 3; r = (t & 7) * 8; dr = 1; dc = 0; }
@@ -100,7 +131,11 @@ __global__ __launch_bounds__(256) void k_mlp_gemm(GemmArgs g) {
         if (SA == 2) sA[SA - 1][mm][kk] = f2bf(v - bf2f(h));
       }
     }
-    {
+    if (full_k && g.sbk == 1 && (g.sbn & 3) == 0 && (((uintptr_t)g.B) & 15) == 0) {
+      const int nn = t >> 2, kc = (t & 3) * 8;
+      stage8<SB>(g.B + (int64_t)(n0 + nn) * g.sbn + k0 + kc, n0 + nn < g.N, &sB[0][nn][kc],
+                 &sB[SB - 1][nn][kc]);
+    } else {
       int r, c, dr, dc;   // r: column n, c: k
       if (g.sbk == 1) { r = t >> 2; c = (t & 3) * 8; dr = 0; dc = 1; }
       else            { c = t >> 3; r = (t & 7) * 8; dr = 1; dc = 0; }
@@ -300,7 +335,7 @@ gm::GemmArgs args0() {
 
 // Forward of layer l: C[n][out] from Ain[n][in]; Z (optional) keeps z.
 int layer_forward(const Net& net, int l, int64_t n, const float* params, const float* Ain,
-                  float* C, float* Z, hipStream_t s) {
+                  float* C, float* Z, int wterms, hipStream_t s) {
   gm::GemmArgs a = args0();
   const int in = net.dims[l], out = net.dims[l + 1];
   a.M = (int)n;
@@ -317,7 +352,7 @@ int layer_forward(const Net& net, int l, int64_t n, const float* params, const f
   a.bias = params + net.boff[l];
   a.Z = Z;
   a.act = l < net.L - 1;
-  return gemm<2, 1, gm::kEpiFwd>(a, 1, s);
+  return wterms == 2 ? gemm<2, 2, gm::kEpiFwd>(a, 1, s) : gemm<2, 1, gm::kEpiFwd>(a, 1, s);
 }
 
 }  // namespace
@@ -351,7 +386,7 @@ int gm_mlp_forward(gm_ctx* ctx, int n_layers, const int* dims, int64_t n, const 
   const float* in = x;
   for (int l = 0; l < net.L; ++l) {
     float* out = l == net.L - 1 ? y : buf[l & 1];
-    if ((rc = layer_forward(net, l, n, params, in, out, nullptr, s))) return rc;
+    if ((rc = layer_forward(net, l, n, params, in, out, nullptr, ctx->tune_mlp_wterms, s))) return rc;
     in = out;
   }
   ctx->mark_end(slot, s);
@@ -409,7 +444,8 @@ int gm_mlp_grad(gm_ctx* ctx, int n_layers, const int* dims, int64_t n, const flo
   // forward
   const float* in = x;
   for (int l = 0; l < L; ++l) {
-    if ((rc = layer_forward(net, l, n, params, in, W + aoff[l], W + zoff[l], s))) return rc;
+    if ((rc = layer_forward(net, l, n, params, in, W + aoff[l], W + zoff[l], ctx->tune_mlp_wterms, s)))
+      return rc;
     in = W + aoff[l];
   }
   // loss and dY
@@ -462,7 +498,9 @@ int gm_mlp_grad(gm_ctx* ctx, int n_layers, const int* dims, int64_t n, const flo
       d.C = gnext;
       d.ldc = fin;
       d.Zprev = W + zoff[l - 1];
-      if ((rc = gemm<2, 1, gm::kEpiBwdAct>(d, 1, s))) return rc;
+      rc = ctx->tune_mlp_wterms == 2 ? gemm<2, 2, gm::kEpiBwdAct>(d, 1, s)
+                                     : gemm<2, 1, gm::kEpiBwdAct>(d, 1, s);
+      if (rc) return rc;
       std::swap(gcur, gnext);
     }
   }

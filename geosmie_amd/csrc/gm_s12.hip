@@ -70,7 +70,7 @@ __global__ __launch_bounds__(256) void k_s12_f64(
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   const int64_t p0 = (int64_t)blockIdx.x * 8;
-  const int j0 = (blockIdx.y * 4 + wave) * 64;
+  const int j0 = (blockIdx.y * (blockDim.x >> 6) + wave) * 64;  // 64 angles per wavefront
   if (j0 >= n_ang) return;  // whole wavefront past the last angle
   const int i = lane & 15;   // A row / B column / D column
   const int kq = lane >> 4;  // k within the 4-step, D row group
@@ -203,7 +203,7 @@ __global__ __launch_bounds__(256) void k_s12_f32(
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   const int64_t p0 = (int64_t)blockIdx.x * 8;
-  const int j0 = (blockIdx.y * 4 + wave) * 64;
+  const int j0 = (blockIdx.y * (blockDim.x >> 6) + wave) * 64;  // 64 angles per wavefront
   if (j0 >= n_ang) return;
   const int i = lane & 15, kq = lane >> 4;
   const int64_t pa = p0 + (i >> 1);
@@ -256,7 +256,7 @@ __global__ __launch_bounds__(256) void k_s12_bf16x3(
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   const int64_t p0 = (int64_t)blockIdx.x * 8;
-  const int j0 = (blockIdx.y * 4 + wave) * 64;
+  const int j0 = (blockIdx.y * (blockDim.x >> 6) + wave) * 64;  // 64 angles per wavefront
   if (j0 >= n_ang) return;
   const int i = lane & 15, kq = lane >> 4;
   const int64_t pa = p0 + (i >> 1);
@@ -334,11 +334,13 @@ int launch_s12(gm_ctx* ctx, const char* who, int64_t n, const int64_t* off, cons
   if (nmax_cap < 1) return gm_fail(GM_EINVAL, "%s: nmax_cap must be >= 1", who);
   hipError_t e = hipSetDevice(ctx->device);
   if (e != hipSuccess) return gm_fail(GM_ERUNTIME, "hipSetDevice: %s", hipGetErrorString(e));
-  dim3 grid((unsigned)((n + 7) / 8), (unsigned)((n_ang + 255) / 256));
+  const int wpb = ctx->tune_s12_waves;
+  dim3 grid((unsigned)((n + 7) / 8), (unsigned)((n_ang + 64 * wpb - 1) / (64 * wpb)));
+  dim3 block(64 * wpb);
   size_t slot;
   if (precision == GM_PREC_FP64) {
     ctx->mark_begin(gm::kKS12, s, &slot);
-    hipLaunchKernelGGL(gm::k_s12_f64, grid, dim3(256), 0, s, n, off, nmax, an, bn, n_ang, pi_t,
+    hipLaunchKernelGGL(gm::k_s12_f64, grid, block, 0, s, n, off, nmax, an, bn, n_ang, pi_t,
                        tau_t, nmax_cap, mode, out0, out1);
     ctx->mark_end(slot, s);
   } else {
@@ -356,10 +358,10 @@ int launch_s12(gm_ctx* ctx, const char* who, int64_t n, const int64_t* off, cons
                        s, n_ang, nmax_cap, kpad, apad, pi_t, tau_t, precision, t32, t16);
     ctx->mark_begin(gm::kKS12, s, &slot);
     if (precision == GM_PREC_FP32)
-      hipLaunchKernelGGL(gm::k_s12_f32, grid, dim3(256), 0, s, n, off, nmax, an, bn, n_ang, apad,
+      hipLaunchKernelGGL(gm::k_s12_f32, grid, block, 0, s, n, off, nmax, an, bn, n_ang, apad,
                          kpad, (const float*)t32, nmax_cap, mode, out0, out1);
     else
-      hipLaunchKernelGGL(gm::k_s12_bf16x3, grid, dim3(256), 0, s, n, off, nmax, an, bn, n_ang,
+      hipLaunchKernelGGL(gm::k_s12_bf16x3, grid, block, 0, s, n, off, nmax, an, bn, n_ang,
                          apad, kpad, (const unsigned short*)t16, nmax_cap, mode, out0, out1);
     ctx->mark_end(slot, s);
   }

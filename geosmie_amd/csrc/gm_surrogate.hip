@@ -13,9 +13,9 @@
 // 2: inside, but its cell of the uniform n_cells grid over the box is not
 // trusted (validation error above the bound).
 // Table rows and network targets (before standardisation) correspond as
-//   t0 = ln Qext,  t1 = ln(1 - SSA + d),  t2 = g,  t_{3+l} = alpha1_l / (2l + 1)
-// with d = 1e-4: the co-albedo is learnt on a log scale down to d, below which
-// SSA is within 1e-4 of 1 and its relative error is bounded by d anyway.
+//   t0 = ln Qext,  t1 = ln SSA,  t2 = ln max(g, 1e-8),  t_{3+l} = alpha1_l / (2l + 1):
+// the squared error of t0..t2 is the squared relative error of Qext, SSA and
+// g, the quantities the gate judges.
 #include <math.h>
 #include <stdint.h>
 
@@ -24,7 +24,7 @@
 namespace gm {
 
 constexpr int kSurrIn = 6;
-constexpr float kSsaFloor = 1e-4f;
+constexpr float kGFloor = 1e-8f;
 
 struct Cells {
   int n[kSurrIn];
@@ -71,8 +71,8 @@ __global__ void k_surr_post(int64_t n, int n_out, const float* __restrict__ y,
   const float t = y[e] * tstd[j] + tmean[j];
   float v;
   if (j == 0) v = expf(t);
-  else if (j == 1) v = 1.0f - (expf(t) - kSsaFloor);
-  else if (j == 2) v = t;
+  else if (j == 1) v = fminf(expf(t), 1.0f);
+  else if (j == 2) v = expf(t);
   else v = (float)(2 * (j - 3) + 1) * t;
   rows[e] = v;
 }
@@ -85,9 +85,8 @@ __global__ void k_surr_targets(int64_t n, int n_out, const float* __restrict__ r
   const int j = (int)(e % n_out);
   const float v = rows[e];
   float t;
-  if (j == 0) t = logf(v);
-  else if (j == 1) t = logf(fmaxf(1.0f - v, 0.0f) + kSsaFloor);
-  else if (j == 2) t = v;
+  if (j == 0 || j == 1) t = logf(v);
+  else if (j == 2) t = logf(fmaxf(v, kGFloor));
   else t = v / (float)(2 * (j - 3) + 1);
   y[e] = (t - tmean[j]) / tstd[j];
 }

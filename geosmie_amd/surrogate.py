@@ -33,7 +33,7 @@ from .lut_sweep import N_MOM, LutSweep, full_grid
 DIMS = (6, 256, 256, 256, 256, 35)
 N_OUT = 3 + N_MOM
 GATE_CELLS = (8, 6, 6, 10, 10, 6)
-SSA_FLOOR = 1e-4       # co-albedo floor of the target transform (gm_surrogate.hip)
+G_FLOOR = 1e-8        # floor of g in its log target (gm_surrogate.hip)
 
 
 def sigma_factor(sigma):
@@ -81,9 +81,10 @@ def init_params(dims, seed=0):
 class MLP(object):
     """FP32 master parameters and Adam moments on the device; HIP kernels."""
 
-    def __init__(self, dims=DIMS, seed=0, params=None):
+    def __init__(self, dims=DIMS, seed=0, params=None, weight_terms=1):
         t = _lib.torch()
         self.t = t
+        self.weight_terms = int(weight_terms)
         self.dims = tuple(int(d) for d in dims)
         self.n_layers = len(self.dims) - 1
         self._cd = (ctypes.c_int * len(self.dims))(*self.dims)
@@ -104,7 +105,12 @@ class MLP(object):
         self.loss = t.zeros(1, dtype=t.float32, device=dev)
         self.step = 0
 
+    def _precision(self):
+        # BF16 terms per weight on the matrix cores (a setting of the context)
+        check(_lib.lib().gm_ctx_tune(_lib.ctx(), 3, self.weight_terms), "gm_ctx_tune")
+
     def forward(self, x, out=None):
+        self._precision()
         n = x.shape[0]
         y = out if out is not None else self.t.empty((n, self.dims[-1]), dtype=self.t.float32,
                                                      device=_lib.device())
@@ -113,6 +119,7 @@ class MLP(object):
         return y
 
     def grad(self, x, target, cw):
+        self._precision()
         check(_lib.lib().gm_mlp_grad(_lib.ctx(), self.n_layers, self._cd, x.shape[0], ptr(self.p),
                                      ptr(x), ptr(target), ptr(cw), ptr(self.g), ptr(self.loss),
                                      stream()), "gm_mlp_grad")
@@ -136,11 +143,12 @@ class Surrogate(object):
     """Trained network + standardisation + gate, with a plain-Mie fallback."""
 
     def __init__(self, params, feat_mean, feat_std, tgt_mean, tgt_std, box_lo, box_hi,
-                 cell_ok, cells=GATE_CELLS, r_dry=None, bound=1e-3, dims=DIMS, info=None):
+                 cell_ok, cells=GATE_CELLS, r_dry=None, bound=1e-3, dims=DIMS, info=None,
+                 weight_terms=1):
         t = _lib.torch()
         self.t = t
         self.dims = tuple(int(d) for d in dims)
-        self.mlp = MLP(self.dims, params=params)
+        self.mlp = MLP(self.dims, params=params, weight_terms=weight_terms)
         self.feat_mean = np.asarray(feat_mean, np.float32)
         self.feat_std = np.asarray(feat_std, np.float32)
         self.tgt_mean = np.asarray(tgt_mean, np.float32)
@@ -218,7 +226,7 @@ class Surrogate(object):
                             feat_std=self.feat_std, tgt_mean=self.tgt_mean, tgt_std=self.tgt_std,
                             box_lo=self.box_lo, box_hi=self.box_hi, cell_ok=self.cell_ok,
                             cells=np.array(self.cells), r_dry=self.r_dry, bound=self.bound,
-                            dims=np.array(self.dims),
+                            dims=np.array(self.dims), weight_terms=self.mlp.weight_terms,
                             info_keys=np.array(list(self.info.keys()), dtype=str),
                             info_vals=np.array([float(v) for v in self.info.values()]))
 
@@ -228,7 +236,8 @@ class Surrogate(object):
         info = dict(zip([str(k) for k in z["info_keys"]], z["info_vals"].tolist()))
         return cls(z["params"], z["feat_mean"], z["feat_std"], z["tgt_mean"], z["tgt_std"],
                    z["box_lo"], z["box_hi"], z["cell_ok"], cells=tuple(z["cells"]),
-                   r_dry=z["r_dry"], bound=float(z["bound"]), dims=tuple(z["dims"]), info=info)
+                   r_dry=z["r_dry"], bound=float(z["bound"]), dims=tuple(z["dims"]), info=info,
+                   weight_terms=int(z["weight_terms"]) if "weight_terms" in z.files else 1)
 
 
 def rel_errors(rows, ref):
@@ -244,66 +253,69 @@ def _features(q):
     return f
 
 
-def train(grid=None, train_stride=(1, 1, 1, 1), n_val_slabs=20000, steps=20000, batch=8192,
-          lr=2e-3, lr_final=2e-5, warmup=500, cw_head=4.0, seed=0, bound=1e-3, safety=2.0,
-          min_count=2, cells=GATE_CELLS, log=print, precision="fp64"):
-    """Build training tables with the GPU sweep, train the network with the HIP
-    kernels, fit the gate on validation slabs; returns (Surrogate, report).
+def _targets_host(R):
+    """Targets (before standardisation) of table rows, as gm_surrogate_targets forms them."""
+    R = np.asarray(R, np.float64)
+    T = np.empty_like(R)
+    T[:, 0] = np.log(R[:, 0])
+    T[:, 1] = np.log(R[:, 1])
+    T[:, 2] = np.log(np.maximum(R[:, 2], G_FLOOR))
+    T[:, 3:] = R[:, 3:] / (2 * np.arange(R.shape[1] - 3) + 1)
+    return T
 
-    train_stride subsamples the (m_r, m_i, lambda, gf) axes of the sweep grid
-    (1 = every value)."""
+
+def sweep_tables(g, mr, mi, lam, gf, pairs=None, precision="fp32", log=print):
+    """Tables of the product grid (mr, mi, lam, gf) x PSDs with the GPU sweep:
+    (queries [N, 6] host float32, rows [N, 35] device float32, seconds).
+    PSDs: the grid's r_mode x sigma_g, or the explicit (r_mode, sigma_g) pairs."""
     t = _lib.torch()
-    g = dict(full_grid() if grid is None else grid)
-    mr, mi, lam, gf = (g[k][::s] for k, s in zip(("mr", "mi", "lam", "gf"), train_stride))
-    rep = {}
-    sweep = LutSweep(g["r"], g["rmode"], g["sigma"], precision=precision)
+    sweep = LutSweep(g["r"], g["rmode"], g["sigma"], precision=precision, pairs=pairs)
     t0 = time.perf_counter()
     tab = sweep.sweep(mr, mi, lam, gf)
     t.cuda.synchronize()
-    rep["train_slabs"] = int(mr.size * mi.size * lam.size * gf.size)
-    rep["train_entries"] = rep["train_slabs"] * sweep.n_psd
-    rep["sweep_s"] = time.perf_counter() - t0
-    log("sweep: %d slabs, %d entries in %.1f s" % (rep["train_slabs"], rep["train_entries"],
-                                                  rep["sweep_s"]))
-    # queries of every entry, order [mr][mi][lam][gf][psd]
+    dt = time.perf_counter() - t0
     pr = np.array([p[0] for p in sweep.pairs])
     ps = np.array([p[1] for p in sweep.pairs])
     A, B, C, D, P = np.meshgrid(np.arange(mr.size), np.arange(mi.size), np.arange(lam.size),
                                 np.arange(gf.size), np.arange(sweep.n_psd), indexing="ij")
     Q = make_queries(pr[P.ravel()], ps[P.ravel()], mr[A.ravel()], mi[B.ravel()], lam[C.ravel()],
                      gf[D.ravel()])
-    del A, B, C, D, P
-    rows = tab.view(-1, N_OUT)
+    log("sweep: %d slabs x %d PSDs = %d entries in %.1f s" % (
+        Q.shape[0] // sweep.n_psd, sweep.n_psd, Q.shape[0], dt))
+    return Q, tab.view(-1, N_OUT), dt
+
+
+def new_surrogate(g, Q, rows, seed=0, cells=GATE_CELLS, bound=1e-3):
+    """Untrained surrogate with standardisation from (a subsample of) the data
+    and the training box of the grid g; every cell trusted until fit_gate."""
+    t = _lib.torch()
     n = rows.shape[0]
-    # standardisation from the data (host statistics on a subsample)
     rng = np.random.default_rng(seed)
-    sub = rng.choice(n, size=min(n, 2_000_000), replace=False)
+    sub = np.sort(rng.choice(n, size=min(n, 2_000_000), replace=False))
     F = _features(Q[sub])
     feat_mean, feat_std = F.mean(0), F.std(0)
     feat_std[feat_std == 0] = 1.0
-    R = rows[t.from_numpy(np.sort(sub)).to(rows.device)].cpu().numpy().astype(np.float64)
-    T = np.empty_like(R)
-    T[:, 0] = np.log(R[:, 0])
-    T[:, 1] = np.log(np.maximum(1.0 - R[:, 1], 0.0) + SSA_FLOOR)
-    T[:, 2] = R[:, 2]
-    T[:, 3:] = R[:, 3:] / (2 * np.arange(N_MOM) + 1)
+    T = _targets_host(rows[t.from_numpy(sub).to(rows.device)].cpu().numpy())
     tgt_mean, tgt_std = T.mean(0), T.std(0)
     tgt_std[tgt_std < 1e-6] = 1.0
-    # training box in gate coordinates, from the grid itself
-    box_lo = np.array([np.log(g["rmode"].min()), g["sigma"].min(), mr.min(), np.log(mi.min()),
-                       np.log(lam.min()), 1 - gf.min() ** -3.0])
-    box_hi = np.array([np.log(g["rmode"].max()), g["sigma"].max(), mr.max(), np.log(mi.max()),
-                       np.log(lam.max()), 1 - gf.max() ** -3.0])
-    # device training set
+    box_lo = np.array([np.log(g["rmode"].min()), g["sigma"].min(), g["mr"].min(),
+                       np.log(g["mi"].min()), np.log(g["lam"].min()), 1 - g["gf"].min() ** -3.0])
+    box_hi = np.array([np.log(g["rmode"].max()), g["sigma"].max(), g["mr"].max(),
+                       np.log(g["mi"].max()), np.log(g["lam"].max()), 1 - g["gf"].max() ** -3.0])
+    return Surrogate(init_params(DIMS, seed), feat_mean, feat_std, tgt_mean, tgt_std,
+                     box_lo - 1e-6, box_hi + 1e-6, np.ones(int(np.prod(cells)), np.uint8),
+                     cells=cells, r_dry=g["r"], bound=bound)
+
+
+def training_arrays(sur, Q, rows):
+    """Standardised features X [N, 6] and targets Y [N, 35] on the device (HIP kernels)."""
+    t = _lib.torch()
+    n = rows.shape[0]
     dev = _lib.device()
     d_q = _f32dev(Q)
-    del Q
     X = t.empty((n, 6), dtype=t.float32, device=dev)
     Y = t.empty((n, N_OUT), dtype=t.float32, device=dev)
     flags = t.empty(n, dtype=t.int32, device=dev)
-    ones = np.ones(int(np.prod(cells)), np.uint8)
-    sur = Surrogate(init_params(DIMS, seed), feat_mean, feat_std, tgt_mean, tgt_std,
-                    box_lo - 1e-6, box_hi + 1e-6, ones, cells=cells, r_dry=g["r"], bound=bound)
     L, h, s = _lib.lib(), _lib.ctx(), stream()
     d = sur._dev
     check(L.gm_surrogate_prep(h, n, ptr(d_q), ptr(d["feat_mean"]), ptr(d["feat_std"]),
@@ -311,66 +323,72 @@ def train(grid=None, train_stride=(1, 1, 1, 1), n_val_slabs=20000, steps=20000, 
                               ptr(X), ptr(flags), s), "gm_surrogate_prep")
     check(L.gm_surrogate_targets(h, n, N_OUT, ptr(rows), ptr(d["tgt_mean"]), ptr(d["tgt_std"]),
                                  ptr(Y), s), "gm_surrogate_targets")
-    del d_q, flags
+    return X, Y
+
+
+def fit(mlp, X, Y, steps, batch=16384, lr=2e-3, lr_final=1e-5, warmup=500, cw_head=20.0,
+        seed=0, log=print):
+    """Adam on shuffled minibatches, linear warmup then cosine decay; returns
+    [(step, loss)] sampled every 1000 steps."""
+    t = _lib.torch()
+    n = X.shape[0]
+    dev = _lib.device()
     cw = np.ones(N_OUT, np.float32)
     cw[:3] = cw_head
     cw[3] = 0.0                                   # alpha1_0 = 1 identically
     cw = _f32dev(cw / cw.sum() * N_OUT)
-    mlp = sur.mlp
-    # training: shuffled epochs, cosine learning rate after a linear warmup
-    t0 = time.perf_counter()
     gen = t.Generator(device=dev)
     gen.manual_seed(seed)
     perm = t.randperm(n, device=dev, generator=gen)
     pos = 0
     hist = []
+    t0 = time.perf_counter()
     for it in range(steps):
         if pos + batch > n:
             perm = t.randperm(n, device=dev, generator=gen)
             pos = 0
         idx = perm[pos:pos + batch]
         pos += batch
-        xb, yb = X[idx], Y[idx]
         if it < warmup:
             lr_t = lr * (it + 1) / warmup
         else:
             c = (it - warmup) / max(1, steps - warmup)
             lr_t = lr_final + 0.5 * (lr - lr_final) * (1 + np.cos(np.pi * c))
-        mlp.grad(xb, yb, cw)
+        mlp.grad(X[idx], Y[idx], cw)
         mlp.adam(lr_t)
         if it % 1000 == 0 or it == steps - 1:
             hist.append((it, float(mlp.loss.item())))
             log("step %6d  loss %.3e  lr %.2e  %.1f s" % (it, hist[-1][1], lr_t,
                                                          time.perf_counter() - t0))
     t.cuda.synchronize()
-    rep["train_s"] = time.perf_counter() - t0
-    rep["train_steps"] = steps
-    rep["batch"] = batch
-    rep["final_loss"] = hist[-1][1]
-    del X, Y
-    # validation slabs at random points inside the grid, random PSDs
-    t0 = time.perf_counter()
-    vr = np.random.default_rng(seed + 1)
-    ns = int(n_val_slabs)
-    v_mr = vr.uniform(mr.min(), mr.max(), ns)
-    v_mi = np.exp(vr.uniform(np.log(mi.min()), np.log(mi.max()), ns))
-    v_lam = np.exp(vr.uniform(np.log(lam.min()), np.log(lam.max()), ns))
-    v_gf = (1 - vr.uniform(0, 1 - gf.max() ** -3.0, ns)) ** (-1.0 / 3.0)
-    n_vp = 40
-    v_rm = np.exp(vr.uniform(np.log(g["rmode"].min()), np.log(g["rmode"].max()), n_vp))
-    v_sg = vr.uniform(g["sigma"].min(), g["sigma"].max(), n_vp)
+    return hist, time.perf_counter() - t0
+
+
+def validation_tables(g, n_slabs, n_psd=40, seed=1, precision="fp32", chunk=2000):
+    """Random slabs and random PSDs inside the grid's box: (queries, rows) host."""
+    r = np.random.default_rng(seed)
+    ns = int(n_slabs)
+    v_mr = r.uniform(g["mr"].min(), g["mr"].max(), ns)
+    v_mi = np.exp(r.uniform(np.log(g["mi"].min()), np.log(g["mi"].max()), ns))
+    v_lam = np.exp(r.uniform(np.log(g["lam"].min()), np.log(g["lam"].max()), ns))
+    v_gf = (1 - r.uniform(0, 1 - g["gf"].max() ** -3.0, ns)) ** (-1.0 / 3.0)
+    v_rm = np.exp(r.uniform(np.log(g["rmode"].min()), np.log(g["rmode"].max()), n_psd))
+    v_sg = r.uniform(g["sigma"].min(), g["sigma"].max(), n_psd)
     vsw = LutSweep(g["r"], None, None, pairs=list(zip(v_rm, v_sg)), precision=precision)
     vq, vrows = [], []
-    chunk = 2000
     for a in range(0, ns, chunk):
         b = min(ns, a + chunk)
         out = vsw.run(v_lam[a:b], v_mr[a:b] + 1j * v_mi[a:b], v_gf[a:b])
         vrows.append(out.view(-1, N_OUT).cpu().numpy())
-        k = np.arange(a, b).repeat(n_vp)
-        p = np.tile(np.arange(n_vp), b - a)
+        k = np.arange(a, b).repeat(n_psd)
+        p = np.tile(np.arange(n_psd), b - a)
         vq.append(make_queries(v_rm[p], v_sg[p], v_mr[k], v_mi[k], v_lam[k], v_gf[k]))
-    vq = np.concatenate(vq)
-    vrows = np.concatenate(vrows)
+    return np.concatenate(vq), np.concatenate(vrows)
+
+
+def evaluate(sur, vq, vrows):
+    """Validation report (per-quantity relative errors) and the per-entry max error."""
+    rep = {}
     pred, _ = sur.predict(vq)
     e3 = rel_errors(pred.cpu().numpy(), vrows)
     err = e3.max(axis=1)
@@ -383,24 +401,74 @@ def train(grid=None, train_stride=(1, 1, 1, 1), n_val_slabs=20000, steps=20000, 
         if sel.any():
             rep["val_err_median_x%g" % lo_] = float(np.median(err[sel]))
     rep["val_entries"] = int(vq.shape[0])
-    rep["val_s"] = time.perf_counter() - t0
     rep["val_err_median"] = float(np.median(err))
     rep["val_err_p90"] = float(np.quantile(err, 0.9))
-    rep["val_frac_within_bound"] = float(np.mean(err <= bound))
-    # gate: per-cell maximum validation error
+    rep["val_frac_within_bound"] = float(np.mean(err <= sur.bound))
+    return rep, err
+
+
+def fit_gate(sur, vq, err, safety=1.0, min_count=3):
+    """Trust a cell when it holds >= min_count validation entries and all are
+    within bound / safety; returns (fraction of cells, fraction of entries) trusted."""
     gc = gate_coords(vq)
     lo, hi = sur.box_lo.astype(np.float64), sur.box_hi.astype(np.float64)
     ci = np.zeros(vq.shape[0], np.int64)
-    for j, nc in enumerate(cells):
+    for j, nc in enumerate(sur.cells):
         c = np.floor((gc[:, j] - lo[j]) / (hi[j] - lo[j]) * nc).astype(np.int64)
         ci = ci * nc + np.clip(c, 0, nc - 1)
-    n_cells = int(np.prod(cells))
+    n_cells = int(np.prod(sur.cells))
     cmax = np.zeros(n_cells)
     np.maximum.at(cmax, ci, err)
     cnt = np.bincount(ci, minlength=n_cells)
-    ok = (cnt >= min_count) & (cmax <= bound / safety)
-    rep["cells_trusted"] = float(ok.mean())
-    rep["val_frac_trusted"] = float(ok[ci].mean())
+    ok = (cnt >= min_count) & (cmax <= sur.bound / safety)
+    sur.cell_ok = ok.astype(np.uint8)
+    sur._cell_ok = _lib.to_dev(sur.cell_ok, np.uint8)
+    return float(ok.mean()), float(ok[ci].mean())
+
+
+def train(grid=None, train_stride=(1, 1, 1, 1), n_val_slabs=20000, steps=20000, batch=16384,
+          lr=2e-3, lr_final=1e-5, warmup=500, cw_head=20.0, seed=0, bound=1e-3, safety=1.0,
+          min_count=3, cells=GATE_CELLS, log=print, precision="fp32", n_aug_psd=0,
+          weight_terms=1):
+    """Build training tables with the GPU sweep, train the network with the HIP
+    kernels, fit the gate on validation slabs; returns (Surrogate, report).
+
+    train_stride subsamples the (m_r, m_i, lambda, gf) axes of the sweep grid
+    (1 = every value).  n_aug_psd > 0 adds a second pass over the same slabs
+    with that many random (r_mode, sigma_g) pairs inside the grid's ranges."""
+    t = _lib.torch()
+    g = dict(full_grid() if grid is None else grid)
+    mr, mi, lam, gf = (g[k][::s] for k, s in zip(("mr", "mi", "lam", "gf"), train_stride))
+    rep = {}
+    Q, rows, dt = sweep_tables(g, mr, mi, lam, gf, precision=precision, log=log)
+    rep["sweep_s"] = dt
+    if n_aug_psd:
+        r = np.random.default_rng(seed + 7)
+        pairs = list(zip(np.exp(r.uniform(np.log(g["rmode"].min()), np.log(g["rmode"].max()),
+                                          n_aug_psd)),
+                         r.uniform(g["sigma"].min(), g["sigma"].max(), n_aug_psd)))
+        Q2, rows2, dt2 = sweep_tables(g, mr, mi, lam, gf, pairs=pairs, precision=precision,
+                                      log=log)
+        Q = np.concatenate([Q, Q2])
+        rows = t.cat([rows, rows2])
+        del Q2, rows2
+        rep["sweep_s"] += dt2
+    rep["train_entries"] = int(rows.shape[0])
+    sur = new_surrogate(g, Q, rows, seed=seed, cells=cells, bound=bound)
+    sur.mlp.weight_terms = int(weight_terms)
+    X, Y = training_arrays(sur, Q, rows)
+    del Q, rows
+    hist, rep["train_s"] = fit(sur.mlp, X, Y, steps, batch=batch, lr=lr, lr_final=lr_final,
+                               warmup=warmup, cw_head=cw_head, seed=seed, log=log)
+    rep["train_steps"], rep["batch"], rep["final_loss"] = steps, batch, hist[-1][1]
+    rep["weight_terms"] = weight_terms
+    del X, Y
+    t0 = time.perf_counter()
+    vq, vrows = validation_tables(g, n_val_slabs, seed=seed + 1, precision=precision)
+    r2, err = evaluate(sur, vq, vrows)
+    rep.update(r2)
+    rep["cells_trusted"], rep["val_frac_trusted"] = fit_gate(sur, vq, err, safety, min_count)
+    rep["val_s"] = time.perf_counter() - t0
     log("validation per quantity: " + ", ".join("%s %.2e/%.2e" % (k, rep["val_%s_median" % k],
                                                                   rep["val_%s_p90" % k])
                                                  for k in ("qext", "ssa", "g")))
@@ -410,7 +478,5 @@ def train(grid=None, train_stride=(1, 1, 1, 1), n_val_slabs=20000, steps=20000, 
         "%.1f%% of validation entries in trusted cells" % (
             rep["val_err_median"], rep["val_err_p90"], 100 * rep["val_frac_within_bound"],
             100 * rep["cells_trusted"], 100 * rep["val_frac_trusted"]))
-    sur.cell_ok = ok.astype(np.uint8)
-    sur._cell_ok = _lib.to_dev(sur.cell_ok, np.uint8)
     sur.info = rep
     return sur, rep

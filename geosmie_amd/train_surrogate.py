@@ -25,7 +25,7 @@ def main(argv=None):
     ap.add_argument("--stride", type=int, nargs=4, default=(1, 1, 1, 1),
                     help="subsampling of the m_r, m_i, lambda, gf axes")
     ap.add_argument("--val-slabs", type=int, default=20000)
-    ap.add_argument("--cw-head", type=float, default=10.0)
+    ap.add_argument("--cw-head", type=float, default=20.0)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--precision", default="fp32",
                     help="S1/S2 precision of the training tables (Q and g are FP64 either way)")

@@ -68,6 +68,21 @@ int gm_ctx_destroy(gm_ctx* ctx);
  * Clears the flags. */
 int gm_ctx_check(gm_ctx* ctx, gm_stream stream, int32_t* flags_host);
 
+/* Launch-shape parameters of a context (no reference counterpart), set by
+ * an autotuner (geosmie_amd/autotune.py) from timings of the caller's own
+ * workload.  Results do not depend on them (fixed reduction orders), except
+ * GM_TUNE_PSD_CHUNK, which regroups the split-K partial sums of the few-PSD
+ * contraction (rounding-level differences). */
+enum gm_tune_param {
+  GM_TUNE_MIE_GRID_MULT = 0,  /* persistent Mie grid = this x resident wavefronts (1..16; 1) */
+  GM_TUNE_S12_WAVES = 1,      /* wavefronts of 64 angles per S1/S2 block (1, 2, 4; 4) */
+  GM_TUNE_PSD_CHUNK = 2,      /* points per split-K chunk, few-PSD contraction (16..2^20; 128) */
+  GM_TUNE_MLP_WEIGHT_TERMS = 3 /* precision, not shape: BF16 terms per MLP weight on the matrix
+                                  cores, 1 = BF16 weights (default), 2 = hi + lo BF16 pair */
+};
+int gm_ctx_tune(gm_ctx* ctx, int param, int value);
+int gm_ctx_get_tune(gm_ctx* ctx, int param, int* value);
+
 /* Kernel timing (no reference counterpart; instrumentation for the
  * benchmark's roofline figure).  When enabled, calls on ctx record a HIP event
  * pair on their launch stream around each main kernel.  gm_ctx_kernel_time
@@ -280,14 +295,14 @@ int gm_surrogate_prep(gm_ctx* ctx, int64_t n, const float* q, const float* feat_
                       gm_stream stream);
 
 /* Network outputs y [n][n_out] (standardised) -> table rows [n][n_out]
- * (Qext, SSA, g, alpha1_0..): Qext = exp(t0), SSA = 1 - (exp(t1) - 1e-4),
- * g = t2, alpha1_l = (2l + 1) t_{3+l}, with t_j = y_j tgt_std[j] + tgt_mean[j]. */
+ * (Qext, SSA, g, alpha1_0..): Qext = exp(t0), SSA = min(exp(t1), 1),
+ * g = exp(t2), alpha1_l = (2l + 1) t_{3+l}, with t_j = y_j tgt_std[j] + tgt_mean[j]. */
 int gm_surrogate_post(gm_ctx* ctx, int64_t n, int n_out, const float* y, const float* tgt_mean,
                       const float* tgt_std, float* rows, gm_stream stream);
 
 /* Inverse of gm_surrogate_post: table rows [n][n_out] -> standardised
- * training targets y [n][n_out] (t0 = ln Qext, t1 = ln(1 - SSA + 1e-4),
- * t2 = g, t_{3+l} = alpha1_l / (2l + 1); y_j = (t_j - tgt_mean[j]) / tgt_std[j]). */
+ * training targets y [n][n_out] (t0 = ln Qext, t1 = ln SSA, t2 = ln max(g, 1e-8),
+ * t_{3+l} = alpha1_l / (2l + 1); y_j = (t_j - tgt_mean[j]) / tgt_std[j]). */
 int gm_surrogate_targets(gm_ctx* ctx, int64_t n, int n_out, const float* rows,
                          const float* tgt_mean, const float* tgt_std, float* y, gm_stream stream);
 

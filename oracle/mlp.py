@@ -6,7 +6,7 @@ kernels (geosmie_amd/csrc/gm_mlp.hip) must reproduce:
 
   * parameters: one FP32 vector, per layer W [out][in] row-major then b [out];
   * the model's weights are bf16(W) (round to nearest even of the FP32 value),
-    biases stay FP32;
+    or, in the two-term setting, bf16(W) + bf16(W - bf16(W)); biases stay FP32;
   * z_l = a_{l-1} bf16(W_l)^T + b_l,  a_l = SiLU(z_l) = z_l / (1 + exp(-z_l))
     for every layer but the last, y = z_L;
   * loss = sum_j cw[j] mean_b (y[b][j] - t[b][j])^2;
@@ -37,11 +37,20 @@ def layout(dims):
     return out, off
 
 
-def unpack(params, dims):
+def weights(w, terms=1):
+    """The weights the network computes with: bf16(W), or with terms=2 the
+    pair bf16(W) + bf16(W - bf16(W))."""
+    hi = bf16_round(w)
+    if terms == 1:
+        return hi
+    return hi + bf16_round((np.asarray(w, np.float32).astype(np.float64) - hi).astype(np.float32))
+
+
+def unpack(params, dims, terms=1):
     lay, total = layout(dims)
     p = np.asarray(params, dtype=np.float32)
     assert p.size == total
-    return [(bf16_round(p[w:b].reshape(o, i)), p[b:b + o].astype(np.float64))
+    return [(weights(p[w:b].reshape(o, i), terms), p[b:b + o].astype(np.float64))
             for w, b, i, o in lay]
 
 
@@ -54,8 +63,8 @@ def dsilu(z):
     return s * (1.0 + z * (1.0 - s))
 
 
-def forward(params, dims, x, keep=False):
-    layers = unpack(params, dims)
+def forward(params, dims, x, keep=False, terms=1):
+    layers = unpack(params, dims, terms)
     a = np.asarray(x, dtype=np.float64)
     zs, acts = [], [a]
     for li, (W, b) in enumerate(layers):
@@ -66,9 +75,9 @@ def forward(params, dims, x, keep=False):
     return (a, zs, acts, layers) if keep else a
 
 
-def loss_grad(params, dims, x, t, cw):
+def loss_grad(params, dims, x, t, cw, terms=1):
     """(loss, flat gradient) in FP64."""
-    y, zs, acts, layers = forward(params, dims, x, keep=True)
+    y, zs, acts, layers = forward(params, dims, x, keep=True, terms=terms)
     n = y.shape[0]
     cw = np.asarray(cw, dtype=np.float64)
     d = y - np.asarray(t, dtype=np.float64)
