@@ -40,8 +40,12 @@ __device__ __forceinline__ int nmax_rule(double s) {
 __device__ __forceinline__ int nmx_rule(int nmax, double zabs) {
   return (int)rint(fmax((double)nmax, zabs) + 16.0);
 }
+// Start of the downward psi-ratio chains: rows n <= nmax agree to < 1e-15
+// with any later start from nmax + 8 + 4 ceil(|z|^(1/3)) on (checked over
+// |z| = 0.01..2000 and the refractive indices of the workloads); the margin
+// below is half again that.
 __device__ __forceinline__ int ratio_start(int nmax, double zabs) {
-  return nmax + 16 + 8 * (int)ceil(cbrt(zabs));
+  return nmax + 12 + 6 * (int)ceil(cbrt(zabs));
 }
 // Whether a downward D_n recurrence started at nmx has converged (relative
 // error < ~1e-17) by the time it reaches n <= |z|: the error decays like
@@ -317,25 +321,29 @@ __global__ __launch_bounds__(64) void k_mie_homog(
     const double zabs = cabs_(z);
     int nmx = nmx_rule(nmax, zabs) + (dn_extra < 0 ? 16 + 10 * (int)ceil(cbrt(zabs)) : dn_extra);
     cdd zinv = crcp_dd(z);
-    // D_n(mx) downward from D_nmx = 0 (mie_coeffs.py:118-121)
-    {
-      cd dd = cd{0.0, 0.0};
-      for (int j = nmx - 1; j >= 1; --j) {
-        cd r = mul_dd((double)(j + 1), zinv);
-        dd = r - crcp_fast(dd + r);
-        if (j <= nmax) Dp[(size_t)(j - 1) * 64 + lane] = dd;
-      }
-    }
     const ddr xinv = rcp_dd(s);
     const int nsw = min((int)s, nmax);
-    // psi ratio psi_{n-1}/psi_n = D_n(x) + n/x above the oscillatory region
-    if (nsw < nmax) {
-      double dd = 0.0;
-      for (int n = ratio_start(nmax, s); n > nsw; --n) {
-        double r = mul_dd((double)n, xinv);
-        double t = dd + r;
-        if (n <= nmax) Rp[(size_t)(n - 1) * 64 + lane] = t;
-        dd = r - rcp_nr(t);
+    // D_n(mx) downward from D_nmx = 0 (mie_coeffs.py:118-121) and the psi ratio
+    // psi_{n-1}/psi_n = D_n(x) + n/x above the oscillatory region: two
+    // independent serial chains advanced in one loop so their latencies overlap.
+    {
+      const int sr = nsw < nmax ? ratio_start(nmax, s) : 0;   // ratio rows (nsw, sr]
+      const int top = max(nmx - 1, sr);
+      cd dd = cd{0.0, 0.0};
+      double dr = 0.0;
+      for (int k = top; k >= 1; --k) {
+        const size_t o = (size_t)(k - 1) * 64 + lane;
+        if (k <= nmx - 1) {
+          const cd r = mul_dd((double)(k + 1), zinv);
+          dd = r - crcp_fast(dd + r);
+          if (k <= nmax) Dp[o] = dd;
+        }
+        if (k <= sr && k > nsw) {
+          const double r = mul_dd((double)k, xinv);
+          const double t = dr + r;
+          if (k <= nmax) Rp[o] = t;
+          dr = r - rcp_nr(t);
+        }
       }
     }
     double sn, cs;
@@ -481,8 +489,47 @@ __global__ __launch_bounds__(64) void k_mie_coated_stable(
     const int nmx =
         nmx_rule(nmax, mxv) + (dn_extra < 0 ? 16 + 10 * (int)ceil(cbrt(mxv)) : dn_extra);
     const double av = cabs_(v), aw = cabs_(w);
-    // ---- pass 1: downward recurrences
-    dn_chain(crcp_dd(u), nmx, nmax, DU, lane, false);
+    // ---- pass 1: downward recurrences.  D_n(u) and the psi ratios of v, w, y
+    // are four independent serial chains; one loop advances all of them so
+    // their dependent FP64 latencies overlap (each chain's own sequence of
+    // operations is unchanged).
+    const int nsw_v = min((int)av, nmax), nsw_w = min((int)aw, nmax), nsw_y = min((int)yy, nmax);
+    {
+      const cdd ui = crcp_dd(u), vi = crcp_dd(v), wi = crcp_dd(w);
+      const ddr yi = rcp_dd(yy);
+      const int sv = nsw_v < nmax ? ratio_start(nmax, av) : 0;   // ratio rows (nsw, s]
+      const int sw = nsw_w < nmax ? ratio_start(nmax, aw) : 0;
+      const int sy = nsw_y < nmax ? ratio_start(nmax, yy) : 0;
+      const int top = max(max(nmx - 1, sv), max(sw, sy));
+      cd du = cd{0.0, 0.0}, dv = cd{0.0, 0.0}, dw = cd{0.0, 0.0};
+      double dy = 0.0;
+      for (int k = top; k >= 1; --k) {
+        const size_t o = (size_t)(k - 1) * 64 + lane;
+        if (k <= nmx - 1) {   // D_k(u) = (k+1)/u - 1/(D_{k+1} + (k+1)/u), D_nmx = 0
+          const cd r = mul_dd((double)(k + 1), ui);
+          du = r - crcp_fast(du + r);
+          if (k <= nmax) DU[o] = du;
+        }
+        if (k <= sv && k > nsw_v) {   // psi_{k-1}(v) / psi_k(v)
+          const cd r = mul_dd((double)k, vi);
+          const cd t = dv + r;
+          if (k <= nmax) RV[o] = t;
+          dv = r - crcp_fast(t);
+        }
+        if (k <= sw && k > nsw_w) {
+          const cd r = mul_dd((double)k, wi);
+          const cd t = dw + r;
+          if (k <= nmax) RW[o] = t;
+          dw = r - crcp_fast(t);
+        }
+        if (k <= sy && k > nsw_y) {
+          const double r = mul_dd((double)k, yi);
+          const double t = dy + r;
+          if (k <= nmax) RY[o] = t;
+          dy = r - rcp_nr(t);
+        }
+      }
+    }
     const bool ddv_on = !converged_start(nmx, av), ddw_on = !converged_start(nmx, aw);
     if (ddv_on) {
       const cdd vi = crcp_dd(v);
@@ -494,10 +541,6 @@ __global__ __launch_bounds__(64) void k_mie_coated_stable(
       dn_chain(wi, nmx, nmax, DDW, lane, false);
       dn_chain(wi, nmx + 16 + 8 * (int)ceil(cbrt(aw)), nmax, DDW, lane, true);
     }
-    const int nsw_v = min((int)av, nmax), nsw_w = min((int)aw, nmax), nsw_y = min((int)yy, nmax);
-    if (nsw_v < nmax) ratio_chain(v, nsw_v, nmax, RV, lane);
-    if (nsw_w < nmax) ratio_chain(w, nsw_w, nmax, RW, lane);
-    if (nsw_y < nmax) ratio_chain(yy, nsw_y, nmax, RY, lane);
     // ---- pass 2: upward Riccati-Bessel recurrences and the combination
     const cdd vinv = crcp_dd(v), winv = crcp_dd(w), minv = crcp_dd(m), m2inv = crcp_dd(m2);
     const ddr yinv = rcp_dd(yy);

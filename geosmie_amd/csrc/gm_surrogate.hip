@@ -6,7 +6,11 @@
 // r_eff = gf r_mode exp(2.5 ln^2 sigma_g); f_w = 1 - gf^-3 the water volume
 // fraction of the humidity growth (dointegration.py:497-507); m_r + i m_i the
 // dry refractive index; lambda [um].
-//   network features  (ln r_eff, sigma_g, m_r, ln m_i, ln lambda, f_w), standardised;
+//   network features  (ln x_eff, sigma_g, Re m_mix, ln Im m_mix, ln(gf / lambda), f_w),
+//                     standardised: the effective size parameter 2 pi r_eff / lambda and
+//                     the volume-mixed index m_mix = m_w + (m - m_w)(1 - f_w) that the
+//                     humidity rule gives (dointegration.py:497-507), gf / lambda locating
+//                     the radius grid's end in size-parameter space, and f_w;
 //   gate coordinates  (ln r_mode, sigma_g, m_r, ln m_i, ln lambda, f_w) with the
 //                     dry mode radius r_mode = r_eff / (gf exp(2.5 ln^2 sigma_g)).
 // flags[i] = 0: served by the network; 1: outside the training box [lo, hi];
@@ -33,17 +37,20 @@ struct Cells {
 __global__ void k_surr_prep(int64_t n, const float* __restrict__ q, const float* __restrict__ fmean,
                             const float* __restrict__ fstd, const float* __restrict__ lo,
                             const float* __restrict__ hi, Cells cells,
-                            const uint8_t* __restrict__ cell_ok, float* __restrict__ x,
-                            int32_t* __restrict__ flags) {
+                            const uint8_t* __restrict__ cell_ok, float mw_re,
+                            float* __restrict__ x, int32_t* __restrict__ flags) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const float* qi = q + i * kSurrIn;
-  float f[kSurrIn] = {qi[0], qi[1], qi[2], qi[3], logf(qi[4]), qi[5]};
+  const float llam = logf(qi[4]);
+  const float dry = 1.0f - qi[5];                 // gf^-3
+  const float lgf = -logf(dry) * (1.0f / 3.0f);   // ln gf
+  float f[kSurrIn] = {qi[0] - llam + 1.8378770664093453f,   // ln(2 pi r_eff / lambda)
+                      qi[1], mw_re + (qi[2] - mw_re) * dry, qi[3] + logf(dry), lgf - llam, qi[5]};
 #pragma unroll
   for (int j = 0; j < kSurrIn; ++j) x[i * kSurrIn + j] = (f[j] - fmean[j]) / fstd[j];
   const float ls = logf(qi[1]);
-  const float gf = 1.0f / cbrtf(1.0f - qi[5]);
-  float gc[kSurrIn] = {qi[0] - logf(gf) - 2.5f * ls * ls, qi[1], qi[2], qi[3], f[4], qi[5]};
+  float gc[kSurrIn] = {qi[0] - lgf - 2.5f * ls * ls, qi[1], qi[2], qi[3], llam, qi[5]};
   int flag = 0;
 #pragma unroll
   for (int j = 0; j < kSurrIn; ++j)
@@ -98,8 +105,8 @@ extern "C" {
 // See include/geosmie.h.
 int gm_surrogate_prep(gm_ctx* ctx, int64_t n, const float* q, const float* feat_mean,
                       const float* feat_std, const float* box_lo, const float* box_hi,
-                      const int* n_cells_host, const uint8_t* cell_ok, float* x, int32_t* flags,
-                      gm_stream stream) {
+                      const int* n_cells_host, const uint8_t* cell_ok, double m_water_re,
+                      float* x, int32_t* flags, gm_stream stream) {
   if (!ctx || n < 0 || !n_cells_host) return gm_fail(GM_EINVAL, "gm_surrogate_prep: bad arguments");
   gm::Cells cells;
   int64_t total = 1;
@@ -116,7 +123,7 @@ int gm_surrogate_prep(gm_ctx* ctx, int64_t n, const float* q, const float* feat_
   if (e != hipSuccess) return gm_fail(GM_ERUNTIME, "hipSetDevice: %s", hipGetErrorString(e));
   hipLaunchKernelGGL(gm::k_surr_prep, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
                      (hipStream_t)stream, n, q, feat_mean, feat_std, box_lo, box_hi, cells,
-                     cell_ok, x, flags);
+                     cell_ok, (float)m_water_re, x, flags);
   e = hipGetLastError();
   if (e != hipSuccess) return gm_fail(GM_ERUNTIME, "k_surr_prep: %s", hipGetErrorString(e));
   return GM_OK;

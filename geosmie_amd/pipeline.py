@@ -7,7 +7,7 @@ runoptics' per-(wavelength, rh) iteration computes
 (src/geosmie/dointegration.py:787-1001):
 
   gm_coef_offsets, gm_mie_batch          a_n, b_n and efficiencies (FP64)
-  gm_angle_tables                         pi'_n, tau'_n at the output angles
+  (gm_angle_tables                        pi'_n, tau'_n: once, at construction)
   gm_s12_mueller                          S1/S2 on MFMA -> s11, s12, s33, s34
   gm_psd_contract (per row)               phase-matrix integrals of integratePSD
   gm_integrate_psd (per row)              the integratePSD scalars
@@ -77,6 +77,11 @@ class RunopticsChain(object):
             self.d_F = t.empty(self.rows * 6 * self.n_ang, dtype=f64, device=dev)
             self.d_pmom = t.empty(self.rows * 6 * self.gsf_ng, dtype=f64, device=dev)
         self.graph = None
+        # pi'/tau' depend only on the angles and nmax_cap: built once, as
+        # MultipleMie.preCalculate does (mie_coated.py:149-151, 160-179)
+        check(_lib.lib().gm_angle_tables(self.ctx, self.n_ang, ptr(self.d_cost), self.nmax_cap,
+                                         ptr(self.d_pi), ptr(self.d_tau), _lib.stream()),
+              "gm_angle_tables")
 
     def step(self):
         """Enqueue one pass on torch's current stream."""
@@ -85,8 +90,6 @@ class RunopticsChain(object):
         check(L.gm_mie_batch(h, self.n, P(self.d_x), None, P(self.d_eps), None, None,
                              self.size_max, P(self.d_off), P(self.d_an), P(self.d_bn),
                              P(self.d_props), P(self.d_nmax), _lib.GM_MIE_SORT, s), "gm_mie_batch")
-        check(L.gm_angle_tables(h, self.n_ang, P(self.d_cost), self.nmax_cap, P(self.d_pi),
-                                P(self.d_tau), s), "gm_angle_tables")
         check(L.gm_s12_mueller(h, self.n, P(self.d_off), P(self.d_nmax), P(self.d_an),
                                P(self.d_bn), self.n_ang, P(self.d_pi), P(self.d_tau),
                                self.nmax_cap, P(self.d_mu), self.precision, s), "gm_s12_mueller")

@@ -9,6 +9,10 @@ Mie theory.  Here a network maps a query
 refractive index, the wavelength, the water volume fraction 1 - gf^-3 of the
 humidity growth) to the table row  Qext, SSA, g, alpha1_0 .. alpha1_31
 that the LUT sweep (geosmie_amd.lut_sweep) computes with plain GPU Mie.
+The network sees the query through the variables Mie theory depends on:
+ln(2 pi r_eff / lambda), sigma_g, the volume-mixed index m_w + (m - m_w)(1 - f_w)
+(real part, log imaginary part), ln(gf / lambda) (where the radius grid ends
+in size-parameter space) and f_w.
 
   * Network: 6 -> 256 -> 256 -> 256 -> 256 -> 35, SiLU, BF16 weights, FP32
     accumulation; forward, backward and Adam are the HIP kernels of
@@ -28,12 +32,13 @@ import numpy as np
 
 from . import _lib
 from ._lib import check, ptr, stream
-from .lut_sweep import N_MOM, LutSweep, full_grid
+from .lut_sweep import M_WATER, N_MOM, LutSweep, full_grid
 
 DIMS = (6, 256, 256, 256, 256, 35)
 N_OUT = 3 + N_MOM
 GATE_CELLS = (8, 6, 6, 10, 10, 6)
 G_FLOOR = 1e-8        # floor of g in its log target (gm_surrogate.hip)
+FEATURES = 2          # version of the feature / target transforms a saved model was trained with
 
 
 def sigma_factor(sigma):
@@ -181,7 +186,8 @@ class Surrogate(object):
         L, h, s = _lib.lib(), _lib.ctx(), stream()
         check(L.gm_surrogate_prep(h, n, ptr(q), ptr(d["feat_mean"]), ptr(d["feat_std"]),
                                   ptr(d["box_lo"]), ptr(d["box_hi"]), self._ccells,
-                                  ptr(self._cell_ok), ptr(x), ptr(flags), s), "gm_surrogate_prep")
+                                  ptr(self._cell_ok), M_WATER.real, ptr(x), ptr(flags), s),
+              "gm_surrogate_prep")
         y = self.mlp.forward(x)
         rows = t.empty_like(y)
         check(L.gm_surrogate_post(h, n, self.dims[-1], ptr(y), ptr(d["tgt_mean"]),
@@ -227,12 +233,17 @@ class Surrogate(object):
                             box_lo=self.box_lo, box_hi=self.box_hi, cell_ok=self.cell_ok,
                             cells=np.array(self.cells), r_dry=self.r_dry, bound=self.bound,
                             dims=np.array(self.dims), weight_terms=self.mlp.weight_terms,
+                            features=FEATURES,
                             info_keys=np.array(list(self.info.keys()), dtype=str),
                             info_vals=np.array([float(v) for v in self.info.values()]))
 
     @classmethod
     def load(cls, path):
         z = np.load(path, allow_pickle=False)
+        ver = int(z["features"]) if "features" in z.files else 1
+        if ver != FEATURES:
+            raise ValueError("%s was trained with feature transform %d, this build uses %d: "
+                             "retrain with python -m geosmie_amd.train_surrogate" % (path, ver, FEATURES))
         info = dict(zip([str(k) for k in z["info_keys"]], z["info_vals"].tolist()))
         return cls(z["params"], z["feat_mean"], z["feat_std"], z["tgt_mean"], z["tgt_std"],
                    z["box_lo"], z["box_hi"], z["cell_ok"], cells=tuple(z["cells"]),
@@ -248,9 +259,15 @@ def rel_errors(rows, ref):
 
 
 def _features(q):
-    f = np.asarray(q, np.float64).copy()
-    f[:, 4] = np.log(f[:, 4])
-    return f
+    """Network features of queries, as gm_surrogate_prep forms them (before
+    standardisation): ln x_eff, sigma_g, Re m_mix, ln Im m_mix, ln(gf/lambda), f_w."""
+    q = np.asarray(q, np.float64)
+    llam = np.log(q[:, 4])
+    dry = 1.0 - q[:, 5]
+    lgf = -np.log(dry) / 3.0
+    mw = M_WATER.real
+    return np.stack([q[:, 0] - llam + np.log(2 * np.pi), q[:, 1], mw + (q[:, 2] - mw) * dry,
+                     q[:, 3] + np.log(dry), lgf - llam, q[:, 5]], axis=-1)
 
 
 def _targets_host(R):
@@ -320,13 +337,13 @@ def training_arrays(sur, Q, rows):
     d = sur._dev
     check(L.gm_surrogate_prep(h, n, ptr(d_q), ptr(d["feat_mean"]), ptr(d["feat_std"]),
                               ptr(d["box_lo"]), ptr(d["box_hi"]), sur._ccells, ptr(sur._cell_ok),
-                              ptr(X), ptr(flags), s), "gm_surrogate_prep")
+                              M_WATER.real, ptr(X), ptr(flags), s), "gm_surrogate_prep")
     check(L.gm_surrogate_targets(h, n, N_OUT, ptr(rows), ptr(d["tgt_mean"]), ptr(d["tgt_std"]),
                                  ptr(Y), s), "gm_surrogate_targets")
     return X, Y
 
 
-def fit(mlp, X, Y, steps, batch=16384, lr=2e-3, lr_final=1e-5, warmup=500, cw_head=20.0,
+def fit(mlp, X, Y, steps, batch=16384, lr=2e-3, lr_final=1e-5, warmup=500, cw_head=1000.0,
         seed=0, log=print):
     """Adam on shuffled minibatches, linear warmup then cosine decay; returns
     [(step, loss)] sampled every 1000 steps."""
@@ -356,7 +373,7 @@ def fit(mlp, X, Y, steps, batch=16384, lr=2e-3, lr_final=1e-5, warmup=500, cw_he
             lr_t = lr_final + 0.5 * (lr - lr_final) * (1 + np.cos(np.pi * c))
         mlp.grad(X[idx], Y[idx], cw)
         mlp.adam(lr_t)
-        if it % 1000 == 0 or it == steps - 1:
+        if it % 10000 == 0 or it == steps - 1:
             hist.append((it, float(mlp.loss.item())))
             log("step %6d  loss %.3e  lr %.2e  %.1f s" % (it, hist[-1][1], lr_t,
                                                          time.perf_counter() - t0))
@@ -364,26 +381,52 @@ def fit(mlp, X, Y, steps, batch=16384, lr=2e-3, lr_final=1e-5, warmup=500, cw_he
     return hist, time.perf_counter() - t0
 
 
-def validation_tables(g, n_slabs, n_psd=40, seed=1, precision="fp32", chunk=2000):
-    """Random slabs and random PSDs inside the grid's box: (queries, rows) host."""
+def validation_tables(g, n_slabs, n_psd=40, seed=1, precision="fp32", chunk=1000):
+    """Random slabs inside the grid's box, each with its own n_psd random
+    (r_mode, sigma_g) pairs (so every part of the box is sampled):
+    (queries, rows) on the host."""
+    t = _lib.torch()
     r = np.random.default_rng(seed)
     ns = int(n_slabs)
     v_mr = r.uniform(g["mr"].min(), g["mr"].max(), ns)
     v_mi = np.exp(r.uniform(np.log(g["mi"].min()), np.log(g["mi"].max()), ns))
     v_lam = np.exp(r.uniform(np.log(g["lam"].min()), np.log(g["lam"].max()), ns))
     v_gf = (1 - r.uniform(0, 1 - g["gf"].max() ** -3.0, ns)) ** (-1.0 / 3.0)
-    v_rm = np.exp(r.uniform(np.log(g["rmode"].min()), np.log(g["rmode"].max()), n_psd))
-    v_sg = r.uniform(g["sigma"].min(), g["sigma"].max(), n_psd)
-    vsw = LutSweep(g["r"], None, None, pairs=list(zip(v_rm, v_sg)), precision=precision)
+    v_rm = np.exp(r.uniform(np.log(g["rmode"].min()), np.log(g["rmode"].max()), (ns, n_psd)))
+    v_sg = r.uniform(g["sigma"].min(), g["sigma"].max(), (ns, n_psd))
+    vsw = LutSweep(g["r"], [1.0], [2.0], precision=precision)
     vq, vrows = [], []
     for a in range(0, ns, chunk):
         b = min(ns, a + chunk)
-        out = vsw.run(v_lam[a:b], v_mr[a:b] + 1j * v_mi[a:b], v_gf[a:b])
+        w = vsw.weights_dev(v_rm[a:b].ravel(), v_sg[a:b].ravel())
+        out = vsw.run(v_lam[a:b], v_mr[a:b] + 1j * v_mi[a:b], v_gf[a:b], w=w)
         vrows.append(out.view(-1, N_OUT).cpu().numpy())
+        del w
         k = np.arange(a, b).repeat(n_psd)
-        p = np.tile(np.arange(n_psd), b - a)
-        vq.append(make_queries(v_rm[p], v_sg[p], v_mr[k], v_mi[k], v_lam[k], v_gf[k]))
+        vq.append(make_queries(v_rm[a:b].ravel(), v_sg[a:b].ravel(), v_mr[k], v_mi[k], v_lam[k],
+                               v_gf[k]))
+    t.cuda.synchronize()
     return np.concatenate(vq), np.concatenate(vrows)
+
+
+def regate(sur, g=None, n_val_slabs=60000, seed=11, safety=1.0, min_count=3, precision="fp32",
+           log=print):
+    """Re-fit the trust gate of a trained surrogate on fresh validation slabs."""
+    g = dict(full_grid() if g is None else g)
+    t0 = time.perf_counter()
+    vq, vrows = validation_tables(g, n_val_slabs, seed=seed, precision=precision)
+    rep, err = evaluate(sur, vq, vrows)
+    rep["cells_trusted"], rep["val_frac_trusted"] = fit_gate(sur, vq, err, safety, min_count)
+    rep["val_s"] = time.perf_counter() - t0
+    log("gate: median err %.2e, %.1f%% within bound; %.1f%% of cells trusted, %.1f%% of "
+        "validation entries in trusted cells" % (rep["val_err_median"],
+                                                 100 * rep["val_frac_within_bound"],
+                                                 100 * rep["cells_trusted"],
+                                                 100 * rep["val_frac_trusted"]))
+    info = dict(sur.info)
+    info.update(rep)
+    sur.info = info
+    return rep
 
 
 def evaluate(sur, vq, vrows):
@@ -427,15 +470,16 @@ def fit_gate(sur, vq, err, safety=1.0, min_count=3):
 
 
 def train(grid=None, train_stride=(1, 1, 1, 1), n_val_slabs=20000, steps=20000, batch=16384,
-          lr=2e-3, lr_final=1e-5, warmup=500, cw_head=20.0, seed=0, bound=1e-3, safety=1.0,
-          min_count=3, cells=GATE_CELLS, log=print, precision="fp32", n_aug_psd=0,
-          weight_terms=1):
+          lr=2e-3, lr_final=1e-5, warmup=500, cw_head=1000.0, seed=0, bound=1e-3, safety=1.0,
+          min_count=3, cells=GATE_CELLS, log=print, precision="fp32", n_aug_psd=40,
+          weight_terms=2, init=None):
     """Build training tables with the GPU sweep, train the network with the HIP
     kernels, fit the gate on validation slabs; returns (Surrogate, report).
 
     train_stride subsamples the (m_r, m_i, lambda, gf) axes of the sweep grid
     (1 = every value).  n_aug_psd > 0 adds a second pass over the same slabs
-    with that many random (r_mode, sigma_g) pairs inside the grid's ranges."""
+    with that many random (r_mode, sigma_g) pairs inside the grid's ranges.
+    init: a Surrogate to continue training (fresh Adam state)."""
     t = _lib.torch()
     g = dict(full_grid() if grid is None else grid)
     mr, mi, lam, gf = (g[k][::s] for k, s in zip(("mr", "mi", "lam", "gf"), train_stride))
@@ -454,14 +498,20 @@ def train(grid=None, train_stride=(1, 1, 1, 1), n_val_slabs=20000, steps=20000, 
         del Q2, rows2
         rep["sweep_s"] += dt2
     rep["train_entries"] = int(rows.shape[0])
-    sur = new_surrogate(g, Q, rows, seed=seed, cells=cells, bound=bound)
-    sur.mlp.weight_terms = int(weight_terms)
+    if init is not None:
+        # continue from a trained model (its standardisation and weight terms)
+        sur = init
+        prev_steps = float(sur.info.get("train_steps", 0.0))
+    else:
+        sur = new_surrogate(g, Q, rows, seed=seed, cells=cells, bound=bound)
+        sur.mlp.weight_terms = int(weight_terms)
+        prev_steps = 0.0
     X, Y = training_arrays(sur, Q, rows)
     del Q, rows
     hist, rep["train_s"] = fit(sur.mlp, X, Y, steps, batch=batch, lr=lr, lr_final=lr_final,
                                warmup=warmup, cw_head=cw_head, seed=seed, log=log)
-    rep["train_steps"], rep["batch"], rep["final_loss"] = steps, batch, hist[-1][1]
-    rep["weight_terms"] = weight_terms
+    rep["train_steps"], rep["batch"], rep["final_loss"] = prev_steps + steps, batch, hist[-1][1]
+    rep["weight_terms"] = sur.mlp.weight_terms
     del X, Y
     t0 = time.perf_counter()
     vq, vrows = validation_tables(g, n_val_slabs, seed=seed + 1, precision=precision)

@@ -18,23 +18,42 @@ import numpy as np
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--out", required=True)
-    ap.add_argument("--steps", type=int, default=100000)
+    ap.add_argument("--init", default=None,
+                    help="continue training this model (its standardisation is kept)")
+    ap.add_argument("--regate", default=None,
+                    help="instead of training: load this model, re-fit its gate on fresh "
+                         "validation slabs (--val-slabs) and write it to --out")
+    ap.add_argument("--steps", type=int, default=400000)
     ap.add_argument("--batch", type=int, default=16384)
     ap.add_argument("--lr", type=float, default=2e-3)
     ap.add_argument("--lr-final", type=float, default=1e-5)
     ap.add_argument("--stride", type=int, nargs=4, default=(1, 1, 1, 1),
                     help="subsampling of the m_r, m_i, lambda, gf axes")
-    ap.add_argument("--val-slabs", type=int, default=20000)
-    ap.add_argument("--cw-head", type=float, default=20.0)
+    ap.add_argument("--val-slabs", type=int, default=30000)
+    ap.add_argument("--cw-head", type=float, default=1000.0,
+                    help="loss weight of Qext, SSA, g relative to each moment output")
+    ap.add_argument("--aug-psd", type=int, default=40,
+                    help="random (r_mode, sigma_g) pairs of a second pass over the grid's slabs")
+    ap.add_argument("--weight-terms", type=int, default=2,
+                    help="BF16 terms per network weight on the matrix cores (1 or 2)")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--precision", default="fp32",
                     help="S1/S2 precision of the training tables (Q and g are FP64 either way)")
     args = ap.parse_args(argv)
     from geosmie_amd import surrogate as S
     t0 = time.time()
+    if args.regate:
+        sur = S.Surrogate.load(args.regate)
+        rep = S.regate(sur, n_val_slabs=args.val_slabs, seed=args.seed + 11,
+                       log=lambda s: print(s, flush=True))
+        sur.save(args.out)
+        print(json.dumps(rep), flush=True)
+        return
     sur, rep = S.train(train_stride=tuple(args.stride), n_val_slabs=args.val_slabs,
                        steps=args.steps, batch=args.batch, lr=args.lr, lr_final=args.lr_final,
                        cw_head=args.cw_head, seed=args.seed, precision=args.precision,
+                       n_aug_psd=args.aug_psd, weight_terms=args.weight_terms,
+                       init=S.Surrogate.load(args.init) if args.init else None,
                        log=lambda s: print(s, flush=True))
     rep["total_s"] = time.time() - t0
     sur.info = rep
