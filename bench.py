@@ -154,16 +154,25 @@ def time_pipeline(pipe, steps, warmup):
     return (t1 - t0) * 1e3 / steps, kt, (t3 - t2) * 1e3 / steps
 
 
-def secondary(steps=5, warmup=1):
+def secondary(steps=5, warmup=1, which=("homog", "largex")):
     from geosmie_amd import autotune
     from geosmie_amd.pipeline import RunopticsChain
     out = []
+    if "homog" in which:
+        out.append(_secondary_homog(steps, warmup, autotune, RunopticsChain))
+    if "largex" in which:
+        out.append(_secondary_largex(steps, warmup, autotune, RunopticsChain))
+    return out
+
+
+def _secondary_homog(steps, warmup, autotune, RunopticsChain):
     x, lam, ang, w, m = homog_lut_workload()
     pipe = RunopticsChain(x, m, ang, w, lam, [(i * 2000, 2000) for i in range(10)])
-    tuned = autotune.tune(pipe.ctx, pipe.step, knobs=("s12_waves", "psd_chunk"), reps=3)
+    tuned = autotune.tune(pipe.ctx, pipe.step, knobs=("s12_points", "s12_waves", "psd_chunk"),
+                          reps=3)
     ms_eager, kt, ms = time_pipeline(pipe, steps, warmup)
     s12_flop = 16.0 * pipe.sum_nmax * ang.size
-    out.append({"workload": "plain homogeneous FP64 Mie, 2000 radii 0.005-20 um x 10 wavelengths "
+    return ({"workload": "plain homogeneous FP64 Mie, 2000 radii 0.005-20 um x 10 wavelengths "
                             "0.3-2.5 um, m=1.53+0.005i, 181 angles, lognormal r_mode 0.5 um "
                             "sigma_g 2.0 integrated per wavelength (Qext/Qsca/g, P11/P12/P33/P34)",
                 "points": int(x.size), "ms_per_step": ms, "value": x.size / (ms * 1e-3),
@@ -171,7 +180,9 @@ def secondary(steps=5, warmup=1):
                 "autotune": {k: v["best"] for k, v in tuned.items()},
                 "unit": "points/s", "kernel_ms": kt,
                 "s12_tflops": s12_flop / (kt.get("s12", ms) * 1e-3) / 1e12})
-    del pipe
+
+
+def _secondary_largex(steps, warmup, autotune, RunopticsChain):
     x, m, ang = largex_workload()
     lam_row = np.repeat(np.linspace(0.3, 1.0, 8), 1)
     from geosmie_amd.lut import getLogNormPSD
@@ -182,7 +193,7 @@ def secondary(steps=5, warmup=1):
         w[i, i * 5000:(i + 1) * 5000] = p / p.sum()
     pipe = RunopticsChain(x, m, ang, w, lam_row, [(i * 5000, 5000) for i in range(8)],
                           precision=0, gsf_ng=300)
-    tuned3 = autotune.tune(pipe.ctx, pipe.step, knobs=("s12_waves",), reps=2)
+    tuned3 = autotune.tune(pipe.ctx, pipe.step, knobs=("s12_points", "s12_waves"), reps=2)
     s12_flop = 16.0 * pipe.sum_nmax * ang.size
     modes = []
     ref = None
@@ -218,7 +229,7 @@ def secondary(steps=5, warmup=1):
             }
         modes.append(ent)
     fp64 = modes[0]
-    out.append({"workload": "large-x homogeneous Mie, 5000 radii 0.1-100 um x 8 wavelengths "
+    return ({"workload": "large-x homogeneous Mie, 5000 radii 0.1-100 um x 8 wavelengths "
                             "0.3-1.0 um (x <= 2094), m_r U(1.33,1.60), m_i 10^U(-6,-2), 1801 "
                             "angles, S1/S2 on MFMA (FP64; FP32 and split-BF16 below), lognormal "
                             "integration (r_mode 5 um, sigma_g 2.0), 300 GSF moments",
@@ -227,7 +238,6 @@ def secondary(steps=5, warmup=1):
                 "s12_flop": s12_flop, "s12_tflops": fp64["s12_tflops"],
                 "autotune": {k: v["best"] for k, v in tuned3.items()},
                 "precision_modes": modes[1:]})
-    return out
 
 
 SURROGATE_FILE = os.path.join(ROOT, "geosmie_amd", "data", "surrogate_lut.npz")
@@ -321,6 +331,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calib", action="store_true",
                     help="also run 3 device copies of 512 MiB (PMC byte-counter calibration)")
+    ap.add_argument("--secondary-only", choices=("homog", "largex", "lut"), default=None,
+                    help="run only this secondary measurement (for profiling) and print it")
     ap.add_argument("--no-secondary", action="store_true",
                     help="skip the homogeneous-LUT and large-x secondary measurements")
     args = ap.parse_args()
@@ -335,6 +347,11 @@ def main():
     from geosmie_amd._lib import check, ptr
 
     torch.cuda.set_device(0)
+    if args.secondary_only:
+        res = lut_surrogate() if args.secondary_only == "lut" else \
+            secondary(which=(args.secondary_only,))[0]
+        print(json.dumps({"secondary_only": args.secondary_only, "result": res}))
+        return
     x, y, m1, m2 = coated_workload()
     n = x.size
     flops, sum_nmax, sum_nmx = coated_flops(x, y, m1, m2)

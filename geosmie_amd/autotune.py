@@ -9,6 +9,7 @@ Knobs (include/geosmie.h, enum gm_tune_param):
   mie_grid_mult  persistent Mie grid as a multiple of the resident wavefronts
   s12_waves      wavefronts (64 angles each) per S1/S2 block
   psd_chunk      points per split-K chunk of the few-PSD contraction
+  s12_points     points per wavefront of the FP64 S1/S2 kernel
 (mlp_weight_terms, the BF16 terms per network weight, is a precision
 setting reached through the same call; it is not tuned for speed.)
 """
@@ -21,6 +22,7 @@ from ._lib import check
 KNOBS = {"mie_grid_mult": (0, (1, 2, 4)),
          "s12_waves": (1, (1, 2, 4)),
          "psd_chunk": (2, (64, 128, 256, 512)),
+         "s12_points": (4, (8, 16)),
          # a precision setting rather than a shape; set explicitly, never tuned for speed
          "mlp_weight_terms": (3, (1, 2))}
 
@@ -35,8 +37,8 @@ def set_knob(ctx, name, value):
     check(_lib.lib().gm_ctx_tune(ctx, KNOBS[name][0], int(value)), "gm_ctx_tune")
 
 
-def tune(ctx, workload, knobs=("mie_grid_mult", "s12_waves", "psd_chunk"), reps=3, warmup=1,
-         candidates=None):
+def tune(ctx, workload, knobs=("mie_grid_mult", "s12_points", "s12_waves", "psd_chunk"), reps=3,
+         warmup=1, candidates=None):
     """Returns {knob: {"best": value, "ms": {value: ms per pass}}}; the context
     keeps the best values."""
     t = _lib.torch()

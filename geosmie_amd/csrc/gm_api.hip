@@ -168,6 +168,11 @@ int gm_ctx_tune(gm_ctx* ctx, int param, int value) {
         return gm_fail(GM_EINVAL, "gm_ctx_tune: split-K chunk %d out of [16, 2^20]", value);
       ctx->tune_psd_chunk = value;
       return GM_OK;
+    case GM_TUNE_S12_POINTS:
+      if (value != 8 && value != 16)
+        return gm_fail(GM_EINVAL, "gm_ctx_tune: S1/S2 points per wavefront must be 8 or 16");
+      ctx->tune_s12_points = value;
+      return GM_OK;
     case GM_TUNE_MLP_WEIGHT_TERMS:
       if (value != 1 && value != 2)
         return gm_fail(GM_EINVAL, "gm_ctx_tune: MLP weight terms must be 1 or 2");
@@ -185,6 +190,7 @@ int gm_ctx_get_tune(gm_ctx* ctx, int param, int* value) {
     case GM_TUNE_S12_WAVES: *value = ctx->tune_s12_waves; return GM_OK;
     case GM_TUNE_PSD_CHUNK: *value = ctx->tune_psd_chunk; return GM_OK;
     case GM_TUNE_MLP_WEIGHT_TERMS: *value = ctx->tune_mlp_wterms; return GM_OK;
+    case GM_TUNE_S12_POINTS: *value = ctx->tune_s12_points; return GM_OK;
     default: return gm_fail(GM_EINVAL, "gm_ctx_get_tune: unknown parameter %d", param);
   }
 }

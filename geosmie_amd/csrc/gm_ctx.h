@@ -24,6 +24,7 @@ struct gm_ctx {
   // launch shapes chosen by gm_ctx_tune (an autotuner's knobs)
   int tune_mie_grid_mult = 1;   // persistent Mie grid = multiple of resident wavefronts
   int tune_s12_waves = 4;       // wavefronts (64 angles each) per S1/S2 block
+  int tune_s12_points = 8;      // points per wavefront of the FP64 S1/S2 kernel (8 or 16)
   int tune_psd_chunk = 128;     // points per split-K chunk of the few-PSD contraction
   int tune_mlp_wterms = 1;      // BF16 terms per MLP weight (1 = BF16 weights, 2 = hi + lo)
   Buf status;                                    // int32 flags + int32 max_nmax + scan scratch

@@ -62,6 +62,9 @@ __device__ __forceinline__ void s12_store(int mode, int64_t p, int j, int n_ang,
   }
 }
 
+// MT = M-tiles (8 points each) per wavefront: with MT = 2 every angle-table
+// load feeds twice as many MFMAs.
+template <int MT>
 __global__ __launch_bounds__(256) void k_s12_f64(
     int64_t n, const int64_t* __restrict__ off, const int32_t* __restrict__ nmax,
     const double* __restrict__ an, const double* __restrict__ bn, int n_ang,
@@ -69,39 +72,53 @@ __global__ __launch_bounds__(256) void k_s12_f64(
     double* __restrict__ out0, double* __restrict__ out1) {
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
-  const int64_t p0 = (int64_t)blockIdx.x * 8;
+  const int64_t p0 = (int64_t)blockIdx.x * 8 * MT;
   const int j0 = (blockIdx.y * (blockDim.x >> 6) + wave) * 64;  // 64 angles per wavefront
   if (j0 >= n_ang) return;  // whole wavefront past the last angle
   const int i = lane & 15;   // A row / B column / D column
   const int kq = lane >> 4;  // k within the 4-step, D row group
   // The f64 16x16x4 result row of (lane group g, element r) is g + 4r.  A row
-  // i = g + 4r is assigned to point p0 + 2g + (r >> 1), component r & 1, so
-  // lane group g ends up holding (re, im) of points p0+2g and p0+2g+1.
-  const int64_t pa = p0 + 2 * (i & 3) + (i >> 3);
-  const bool pa_ok = pa < n;
-  const int nm_a = pa_ok ? nmax[pa] : 0;
-  const int64_t oa = pa_ok ? off[pa] : 0;
+  // i = g + 4r of M-tile mt is assigned to point p0 + 8 mt + 2g + (r >> 1),
+  // component r & 1, so lane group g ends up holding (re, im) of points
+  // p0 + 8 mt + 2g and p0 + 8 mt + 2g + 1.
   const int comp = (i >> 2) & 1;
-  // tile K = max nmax of the 8 points
+  int nm_a[MT];
+  int64_t oa[MT];
+#pragma unroll
+  for (int mt = 0; mt < MT; ++mt) {
+    const int64_t pa = p0 + 8 * mt + 2 * (i & 3) + (i >> 3);
+    const bool ok = pa < n;
+    nm_a[mt] = ok ? nmax[pa] : 0;
+    oa[mt] = ok ? off[pa] : 0;
+  }
+  // tile K = max nmax of the 8 MT points
   int kmax = 0;
-  for (int q = 0; q < 8; ++q) {
+  for (int q = 0; q < 8 * MT; ++q) {
     int64_t pq = p0 + q;
     if (pq < n) kmax = max(kmax, nmax[pq]);
   }
   kmax = min(kmax, nmax_cap);
-  double4_ accp[4], accm[4];
-  for (int t = 0; t < 4; ++t) {
-    accp[t] = (double4_){0.0, 0.0, 0.0, 0.0};
-    accm[t] = (double4_){0.0, 0.0, 0.0, 0.0};
-  }
+  double4_ accp[MT][4], accm[MT][4];
+#pragma unroll
+  for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      accp[mt][t] = (double4_){0.0, 0.0, 0.0, 0.0};
+      accm[mt][t] = (double4_){0.0, 0.0, 0.0, 0.0};
+    }
   for (int k0 = 0; k0 < kmax; k0 += 4) {
     const int kk = k0 + kq;
-    double ua = 0.0, va = 0.0;
-    if (kk < nm_a) {
-      double ar = an[2 * (oa + kk) + comp];
-      double br = bn[2 * (oa + kk) + comp];
-      ua = ar + br;
-      va = ar - br;
+    double ua[MT], va[MT];
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt) {
+      ua[mt] = 0.0;
+      va[mt] = 0.0;
+      if (kk < nm_a[mt]) {
+        const double ar = an[2 * (oa[mt] + kk) + comp];
+        const double br = bn[2 * (oa[mt] + kk) + comp];
+        ua[mt] = ar + br;
+        va[mt] = ar - br;
+      }
     }
     const bool krow = kk < kmax;
 #pragma unroll
@@ -112,23 +129,29 @@ __global__ __launch_bounds__(256) void k_s12_f64(
         pp = pi_t[(size_t)kk * n_ang + j];
         tt = tau_t[(size_t)kk * n_ang + j];
       }
-      accp[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(ua, pp + tt, accp[t], 0, 0, 0);
-      accm[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(va, pp - tt, accm[t], 0, 0, 0);
+      const double P = pp + tt, M = pp - tt;
+#pragma unroll
+      for (int mt = 0; mt < MT; ++mt) {
+        accp[mt][t] = __builtin_amdgcn_mfma_f64_16x16x4f64(ua[mt], P, accp[mt][t], 0, 0, 0);
+        accm[mt][t] = __builtin_amdgcn_mfma_f64_16x16x4f64(va[mt], M, accm[mt][t], 0, 0, 0);
+      }
     }
   }
-  // epilogue: elements (0,1) = point p0+2kq (re,im), (2,3) = point p0+2kq+1 (re,im)
+  // epilogue: elements (0,1) = point p0+8mt+2kq (re,im), (2,3) = the next point
 #pragma unroll
-  for (int t = 0; t < 4; ++t) {
-    const int j = j0 + t * 16 + i;
-    if (j >= n_ang) continue;
+  for (int mt = 0; mt < MT; ++mt)
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      const int64_t p = p0 + 2 * kq + h;
-      if (p >= n) continue;
-      s12_store(mode, p, j, n_ang, cd{accp[t][2 * h], accp[t][2 * h + 1]},
-                cd{accm[t][2 * h], accm[t][2 * h + 1]}, out0, out1);
+    for (int t = 0; t < 4; ++t) {
+      const int j = j0 + t * 16 + i;
+      if (j >= n_ang) continue;
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const int64_t p = p0 + 8 * mt + 2 * kq + h;
+        if (p >= n) continue;
+        s12_store(mode, p, j, n_ang, cd{accp[mt][t][2 * h], accp[mt][t][2 * h + 1]},
+                  cd{accm[mt][t][2 * h], accm[mt][t][2 * h + 1]}, out0, out1);
+      }
     }
-  }
 }
 
 // ---- reduced-precision paths -------------------------------------------------
@@ -340,8 +363,14 @@ int launch_s12(gm_ctx* ctx, const char* who, int64_t n, const int64_t* off, cons
   size_t slot;
   if (precision == GM_PREC_FP64) {
     ctx->mark_begin(gm::kKS12, s, &slot);
-    hipLaunchKernelGGL(gm::k_s12_f64, grid, block, 0, s, n, off, nmax, an, bn, n_ang, pi_t,
-                       tau_t, nmax_cap, mode, out0, out1);
+    if (ctx->tune_s12_points == 16) {
+      dim3 grid16((unsigned)((n + 15) / 16), grid.y);
+      hipLaunchKernelGGL(gm::k_s12_f64<2>, grid16, block, 0, s, n, off, nmax, an, bn, n_ang,
+                         pi_t, tau_t, nmax_cap, mode, out0, out1);
+    } else {
+      hipLaunchKernelGGL(gm::k_s12_f64<1>, grid, block, 0, s, n, off, nmax, an, bn, n_ang, pi_t,
+                         tau_t, nmax_cap, mode, out0, out1);
+    }
     ctx->mark_end(slot, s);
   } else {
     const int kpad = (nmax_cap + gm::kLpK - 1) / gm::kLpK * gm::kLpK;

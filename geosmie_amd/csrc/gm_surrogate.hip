@@ -105,17 +105,23 @@ extern "C" {
 // See include/geosmie.h.
 int gm_surrogate_prep(gm_ctx* ctx, int64_t n, const float* q, const float* feat_mean,
                       const float* feat_std, const float* box_lo, const float* box_hi,
-                      const int* n_cells_host, const uint8_t* cell_ok, double m_water_re,
-                      float* x, int32_t* flags, gm_stream stream) {
-  if (!ctx || n < 0 || !n_cells_host) return gm_fail(GM_EINVAL, "gm_surrogate_prep: bad arguments");
+                      const int* n_cells_host, const uint8_t* cell_ok, int64_t n_cell_ok,
+                      double m_water_re, float* x, int32_t* flags, gm_stream stream) {
+  if (!n_cells_host) return gm_fail(GM_EINVAL, "gm_surrogate_prep: n_cells is NULL");
   gm::Cells cells;
   int64_t total = 1;
   for (int j = 0; j < gm::kSurrIn; ++j) {
-    if (n_cells_host[j] < 1) return gm_fail(GM_EINVAL, "gm_surrogate_prep: n_cells must be >= 1");
+    if (n_cells_host[j] < 1 || n_cells_host[j] > (1 << 16))
+      return gm_fail(GM_EINVAL, "gm_surrogate_prep: n_cells must be in [1, 65536]");
     cells.n[j] = n_cells_host[j];
     total *= n_cells_host[j];
+    if (total > ((int64_t)1 << 31)) return gm_fail(GM_EINVAL, "gm_surrogate_prep: too many cells");
   }
-  if (total > ((int64_t)1 << 31)) return gm_fail(GM_EINVAL, "gm_surrogate_prep: too many cells");
+  // every cell index the kernel can form must lie inside cell_ok
+  if (n_cell_ok != total)
+    return gm_fail(GM_EINVAL, "gm_surrogate_prep: cell_ok has %lld entries, the cell grid %lld",
+                   (long long)n_cell_ok, (long long)total);
+  if (!ctx || n < 0) return gm_fail(GM_EINVAL, "gm_surrogate_prep: bad arguments");
   if (n == 0) return GM_OK;
   if (!q || !feat_mean || !feat_std || !box_lo || !box_hi || !cell_ok || !x || !flags)
     return gm_fail(GM_EINVAL, "gm_surrogate_prep: NULL array");

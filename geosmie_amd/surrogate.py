@@ -173,6 +173,19 @@ class Surrogate(object):
         self._ccells = (ctypes.c_int * 6)(*self.cells)
         self._fallback = None
 
+    def set_cells(self, cells, cell_ok=None):
+        """Replace the gate's cell grid (and its trusted-cell table, all
+        trusted when not given) on the host and the device together."""
+        cells = tuple(int(c) for c in cells)
+        n_cells = int(np.prod(cells))
+        ok = np.ones(n_cells, np.uint8) if cell_ok is None else np.asarray(cell_ok, np.uint8).ravel()
+        if len(cells) != 6 or ok.size != n_cells:
+            raise ValueError("cell_ok must have prod(cells) entries for 6 cell counts")
+        self.cells = cells
+        self.cell_ok = ok
+        self._cell_ok = _lib.to_dev(ok, np.uint8)
+        self._ccells = (ctypes.c_int * 6)(*cells)
+
     # -- network path ------------------------------------------------------
     def predict(self, q):
         """Network rows [n, 35] (float32, device) and gate flags [n] (int32, device)."""
@@ -186,8 +199,8 @@ class Surrogate(object):
         L, h, s = _lib.lib(), _lib.ctx(), stream()
         check(L.gm_surrogate_prep(h, n, ptr(q), ptr(d["feat_mean"]), ptr(d["feat_std"]),
                                   ptr(d["box_lo"]), ptr(d["box_hi"]), self._ccells,
-                                  ptr(self._cell_ok), M_WATER.real, ptr(x), ptr(flags), s),
-              "gm_surrogate_prep")
+                                  ptr(self._cell_ok), self._cell_ok.numel(), M_WATER.real,
+                                  ptr(x), ptr(flags), s), "gm_surrogate_prep")
         y = self.mlp.forward(x)
         rows = t.empty_like(y)
         check(L.gm_surrogate_post(h, n, self.dims[-1], ptr(y), ptr(d["tgt_mean"]),
@@ -337,7 +350,8 @@ def training_arrays(sur, Q, rows):
     d = sur._dev
     check(L.gm_surrogate_prep(h, n, ptr(d_q), ptr(d["feat_mean"]), ptr(d["feat_std"]),
                               ptr(d["box_lo"]), ptr(d["box_hi"]), sur._ccells, ptr(sur._cell_ok),
-                              M_WATER.real, ptr(X), ptr(flags), s), "gm_surrogate_prep")
+                              sur._cell_ok.numel(), M_WATER.real, ptr(X), ptr(flags), s),
+          "gm_surrogate_prep")
     check(L.gm_surrogate_targets(h, n, N_OUT, ptr(rows), ptr(d["tgt_mean"]), ptr(d["tgt_std"]),
                                  ptr(Y), s), "gm_surrogate_targets")
     return X, Y
@@ -410,9 +424,12 @@ def validation_tables(g, n_slabs, n_psd=40, seed=1, precision="fp32", chunk=1000
 
 
 def regate(sur, g=None, n_val_slabs=60000, seed=11, safety=1.0, min_count=3, precision="fp32",
-           log=print):
-    """Re-fit the trust gate of a trained surrogate on fresh validation slabs."""
+           cells=None, log=print):
+    """Re-fit the trust gate of a trained surrogate on fresh validation slabs
+    (optionally on a new cell grid)."""
     g = dict(full_grid() if g is None else g)
+    if cells is not None:
+        sur.set_cells(cells)
     t0 = time.perf_counter()
     vq, vrows = validation_tables(g, n_val_slabs, seed=seed, precision=precision)
     rep, err = evaluate(sur, vq, vrows)
@@ -464,8 +481,7 @@ def fit_gate(sur, vq, err, safety=1.0, min_count=3):
     np.maximum.at(cmax, ci, err)
     cnt = np.bincount(ci, minlength=n_cells)
     ok = (cnt >= min_count) & (cmax <= sur.bound / safety)
-    sur.cell_ok = ok.astype(np.uint8)
-    sur._cell_ok = _lib.to_dev(sur.cell_ok, np.uint8)
+    sur.set_cells(sur.cells, ok)
     return float(ok.mean()), float(ok[ci].mean())
 
 

@@ -30,6 +30,8 @@ def main(argv=None):
     ap.add_argument("--stride", type=int, nargs=4, default=(1, 1, 1, 1),
                     help="subsampling of the m_r, m_i, lambda, gf axes")
     ap.add_argument("--val-slabs", type=int, default=30000)
+    ap.add_argument("--cells", type=int, nargs=6, default=None,
+                    help="gate cells along ln r_mode, sigma_g, m_r, ln m_i, ln lambda, f_w")
     ap.add_argument("--cw-head", type=float, default=1000.0,
                     help="loss weight of Qext, SSA, g relative to each moment output")
     ap.add_argument("--aug-psd", type=int, default=40,
@@ -44,7 +46,7 @@ def main(argv=None):
     t0 = time.time()
     if args.regate:
         sur = S.Surrogate.load(args.regate)
-        rep = S.regate(sur, n_val_slabs=args.val_slabs, seed=args.seed + 11,
+        rep = S.regate(sur, n_val_slabs=args.val_slabs, seed=args.seed + 11, cells=args.cells,
                        log=lambda s: print(s, flush=True))
         sur.save(args.out)
         print(json.dumps(rep), flush=True)

@@ -77,8 +77,9 @@ enum gm_tune_param {
   GM_TUNE_MIE_GRID_MULT = 0,  /* persistent Mie grid = this x resident wavefronts (1..16; 1) */
   GM_TUNE_S12_WAVES = 1,      /* wavefronts of 64 angles per S1/S2 block (1, 2, 4; 4) */
   GM_TUNE_PSD_CHUNK = 2,      /* points per split-K chunk, few-PSD contraction (16..2^20; 128) */
-  GM_TUNE_MLP_WEIGHT_TERMS = 3 /* precision, not shape: BF16 terms per MLP weight on the matrix
-                                  cores, 1 = BF16 weights (default), 2 = hi + lo BF16 pair */
+  GM_TUNE_MLP_WEIGHT_TERMS = 3, /* precision, not shape: BF16 terms per MLP weight on the matrix
+                                   cores, 1 = BF16 weights (default), 2 = hi + lo BF16 pair */
+  GM_TUNE_S12_POINTS = 4        /* points per wavefront of the FP64 S1/S2 kernel (8, 16; 8) */
 };
 int gm_ctx_tune(gm_ctx* ctx, int param, int value);
 int gm_ctx_get_tune(gm_ctx* ctx, int param, int* value);
@@ -283,16 +284,19 @@ int gm_mlp_adam(gm_ctx* ctx, int64_t n_params, float* params, const float* grads
  * number distribution (r_eff = gf r_mode exp(2.5 ln^2 sigma_g)), its geometric
  * width, the dry refractive index, the wavelength [um] and the water volume
  * fraction f_w = 1 - gf^-3 of the humidity growth (dointegration.py:497-507).
- * x [n][6]: network features (ln r_eff, sigma_g, m_r, ln m_i, ln lambda, f_w)
- * standardised with feat_mean/feat_std.  flags [n]: 0 = served by the network,
+ * x [n][6]: network features (ln(2 pi r_eff / lambda), sigma_g, Re m_mix,
+ * ln Im m_mix, ln(gf / lambda), f_w) with the volume-mixed index
+ * m_mix = m_w + (m - m_w)(1 - f_w) (real water index m_water_re), standardised
+ * with feat_mean/feat_std.  flags [n]: 0 = served by the network,
  * 1 = outside the box [box_lo, box_hi] of the gate coordinates (ln r_mode,
  * sigma_g, m_r, ln m_i, ln lambda, f_w), 2 = inside but in a cell of the
- * uniform n_cells_host[6] grid whose cell_ok byte is 0.  All arrays but
- * n_cells_host are device arrays. */
+ * uniform n_cells_host[6] grid whose cell_ok byte is 0.  cell_ok holds
+ * n_cell_ok bytes, which must equal the product of n_cells_host (checked).
+ * All arrays but n_cells_host are device arrays. */
 int gm_surrogate_prep(gm_ctx* ctx, int64_t n, const float* q, const float* feat_mean,
                       const float* feat_std, const float* box_lo, const float* box_hi,
-                      const int* n_cells_host, const uint8_t* cell_ok, float* x, int32_t* flags,
-                      gm_stream stream);
+                      const int* n_cells_host, const uint8_t* cell_ok, int64_t n_cell_ok,
+                      double m_water_re, float* x, int32_t* flags, gm_stream stream);
 
 /* Network outputs y [n][n_out] (standardised) -> table rows [n][n_out]
  * (Qext, SSA, g, alpha1_0..): Qext = exp(t0), SSA = min(exp(t1), 1),

@@ -14,21 +14,23 @@ def test_knobs_do_not_change_results(gpu):
     h = _lib.ctx()
     x = np.logspace(-2, 2.5, 700)
     ref = None
-    for mult in (1, 2, 4):
-        for waves in (1, 2, 4):
-            autotune.set_knob(h, "mie_grid_mult", mult)
-            autotune.set_knob(h, "s12_waves", waves)
-            r = mie_batch(x, m=1.5 + 0.02j, coeffs=True)
-            s1, s2 = s12_batch(r["an"][::37], r["bn"][::37],
-                               cost=np.cos(np.radians(np.linspace(0, 180, 300))))
-            got = (np.stack([r[k] for k in ("qext", "qsca", "asy")]), s1, s2)
-            if ref is None:
-                ref = got
-            else:
-                for a, b in zip(ref, got):
-                    assert np.array_equal(a, b)
+    for mult, waves, pts in [(1, 4, 8), (2, 4, 8), (4, 4, 8), (1, 1, 8), (1, 2, 16),
+                             (2, 4, 16), (1, 1, 16)]:
+        autotune.set_knob(h, "mie_grid_mult", mult)
+        autotune.set_knob(h, "s12_waves", waves)
+        autotune.set_knob(h, "s12_points", pts)
+        r = mie_batch(x, m=1.5 + 0.02j, coeffs=True)
+        s1, s2 = s12_batch(r["an"][::37], r["bn"][::37],
+                           cost=np.cos(np.radians(np.linspace(0, 180, 300))))
+        got = (np.stack([r[k] for k in ("qext", "qsca", "asy")]), s1, s2)
+        if ref is None:
+            ref = got
+        else:
+            for a, b in zip(ref, got):
+                assert np.array_equal(a, b)
     autotune.set_knob(h, "mie_grid_mult", 1)
     autotune.set_knob(h, "s12_waves", 4)
+    autotune.set_knob(h, "s12_points", 8)
 
 
 def test_tune_picks_a_candidate(gpu):

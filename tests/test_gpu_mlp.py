@@ -187,3 +187,17 @@ def test_targets_post_roundtrip(gpu):
     b = back.cpu().numpy()
     np.testing.assert_allclose(b[:, :3], rows[:, :3], rtol=1e-5)
     np.testing.assert_allclose(b[:, 3:], rows[:, 3:], rtol=1e-5, atol=1e-5)
+
+
+def test_gate_on_a_finer_cell_grid(gpu):
+    """Replacing the cell grid replaces the device cell table with it (a table
+    sized for the old grid would be indexed out of bounds)."""
+    sur = _toy_surrogate()
+    fine = (16, 6, 6, 12, 20, 6)
+    r = np.random.default_rng(9)
+    sur.set_cells(fine, (r.uniform(size=int(np.prod(fine))) < 0.7).astype(np.uint8))
+    assert sur._cell_ok.numel() == int(np.prod(fine))
+    q = _queries(4000, 5)
+    _, flags = sur.predict(q)
+    want = _host_flags(sur, q.astype(np.float64))
+    assert np.mean(flags.cpu().numpy() == want) > 0.999

@@ -48,3 +48,19 @@ def test_no_gpu_raises_loudly():
     from geosmie_amd.mie import mie_batch
     with pytest.raises(_lib.GMError):
         mie_batch([1.0], m=1.5)
+
+
+def test_surrogate_prep_rejects_cell_table_size_mismatch():
+    """The gate's cell table must cover every cell index the kernel can form:
+    a table sized for another cell grid is refused before anything launches."""
+    import ctypes
+    import pytest
+    from geosmie_amd import _lib
+    L = _lib.lib()
+    fine = (ctypes.c_int * 6)(16, 6, 6, 12, 20, 6)       # 829440 cells
+    rc = L.gm_surrogate_prep(None, 10, None, None, None, None, None, fine, None, 172800, 1.33,
+                             None, None, None)
+    assert rc < 0 and b"cell_ok has 172800 entries, the cell grid 829440" in L.gm_last_error()
+    rc = L.gm_surrogate_prep(None, 10, None, None, None, None, None, fine, None, 829440, 1.33,
+                             None, None, None)
+    assert rc < 0 and b"bad arguments" in L.gm_last_error()
