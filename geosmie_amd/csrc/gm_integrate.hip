@@ -55,6 +55,45 @@ __global__ void k_reduce_splits(int n_split, int64_t n_out, const double* __rest
   out[i] = s;
 }
 
+// Segmented contraction: row r integrates only its own points
+// [start_r, start_r + count_r):  out[r][c] = sum_p w[r][p] v[p][c].  Each
+// row's range is cut into chunks; block (column block, chunk q) writes the
+// chunk's partial sum, then k_reduce_segs adds a row's chunks in order.
+constexpr int kMaxSegs = 64;
+struct SegArgs {
+  int n;                          // rows
+  int chunk;                      // points per chunk
+  int64_t start[kMaxSegs], count[kMaxSegs];
+  int first[kMaxSegs + 1];        // first chunk of each row; first[n] = total
+};
+
+__global__ __launch_bounds__(256) void k_contract_segs(SegArgs a, int64_t n_pts, int n_cols,
+                                                       const double* __restrict__ w,
+                                                       const double* __restrict__ v,
+                                                       double* __restrict__ part) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  const int q = blockIdx.y;
+  int r = 0;
+  while (r + 1 < a.n && a.first[r + 1] <= q) ++r;
+  const int64_t p0 = a.start[r] + (int64_t)(q - a.first[r]) * a.chunk;
+  const int64_t p1 = min(p0 + (int64_t)a.chunk, a.start[r] + a.count[r]);
+  if (c >= n_cols) return;
+  const double* wr = w + (size_t)r * n_pts;
+  double acc = 0.0;
+  for (int64_t p = p0; p < p1; ++p) acc = fma(wr[p], v[(size_t)p * n_cols + c], acc);
+  part[(size_t)q * n_cols + c] = acc;
+}
+
+__global__ void k_reduce_segs(SegArgs a, int n_cols, const double* __restrict__ part,
+                              double* __restrict__ out) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  const int r = blockIdx.y;
+  if (c >= n_cols || r >= a.n) return;
+  double s = 0.0;
+  for (int q = a.first[r]; q < a.first[r + 1]; ++q) s += part[(size_t)q * n_cols + c];
+  out[(size_t)r * n_cols + c] = s;
+}
+
 // Batched PSDs on FP64 MFMA: a wavefront owns 16 PSDs x 64 columns.
 // A = w (k rows, p = K), B = v (p = K, c columns); D[g + 4r][j] in lane j + 16g.
 // Split-K: blockIdx.z covers points [z * kSplitChunkMfma, ...); partial sums
@@ -104,12 +143,14 @@ __global__ __launch_bounds__(256) void k_contract_mfma(int n_psd, int64_t n_pts,
 //   7 qb r^2   8 g qsca r^2   9 csca qsca r^2   10 cext qsca r^2
 // (csca = qsca pi r^2, cext = qext pi r^2, dointegration.py:1219-1220)
 constexpr int kNFeat = 11;
+// lam_pts (optional): a wavelength per point, for batches of several rows.
 __global__ void k_scalar_features(int64_t n, const double* __restrict__ xx, double lam,
+                                  const double* __restrict__ lam_pts,
                                   const double* __restrict__ props, double* __restrict__ v) {
   int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n) return;
   const double pi = 3.14159265358979323846;
-  double r = xx[p] * lam / (2.0 * pi);
+  double r = xx[p] * (lam_pts ? lam_pts[p] : lam) / (2.0 * pi);
   double r2 = r * r;
   double qext = props[p], qsca = props[n + p], qabs = props[2 * n + p], qb = props[3 * n + p];
   double g = props[4 * n + p];
@@ -132,11 +173,9 @@ __global__ void k_scalar_features(int64_t n, const double* __restrict__ xx, doub
 // order.  out (kNOut): num, area, volume, mass, rEff, rMass, qext, qsca, qabs,
 // qb, g, csca, cext, bsca, bext, bbck.
 constexpr int kNOut = 16;
-__global__ void k_scalar_finish(int n_modes, const double* __restrict__ sums,
-                                const double* __restrict__ fracs,
-                                const double* __restrict__ reff0, double rhop0, double rhop,
-                                double* __restrict__ out) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+__device__ void finish_modes(int n_modes, const double* __restrict__ sums,
+                             const double* __restrict__ fracs, const double* __restrict__ reff0,
+                             double rhop0, double rhop, double* __restrict__ out) {
   const double pi = 3.14159265358979323846;
   double acc[kNOut];
   for (int q = 0; q < kNOut; ++q) acc[q] = 0.0;
@@ -166,9 +205,26 @@ __global__ void k_scalar_finish(int n_modes, const double* __restrict__ sums,
     th[13] = 0.75 * th[7] * mc;               // bsca
     th[14] = 0.75 * th[6] * mc;               // bext
     th[15] = 0.75 * th[9] * mc / (4.0 * pi);  // bbck
-    for (int q = 0; q < kNOut; ++q) acc[q] += fracs[k] * th[q];
+    for (int q = 0; q < kNOut; ++q) acc[q] += (fracs ? fracs[k] : 1.0) * th[q];
   }
   for (int q = 0; q < kNOut; ++q) out[q] = acc[q];
+}
+
+__global__ void k_scalar_finish(int n_modes, const double* __restrict__ sums,
+                                const double* __restrict__ fracs,
+                                const double* __restrict__ reff0, double rhop0, double rhop,
+                                double* __restrict__ out) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  finish_modes(n_modes, sums, fracs, reff0, rhop0, rhop, out);
+}
+
+// One row per thread: rows are independent single-mode integrations.
+__global__ void k_scalar_finish_rows(int64_t n_rows, const double* __restrict__ sums,
+                                     const double* __restrict__ reff0, double rhop0, double rhop,
+                                     double* __restrict__ out) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n_rows) return;
+  finish_modes(1, sums + r * kNFeat, nullptr, reff0 + r, rhop0, rhop, out + r * kNOut);
 }
 
 // Phase-matrix normalisation, dointegration.py:978-988, per entry:
@@ -279,7 +335,7 @@ int gm_integrate_psd(gm_ctx* ctx, int64_t n_pts, int n_modes, const double* xx, 
   double* v = (double*)ctx->feat_tmp.ptr;
   double* sums = v + (size_t)n_pts * gm::kNFeat;
   hipLaunchKernelGGL(gm::k_scalar_features, dim3((unsigned)((n_pts + 255) / 256)), dim3(256), 0, s,
-                     n_pts, xx, lam, props, v);
+                     n_pts, xx, lam, (const double*)nullptr, props, v);
   rc = gm_psd_contract(ctx, n_modes, n_pts, gm::kNFeat, psd, v, sums, stream);
   if (rc) return rc;
   hipLaunchKernelGGL(gm::k_scalar_finish, dim3(1), dim3(64), 0, s, n_modes, sums, fracs, reff0,
@@ -300,6 +356,75 @@ int gm_pmatrix_normalize(gm_ctx* ctx, int64_t n_entries, int n_ang, const double
                      (hipStream_t)stream, n_entries, n_ang, theta_rad, pmat, pback);
   e = hipGetLastError();
   if (e != hipSuccess) return gm_fail(1, "gm_pmatrix_normalize: %s", hipGetErrorString(e));
+  return 0;
+}
+
+int gm_integrate_psd_rows(gm_ctx* ctx, int64_t n_rows, int64_t n_pts, const double* xx,
+                          const double* lam_pts, const double* props, const double* psd,
+                          const double* reff0, double rhop0, double rhop, double* out,
+                          gm_stream stream) {
+  if (!ctx || n_pts <= 0 || n_rows <= 0 || n_rows > 0x7fffffff || !xx || !lam_pts || !props ||
+      !psd || !reff0 || !out)
+    return gm_fail(-1, "gm_integrate_psd_rows: bad arguments");
+  hipError_t e = hipSetDevice(ctx->device);
+  if (e != hipSuccess) return gm_fail(1, "hipSetDevice: %s", hipGetErrorString(e));
+  hipStream_t s = (hipStream_t)stream;
+  size_t feat_bytes = sizeof(double) * (size_t)n_pts * gm::kNFeat;
+  size_t sum_bytes = sizeof(double) * (size_t)n_rows * gm::kNFeat;
+  int rc = ctx->reserve(ctx->feat_tmp, feat_bytes + sum_bytes);
+  if (rc) return rc;
+  double* v = (double*)ctx->feat_tmp.ptr;
+  double* sums = v + (size_t)n_pts * gm::kNFeat;
+  hipLaunchKernelGGL(gm::k_scalar_features, dim3((unsigned)((n_pts + 255) / 256)), dim3(256), 0, s,
+                     n_pts, xx, 0.0, lam_pts, props, v);
+  rc = gm_psd_contract(ctx, (int)n_rows, n_pts, gm::kNFeat, psd, v, sums, stream);
+  if (rc) return rc;
+  hipLaunchKernelGGL(gm::k_scalar_finish_rows, dim3((unsigned)((n_rows + 63) / 64)), dim3(64), 0, s,
+                     n_rows, sums, reff0, rhop0, rhop, out);
+  e = hipGetLastError();
+  if (e != hipSuccess) return gm_fail(1, "gm_integrate_psd_rows: %s", hipGetErrorString(e));
+  return 0;
+}
+
+int gm_psd_contract_rows(gm_ctx* ctx, int n_rows, const int64_t* start_host,
+                         const int64_t* count_host, int64_t n_pts, int n_cols, const double* w,
+                         const double* v, double* out, gm_stream stream) {
+  if (!ctx || n_rows < 0 || n_rows > gm::kMaxSegs || n_pts < 0 || n_cols < 0)
+    return gm_fail(-1, "gm_psd_contract_rows: bad arguments (at most %d rows)", gm::kMaxSegs);
+  if (n_rows == 0 || n_cols == 0) return 0;
+  if (!start_host || !count_host || !w || !v || !out)
+    return gm_fail(-1, "gm_psd_contract_rows: NULL array");
+  gm::SegArgs a;
+  a.n = n_rows;
+  a.chunk = ctx->tune_psd_chunk;
+  a.first[0] = 0;
+  for (int r = 0; r < n_rows; ++r) {
+    if (start_host[r] < 0 || count_host[r] < 1 || start_host[r] + count_host[r] > n_pts)
+      return gm_fail(-1, "gm_psd_contract_rows: row %d range outside [0, n_pts)", r);
+    a.start[r] = start_host[r];
+    a.count[r] = count_host[r];
+    const int64_t nc = (count_host[r] + a.chunk - 1) / a.chunk;
+    if (a.first[r] + nc > 0x7fffffffLL) return gm_fail(-1, "gm_psd_contract_rows: too many chunks");
+    a.first[r + 1] = a.first[r] + (int)nc;
+  }
+  for (int r = n_rows; r < gm::kMaxSegs; ++r) a.start[r] = a.count[r] = 0;
+  hipError_t e = hipSetDevice(ctx->device);
+  if (e != hipSuccess) return gm_fail(1, "hipSetDevice: %s", hipGetErrorString(e));
+  hipStream_t s = (hipStream_t)stream;
+  const int total = a.first[n_rows];
+  int rc = ctx->reserve(ctx->gemm_tmp, sizeof(double) * (size_t)total * n_cols);
+  if (rc) return rc;
+  double* part = (double*)ctx->gemm_tmp.ptr;
+  size_t slot;
+  ctx->mark_begin(gm::kKPsd, s, &slot);
+  const unsigned cb = (unsigned)((n_cols + 255) / 256);
+  hipLaunchKernelGGL(gm::k_contract_segs, dim3(cb, (unsigned)total), dim3(256), 0, s, a, n_pts,
+                     n_cols, w, v, part);
+  hipLaunchKernelGGL(gm::k_reduce_segs, dim3(cb, (unsigned)n_rows), dim3(256), 0, s, a, n_cols,
+                     (const double*)part, out);
+  ctx->mark_end(slot, s);
+  e = hipGetLastError();
+  if (e != hipSuccess) return gm_fail(1, "gm_psd_contract_rows: %s", hipGetErrorString(e));
   return 0;
 }
 

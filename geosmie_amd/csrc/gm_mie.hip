@@ -338,11 +338,11 @@ __global__ __launch_bounds__(64) void k_mie_homog(
           dd = r - crcp_fast(dd + r);
           if (k <= nmax) Dp[o] = dd;
         }
-        if (k <= sr && k > nsw) {
+        if (k <= sr && k > nsw) {   // keeps psi_k / psi_{k-1} = 1/t
           const double r = mul_dd((double)k, xinv);
-          const double t = dr + r;
-          if (k <= nmax) Rp[o] = t;
-          dr = r - rcp_nr(t);
+          const double it = rcp_nr(dr + r);
+          if (k <= nmax) Rp[o] = it;
+          dr = r - it;
         }
       }
     }
@@ -366,7 +366,7 @@ __global__ __launch_bounds__(64) void k_mie_homog(
         R_nx = Rp[(size_t)n * 64 + lane];
       }
       double f = mul_dd((double)(2 * n - 1), xinv);
-      double psi = (n <= nsw) ? fma(f, psi_p, -psi_pp) : psi_p * rcp_nr(R);
+      double psi = (n <= nsw) ? fma(f, psi_p, -psi_pp) : psi_p * R;
       double chi = fma(f, chi_p, -chi_pp);
       double nx = mul_dd((double)n, xinv);
       cd da = mul_dd(D, minv) + nx;
@@ -408,30 +408,6 @@ __device__ __forceinline__ void dn_chain(cdd zi, int nstart, int nmax, cd* __res
     }
   }
 }
-// psi_{n-1}(z)/psi_n(z) = D_n(z) + n/z for nsw < n <= nmax, converged start.
-__device__ __forceinline__ void ratio_chain(cd z, int nsw, int nmax, cd* __restrict__ plane,
-                                            int lane) {
-  const cdd zi = crcp_dd(z);
-  cd dd = cd{0.0, 0.0};
-  for (int n = ratio_start(nmax, cabs_(z)); n > nsw; --n) {
-    cd r = mul_dd((double)n, zi);
-    cd t = dd + r;
-    if (n <= nmax) plane[(size_t)(n - 1) * 64 + lane] = t;
-    dd = r - crcp_fast(t);
-  }
-}
-__device__ __forceinline__ void ratio_chain(double z, int nsw, int nmax, double* __restrict__ plane,
-                                            int lane) {
-  const ddr zi = rcp_dd(z);
-  double dd = 0.0;
-  for (int n = ratio_start(nmax, z); n > nsw; --n) {
-    double r = mul_dd((double)n, zi);
-    double t = dd + r;
-    if (n <= nmax) plane[(size_t)(n - 1) * 64 + lane] = t;
-    dd = r - rcp_nr(t);
-  }
-}
-
 // ---------------------------------------------------------------------------
 // Coated sphere, real or weakly absorbing shell (|Im m2| y <= 1)
 // ---------------------------------------------------------------------------
@@ -510,23 +486,25 @@ __global__ __launch_bounds__(64) void k_mie_coated_stable(
           du = r - crcp_fast(du + r);
           if (k <= nmax) DU[o] = du;
         }
-        if (k <= sv && k > nsw_v) {   // psi_{k-1}(v) / psi_k(v)
+        // ratio chains keep psi_k / psi_{k-1} = 1/t (t = psi_{k-1}/psi_k), which
+        // the chain forms anyway; the upward pass then multiplies
+        if (k <= sv && k > nsw_v) {
           const cd r = mul_dd((double)k, vi);
-          const cd t = dv + r;
-          if (k <= nmax) RV[o] = t;
-          dv = r - crcp_fast(t);
+          const cd it = crcp_fast(dv + r);
+          if (k <= nmax) RV[o] = it;
+          dv = r - it;
         }
         if (k <= sw && k > nsw_w) {
           const cd r = mul_dd((double)k, wi);
-          const cd t = dw + r;
-          if (k <= nmax) RW[o] = t;
-          dw = r - crcp_fast(t);
+          const cd it = crcp_fast(dw + r);
+          if (k <= nmax) RW[o] = it;
+          dw = r - it;
         }
         if (k <= sy && k > nsw_y) {
           const double r = mul_dd((double)k, yi);
-          const double t = dy + r;
-          if (k <= nmax) RY[o] = t;
-          dy = r - rcp_nr(t);
+          const double it = rcp_nr(dy + r);
+          if (k <= nmax) RY[o] = it;
+          dy = r - it;
         }
       }
     }
@@ -562,12 +540,12 @@ __global__ __launch_bounds__(64) void k_mie_coated_stable(
       if (n < nmax) du_nx = DU[o + 64];
       const double fn = (double)(2 * n - 1), dn = (double)n;
       const cd fv = mul_dd(fn, vinv), fw = mul_dd(fn, winv);
-      const cd pv = (n <= nsw_v) ? fv * pv_p - pv_pp : cdiv(pv_p, RV[o]);
+      const cd pv = (n <= nsw_v) ? fv * pv_p - pv_pp : pv_p * RV[o];
       const cd chv = fv * cv_p - cv_pp;
-      const cd pw = (n <= nsw_w) ? fw * pw_p - pw_pp : cdiv(pw_p, RW[o]);
+      const cd pw = (n <= nsw_w) ? fw * pw_p - pw_pp : pw_p * RW[o];
       const cd chw = fw * cw_p - cw_pp;
       const double f = mul_dd(fn, yinv);
-      const double py = (n <= nsw_y) ? fma(f, py_p, -py_pp) : py_p * rcp_nr(RY[o]);
+      const double py = (n <= nsw_y) ? fma(f, py_p, -py_pp) : py_p * RY[o];
       const double chy = fma(f, cy_p, -cy_pp);
       // derivatives psi'_n = psi_{n-1} - (n/z) psi_n
       const cd dpv = pv_p - mul_dd(dn, vinv) * pv;

@@ -9,8 +9,8 @@ runoptics' per-(wavelength, rh) iteration computes
   gm_coef_offsets, gm_mie_batch          a_n, b_n and efficiencies (FP64)
   (gm_angle_tables                        pi'_n, tau'_n: once, at construction)
   gm_s12_mueller                          S1/S2 on MFMA -> s11, s12, s33, s34
-  gm_psd_contract (per row)               phase-matrix integrals of integratePSD
-  gm_integrate_psd (per row)              the integratePSD scalars
+  gm_psd_contract_rows                    phase-matrix integrals of integratePSD
+  gm_integrate_psd_rows                   the integratePSD scalars of every row
   gm_pmatrix_normalize                    P normalisation and pback
   gm_gsf_expand (optional)                GSF moments of every row
 
@@ -32,7 +32,7 @@ class RunopticsChain(object):
     def __init__(self, x, m, ang, w, lam_of_row, ranges, precision=0, gsf_ng=0, rhop0=1000.0,
                  rhop=1000.0):
         """x [n] size parameters, m complex index (scalar or [n]), ang [n_ang]
-        degrees, w [rows, n] number weights (row r nonzero on points
+        degrees, w [rows, n] number weights (row r nonzero only on its points
         ranges[r] = (start, count)), lam_of_row [rows] wavelengths."""
         t = _lib.torch()
         self.t = t
@@ -46,6 +46,8 @@ class RunopticsChain(object):
         self.rows = w.shape[0]
         self.lam = np.asarray(lam_of_row, dtype=np.float64)
         self.ranges = [(int(a), int(c)) for a, c in ranges]
+        if len(self.ranges) != w.shape[0] or not 1 <= w.shape[0] <= 64:
+            raise ValueError("one (start, count) range per weight row, 1 to 64 rows")
         self.precision = int(precision)
         self.gsf_ng = int(gsf_ng)
         self.rhop0, self.rhop = float(rhop0), float(rhop)
@@ -70,9 +72,14 @@ class RunopticsChain(object):
         self.d_pm = t.empty(self.rows * 4 * self.n_ang, dtype=f64, device=dev)
         self.d_back = t.empty(self.rows * 6, dtype=f64, device=dev)
         self.d_sc = t.empty(self.rows * 16, dtype=f64, device=dev)
-        self.d_one = _lib.to_dev(np.array([1.0]))
         rr = x[None, :] * self.lam[:, None] / (2 * np.pi)
         self.d_reff0 = _lib.to_dev((w * rr ** 4).sum(1) / (w * rr ** 3).sum(1))
+        lam_pts = np.ones(self.n)
+        for r, (a, c) in enumerate(self.ranges):
+            lam_pts[a:a + c] = self.lam[r]
+        self.d_lam_pts = _lib.to_dev(lam_pts)
+        self._seg_start = (ctypes.c_int64 * self.rows)(*[a for a, _ in self.ranges])
+        self._seg_count = (ctypes.c_int64 * self.rows)(*[c for _, c in self.ranges])
         if self.gsf_ng:
             self.d_F = t.empty(self.rows * 6 * self.n_ang, dtype=f64, device=dev)
             self.d_pmom = t.empty(self.rows * 6 * self.gsf_ng, dtype=f64, device=dev)
@@ -93,15 +100,13 @@ class RunopticsChain(object):
         check(L.gm_s12_mueller(h, self.n, P(self.d_off), P(self.d_nmax), P(self.d_an),
                                P(self.d_bn), self.n_ang, P(self.d_pi), P(self.d_tau),
                                self.nmax_cap, P(self.d_mu), self.precision, s), "gm_s12_mueller")
-        nc = 4 * self.n_ang
-        for r, (a, c) in enumerate(self.ranges):
-            check(L.gm_psd_contract(h, 1, c, nc, P(self.d_w[r, a:a + c]), P(self.d_mu[a * nc:]),
-                                    P(self.d_pm[r * nc:]), s), "gm_psd_contract")
-        for r in range(self.rows):
-            check(L.gm_integrate_psd(h, self.n, 1, P(self.d_x), P(self.d_props), P(self.d_w[r]),
-                                     P(self.d_one), P(self.d_reff0[r:r + 1]), float(self.lam[r]),
-                                     self.rhop0, self.rhop, P(self.d_sc[16 * r:16 * r + 16]), s),
-                  "gm_integrate_psd")
+        # one launch integrates every row over its own points
+        check(L.gm_psd_contract_rows(h, self.rows, self._seg_start, self._seg_count, self.n,
+                                     4 * self.n_ang, P(self.d_w), P(self.d_mu), P(self.d_pm), s),
+              "gm_psd_contract_rows")
+        check(L.gm_integrate_psd_rows(h, self.rows, self.n, P(self.d_x), P(self.d_lam_pts),
+                                      P(self.d_props), P(self.d_w), P(self.d_reff0), self.rhop0,
+                                      self.rhop, P(self.d_sc), s), "gm_integrate_psd_rows")
         check(L.gm_pmatrix_normalize(h, self.rows, self.n_ang, P(self.d_theta), P(self.d_pm),
                                      P(self.d_back), s), "gm_pmatrix_normalize")
         if self.gsf_ng:

@@ -178,6 +178,16 @@ int gm_s12_p11(gm_ctx* ctx, int64_t n, const int64_t* coef_off, const int32_t* n
 int gm_psd_contract(gm_ctx* ctx, int n_psd, int64_t n_pts, int n_cols, const double* w,
                     const double* v, double* out, gm_stream stream);
 
+/* Rows over their own point ranges: out[r][c] = sum over p in
+ * [start_host[r], start_host[r] + count_host[r]) of w[r][p] v[p][c], for
+ * n_rows <= 64 rows (one runoptics slab per wavelength: integratePSD's
+ * phase-matrix sums, dointegration.py:1164-1166).  w [n_rows][n_pts] and v
+ * [n_pts][n_cols] device; the ranges are host arrays.  Split-K over chunks of
+ * GM_TUNE_PSD_CHUNK points, chunks of a row added in order. */
+int gm_psd_contract_rows(gm_ctx* ctx, int n_rows, const int64_t* start_host,
+                         const int64_t* count_host, int64_t n_pts, int n_cols, const double* w,
+                         const double* v, double* out, gm_stream stream);
+
 /* integratePSD scalars (src/geosmie/dointegration.py:1064-1209, mixing
  * 'dumix', including its Qsca weighting of g, csca, cext from the aliased
  * thisweight, :1111-1112 and :1154-1157, and the qb back-out, :1133-1169).
@@ -189,6 +199,15 @@ int gm_psd_contract(gm_ctx* ctx, int n_psd, int64_t n_pts, int n_cols, const dou
 int gm_integrate_psd(gm_ctx* ctx, int64_t n_pts, int n_modes, const double* xx, const double* props,
                      const double* psd, const double* fracs, const double* reff0, double lam,
                      double rhop0, double rhop, double* out, gm_stream stream);
+
+/* integratePSD for n_rows independent single-mode rows at once (one
+ * runoptics slab per wavelength, fracs = 1): psd [n_rows][n_pts] number
+ * weights, each row nonzero on its own points; lam_pts [n_pts] the wavelength
+ * of each point's row; reff0 [n_rows]; out [n_rows][16] as gm_integrate_psd. */
+int gm_integrate_psd_rows(gm_ctx* ctx, int64_t n_rows, int64_t n_pts, const double* xx,
+                          const double* lam_pts, const double* props, const double* psd,
+                          const double* reff0, double rhop0, double rhop, double* out,
+                          gm_stream stream);
 
 /* Phase-matrix normalisation (dointegration.py:978-988) in place on
  * pmat [n_entries][4][n_ang] (s11, s12, s33, s34): P11 -> 2 P11 / trapz(P11

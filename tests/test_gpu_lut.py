@@ -118,3 +118,75 @@ def test_psd_contract_mfma_and_small(gpu):
         got = _contract(w, v)
         ref = w @ v
         assert np.allclose(got, ref, rtol=1e-12, atol=1e-12 * np.abs(ref).max()), n_psd
+
+
+def test_integrate_psd_rows_matches_per_row(gpu):
+    """gm_integrate_psd_rows (all rows in one pass) equals gm_integrate_psd run
+    row by row, and the one-pass contraction equals per-row contractions."""
+    import ctypes
+    from geosmie_amd import _lib
+    from geosmie_amd._lib import check, ptr, stream
+    from geosmie_amd.lut import getDR, getLogNormPSD
+    t = _lib.torch()
+    rng = np.random.default_rng(4)
+    r = np.logspace(-2, 1, 700)
+    lams = np.array([0.35, 0.6, 1.1, 2.2])
+    x = np.concatenate([2 * np.pi * r / L for L in lams])
+    n = x.size
+    W = np.zeros((lams.size, n))
+    for i, L in enumerate(lams):
+        p = getLogNormPSD(rng.uniform(0.1, 1.0), rng.uniform(1.5, 2.2), 2 * np.pi * r / L, L,
+                          1e9, 0.0) * (2 * np.pi / L) * getDR(r)
+        W[i, i * r.size:(i + 1) * r.size] = p / p.sum()
+    from geosmie_amd.mie import mie_batch
+    res = mie_batch(x, m=1.5 + 0.01j)
+    props = np.stack([res[k] for k in ("qext", "qsca", "qabs", "qb", "asy", "qratio")])
+    rr = x[None, :] * np.repeat(lams, r.size)[None, :] / (2 * np.pi)
+    reff0 = (W * rr ** 4).sum(1) / (W * rr ** 3).sum(1)
+    d_x, d_props, d_w = _lib.to_dev(x), _lib.to_dev(props), _lib.to_dev(W)
+    d_lp = _lib.to_dev(np.repeat(lams, r.size))
+    d_r0 = _lib.to_dev(reff0)
+    out = t.empty(lams.size * 16, dtype=t.float64, device=_lib.device())
+    L, h, s = _lib.lib(), _lib.ctx(), stream()
+    check(L.gm_integrate_psd_rows(h, lams.size, n, ptr(d_x), ptr(d_lp), ptr(d_props), ptr(d_w),
+                                  ptr(d_r0), 1000.0, 1700.0, ptr(out), s))
+    rows = out.view(lams.size, 16).cpu().numpy()
+    one = _lib.to_dev(np.array([1.0]))
+    for i, Lm in enumerate(lams):
+        o1 = t.empty(16, dtype=t.float64, device=_lib.device())
+        check(L.gm_integrate_psd(h, n, 1, ptr(d_x), ptr(d_props), ptr(d_w[i]), ptr(one),
+                                 ptr(d_r0[i:i + 1]), float(Lm), 1000.0, 1700.0, ptr(o1), s))
+        np.testing.assert_allclose(rows[i], o1.cpu().numpy(), rtol=1e-12)
+
+
+@pytest.mark.parametrize("chunk", [64, 512])
+def test_psd_contract_rows_matches_per_row(gpu, chunk):
+    """The segmented contraction (every row over its own points, one launch)
+    equals per-row gm_psd_contract on the same ranges."""
+    import ctypes
+    from geosmie_amd import _lib, autotune
+    from geosmie_amd._lib import check, ptr, stream
+    t = _lib.torch()
+    rng = np.random.default_rng(chunk)
+    n_pts, n_cols = 3000, 333
+    starts = np.array([0, 700, 701, 1500, 2999])
+    counts = np.array([700, 1, 799, 1499, 1])
+    W = rng.uniform(0, 1, (starts.size, n_pts))
+    V = rng.standard_normal((n_pts, n_cols))
+    d_w, d_v = _lib.to_dev(W), _lib.to_dev(V)
+    out = t.empty(starts.size * n_cols, dtype=t.float64, device=_lib.device())
+    L, h, s = _lib.lib(), _lib.ctx(), stream()
+    autotune.set_knob(h, "psd_chunk", chunk)
+    st = (ctypes.c_int64 * starts.size)(*starts.tolist())
+    ct = (ctypes.c_int64 * starts.size)(*counts.tolist())
+    check(L.gm_psd_contract_rows(h, starts.size, st, ct, n_pts, n_cols, ptr(d_w), ptr(d_v),
+                                 ptr(out), s))
+    got = out.view(starts.size, n_cols).cpu().numpy()
+    for r, (a, c) in enumerate(zip(starts, counts)):
+        want = W[r, a:a + c] @ V[a:a + c]
+        np.testing.assert_allclose(got[r], want, rtol=1e-12, atol=1e-12 * np.abs(want).max())
+    bad = (ctypes.c_int64 * 1)(2900)
+    cnt = (ctypes.c_int64 * 1)(200)
+    with pytest.raises(ValueError):
+        check(L.gm_psd_contract_rows(h, 1, bad, cnt, n_pts, n_cols, ptr(d_w), ptr(d_v), ptr(out), s))
+    autotune.set_knob(h, "psd_chunk", 128)
