@@ -219,12 +219,15 @@ __global__ void k_scalar_finish(int n_modes, const double* __restrict__ sums,
 }
 
 // One row per thread: rows are independent single-mode integrations.
+// rhop_rows (or nullptr: rhop for every row) is each row's wet density.
 __global__ void k_scalar_finish_rows(int64_t n_rows, const double* __restrict__ sums,
                                      const double* __restrict__ reff0, double rhop0, double rhop,
+                                     const double* __restrict__ rhop_rows,
                                      double* __restrict__ out) {
   const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= n_rows) return;
-  finish_modes(1, sums + r * kNFeat, nullptr, reff0 + r, rhop0, rhop, out + r * kNOut);
+  finish_modes(1, sums + r * kNFeat, nullptr, reff0 + r, rhop0, rhop_rows ? rhop_rows[r] : rhop,
+               out + r * kNOut);
 }
 
 // Phase-matrix normalisation, dointegration.py:978-988, per entry:
@@ -361,8 +364,8 @@ int gm_pmatrix_normalize(gm_ctx* ctx, int64_t n_entries, int n_ang, const double
 
 int gm_integrate_psd_rows(gm_ctx* ctx, int64_t n_rows, int64_t n_pts, const double* xx,
                           const double* lam_pts, const double* props, const double* psd,
-                          const double* reff0, double rhop0, double rhop, double* out,
-                          gm_stream stream) {
+                          const double* reff0, double rhop0, double rhop,
+                          const double* rhop_rows, double* out, gm_stream stream) {
   if (!ctx || n_pts <= 0 || n_rows <= 0 || n_rows > 0x7fffffff || !xx || !lam_pts || !props ||
       !psd || !reff0 || !out)
     return gm_fail(-1, "gm_integrate_psd_rows: bad arguments");
@@ -380,7 +383,7 @@ int gm_integrate_psd_rows(gm_ctx* ctx, int64_t n_rows, int64_t n_pts, const doub
   rc = gm_psd_contract(ctx, (int)n_rows, n_pts, gm::kNFeat, psd, v, sums, stream);
   if (rc) return rc;
   hipLaunchKernelGGL(gm::k_scalar_finish_rows, dim3((unsigned)((n_rows + 63) / 64)), dim3(64), 0, s,
-                     n_rows, sums, reff0, rhop0, rhop, out);
+                     n_rows, sums, reff0, rhop0, rhop, rhop_rows, out);
   e = hipGetLastError();
   if (e != hipSuccess) return gm_fail(1, "gm_integrate_psd_rows: %s", hipGetErrorString(e));
   return 0;

@@ -42,7 +42,7 @@ def getDR(arr):
     arr = np.asarray(arr, dtype=np.float64)
     d = np.empty_like(arr)
     d[0] = arr[1] - arr[0]
-    d[1:-1] = 0.5 * (arr[2:] - arr[:-2])
+    d[1:-1] = ((arr[1:-1] - arr[:-2]) + (arr[2:] - arr[1:-1])) / 2.
     d[-1] = arr[-1] - arr[-2]
     return d
 
@@ -54,8 +54,12 @@ def getXArrCarma(minx, maxx, nbinperdecade):
     rmin = minx * ((1. + rmrat) / 2.) ** (1. / 3.)
     cpi = 4. / 3. * np.pi
     vrfact = ((3. / 2. / np.pi / (rmrat + 1)) ** (1. / 3.)) * (rmrat ** (1. / 3.) - 1.)
-    rvol = cpi * rmin ** 3. * rmrat ** np.arange(int(nbin))
-    return (rvol / cpi) ** (1. / 3.), vrfact * rvol ** (1. / 3.)
+    rvolmin = cpi * rmin ** 3.
+    # scalar powers (libm), not numpy's vector pow: the grid matches runoptics' bit for bit
+    rvol = [float(rvolmin) * float(rmrat) ** i for i in range(int(nbin))]
+    r = np.array([(v / cpi) ** (1. / 3.) for v in rvol])
+    dr = np.array([vrfact * v ** (1. / 3.) for v in rvol])
+    return r, dr
 
 
 def getLogNormPSD(rmode, sigma, xxArr, lambd, rmax, rmin):

@@ -106,7 +106,7 @@ class RunopticsChain(object):
               "gm_psd_contract_rows")
         check(L.gm_integrate_psd_rows(h, self.rows, self.n, P(self.d_x), P(self.d_lam_pts),
                                       P(self.d_props), P(self.d_w), P(self.d_reff0), self.rhop0,
-                                      self.rhop, P(self.d_sc), s), "gm_integrate_psd_rows")
+                                      self.rhop, None, P(self.d_sc), s), "gm_integrate_psd_rows")
         check(L.gm_pmatrix_normalize(h, self.rows, self.n_ang, P(self.d_theta), P(self.d_pm),
                                      P(self.d_back), s), "gm_pmatrix_normalize")
         if self.gsf_ng:

@@ -1,0 +1,188 @@
+"""Optics table writer in runoptics' layout.
+
+Follows createNCDF and the per-(bin, wavelength, rh) writes of
+src/geosmie/dointegration.py:205-422, 1006-1031, and the pmom variable that
+the GSF step adds (src/gsf/convertncdf.py:70-79, 381-394, zero-padded to m):
+
+  dimensions   rh, wavelength [m], bin (1-indexed), p = [11,12,33,34,22,44], ang
+               (legacy layout: rh, lambda, radius, nPol, ang)
+  variables    p11..p44 (bin, wavelength, rh, ang); the scalars
+               (bin, wavelength, rh); pback (bin, wavelength, rh, p); the
+               per-bin-RH scalars (bin, rh); pmom (bin, wavelength, rh, p, m)
+               (legacy: radius, rh, lambda order; pback (nPol, radius, rh, lambda);
+               pmom (nPol, nMom, radius, rh, lambda))
+
+Differences from the reference files, forced by this image: netCDF4/HDF5 is
+not installed, so the file is NetCDF-3 (64-bit offset, scipy.io.netcdf_file),
+without zlib compression, and the integer index variables are int32 (NetCDF-3
+has no int64).  Names, dimension orders, units and long names are the
+reference's.
+"""
+import numpy as np
+
+P_ORDER = [11, 12, 33, 34, 22, 44]
+SCAT_KEYS = ["p11", "p12", "p22", "p33", "p34", "p44"]
+SCALAR_KEYS = ["qsca", "qabs", "qext", "g", "ssa", "qb", "bsca", "bext", "csca", "cext", "bbck",
+               "lidar_ratio", "refreal", "refimag"]
+NL_SCALAR_KEYS = ["mass", "volume", "area", "rEff", "rMass", "rUp", "rLow", "rhop",
+                  "growth_factor"]
+
+# (units, long_name) of every variable (dointegration.py:224-334)
+META = {
+    "p11": ("dimensionless", "P11 element of the normalized scattering matrix"),
+    "p12": ("dimensionless", "P12 element of the normalized scattering matrix"),
+    "p22": ("dimensionless", "P22 element of the normalized scattering matrix"),
+    "p33": ("dimensionless", "P33 element of the normalized scattering matrix"),
+    "p34": ("dimensionless", "P34 element of the normalized scattering matrix"),
+    "p44": ("dimensionless", "P44 element of the normalized scattering matrix"),
+    "qsca": ("dimensionless", "scattering efficiency"),
+    "qabs": ("dimensionless", "absorption efficiency"),
+    "qext": ("dimensionless", "extinction efficiency"),
+    "g": ("dimensionless", "asymmetry factor"),
+    "ssa": ("dimensionless", "single-scattering albedo"),
+    "bsca": ("m2 (kg dry mass)-1", "mass scattering efficiency"),
+    "bext": ("m2 (kg dry mass)-1", "mass extinction efficiency"),
+    "csca": ("m2", "mass scattering cross-section"),
+    "cext": ("m2", "mass extinction cross-section"),
+    "bbck": ("m2 (kg dry mass)-1 sr-1", "mass backscatter efficiency"),
+    "lidar_ratio": ("", "lidar ratio"),
+    "mass": ("kg", "particle mass"),
+    "volume": ("m3 kg-1", "particle volume per kg dry mass"),
+    "area": ("m2 kg-1", "particle cross sectional area per kg dry mass"),
+    "rEff": ("m", "effective radius of bin"),
+    "rMass": ("kg", "effective mass of wet particle"),
+    "rUp": ("m", "upper edge of radius bin"),
+    "rLow": ("m", "lower edge of radius bin"),
+    "pback": ("dimensionless", "phase function in backscatter direction, ordered as P11, P12, "
+                               "P33, P34, P22, P44"),
+    "qb": ("dimensionless", "backscattering efficiency"),
+    "rhop": ("kg m-3", "wet particle density"),
+    "growth_factor": ("fraction", "growth factor = ratio of wet to dry particle radius"),
+    "refreal": ("dimensionless", "real refractive index of wet particle"),
+    "refimag": ("dimensionless", "imaginary refractive index of wet particle"),
+    "pmom": (None, "Moments of the generalized spherical functions order over nPol as the 11, 12, "
+                   "33, 34, 22, 44 elements of the scattering law matrix for a Fourier "
+                   "decomposition of the phase matrix such as those used in discrete ordinate "
+                   "solutions to the RTE"),
+}
+
+
+class OpticsTable(object):
+    """In-memory optics table; fill with set_entry / set_bin_rh, write with write()."""
+
+    def __init__(self, n_bin, rh, wavelength_m, ang_deg, classic=False, n_mom=0):
+        self.classic = bool(classic)
+        self.rh = np.asarray(rh, np.float64)
+        self.lam = np.asarray(wavelength_m, np.float64)
+        self.ang = np.asarray(ang_deg, np.float64)
+        self.n_bin, self.n_rh, self.n_lam, self.n_ang = int(n_bin), self.rh.size, self.lam.size, \
+            self.ang.size
+        self.n_mom = int(n_mom)
+        self.radius = "radius" if classic else "bin"
+        self.lamname = "lambda" if classic else "wavelength"
+        self.pname = "nPol" if classic else "p"
+        self.mname = "nMom" if classic else "m"
+        r, lm, p, mo = self.radius, self.lamname, self.pname, self.mname
+        if classic:   # dointegration.py:356-363
+            scal = (r, "rh", lm)
+            fun = (r, "rh", lm, "ang")
+            ele = (p, r, "rh", lm)
+        else:         # :364-371
+            scal = (r, lm, "rh")
+            fun = (r, lm, "rh", "ang")
+            ele = (r, lm, "rh", p)
+        self.dims = {}
+        for k in SCAT_KEYS:
+            self.dims[k] = fun
+        for k in SCALAR_KEYS:
+            self.dims[k] = scal
+        for k in NL_SCALAR_KEYS:
+            self.dims[k] = (r, "rh")
+        self.dims["pback"] = ele
+        if self.n_mom:   # convertncdf.py:70-79
+            self.dims["pmom"] = (p, mo, r, "rh", lm) if classic else (r, lm, "rh", p, mo)
+        size = {r: self.n_bin, "rh": self.n_rh, lm: self.n_lam, "ang": self.n_ang, p: 6,
+                mo: self.n_mom}
+        self.size = size
+        self.data = {k: np.zeros([size[d] for d in dims]) for k, dims in self.dims.items()}
+
+    def set_entry(self, radind, li, rhi, ret, pmom=None):
+        """Write one (bin, wavelength, rh) result dict as the reference does
+        (dointegration.py:1006-1031): p11..p44 [ang], the scalars, pback [6]
+        in p order; pmom [6, <= n_mom] optional, zero-padded to n_mom."""
+        for k in SCAT_KEYS:
+            if k in ret:
+                self._put(k, radind, li, rhi, np.asarray(ret[k]))
+        for k in SCALAR_KEYS:
+            if k in ret:
+                self._put(k, radind, li, rhi, float(ret[k]))
+        if "pback" in ret:
+            pb = np.asarray(ret["pback"], np.float64)
+            if self.classic:
+                self.data["pback"][:, radind, rhi, li] = pb
+            else:
+                self.data["pback"][radind, li, rhi, :] = pb
+        if pmom is not None:
+            src = np.asarray(pmom, np.float64)[:, :self.n_mom]
+            pm = np.zeros((6, self.n_mom))
+            pm[:, :src.shape[1]] = src
+            if self.classic:
+                self.data["pmom"][:, :, radind, rhi, li] = pm
+            else:
+                self.data["pmom"][radind, li, rhi] = pm
+
+    def set_bin_rh(self, radind, rhi, ret):
+        """Per-(bin, rh) scalars (mass, volume, area, rEff, rMass, rUp, rLow,
+        rhop, growth_factor)."""
+        for k in NL_SCALAR_KEYS:
+            if k in ret:
+                self.data[k][radind, rhi] = float(ret[k])
+
+    def _put(self, key, radind, li, rhi, value):
+        if self.classic:
+            self.data[key][radind, rhi, li] = value
+        else:
+            self.data[key][radind, li, rhi] = value
+
+    def write(self, path):
+        from scipy.io import netcdf_file
+        f = netcdf_file(path, "w", version=2)
+        try:
+            r, lm, p = self.radius, self.lamname, self.pname
+            for d in ("rh", lm, r, p, "ang"):
+                f.createDimension(d, self.size[d])
+            if self.n_mom:
+                f.createDimension(self.mname, self.n_mom)
+            coords = {r: (np.arange(1, self.n_bin + 1, dtype=np.int32), "i",
+                          ("dimensionless", "radius bin index (1-indexed)")),
+                      "rh": (self.rh, "d", ("fraction", "relative humidity")),
+                      lm: (self.lam, "d", ("m", "wavelength")),
+                      "ang": (self.ang, "d", ("degrees", "scattering angle")),
+                      p: (np.array(P_ORDER, dtype=np.int32), "i",
+                          ("dimensionless", "Scattering matrix element index, ordered as P11, "
+                                            "P12, P33, P34, P22, P44"))}
+            for name, (vals, typ, (units, lname)) in coords.items():
+                v = f.createVariable(name, typ, (name,))
+                v[:] = vals
+                v.units = units
+                v.long_name = lname
+            for k, dims in self.dims.items():
+                v = f.createVariable(k, "d", dims)
+                v[:] = self.data[k]
+                units, v.long_name = META[k]
+                if units is not None:
+                    v.units = units
+        finally:
+            f.close()
+
+
+def read(path):
+    """Variables of a written table as numpy arrays (with their dimension names)."""
+    from scipy.io import netcdf_file
+    f = netcdf_file(path, "r", mmap=False)
+    try:
+        return {k: (v[:].copy(), v.dimensions, getattr(v, "units", b"").decode()
+                    if isinstance(getattr(v, "units", b""), bytes) else getattr(v, "units", ""))
+                for k, v in f.variables.items()}
+    finally:
+        f.close()

@@ -203,11 +203,13 @@ int gm_integrate_psd(gm_ctx* ctx, int64_t n_pts, int n_modes, const double* xx, 
 /* integratePSD for n_rows independent single-mode rows at once (one
  * runoptics slab per wavelength, fracs = 1): psd [n_rows][n_pts] number
  * weights, each row nonzero on its own points; lam_pts [n_pts] the wavelength
- * of each point's row; reff0 [n_rows]; out [n_rows][16] as gm_integrate_psd. */
+ * of each point's row; reff0 [n_rows]; rhop_rows [n_rows] device, each row's
+ * wet density (runoptics: one row per rh, dointegration.py:867-868), or NULL
+ * for rhop on every row; out [n_rows][16] as gm_integrate_psd. */
 int gm_integrate_psd_rows(gm_ctx* ctx, int64_t n_rows, int64_t n_pts, const double* xx,
                           const double* lam_pts, const double* props, const double* psd,
-                          const double* reff0, double rhop0, double rhop, double* out,
-                          gm_stream stream);
+                          const double* reff0, double rhop0, double rhop,
+                          const double* rhop_rows, double* out, gm_stream stream);
 
 /* Phase-matrix normalisation (dointegration.py:978-988) in place on
  * pmat [n_entries][4][n_ang] (s11, s12, s33, s34): P11 -> 2 P11 / trapz(P11

@@ -53,9 +53,9 @@ def getXArrCarma(minx, maxx, nbinperdecade):
     cpi = 4. / 3. * np.pi
     rvolmin = cpi * rMin ** 3.
     vrfact = ((3. / 2. / np.pi / (rmRat + 1)) ** (1. / 3.)) * (rmRat ** (1. / 3.) - 1.)
-    i = np.arange(int(nbin))
-    rvol = rvolmin * rmRat ** i
-    return (rvol / cpi) ** (1. / 3.), vrfact * rvol ** (1. / 3.)
+    rvol = [float(rvolmin) * float(rmRat) ** i for i in range(int(nbin))]
+    return (np.array([(v / cpi) ** (1. / 3.) for v in rvol]),
+            np.array([vrfact * v ** (1. / 3.) for v in rvol]))
 
 
 def getLogNormPSD(rmode, sigma, xxArr, lambd, rmax, rmin):

@@ -149,14 +149,25 @@ def test_integrate_psd_rows_matches_per_row(gpu):
     out = t.empty(lams.size * 16, dtype=t.float64, device=_lib.device())
     L, h, s = _lib.lib(), _lib.ctx(), stream()
     check(L.gm_integrate_psd_rows(h, lams.size, n, ptr(d_x), ptr(d_lp), ptr(d_props), ptr(d_w),
-                                  ptr(d_r0), 1000.0, 1700.0, ptr(out), s))
+                                  ptr(d_r0), 1000.0, 1700.0, None, ptr(out), s))
     rows = out.view(lams.size, 16).cpu().numpy()
+    # per-row wet densities (one per rh row in runoptics)
+    rho = 1100.0 + 150.0 * np.arange(lams.size)
+    d_rho = _lib.to_dev(rho)
+    out2 = t.empty_like(out)
+    check(L.gm_integrate_psd_rows(h, lams.size, n, ptr(d_x), ptr(d_lp), ptr(d_props), ptr(d_w),
+                                  ptr(d_r0), 1000.0, 1700.0, ptr(d_rho), ptr(out2), s))
+    rows2 = out2.view(lams.size, 16).cpu().numpy()
     one = _lib.to_dev(np.array([1.0]))
     for i, Lm in enumerate(lams):
         o1 = t.empty(16, dtype=t.float64, device=_lib.device())
         check(L.gm_integrate_psd(h, n, 1, ptr(d_x), ptr(d_props), ptr(d_w[i]), ptr(one),
                                  ptr(d_r0[i:i + 1]), float(Lm), 1000.0, 1700.0, ptr(o1), s))
         np.testing.assert_allclose(rows[i], o1.cpu().numpy(), rtol=1e-12)
+        check(L.gm_integrate_psd(h, n, 1, ptr(d_x), ptr(d_props), ptr(d_w[i]), ptr(one),
+                                 ptr(d_r0[i:i + 1]), float(Lm), 1000.0, float(rho[i]), ptr(o1),
+                                 s))
+        np.testing.assert_allclose(rows2[i], o1.cpu().numpy(), rtol=1e-12)
 
 
 @pytest.mark.parametrize("chunk", [64, 512])
