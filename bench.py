@@ -323,6 +323,89 @@ def lut_surrogate(n_queries=1_000_000, n_sample=4096, seed=20261202):
     return out
 
 
+def oc_like_particle():
+    """A GOCART organic-carbon-like particle description (one lognormal bin,
+    r0 0.0212 um, sigma 2.2, 0.1 nm-0.3 um, 800 grid points per decade, 36 rh
+    with simple growth factors) on a 61-wavelength 0.25-40 um table with a
+    smooth made-up refractive index; water index made up as well."""
+    lam = np.concatenate([np.linspace(0.25, 1.0, 16), np.linspace(1.25, 10.0, 30),
+                          np.linspace(11.0, 40.0, 15)]) * 1e-6
+    mr = 1.53 - 0.05 * np.log(lam / 1e-6) / np.log(40.0)
+    mi = -(0.005 + 0.05 * (lam / 40e-6))
+    rh = [0.00, 0.05, 0.10, 0.15, 0.20, 0.25, 0.30, 0.35, 0.40, 0.45, 0.50, 0.55, 0.60, 0.65,
+          0.70, 0.75, 0.80, 0.81, 0.82, 0.83, 0.84, 0.85, 0.86, 0.87, 0.88, 0.89, 0.90, 0.91,
+          0.92, 0.93, 0.94, 0.95, 0.96, 0.97, 0.98, 0.99]
+    gf = list(1.0 + 1.52 * np.array(rh) ** 4)
+    params = {"rhop0": 1800.0, "rh": rh, "rhDep": {"type": "simple", "params": {"gf": gf}},
+              "psd": {"type": "lognorm", "params": {
+                  "r0": [[0.0212e-6]], "rmax0": [[0.3e-6]], "rmin0": [[1e-10]],
+                  "sigma": [[2.20]], "numperdec": [800], "fracs": [[1.0]]}},
+              "mList": [np.array([lam, mr, mi])]}
+    wl = np.linspace(0.2e-6, 45e-6, 60)
+    water = np.array([wl, 1.33 + 0.0 * wl, 1e-8 + 0.3 * (wl / 45e-6)])
+    return params, water
+
+
+def table_build(cpu_sample=True):
+    """A whole runoptics table on the GPU (build_table: front end, one device
+    pass per wavelength, mixing, normalisation) against the CPU oracle's
+    per-entry time for the same entries."""
+    import torch
+    from geosmie_amd.runoptics import build_table
+    params, water = oc_like_particle()
+    small = dict(params, mList=[params["mList"][0][:, :2]])
+    build_table(small, water=water)               # warm: library, tables, workspaces
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    tab = build_table(params, water=water)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    n_ent = tab.n_bin * tab.n_lam * tab.n_rh
+    from geosmie_amd.particle import initializeXarr
+    xx, _ = initializeXarr(params, 0, params["mList"][0][0][0], params["mList"][0][0][-1])
+    out = {"workload": "organic-carbon-like runoptics table: 1 bin, %d size points, 61 "
+                       "wavelengths, 36 rh, 371 angles (%.3g Mie points)" % (
+                           xx.size, xx.size * 61 * 36),
+           "unit": "table entries/s", "entries": n_ent, "s": dt, "value": n_ent / dt,
+           "mie_points_per_s": xx.size * n_ent / dt}
+    if cpu_sample:
+        sys.path.insert(0, ROOT)
+        from oracle import lut as OL, mie as OM
+        cost = np.cos(np.radians(OL.angle_grid()))
+        t0 = time.perf_counter()
+        OM.size_range(xx, cost, 1.55, 0.01)       # rawMie of one entry (the per-entry work)
+        t_ent = time.perf_counter() - t0
+        out["cpu_oracle"] = {"s_per_entry": t_ent, "table_s_est": t_ent * n_ent, "cores": 1,
+                             "kind": "port", "sample": "one entry's Mie over the size grid "
+                                                       "(numpy oracle)"}
+        out["speedup_vs_cpu_oracle"] = t_ent * n_ent / dt
+    # runbands on the same table: every (variable, bin, rh, band) mean in one launch
+    from geosmie_amd.bands import VARS, average_table
+    var = {v: tab.data[v] for v in VARS}
+    average_table(var, tab.lam, "RRTMG")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(5):
+        average_table(var, tab.lam, "RRTMG")
+    torch.cuda.synchronize()
+    t_band = (time.perf_counter() - t0) / 5
+    band = {"workload": "RRTMG band averaging of that table (8 variables x 36 rh x 30 bands)",
+            "s": t_band}
+    if cpu_sample:
+        from oracle.bands import do_average
+        from geosmie_amd.bands import getBands
+        lo, up, _, use_wn, _ = getBands("RRTMG")
+        t0 = time.perf_counter()
+        for v in VARS[:2]:
+            for irh in range(tab.n_rh):
+                for b in range(len(lo)):
+                    do_average(tab.lam, var[v][0, :, irh], lo[b], up[b], use_wn)
+        band["cpu_oracle_s_est"] = (time.perf_counter() - t0) * len(VARS) / 2
+        band["cpu_note"] = "the reference's loop (one interp1d per mean), timed on 2 of 8 variables"
+    out["band_average"] = band
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -331,7 +414,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calib", action="store_true",
                     help="also run 3 device copies of 512 MiB (PMC byte-counter calibration)")
-    ap.add_argument("--secondary-only", choices=("homog", "largex", "lut"), default=None,
+    ap.add_argument("--secondary-only", choices=("homog", "largex", "lut", "table"), default=None,
                     help="run only this secondary measurement (for profiling) and print it")
     ap.add_argument("--no-secondary", action="store_true",
                     help="skip the homogeneous-LUT and large-x secondary measurements")
@@ -348,8 +431,12 @@ def main():
 
     torch.cuda.set_device(0)
     if args.secondary_only:
-        res = lut_surrogate() if args.secondary_only == "lut" else \
-            secondary(which=(args.secondary_only,))[0]
+        if args.secondary_only == "lut":
+            res = lut_surrogate()
+        elif args.secondary_only == "table":
+            res = table_build()
+        else:
+            res = secondary(which=(args.secondary_only,))[0]
         print(json.dumps({"secondary_only": args.secondary_only, "result": res}))
         return
     x, y, m1, m2 = coated_workload()
@@ -434,6 +521,7 @@ def main():
         del d_x, d_y, d_eps, d_eps2, d_props
         out["secondary"] = secondary()
         out["lut"] = lut_surrogate()
+        out["table"] = table_build()
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(x, y, m1, m2)
     print(json.dumps(out))

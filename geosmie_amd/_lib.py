@@ -57,6 +57,8 @@ SIGNATURES = {
                                             c_vp, c_vp, c_vp, c_vp]),
     "gm_integrate_psd_rows": (ctypes.c_int, [c_vp, c_i64, c_i64, c_vp, c_vp, c_vp, c_vp, c_vp,
                                              c_dbl, c_dbl, c_vp, c_vp, c_vp]),
+    "gm_band_average": (ctypes.c_int, [c_vp, c_i64, ctypes.c_int, c_vp, c_vp, ctypes.c_int,
+                                       ctypes.c_int, c_vp, c_vp, c_vp]),
     "gm_pmatrix_normalize": (ctypes.c_int, [c_vp, c_i64, ctypes.c_int, c_vp, c_vp, c_vp, c_vp]),
     "gm_gsf_expand": (ctypes.c_int, [c_vp, c_i64, ctypes.c_int, c_vp, c_vp, ctypes.c_int,
                                      ctypes.c_int, c_vp, c_vp]),

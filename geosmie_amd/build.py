@@ -13,9 +13,12 @@ INCLUDE = os.path.join(os.path.dirname(HERE), "include")
 
 SOURCES = ["gm_api.hip", "gm_util.hip", "gm_mie.hip", "gm_angles.hip", "gm_s12.hip",
            "gm_integrate.hip", "gm_gsf.hip", "gm_lut.hip", "gm_mlp.hip",
-           "gm_surrogate.hip"]
+           "gm_surrogate.hip", "gm_band.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+
+# sources whose results must match numpy bit for bit: no fused multiply-add
+NO_CONTRACT = {"gm_band.hip"}
 
 # -packed-fp32-ops: no v_pk_*_f32 from the compiler (build notes)
 FLAGS = ["-O3", "-std=c++17", "--offload-arch=gfx950", "-fPIC", "-shared",
@@ -35,7 +38,10 @@ def build(verbose=False):
     procs = []
     for src in sources():
         obj = os.path.join(tmp, os.path.basename(src) + ".o")
-        cmd = [HIPCC] + [f for f in FLAGS if f != "-shared"] + ["-c", src, "-o", obj]
+        flags = [f for f in FLAGS if f != "-shared"]
+        if os.path.basename(src) in NO_CONTRACT:
+            flags = [("-ffp-contract=off" if f.startswith("-ffp-contract") else f) for f in flags]
+        cmd = [HIPCC] + flags + ["-c", src, "-o", obj]
         if verbose:
             print(" ".join(cmd))
         procs.append((subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT), src))

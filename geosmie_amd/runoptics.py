@@ -1,0 +1,218 @@
+"""runoptics on the GPU: a whole optics table from a particle description.
+
+Drop-in for the Mie branch of src/geosmie/dointegration.py:fun (672-1036)
+and the runoptics command (src/geosmie/runoptics.py):
+
+  * the front end is geosmie_amd.particle (readers, humidity rules, PSDs);
+  * per bin the size-parameter grid is fixed, so the angle tables are built
+    once per bin; per wavelength ONE device pass covers every rh of that
+    wavelength: the points are the grid repeated once per (rh, refractive
+    index), each with its own wet index, so the Mie batch, S1/S2 and Mueller
+    planes of all rh run as one launch sequence;
+  * every (rh, mode) is one weight row over its own block of points: the
+    phase-matrix integrals come from gm_psd_contract_rows and the
+    integratePSD scalars from gm_integrate_psd_rows with each row's wet
+    density; modes are mixed with their fractions in the order integratePSD
+    adds them (dointegration.py:1199-1200), then P is normalised on the
+    device (gm_pmatrix_normalize) and the post-integration values follow
+    (:950-1001);
+  * the result goes to an OpticsTable (geosmie_amd.table) in either layout.
+
+Mie mode only ('mode': 'mie', psd types lognorm and ss); GRASP kernel files
+('mode': 'kernel') and the 'du' size grid are not part of this path.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+from . import _lib, particle
+from ._lib import check, ptr, stream
+from .lut import SCALAR_OUT, _DevMie, angle_grid
+from .table import OpticsTable
+
+RHOW = 1000.          # density of water (dointegration.py:866)
+ROWS_PER_CALL = 64    # gm_psd_contract_rows row limit
+
+
+def _interp(xp, fp):
+    from scipy.interpolate import interp1d
+    return interp1d(xp, fp)
+
+
+class _WavelengthPass(object):
+    """Device pass over every rh of one (bin, wavelength)."""
+
+    def __init__(self, xx, pi_t, tau_t, n_ang, precision):
+        self.xx, self.pi_t, self.tau_t = xx, pi_t, tau_t
+        self.n_ang, self.precision = n_ang, precision
+
+    def run(self, lam, m_blocks, rows, rhop0):
+        """m_blocks: complex index per point block (each block = the whole
+        grid); rows: list of (block, psd [N_x], reff0, rhop).  Returns the
+        un-normalised phase-matrix integrals [rows][4][n_ang] and the scalars
+        [rows][16] (gm_integrate_psd order)."""
+        t = _lib.torch()
+        nx, nb = self.xx.size, len(m_blocks)
+        n = nx * nb
+        x = np.tile(self.xx, nb)
+        m = np.repeat(np.asarray(m_blocks, np.complex128), nx)
+        dm = _DevMie(x, m)
+        d_mu = dm.mueller(self.pi_t, self.tau_t, self.precision)
+        nr = len(rows)
+        w = np.zeros((nr, n))
+        for r, (b, psd, _, _) in enumerate(rows):
+            w[r, b * nx:(b + 1) * nx] = psd
+        d_w = _lib.to_dev(w)
+        dev = _lib.device()
+        n_cols = 4 * self.n_ang
+        d_pm = t.empty(nr * n_cols, dtype=t.float64, device=dev)
+        L, h, s = _lib.lib(), _lib.ctx(), stream()
+        for r0 in range(0, nr, ROWS_PER_CALL):
+            k = min(ROWS_PER_CALL, nr - r0)
+            starts = (ctypes.c_int64 * k)(*[rows[r0 + i][0] * nx for i in range(k)])
+            counts = (ctypes.c_int64 * k)(*([nx] * k))
+            check(L.gm_psd_contract_rows(h, k, starts, counts, n, n_cols,
+                                         ptr(d_w[r0:r0 + k]), ptr(d_mu),
+                                         ptr(d_pm[r0 * n_cols:(r0 + k) * n_cols]), s),
+                  "gm_psd_contract_rows")
+        d_lp = _lib.to_dev(np.full(n, float(lam)))
+        d_r0 = _lib.to_dev(np.array([r[2] for r in rows], np.float64))
+        d_rho = _lib.to_dev(np.array([r[3] for r in rows], np.float64))
+        d_sc = t.empty(nr * len(SCALAR_OUT), dtype=t.float64, device=dev)
+        check(L.gm_integrate_psd_rows(h, nr, n, ptr(dm.d_x), ptr(d_lp), ptr(dm.d_props), ptr(d_w),
+                                      ptr(d_r0), float(rhop0), 0.0, ptr(d_rho), ptr(d_sc), s),
+              "gm_integrate_psd_rows")
+        pm = d_pm.cpu().numpy().reshape(nr, 4, self.n_ang)
+        sc = d_sc.cpu().numpy().reshape(nr, len(SCALAR_OUT))
+        return pm, sc
+
+
+def _normalize(pm, ang):
+    """P normalisation and pback of entries pm [e][4][n_ang] on the device."""
+    t = _lib.torch()
+    d_p = _lib.to_dev(np.ascontiguousarray(pm))
+    d_b = t.empty(pm.shape[0] * 6, dtype=t.float64, device=_lib.device())
+    d_th = _lib.to_dev(np.radians(ang))
+    check(_lib.lib().gm_pmatrix_normalize(_lib.ctx(), pm.shape[0], pm.shape[2], ptr(d_th),
+                                          ptr(d_p), ptr(d_b), stream()), "gm_pmatrix_normalize")
+    return d_p.cpu().numpy().reshape(pm.shape), d_b.cpu().numpy().reshape(-1, 6)
+
+
+def build_table(params, water=None, classic=False, precision=0, ang=None):
+    """An OpticsTable for a particle description (getPPJSON's dict with
+    'mList'); water = getWaterM() array (needed unless rhDep is 'trivial')."""
+    mode = params.get("mode", "mie")
+    if mode != "mie":
+        raise ValueError("mode %s is not supported (Mie tables only)" % mode)
+    kind = params["psd"]["type"]
+    pp = params["psd"]["params"]
+    if kind == "lognorm":
+        radii = list(pp["r0"])
+    elif kind == "ss":
+        radii = list(pp["rMinMaj"])
+    else:
+        raise ValueError("psd type %s is not supported by the Mie table path" % kind)
+    mlist = params["mList"]
+    lams = np.asarray(mlist[0][0])
+    part_mr = [_interp(m[0], m[1]) for m in mlist]
+    part_mi = [_interp(m[0], m[2]) for m in mlist]
+    rhdep = params["rhDep"]["type"]
+    if rhdep != "trivial":
+        if water is None:
+            raise ValueError("the water refractive index table is required for rhDep %s" % rhdep)
+        w_mr, w_mi = _interp(water[0], water[1]), _interp(water[0], water[2])
+    ang = angle_grid() if ang is None else np.asarray(ang, np.float64)
+    rh_table = list(params["rh"])
+    table = OpticsTable(len(radii), rh_table, lams, ang, classic=classic)
+    rhop00 = params["rhop0"]
+    cost = np.cos(np.radians(ang))
+    from .s12 import angle_tables_dev
+    for radind in range(len(radii)):
+        rhop0 = rhop00[radind] if isinstance(rhop00, list) else rhop00
+        xx, dr = particle.initializeXarr(params, radind, lams[0], lams[-1])
+        nmax_cap = int(np.rint(2 + xx.max() + 4 * xx.max() ** (1. / 3.)))
+        pi_t, tau_t = angle_tables_dev(cost, nmax_cap)
+        dev_pass = _WavelengthPass(xx, pi_t, tau_t, ang.size, precision)
+        fracs = pp["fracs"][radind]
+        for li, lam in enumerate(lams):
+            _wavelength(table, params, radind, li, lam, xx, dr, fracs, rhop0, dev_pass, ang,
+                        part_mr, part_mi, (w_mr, w_mi) if rhdep != "trivial" else None)
+    return table
+
+
+def _wavelength(table, params, radind, li, lam, xx, dr, fracs, rhop0, dev_pass, ang, part_mr,
+                part_mi, water):
+    """All rh entries of one (bin, wavelength) (dointegration.py:811-1031)."""
+    nref0 = [complex(f(lam), -g(lam)) for f, g in zip(part_mr, part_mi)]
+    nw = complex(water[0](lam), water[1](lam)) if water is not None else complex(1, 0)
+    rh = list(params["rh"])
+    if "maxrh" in params:     # rh above maxrh computed at maxrh (:824-828)
+        rh = list(np.where(np.array(rh) > params["maxrh"], params["maxrh"], np.array(rh)))
+    trivial = params["rhDep"]["type"] == "trivial"
+    # dry state: effective radii the mass efficiencies are referred to (:835-836)
+    _, _, _, rrat0 = particle.getHumidRefractiveIndex(params, radind, 0, rh, nref0, nw)
+    _, reff0, _, _ = particle.calculatePSD(params, radind, 0., rh, xx, dr, rrat0, lam)
+    todo = [0] if trivial else list(range(len(rh)))
+    blocks, rows, meta = [], [], []
+    for rhi in todo:
+        mr, mi, gf, rrat = particle.getHumidRefractiveIndex(params, radind, rhi, rh, nref0, nw)
+        psd, _, rlo, rup = particle.calculatePSD(params, radind, rh[rhi], rh, xx, dr, rrat, lam)
+        rhop = rrat ** 3. * rhop0 + (1. - rrat ** 3.) * RHOW
+        b0 = len(blocks)
+        blocks.extend(complex(a, b) for a, b in zip(mr, mi))
+        n_ri = len(mr)
+        mode_rows = []
+        for k in range(len(fracs)):
+            mode_rows.append(len(rows))
+            rows.append((b0 + (k if n_ri > 1 else 0), psd[k], reff0[k], rhop))
+        meta.append((rhi, mr, mi, gf, rhop, rlo, rup, mode_rows))
+    pm, sc = dev_pass.run(lam, blocks, rows, rhop0)
+    # mix modes in integratePSD's order, then normalise every entry on the device
+    mixed_p = np.zeros((len(meta), 4, ang.size))
+    mixed_s = np.zeros((len(meta), len(SCALAR_OUT)))
+    for e, m in enumerate(meta):
+        for k, r in enumerate(m[7]):
+            mixed_p[e] += fracs[k] * pm[r]
+            mixed_s[e] += fracs[k] * sc[r]
+    p_norm, pback = _normalize(mixed_p, ang)
+    mass0 = None
+    for e, (rhi, mr, mi, gf, rhop, rlo, rup, _) in enumerate(meta):
+        s = dict(zip(SCALAR_OUT, mixed_s[e]))
+        ret = {"qext": s["qext"], "qsca": s["qsca"], "qabs": s["qabs"], "qb": s["qb"],
+               "g": s["g"], "csca": s["csca"], "cext": s["cext"], "bsca": s["bsca"],
+               "bext": s["bext"], "bbck": s["bbck"]}
+        ret["lidar_ratio"] = s["qext"] / s["qb"] * 4 * np.pi
+        ret["ssa"] = s["qsca"] / s["qext"]
+        p11, p12, p33, p34 = p_norm[e]
+        ret.update(p11=p11, p12=p12, p22=p11.copy(), p33=p33, p34=p34, p44=p33.copy())
+        ret["pback"] = pback[e]
+        if rhi == 0:
+            mass0 = s["volume"] * rhop0
+        nl = {"mass": s["mass"], "rEff": s["rEff"], "rMass": s["rMass"], "rLow": rlo, "rUp": rup,
+              "growth_factor": gf, "rhop": rhop, "area": s["area"] / mass0,
+              "volume": s["volume"] / mass0}
+        ret["refreal"] = mr[0]
+        ret["refimag"] = -np.abs(mi[0])
+        table.set_entry(radind, li, rhi, ret)
+        table.set_bin_rh(radind, rhi, nl)
+    if trivial:   # copyDryValues (:462-490): every rh repeats the dry entry
+        for rhi in range(1, len(rh)):
+            table.copy_rh(radind, li, 0, rhi)
+
+
+def fun(partID0, oppfx=".", oppclassic=False, data_dir=None, precision=0):
+    """runoptics for one particle file: builds the table and writes
+    optics_<name>.nomom.nc4 (or .nomom.legacy.nc4) under oppfx
+    (dointegration.py:672-1036, 205-213)."""
+    part = os.path.basename(partID0).replace(".json", "")
+    src = partID0.replace("-orig", "") if "-orig" in part else partID0
+    params = particle.getPPJSON(src, data_dir)
+    water = None
+    if params["rhDep"]["type"] != "trivial":
+        water = particle.getWaterM(data_dir=data_dir)
+    table = build_table(params, water=water, classic=oppclassic, precision=precision)
+    name = "optics_%s.nomom.legacy.nc4" % part if oppclassic else "optics_%s.nomom.nc4" % part
+    path = os.path.join(oppfx, name)
+    table.write(path)
+    return path

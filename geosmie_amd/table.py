@@ -138,6 +138,23 @@ class OpticsTable(object):
             if k in ret:
                 self.data[k][radind, rhi] = float(ret[k])
 
+    def copy_rh(self, radind, li, src, dst):
+        """Entry (radind, li, dst) and the (radind, dst) scalars := those at rh
+        index src (copyDryValues, dointegration.py:462-490)."""
+        for k, dims in self.dims.items():
+            a = self.data[k]
+            idx_s, idx_d = [], []
+            for d in dims:
+                if d == self.radius:
+                    idx_s.append(radind), idx_d.append(radind)
+                elif d == "rh":
+                    idx_s.append(src), idx_d.append(dst)
+                elif d == self.lamname:
+                    idx_s.append(li), idx_d.append(li)
+                else:
+                    idx_s.append(slice(None)), idx_d.append(slice(None))
+            a[tuple(idx_d)] = a[tuple(idx_s)]
+
     def _put(self, key, radind, li, rhi, value):
         if self.classic:
             self.data[key][radind, rhi, li] = value

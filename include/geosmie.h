@@ -232,6 +232,17 @@ int gm_pmatrix_normalize(gm_ctx* ctx, int64_t n_entries, int n_ang, const double
 int gm_gsf_expand(gm_ctx* ctx, int64_t n_entries, int n_ang, const double* ang_deg_host,
                   const double* F, int ng, int quantize, double* out, gm_stream stream);
 
+/* Band means of table spectra (bandaverage.doAverage, src/geosmie/
+ * bandaverage.py:18-50): out[s][b] = mean over the n_sub (<= 128) sample
+ * wavenumbers samples[b][0..n_sub) of numpy.interp(sample, wavenumber,
+ * y[s]) (the rule scipy's interp1d applies to 1-D float64 data), summed in
+ * numpy's pairwise order; bit-identical to the reference.  wavenumber
+ * [n_lam] ascending, y [n_series][n_lam] in that order, samples already
+ * clipped to the wavenumber range (bandaverage.py:38-41); all device. */
+int gm_band_average(gm_ctx* ctx, int64_t n_series, int n_lam, const double* wavenumber,
+                    const double* y, int n_band, int n_sub, const double* samples, double* out,
+                    gm_stream stream);
+
 /* out[s][i] = sum_l coef[s][l] P_l(x[i]) for n_series series of L terms
  * (hbleg, src/gsf/convertncdf.py:32-43). */
 int gm_legendre_eval(gm_ctx* ctx, int64_t n_x, const double* x, int n_series, int L,

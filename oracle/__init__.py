@@ -16,6 +16,8 @@ gsf        numpy restatement of ``spher_expan.f`` (Gauss nodes, linear
            of the ``hbleg`` Legendre evaluator.
 mpmie      arbitrary-precision Mie (Bohren & Huffman) on mpmath, an
            independent check of the FP64 restatement at any size.
+runoptics  the Mie table loop of runoptics (every (bin, wavelength, rh)
+           entry of one particle table) over the pieces above.
 cmie       ctypes loader for ``liboracle_mie.so`` built from ``cmie.c``: the
            same homogeneous/coated algorithm in C (OpenMP), used as the fast
            oracle for large parity sizes and as the host-CPU baseline.

@@ -1,0 +1,88 @@
+"""runoptics tables on the GPU (geosmie_amd.runoptics) against the oracle's
+CPU restatement of the same loop (oracle/runoptics.py): every variable of
+every (bin, wavelength, rh) entry, both layouts, and the file written by
+fun() from a particle file on disk."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from tests.runoptics_cases import LAM, NAMES, params, water
+
+pytestmark = pytest.mark.gpu
+SCAL = ["qext", "qsca", "qabs", "g", "ssa", "qb", "bsca", "bext", "csca", "cext", "bbck",
+        "lidar_ratio", "refreal", "refimag"]
+NL = ["mass", "volume", "area", "rEff", "rMass", "rUp", "rLow", "rhop", "growth_factor"]
+P = ["p11", "p12", "p22", "p33", "p34", "p44"]
+
+
+@pytest.fixture(scope="module")
+def oracle_tables():
+    from oracle import runoptics as ORO
+    return {n: ORO.table(params(n), water()) for n in NAMES}
+
+
+@pytest.mark.parametrize("name", NAMES)
+def test_table_vs_oracle(gpu, oracle_tables, name):
+    from geosmie_amd.runoptics import build_table
+    print("runoptics case", name, flush=True)
+    tab = build_table(params(name), water=water())
+    ent, nl = oracle_tables[name]
+    d = tab.data
+    for (b, li, rhi), e in ent.items():
+        scale = np.abs(e["p11"]).max()
+        for k in P:
+            err = np.abs(d[k][b, li, rhi] - e[k]).max()
+            assert err <= 1e-9 * scale, (name, b, li, rhi, k, err)
+        np.testing.assert_allclose(d["pback"][b, li, rhi], e["pback"], rtol=1e-9,
+                                   atol=1e-9 * scale)
+        for k in SCAL:
+            assert abs(d[k][b, li, rhi] - e[k]) <= 1e-9 * abs(e[k]) + 1e-300, (name, k)
+    for (b, rhi), v in nl.items():
+        for k in NL:
+            assert abs(d[k][b, rhi] - v[k]) <= 1e-9 * abs(v[k]), (name, b, rhi, k)
+
+
+def test_classic_layout_and_file(gpu, tmp_path):
+    from geosmie_amd.runoptics import build_table
+    from geosmie_amd.table import read
+    new = build_table(params("bimodal_trivial"))
+    old = build_table(params("bimodal_trivial"), classic=True)
+    np.testing.assert_array_equal(old.data["p34"], new.data["p34"].transpose(0, 2, 1, 3))
+    np.testing.assert_array_equal(old.data["qext"], new.data["qext"].transpose(0, 2, 1))
+    np.testing.assert_array_equal(old.data["pback"], new.data["pback"].transpose(3, 0, 2, 1))
+    path = str(tmp_path / "t.nc4")
+    new.write(path)
+    v = read(path)
+    np.testing.assert_array_equal(v["g"][0], new.data["g"])
+    assert v["p11"][1] == ("bin", "wavelength", "rh", "ang")
+
+
+def test_fun_from_particle_file(gpu, tmp_path):
+    """runoptics' entry point: particle JSON + wsv refractive-index table +
+    water table on disk -> optics_<name>.nomom.nc4 equal to build_table."""
+    from geosmie_amd.runoptics import build_table, fun
+    from geosmie_amd.table import read
+    p = params("lognorm_simple")
+    m = p.pop("mList")[0]
+    os.makedirs(tmp_path / "data")
+    np.savetxt(tmp_path / "data" / "ri-test.wsv", np.array([m[0] * 1e6, m[1], m[2]]).T,
+               header="lambda[um] m_real m_imaginary")
+    w = water()
+    with open(tmp_path / "data" / "refrac.water.txt", "w") as f:
+        f.write("\n" * 13)
+        for row in np.array([w[0] * 1e6, w[1], w[2]]).T:
+            f.write("  %.6f  %.6f  %.6e\n" % tuple(row))
+    p["ri"] = {"format": "wsv", "path": ["data/ri-test.wsv"]}
+    with open(tmp_path / "tst.json", "w") as f:
+        json.dump(p, f)
+    out = fun(str(tmp_path / "tst.json"), oppfx=str(tmp_path), data_dir=str(tmp_path))
+    assert os.path.basename(out) == "optics_tst.nomom.nc4"
+    v = read(out)
+    from geosmie_amd import particle
+    pj = particle.getPPJSON(str(tmp_path / "tst.json"), str(tmp_path))
+    ref = build_table(pj, water=particle.getWaterM(data_dir=str(tmp_path)))
+    for k in ("p11", "qext", "bbck", "rhop", "pback"):
+        np.testing.assert_array_equal(v[k][0], ref.data[k])
+    np.testing.assert_allclose(v["wavelength"][0], LAM, rtol=1e-12)
