@@ -304,6 +304,14 @@ def lut_surrogate(n_queries=1_000_000, n_sample=4096, seed=20261202):
     net = rows[torch.from_numpy(pick).to(rows.device)].cpu().numpy()
     err = rel_errors(net, plain.cpu().numpy())
     t_total = t_net + n_fb * per_query
+    # end to end on the sample: network + gate + plain Mie re-solve of the flagged queries
+    qs = qd[torch.from_numpy(pick).to(qd.device)]
+    sur.query(qs[:256])
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    _, nf_s = sur.query(qs)
+    torch.cuda.synchronize()
+    t_e2e = time.perf_counter() - t0
     out["surrogate"] = {
         "queries": n, "served_by_network": n - n_fb, "fallback": n_fb,
         "network_s": t_net, "network_entries_per_s": n / t_net,
@@ -313,6 +321,8 @@ def lut_surrogate(n_queries=1_000_000, n_sample=4096, seed=20261202):
                       "time per query (plain timed on %d random queries)" % pick.size,
         "speedup_vs_plain_per_query": (n / t_total) * per_query,
         "speedup_vs_plain_grid": (n / t_total) / out["plain_grid"]["value"],
+        "sample_end_to_end": {"queries": int(pick.size), "fallback": int(nf_s), "s": t_e2e,
+                              "plain_s": t_plain, "speedup_vs_plain": t_plain / t_e2e},
         "served_err_vs_plain": None if not served.any() else {
             "qext_max": float(err[served, 0].max()), "ssa_max": float(err[served, 1].max()),
             "g_max": float(err[served, 2].max()),

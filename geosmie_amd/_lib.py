@@ -81,10 +81,10 @@ SIGNATURES = {
                                    ctypes.c_float, ctypes.c_float, ctypes.c_float, c_i64, c_vp]),
     "gm_surrogate_prep": (ctypes.c_int, [c_vp, c_i64, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp,
                                          c_i64, c_dbl, c_vp, c_vp, c_vp]),
-    "gm_surrogate_post": (ctypes.c_int, [c_vp, c_i64, ctypes.c_int, c_vp, c_vp, c_vp, c_vp,
-                                         c_vp]),
-    "gm_surrogate_targets": (ctypes.c_int, [c_vp, c_i64, ctypes.c_int, c_vp, c_vp, c_vp, c_vp,
-                                            c_vp]),
+    "gm_surrogate_post": (ctypes.c_int, [c_vp, c_i64, ctypes.c_int, c_vp, c_dbl, c_vp, c_vp,
+                                         c_vp, c_vp, c_vp]),
+    "gm_surrogate_targets": (ctypes.c_int, [c_vp, c_i64, ctypes.c_int, c_vp, c_dbl, c_vp, c_vp,
+                                            c_vp, c_vp, c_vp]),
 }
 
 

@@ -17,9 +17,17 @@
 // 2: inside, but its cell of the uniform n_cells grid over the box is not
 // trusted (validation error above the bound).
 // Table rows and network targets (before standardisation) correspond as
-//   t0 = ln Qext,  t1 = ln SSA,  t2 = ln max(g, 1e-8),  t_{3+l} = alpha1_l / (2l + 1):
+//   t0 = ln Qext - ln B,  t1 = ln SSA,  t2 = ln max(g, 1e-8),  t_{3+l} = alpha1_l / (2l + 1):
 // the squared error of t0..t2 is the squared relative error of Qext, SSA and
-// g, the quantities the gate judges.
+// g, the quantities the gate judges.  B is the bulk Rayleigh extinction of
+// the query's lognormal, saturated at 2 (models trained without it pass no
+// queries and use B = 1):
+//   Q_R = 8/3 |K|^2 x_eff^4 exp(6 s^2) + 4 Im(K) x_eff,   B = Q_R / (1 + Q_R / 2),
+// K = (m^2 - 1) / (m^2 + 2) at the volume-mixed index, s = ln sigma_g.  These
+// are the small-particle efficiencies Q_sca = 8/3 |K|^2 x^4, Q_abs = 4 Im(K) x
+// averaged over the area-weighted lognormal: <r^6>/<r^2> = r_eff^4 exp(6 s^2),
+// <r^3>/<r^2> = r_eff.  Below x_eff ~ 0.3 bulk Qext spans many decades and
+// follows Q_R to O(x^2); the network then only learns the PSD/grid residual.
 #include <math.h>
 #include <stdint.h>
 
@@ -69,30 +77,48 @@ __global__ void k_surr_prep(int64_t n, const float* __restrict__ q, const float*
   flags[i] = flag;
 }
 
-__global__ void k_surr_post(int64_t n, int n_out, const float* __restrict__ y,
-                            const float* __restrict__ tmean, const float* __restrict__ tstd,
-                            float* __restrict__ rows) {
+// ln B of one query (see the header comment); FP32 throughout, log domain.
+__device__ __forceinline__ float rayleigh_log_b(const float* __restrict__ qi, float mw_re) {
+  const float dry = 1.0f - qi[5];
+  const float lx = qi[0] - logf(qi[4]) + 1.8378770664093453f;   // ln x_eff
+  const float ls = logf(qi[1]);
+  const float mr = mw_re + (qi[2] - mw_re) * dry, mi = expf(qi[3]) * dry;
+  const float a = mr * mr - mi * mi, b = 2.0f * mr * mi;          // m^2 = a + i b
+  const float den = (a + 2.0f) * (a + 2.0f) + b * b;
+  const float k2 = ((a - 1.0f) * (a - 1.0f) + b * b) / den;      // |K|^2
+  const float ln_sca = 0.98082925301172f + logf(k2) + 4.0f * lx + 6.0f * ls * ls;   // ln 8/3
+  const float ln_abs = 1.38629436111989f + logf(3.0f * b / den) + lx;               // ln 4
+  const float hi = fmaxf(ln_sca, ln_abs), lo = fminf(ln_sca, ln_abs);
+  const float lq = hi + log1pf(expf(lo - hi));                   // ln Q_R
+  const float u = lq - 0.69314718055995f;                        // ln(Q_R / 2)
+  return lq - (fmaxf(u, 0.0f) + log1pf(expf(-fabsf(u))));        // ln Q_R - ln(1 + Q_R/2)
+}
+
+__global__ void k_surr_post(int64_t n, int n_out, const float* __restrict__ q, float mw_re,
+                            const float* __restrict__ y, const float* __restrict__ tmean,
+                            const float* __restrict__ tstd, float* __restrict__ rows) {
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= n * n_out) return;
   const int j = (int)(e % n_out);
   const float t = y[e] * tstd[j] + tmean[j];
   float v;
-  if (j == 0) v = expf(t);
+  if (j == 0) v = expf(q ? t + rayleigh_log_b(q + (e / n_out) * kSurrIn, mw_re) : t);
   else if (j == 1) v = fminf(expf(t), 1.0f);
   else if (j == 2) v = expf(t);
   else v = (float)(2 * (j - 3) + 1) * t;
   rows[e] = v;
 }
 
-__global__ void k_surr_targets(int64_t n, int n_out, const float* __restrict__ rows,
-                               const float* __restrict__ tmean, const float* __restrict__ tstd,
-                               float* __restrict__ y) {
+__global__ void k_surr_targets(int64_t n, int n_out, const float* __restrict__ q, float mw_re,
+                               const float* __restrict__ rows, const float* __restrict__ tmean,
+                               const float* __restrict__ tstd, float* __restrict__ y) {
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= n * n_out) return;
   const int j = (int)(e % n_out);
   const float v = rows[e];
   float t;
-  if (j == 0 || j == 1) t = logf(v);
+  if (j == 0) t = q ? logf(v) - rayleigh_log_b(q + (e / n_out) * kSurrIn, mw_re) : logf(v);
+  else if (j == 1) t = logf(v);
   else if (j == 2) t = logf(fmaxf(v, kGFloor));
   else t = v / (float)(2 * (j - 3) + 1);
   y[e] = (t - tmean[j]) / tstd[j];
@@ -135,8 +161,9 @@ int gm_surrogate_prep(gm_ctx* ctx, int64_t n, const float* q, const float* feat_
   return GM_OK;
 }
 
-int gm_surrogate_post(gm_ctx* ctx, int64_t n, int n_out, const float* y, const float* tgt_mean,
-                      const float* tgt_std, float* rows, gm_stream stream) {
+int gm_surrogate_post(gm_ctx* ctx, int64_t n, int n_out, const float* q, double m_water_re,
+                      const float* y, const float* tgt_mean, const float* tgt_std, float* rows,
+                      gm_stream stream) {
   if (!ctx || n < 0 || n_out < 3) return gm_fail(GM_EINVAL, "gm_surrogate_post: bad arguments");
   if (n == 0) return GM_OK;
   if (!y || !tgt_mean || !tgt_std || !rows) return gm_fail(GM_EINVAL, "gm_surrogate_post: NULL array");
@@ -144,14 +171,16 @@ int gm_surrogate_post(gm_ctx* ctx, int64_t n, int n_out, const float* y, const f
   if (e != hipSuccess) return gm_fail(GM_ERUNTIME, "hipSetDevice: %s", hipGetErrorString(e));
   const int64_t tot = n * n_out;
   hipLaunchKernelGGL(gm::k_surr_post, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0,
-                     (hipStream_t)stream, n, n_out, y, tgt_mean, tgt_std, rows);
+                     (hipStream_t)stream, n, n_out, q, (float)m_water_re, y, tgt_mean, tgt_std,
+                     rows);
   e = hipGetLastError();
   if (e != hipSuccess) return gm_fail(GM_ERUNTIME, "k_surr_post: %s", hipGetErrorString(e));
   return GM_OK;
 }
 
-int gm_surrogate_targets(gm_ctx* ctx, int64_t n, int n_out, const float* rows,
-                         const float* tgt_mean, const float* tgt_std, float* y, gm_stream stream) {
+int gm_surrogate_targets(gm_ctx* ctx, int64_t n, int n_out, const float* q, double m_water_re,
+                         const float* rows, const float* tgt_mean, const float* tgt_std, float* y,
+                         gm_stream stream) {
   if (!ctx || n < 0 || n_out < 3) return gm_fail(GM_EINVAL, "gm_surrogate_targets: bad arguments");
   if (n == 0) return GM_OK;
   if (!rows || !tgt_mean || !tgt_std || !y)
@@ -160,7 +189,8 @@ int gm_surrogate_targets(gm_ctx* ctx, int64_t n, int n_out, const float* rows,
   if (e != hipSuccess) return gm_fail(GM_ERUNTIME, "hipSetDevice: %s", hipGetErrorString(e));
   const int64_t tot = n * n_out;
   hipLaunchKernelGGL(gm::k_surr_targets, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0,
-                     (hipStream_t)stream, n, n_out, rows, tgt_mean, tgt_std, y);
+                     (hipStream_t)stream, n, n_out, q, (float)m_water_re, rows, tgt_mean,
+                     tgt_std, y);
   e = hipGetLastError();
   if (e != hipSuccess) return gm_fail(GM_ERUNTIME, "k_surr_targets: %s", hipGetErrorString(e));
   return GM_OK;

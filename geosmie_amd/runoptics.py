@@ -60,11 +60,13 @@ class _WavelengthPass(object):
         dm = _DevMie(x, m)
         d_mu = dm.mueller(self.pi_t, self.tau_t, self.precision)
         nr = len(rows)
-        w = np.zeros((nr, n))
-        for r, (b, psd, _, _) in enumerate(rows):
-            w[r, b * nx:(b + 1) * nx] = psd
-        d_w = _lib.to_dev(w)
         dev = _lib.device()
+        # dense [rows][points] weights, zero outside each row's block, assembled
+        # on the device from the compact [rows][grid] PSDs
+        d_psd = _lib.to_dev(np.stack([r[1] for r in rows]))
+        d_w = t.zeros((nr, nb, nx), dtype=t.float64, device=dev)
+        d_w[t.arange(nr, device=dev), t.tensor([r[0] for r in rows], device=dev)] = d_psd
+        d_w = d_w.view(nr, n)
         n_cols = 4 * self.n_ang
         d_pm = t.empty(nr * n_cols, dtype=t.float64, device=dev)
         L, h, s = _lib.lib(), _lib.ctx(), stream()
@@ -76,7 +78,7 @@ class _WavelengthPass(object):
                                          ptr(d_w[r0:r0 + k]), ptr(d_mu),
                                          ptr(d_pm[r0 * n_cols:(r0 + k) * n_cols]), s),
                   "gm_psd_contract_rows")
-        d_lp = _lib.to_dev(np.full(n, float(lam)))
+        d_lp = t.full((n,), float(lam), dtype=t.float64, device=dev)
         d_r0 = _lib.to_dev(np.array([r[2] for r in rows], np.float64))
         d_rho = _lib.to_dev(np.array([r[3] for r in rows], np.float64))
         d_sc = t.empty(nr * len(SCALAR_OUT), dtype=t.float64, device=dev)

@@ -38,7 +38,8 @@ DIMS = (6, 256, 256, 256, 256, 35)
 N_OUT = 3 + N_MOM
 GATE_CELLS = (8, 6, 6, 10, 10, 6)
 G_FLOOR = 1e-8        # floor of g in its log target (gm_surrogate.hip)
-FEATURES = 2          # version of the feature / target transforms a saved model was trained with
+FEATURES = 3          # version of the feature / target transforms new models are trained with
+# 2: Qext target ln Qext; 3: ln(Qext / B), B the bulk Rayleigh extinction saturated at 2
 
 
 def sigma_factor(sigma):
@@ -149,9 +150,12 @@ class Surrogate(object):
 
     def __init__(self, params, feat_mean, feat_std, tgt_mean, tgt_std, box_lo, box_hi,
                  cell_ok, cells=GATE_CELLS, r_dry=None, bound=1e-3, dims=DIMS, info=None,
-                 weight_terms=1):
+                 weight_terms=1, features=FEATURES):
         t = _lib.torch()
         self.t = t
+        if features not in (2, 3):
+            raise ValueError("unknown feature transform %d" % features)
+        self.features = int(features)
         self.dims = tuple(int(d) for d in dims)
         self.mlp = MLP(self.dims, params=params, weight_terms=weight_terms)
         self.feat_mean = np.asarray(feat_mean, np.float32)
@@ -203,9 +207,14 @@ class Surrogate(object):
                                   ptr(x), ptr(flags), s), "gm_surrogate_prep")
         y = self.mlp.forward(x)
         rows = t.empty_like(y)
-        check(L.gm_surrogate_post(h, n, self.dims[-1], ptr(y), ptr(d["tgt_mean"]),
-                                  ptr(d["tgt_std"]), ptr(rows), s), "gm_surrogate_post")
+        check(L.gm_surrogate_post(h, n, self.dims[-1], self._baseline_q(q), M_WATER.real, ptr(y),
+                                  ptr(d["tgt_mean"]), ptr(d["tgt_std"]), ptr(rows), s),
+              "gm_surrogate_post")
         return rows, flags
+
+    def _baseline_q(self, q):
+        """The queries the Qext baseline is formed from (None: no baseline)."""
+        return ptr(q) if self.features >= 3 else None
 
     # -- plain GPU Mie -----------------------------------------------------
     def solver(self):
@@ -246,7 +255,7 @@ class Surrogate(object):
                             box_lo=self.box_lo, box_hi=self.box_hi, cell_ok=self.cell_ok,
                             cells=np.array(self.cells), r_dry=self.r_dry, bound=self.bound,
                             dims=np.array(self.dims), weight_terms=self.mlp.weight_terms,
-                            features=FEATURES,
+                            features=self.features,
                             info_keys=np.array(list(self.info.keys()), dtype=str),
                             info_vals=np.array([float(v) for v in self.info.values()]))
 
@@ -254,14 +263,15 @@ class Surrogate(object):
     def load(cls, path):
         z = np.load(path, allow_pickle=False)
         ver = int(z["features"]) if "features" in z.files else 1
-        if ver != FEATURES:
-            raise ValueError("%s was trained with feature transform %d, this build uses %d: "
-                             "retrain with python -m geosmie_amd.train_surrogate" % (path, ver, FEATURES))
+        if ver not in (2, 3):
+            raise ValueError("%s was trained with feature transform %d, this build reads 2 and "
+                             "3: retrain with python -m geosmie_amd.train_surrogate" % (path, ver))
         info = dict(zip([str(k) for k in z["info_keys"]], z["info_vals"].tolist()))
         return cls(z["params"], z["feat_mean"], z["feat_std"], z["tgt_mean"], z["tgt_std"],
                    z["box_lo"], z["box_hi"], z["cell_ok"], cells=tuple(z["cells"]),
                    r_dry=z["r_dry"], bound=float(z["bound"]), dims=tuple(z["dims"]), info=info,
-                   weight_terms=int(z["weight_terms"]) if "weight_terms" in z.files else 1)
+                   weight_terms=int(z["weight_terms"]) if "weight_terms" in z.files else 1,
+                   features=ver)
 
 
 def rel_errors(rows, ref):
@@ -283,11 +293,29 @@ def _features(q):
                      q[:, 3] + np.log(dry), lgf - llam, q[:, 5]], axis=-1)
 
 
-def _targets_host(R):
-    """Targets (before standardisation) of table rows, as gm_surrogate_targets forms them."""
+def rayleigh_log_b(q):
+    """ln B of queries (FP64), B = Q_R / (1 + Q_R / 2) with Q_R the bulk
+    Rayleigh extinction 8/3 |K|^2 x_eff^4 exp(6 s^2) + 4 Im(K) x_eff at the
+    volume-mixed index (gm_surrogate_post forms the same in FP32)."""
+    q = np.asarray(q, np.float64)
+    dry = 1.0 - q[:, 5]
+    lx = q[:, 0] - np.log(q[:, 4]) + np.log(2 * np.pi)
+    ls = np.log(q[:, 1])
+    mw = M_WATER.real
+    m = (mw + (q[:, 2] - mw) * dry) + 1j * np.exp(q[:, 3]) * dry
+    K = (m * m - 1) / (m * m + 2)
+    ln_sca = np.log(8. / 3.) + np.log(np.abs(K) ** 2) + 4 * lx + 6 * ls * ls
+    ln_abs = np.log(4.) + np.log(K.imag) + lx
+    lq = np.logaddexp(ln_sca, ln_abs)
+    return lq - np.logaddexp(0.0, lq - np.log(2.0))
+
+
+def _targets_host(R, Q=None):
+    """Targets (before standardisation) of table rows, as gm_surrogate_targets
+    forms them (Q given: with the Rayleigh baseline of transform 3)."""
     R = np.asarray(R, np.float64)
     T = np.empty_like(R)
-    T[:, 0] = np.log(R[:, 0])
+    T[:, 0] = np.log(R[:, 0]) - (rayleigh_log_b(Q) if Q is not None else 0.0)
     T[:, 1] = np.log(R[:, 1])
     T[:, 2] = np.log(np.maximum(R[:, 2], G_FLOOR))
     T[:, 3:] = R[:, 3:] / (2 * np.arange(R.shape[1] - 3) + 1)
@@ -325,7 +353,7 @@ def new_surrogate(g, Q, rows, seed=0, cells=GATE_CELLS, bound=1e-3):
     F = _features(Q[sub])
     feat_mean, feat_std = F.mean(0), F.std(0)
     feat_std[feat_std == 0] = 1.0
-    T = _targets_host(rows[t.from_numpy(sub).to(rows.device)].cpu().numpy())
+    T = _targets_host(rows[t.from_numpy(sub).to(rows.device)].cpu().numpy(), Q[sub])
     tgt_mean, tgt_std = T.mean(0), T.std(0)
     tgt_std[tgt_std < 1e-6] = 1.0
     box_lo = np.array([np.log(g["rmode"].min()), g["sigma"].min(), g["mr"].min(),
@@ -352,8 +380,9 @@ def training_arrays(sur, Q, rows):
                               ptr(d["box_lo"]), ptr(d["box_hi"]), sur._ccells, ptr(sur._cell_ok),
                               sur._cell_ok.numel(), M_WATER.real, ptr(X), ptr(flags), s),
           "gm_surrogate_prep")
-    check(L.gm_surrogate_targets(h, n, N_OUT, ptr(rows), ptr(d["tgt_mean"]), ptr(d["tgt_std"]),
-                                 ptr(Y), s), "gm_surrogate_targets")
+    check(L.gm_surrogate_targets(h, n, N_OUT, sur._baseline_q(d_q), M_WATER.real, ptr(rows),
+                                 ptr(d["tgt_mean"]), ptr(d["tgt_std"]), ptr(Y), s),
+          "gm_surrogate_targets")
     return X, Y
 
 

@@ -331,16 +331,23 @@ int gm_surrogate_prep(gm_ctx* ctx, int64_t n, const float* q, const float* feat_
                       double m_water_re, float* x, int32_t* flags, gm_stream stream);
 
 /* Network outputs y [n][n_out] (standardised) -> table rows [n][n_out]
- * (Qext, SSA, g, alpha1_0..): Qext = exp(t0), SSA = min(exp(t1), 1),
- * g = exp(t2), alpha1_l = (2l + 1) t_{3+l}, with t_j = y_j tgt_std[j] + tgt_mean[j]. */
-int gm_surrogate_post(gm_ctx* ctx, int64_t n, int n_out, const float* y, const float* tgt_mean,
-                      const float* tgt_std, float* rows, gm_stream stream);
+ * (Qext, SSA, g, alpha1_0..): Qext = B exp(t0), SSA = min(exp(t1), 1),
+ * g = exp(t2), alpha1_l = (2l + 1) t_{3+l}, with t_j = y_j tgt_std[j] + tgt_mean[j].
+ * B is the bulk Rayleigh extinction of query q[i] (device [n][6], the
+ * gm_surrogate_prep queries) saturated at 2:
+ *   Q_R = 8/3 |K|^2 x_eff^4 exp(6 ln^2 sigma_g) + 4 Im(K) x_eff,  B = Q_R / (1 + Q_R / 2),
+ * K = (m^2 - 1)/(m^2 + 2) at the index m_water_re + (m_r - m_water_re)(1 - f_w)
+ * + i m_i (1 - f_w); q = NULL gives B = 1 (models trained without it). */
+int gm_surrogate_post(gm_ctx* ctx, int64_t n, int n_out, const float* q, double m_water_re,
+                      const float* y, const float* tgt_mean, const float* tgt_std, float* rows,
+                      gm_stream stream);
 
 /* Inverse of gm_surrogate_post: table rows [n][n_out] -> standardised
- * training targets y [n][n_out] (t0 = ln Qext, t1 = ln SSA, t2 = ln max(g, 1e-8),
+ * training targets y [n][n_out] (t0 = ln Qext - ln B, t1 = ln SSA, t2 = ln max(g, 1e-8),
  * t_{3+l} = alpha1_l / (2l + 1); y_j = (t_j - tgt_mean[j]) / tgt_std[j]). */
-int gm_surrogate_targets(gm_ctx* ctx, int64_t n, int n_out, const float* rows,
-                         const float* tgt_mean, const float* tgt_std, float* y, gm_stream stream);
+int gm_surrogate_targets(gm_ctx* ctx, int64_t n, int n_out, const float* q, double m_water_re,
+                         const float* rows, const float* tgt_mean, const float* tgt_std, float* y,
+                         gm_stream stream);
 
 #ifdef __cplusplus
 }

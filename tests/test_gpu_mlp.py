@@ -182,11 +182,36 @@ def test_targets_post_roundtrip(gpu):
     y = t.empty_like(d_rows)
     back = t.empty_like(d_rows)
     L, h, s = _lib.lib(), _lib.ctx(), stream()
-    check(L.gm_surrogate_targets(h, n, 35, ptr(d_rows), ptr(d_m), ptr(d_s), ptr(y), s))
-    check(L.gm_surrogate_post(h, n, 35, ptr(y), ptr(d_m), ptr(d_s), ptr(back), s))
-    b = back.cpu().numpy()
-    np.testing.assert_allclose(b[:, :3], rows[:, :3], rtol=1e-5)
-    np.testing.assert_allclose(b[:, 3:], rows[:, 3:], rtol=1e-5, atol=1e-5)
+    from geosmie_amd.surrogate import M_WATER, _targets_host
+    q = _queries(n, 12)
+    d_q = f(q)
+    for qq in (None, d_q):     # transform 2 (no baseline) and 3 (Rayleigh baseline)
+        check(L.gm_surrogate_targets(h, n, 35, ptr(qq), M_WATER.real, ptr(d_rows), ptr(d_m),
+                                     ptr(d_s), ptr(y), s))
+        check(L.gm_surrogate_post(h, n, 35, ptr(qq), M_WATER.real, ptr(y), ptr(d_m), ptr(d_s),
+                                  ptr(back), s))
+        b = back.cpu().numpy()
+        np.testing.assert_allclose(b[:, :3], rows[:, :3], rtol=1e-5)
+        np.testing.assert_allclose(b[:, 3:], rows[:, 3:], rtol=1e-5, atol=1e-5)
+        # the device targets are the host formula's (FP32 vs FP64)
+        want = (_targets_host(rows, q if qq is not None else None) - mean) / std
+        np.testing.assert_allclose(y.cpu().numpy(), want, rtol=2e-5, atol=2e-4)
+
+
+def test_rayleigh_baseline_is_the_small_particle_limit(gpu):
+    """Bulk Qext of plain GPU Mie over a narrow lognormal of small particles
+    (x_eff <= 0.05) equals the baseline B to O(x^2) plus the grid's truncation."""
+    from geosmie_amd.surrogate import make_queries, rayleigh_log_b
+    sur = _toy_surrogate()
+    n = 64
+    r = np.random.default_rng(13)
+    q = make_queries(np.exp(r.uniform(np.log(0.02), np.log(0.04), n)), r.uniform(1.4, 1.5, n),
+                     r.uniform(1.3, 1.8, n), np.exp(r.uniform(np.log(1e-6), 0.0, n)),
+                     r.uniform(20.0, 40.0, n), np.ones(n))
+    plain = sur.solve_plain(q).cpu().numpy()
+    b = np.exp(rayleigh_log_b(q))
+    ratio = plain[:, 0] / (1.0 / (1.0 / b - 0.5))      # Q_R from B = Q_R / (1 + Q_R / 2)
+    assert np.all(np.abs(ratio - 1) < 0.03), ratio
 
 
 def test_gate_on_a_finer_cell_grid(gpu):

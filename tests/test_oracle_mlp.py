@@ -82,3 +82,34 @@ def test_adam_matches_definition():
     p1, m1, v1 = O.adam(p, g, m, v, 1e-3, 0.9, 0.999, 1e-8, 1)
     # first step of bias-corrected Adam moves every weight by lr * sign(g) (up to eps)
     np.testing.assert_allclose(p - p1, 1e-3 * np.sign(g), rtol=1e-6)
+
+
+def test_rayleigh_baseline_closed_forms():
+    """The surrogate's Qext baseline: one sphere (sigma_g = 1) at small x is
+    the Rayleigh extinction of the Mie series (oracle Mie to O(x^2)); B
+    saturates at 2 for large particles; a dropped factor of the moments
+    (exp(6 s^2)) changes B for broad distributions."""
+    from geosmie_amd.surrogate import M_WATER, make_queries, rayleigh_log_b
+    from oracle import mie as OM
+    for m in (1.5 + 0.01j, 1.33 + 1e-6j, 1.8 + 0.5j):
+        for x in (1e-3, 3e-3):
+            lam = 10.0
+            q = make_queries(np.array([x * lam / (2 * np.pi)]), np.array([1.0 + 1e-12]),
+                             np.array([m.real]), np.array([m.imag]), np.array([lam]),
+                             np.array([1.0]))
+            an, bn, nmax = OM.single_mie_coeff(m * m, 1.0, x)
+            qext = OM.mie_props_raw(an, bn, nmax, x)[0]
+            b = np.exp(rayleigh_log_b(q))[0]
+            q_r = 1.0 / (1.0 / b - 0.5)          # B = Q_R / (1 + Q_R / 2)
+            # queries are float32: ~1e-5 from the rounded radius and index
+            assert abs(q_r / qext - 1) < 10 * x * x + 5e-5, (m, x, q_r, qext)
+    big = make_queries(np.array([5.0]), np.array([2.0]), np.array([1.5]), np.array([1e-3]),
+                       np.array([0.2]), np.array([1.0]))
+    assert abs(np.exp(rayleigh_log_b(big))[0] - 2) < 1e-6
+    # broad distribution: Q_sca scales with exp(6 ln^2 sigma) at fixed r_eff
+    q1 = make_queries(np.array([0.01]), np.array([1.0 + 1e-12]), np.array([1.5]),
+                      np.array([1e-9]), np.array([3.0]), np.array([1.0]))
+    q2 = make_queries(np.array([0.01 * np.exp(-2.5 * np.log(2.0) ** 2)]), np.array([2.0]),
+                      np.array([1.5]), np.array([1e-9]), np.array([3.0]), np.array([1.0]))
+    d = rayleigh_log_b(q2)[0] - rayleigh_log_b(q1)[0]
+    assert abs(d - 6 * np.log(2.0) ** 2) < 1e-3
