@@ -91,3 +91,27 @@ def test_served_within_bound_and_faster(gpu, sur):
     assert frac >= 0.95
     assert np.median(err) <= sur.bound
     assert t_plain > 20 * t_net
+
+
+def test_end_to_end_gain_over_plain(gpu, sur):
+    """Uniform queries answered by query() (network, gate, plain re-solve of
+    the flagged ones) against plain GPU Mie for all of them: the surrogate
+    path with its fallback is several times faster, and every answer is
+    either the network's (served) or exactly plain Mie's."""
+    t = __import__("torch")
+    q = _uniform_queries(2048, 3)
+    sur.query(q[:128])
+    sur.solve_plain(q[:128])
+    t.cuda.synchronize()
+    t0 = time.perf_counter()
+    rows, nf = sur.query(q)
+    t.cuda.synchronize()
+    t_sur = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    plain = sur.solve_plain(q)
+    t.cuda.synchronize()
+    t_plain = time.perf_counter() - t0
+    print("uniform queries: %d of %d re-solved by plain Mie; surrogate+fallback %.4f s, plain "
+          "GPU Mie %.4f s (%.1fx)" % (nf, len(q), t_sur, t_plain, t_plain / t_sur))
+    assert nf < 0.2 * len(q)
+    assert t_plain > 4 * t_sur
