@@ -110,11 +110,19 @@ int gm_ctx_create(int device, gm_ctx** out) {
   c->n_cu = prop.multiProcessorCount;
   // resident one-wavefront blocks per CU of each persistent Mie kernel
   int occ = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, gm::k_mie_homog, 64, 0) == hipSuccess)
-    c->occ_homog = occ;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, gm::k_mie_coated_stable, 64, 0) ==
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, gm::k_mie_homog<1>, 64, 0) == hipSuccess)
+    c->occ_homog[0] = c->occ_homog[1] = occ;   // the unconstrained homog kernel allows 3 already
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, gm::k_mie_homog<4>, 64, 0) == hipSuccess)
+    c->occ_homog[2] = occ;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, gm::k_mie_coated_stable<1>, 64, 0) ==
       hipSuccess)
-    c->occ_stable = occ;
+    c->occ_stable[0] = occ;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, gm::k_mie_coated_stable<3>, 64, 0) ==
+      hipSuccess)
+    c->occ_stable[1] = occ;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, gm::k_mie_coated_stable<4>, 64, 0) ==
+      hipSuccess)
+    c->occ_stable[2] = occ;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, gm::k_mie_coated_literal, 64, 0) ==
       hipSuccess)
     c->occ_literal = occ;
@@ -173,6 +181,11 @@ int gm_ctx_tune(gm_ctx* ctx, int param, int value) {
         return gm_fail(GM_EINVAL, "gm_ctx_tune: S1/S2 points per wavefront must be 8 or 16");
       ctx->tune_s12_points = value;
       return GM_OK;
+    case GM_TUNE_MIE_WAVES:
+      if (value != 0 && value != 3 && value != 4)
+        return gm_fail(GM_EINVAL, "gm_ctx_tune: Mie waves per SIMD must be 0, 3 or 4");
+      ctx->tune_mie_waves = value;
+      return GM_OK;
     case GM_TUNE_MLP_WEIGHT_TERMS:
       if (value != 1 && value != 2)
         return gm_fail(GM_EINVAL, "gm_ctx_tune: MLP weight terms must be 1 or 2");
@@ -191,6 +204,7 @@ int gm_ctx_get_tune(gm_ctx* ctx, int param, int* value) {
     case GM_TUNE_PSD_CHUNK: *value = ctx->tune_psd_chunk; return GM_OK;
     case GM_TUNE_MLP_WEIGHT_TERMS: *value = ctx->tune_mlp_wterms; return GM_OK;
     case GM_TUNE_S12_POINTS: *value = ctx->tune_s12_points; return GM_OK;
+    case GM_TUNE_MIE_WAVES: *value = ctx->tune_mie_waves; return GM_OK;
     default: return gm_fail(GM_EINVAL, "gm_ctx_get_tune: unknown parameter %d", param);
   }
 }
@@ -300,8 +314,9 @@ int gm_mie_batch(gm_ctx* ctx, int64_t n, const double* x, const double* y, const
     g = std::min<int64_t>(g, std::max<int64_t>(1, (int64_t)(ctx->scratch_budget / slot_bytes)));
     return std::make_pair(g, (size_t)g * slot_bytes);
   };
-  auto gh = grid_for(ctx->occ_homog, gm::kHomogRowDoubles);
-  auto gs = grid_for(ctx->occ_stable, gm::kCoatedRowDoubles);
+  const int var = ctx->tune_mie_waves == 4 ? 2 : ctx->tune_mie_waves == 3 ? 1 : 0;
+  auto gh = grid_for(ctx->occ_homog[var], gm::kHomogRowDoubles);
+  auto gs = grid_for(ctx->occ_stable[var], gm::kCoatedRowDoubles);
   auto gl = grid_for(ctx->occ_literal, gm::kCoatedRowDoubles);
   size_t need = gh.second;
   if (y) need = std::max(need, std::max(gs.second, gl.second));
@@ -309,16 +324,19 @@ int gm_mie_batch(gm_ctx* ctx, int64_t n, const double* x, const double* y, const
   double* scratch = (double*)ctx->scratch.ptr;
   size_t slot;
   ctx->mark_begin(gm::kKMieHomog, s, &slot);
-  hipLaunchKernelGGL(gm::k_mie_homog, dim3((unsigned)gh.first), dim3(64), 0, s, order, range, n,
-                     x, y, eps, mu, eps2, coef_off, an, bn, props, status, scratch, rows_cap,
-                     dn_extra);
+  auto homog = var == 2 ? gm::k_mie_homog<4> : gm::k_mie_homog<1>;
+  hipLaunchKernelGGL(homog, dim3((unsigned)gh.first), dim3(64), 0, s, order, range, n, x, y, eps,
+                     mu, eps2, coef_off, an, bn, props, status, scratch, rows_cap, dn_extra);
   ctx->mark_end(slot, s);
   HIP_OK(hipGetLastError());
   if (y) {
     ctx->mark_begin(gm::kKMieCoated, s, &slot);
-    hipLaunchKernelGGL(gm::k_mie_coated_stable, dim3((unsigned)gs.first), dim3(64), 0, s, order,
-                       range, n, x, y, eps, mu, eps2, coef_off, an, bn, props, status, scratch,
-                       rows_cap, dn_extra);
+    auto stable = var == 2   ? gm::k_mie_coated_stable<4>
+                  : var == 1 ? gm::k_mie_coated_stable<3>
+                             : gm::k_mie_coated_stable<1>;
+    hipLaunchKernelGGL(stable, dim3((unsigned)gs.first), dim3(64), 0, s, order, range, n, x, y,
+                       eps, mu, eps2, coef_off, an, bn, props, status, scratch, rows_cap,
+                       dn_extra);
     ctx->mark_end(slot, s);
     HIP_OK(hipGetLastError());
     ctx->mark_begin(gm::kKMieCoatedLit, s, &slot);

@@ -19,7 +19,9 @@ struct Buf {
 struct gm_ctx {
   int device = 0;
   int n_cu = 256;
-  int occ_homog = 8, occ_stable = 8, occ_literal = 4;  // resident Mie wavefronts per CU
+  // resident Mie wavefronts per CU, per register-allocation variant (index
+  // 0: unconstrained, 1: >= 3 waves per SIMD, 2: >= 4 waves per SIMD)
+  int occ_homog[3] = {8, 8, 8}, occ_stable[3] = {8, 8, 8}, occ_literal = 4;
   size_t scratch_budget = (size_t)3 << 30;       // bytes of recurrence scratch
   // launch shapes chosen by gm_ctx_tune (an autotuner's knobs)
   int tune_mie_grid_mult = 1;   // persistent Mie grid = multiple of resident wavefronts
@@ -27,6 +29,7 @@ struct gm_ctx {
   int tune_s12_points = 8;      // points per wavefront of the FP64 S1/S2 kernel (8 or 16)
   int tune_psd_chunk = 128;     // points per split-K chunk of the few-PSD contraction
   int tune_mlp_wterms = 1;      // BF16 terms per MLP weight (1 = BF16 weights, 2 = hi + lo)
+  int tune_mie_waves = 0;       // Mie register variant: 0 unconstrained, 3 or 4 waves per SIMD
   Buf status;                                    // int32 flags + int32 max_nmax + scan scratch
   Buf order, key, hist, scratch, scan_tmp;
   Buf tab_pi, tab_tau, gemm_tmp, feat_tmp, gsf_tab, tab_lp, lut_tmp, mlp_ws;

@@ -137,6 +137,46 @@ __global__ __launch_bounds__(256) void k_contract_mfma(int n_psd, int64_t n_pts,
   }
 }
 
+// Narrow variant (n_cols <= 16: the integratePSD features): one wavefront per
+// (16 PSDs, chunk of kNarrowChunk points), so a batch of 10^4..10^5 points
+// spreads over hundreds of wavefronts; the loads of kNarrowAhead MFMA steps
+// are issued before their products, hiding their latency inside the wave.
+constexpr int kNarrowChunk = 256;
+constexpr int kNarrowAhead = 8;
+__global__ __launch_bounds__(64) void k_contract_mfma_narrow(int n_psd, int64_t n_pts, int n_cols,
+                                                             const double* __restrict__ w,
+                                                             const double* __restrict__ v,
+                                                             double* __restrict__ part) {
+  const int lane = threadIdx.x;
+  const int k0 = blockIdx.x * 16;
+  const int i = lane & 15, kq = lane >> 4;
+  const int ka = k0 + i;
+  const int64_t pbeg = (int64_t)blockIdx.y * kNarrowChunk;
+  const int64_t pend = min(pbeg + (int64_t)kNarrowChunk, n_pts);
+  double4_ acc = (double4_){0.0, 0.0, 0.0, 0.0};
+  for (int64_t p0 = pbeg; p0 < pend; p0 += 4 * kNarrowAhead) {
+    double a[kNarrowAhead], b[kNarrowAhead];
+#pragma unroll
+    for (int u = 0; u < kNarrowAhead; ++u) {
+      const int64_t p = p0 + 4 * u + kq;
+      const bool pin = p < pend;
+      a[u] = (pin && ka < n_psd) ? w[(size_t)ka * n_pts + p] : 0.0;
+      b[u] = (pin && i < n_cols) ? v[(size_t)p * n_cols + i] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < kNarrowAhead; ++u)
+      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u], b[u], acc, 0, 0, 0);
+  }
+  double* out = part + (size_t)blockIdx.y * n_psd * n_cols;
+  if (i < n_cols) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int k = k0 + kq + 4 * r;
+      if (k < n_psd) out[(size_t)k * n_cols + i] = acc[r];
+    }
+  }
+}
+
 // Per-size features for integratePSD (r = x lam / 2 pi; props planes
 // qext, qsca, qabs, qb, asy, qratio):
 //   0 1   1 r^2   2 r^3   3 r^4   4 qext r^2   5 qsca r^2   6 qabs r^2
@@ -300,6 +340,19 @@ int gm_psd_contract(gm_ctx* ctx, int n_psd, int64_t n_pts, int n_cols, const dou
     double* part = (double*)ctx->gemm_tmp.ptr;
     hipLaunchKernelGGL(gm::k_contract_small, dim3((n_cols + 255) / 256, nsplit), dim3(256), 0, s,
                        n_psd, n_pts, n_cols, chunk, w, v, part);
+    int64_t n_out = (int64_t)n_psd * n_cols;
+    hipLaunchKernelGGL(gm::k_reduce_splits, dim3((unsigned)((n_out + 255) / 256)), dim3(256), 0,
+                       s, nsplit, n_out, part, out);
+  } else if (n_cols <= 16) {
+    const int64_t nsplit64 = (n_pts + gm::kNarrowChunk - 1) / gm::kNarrowChunk;
+    if (nsplit64 > 65535) return gm_fail(-1, "gm_psd_contract: too many points for one call");
+    const int nsplit = (int)std::max<int64_t>(1, nsplit64);
+    size_t bytes = sizeof(double) * (size_t)nsplit * n_psd * n_cols;
+    int rc = ctx->reserve(ctx->gemm_tmp, bytes);
+    if (rc) return rc;
+    double* part = (double*)ctx->gemm_tmp.ptr;
+    hipLaunchKernelGGL(gm::k_contract_mfma_narrow, dim3((n_psd + 15) / 16, nsplit), dim3(64), 0,
+                       s, n_psd, n_pts, n_cols, w, v, part);
     int64_t n_out = (int64_t)n_psd * n_cols;
     hipLaunchKernelGGL(gm::k_reduce_splits, dim3((unsigned)((n_out + 255) / 256)), dim3(256), 0,
                        s, nsplit, n_out, part, out);

@@ -22,11 +22,13 @@ __global__ void k_plan_keys(int64_t n, const double* x, const double* y, const d
 __global__ void k_plan_scan(const int64_t* hist, int64_t* cursor, int64_t* range);
 __global__ void k_plan_scatter(int64_t n, const int32_t* key, int64_t* cursor, int64_t* order);
 __global__ void k_plan_identity(int64_t n, int64_t* order, int64_t* range);
+template <int MINW>
 __global__ void k_mie_homog(const int64_t* order, const int64_t* range, int64_t n_all,
                             const double* x, const double* y, const double* eps, const double* mu,
                             const double* eps2, const int64_t* coef_off, double* an, double* bn,
                             double* props, int32_t* status, double* scratch, int rows_cap,
                             int dn_extra);
+template <int MINW>
 __global__ void k_mie_coated_stable(const int64_t* order, const int64_t* range, int64_t n_all,
                                     const double* x, const double* y, const double* eps,
                                     const double* mu, const double* eps2, const int64_t* coef_off,

@@ -288,7 +288,10 @@ struct Props {
 // ---------------------------------------------------------------------------
 // Homogeneous sphere, mie_coeffs.py:83-130 / 132-180 + mie_props.py:28-70
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(64) void k_mie_homog(
+// MINW: resident wavefronts per SIMD the register allocation must allow
+// (1 = unconstrained; 3 and 4 trade a few spills for latency hiding).
+template <int MINW>
+__global__ __launch_bounds__(64, MINW) void k_mie_homog(
     const int64_t* __restrict__ order, const int64_t* __restrict__ range, int64_t n_all,
     const double* __restrict__ x, const double* __restrict__ y, const double* __restrict__ eps,
     const double* __restrict__ mu, const double* __restrict__ eps2,
@@ -426,7 +429,8 @@ __device__ __forceinline__ void dn_chain(cdd zi, int nstart, int nmax, cd* __res
 // that start is within a few Airy widths of |v| or |w|, and is only then
 // computed (two extra recurrences).  D_n(m2 x) and D_n(m2 y) are otherwise
 // not needed at all: pass 1 runs only the D_n(m1 x) recurrence.
-__global__ __launch_bounds__(64) void k_mie_coated_stable(
+template <int MINW>
+__global__ __launch_bounds__(64, MINW) void k_mie_coated_stable(
     const int64_t* __restrict__ order, const int64_t* __restrict__ range, int64_t n_all,
     const double* __restrict__ x, const double* __restrict__ y, const double* __restrict__ eps,
     const double* __restrict__ mu, const double* __restrict__ eps2,
@@ -764,6 +768,17 @@ __global__ __launch_bounds__(64) void k_mie_coated_literal(
 }
 
 // n_max per point and its exclusive prefix (coefficient offsets), one block.
+#define GM_MIE_ARGS                                                                         \
+  const int64_t*, const int64_t*, int64_t, const double*, const double*, const double*,    \
+      const double*, const double*, const int64_t*, double*, double*, double*, int32_t*,    \
+      double*, int, int
+template __global__ void k_mie_homog<1>(GM_MIE_ARGS);
+template __global__ void k_mie_homog<4>(GM_MIE_ARGS);
+template __global__ void k_mie_coated_stable<1>(GM_MIE_ARGS);
+template __global__ void k_mie_coated_stable<3>(GM_MIE_ARGS);
+template __global__ void k_mie_coated_stable<4>(GM_MIE_ARGS);
+#undef GM_MIE_ARGS
+
 __global__ void k_nmax_of(int64_t n, const double* __restrict__ size, int64_t* __restrict__ cnt) {
   int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (p < n) cnt[p] = nmax_rule(size[p]);

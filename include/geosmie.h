@@ -71,15 +71,19 @@ int gm_ctx_check(gm_ctx* ctx, gm_stream stream, int32_t* flags_host);
 /* Launch-shape parameters of a context (no reference counterpart), set by
  * an autotuner (geosmie_amd/autotune.py) from timings of the caller's own
  * workload.  Results do not depend on them (fixed reduction orders), except
- * GM_TUNE_PSD_CHUNK, which regroups the split-K partial sums of the few-PSD
- * contraction (rounding-level differences). */
+ * at rounding level for GM_TUNE_PSD_CHUNK, which regroups the split-K partial
+ * sums of the few-PSD contraction, and GM_TUNE_MIE_WAVES, whose register
+ * budgets let the compiler fuse a different set of multiply-adds. */
 enum gm_tune_param {
   GM_TUNE_MIE_GRID_MULT = 0,  /* persistent Mie grid = this x resident wavefronts (1..16; 1) */
   GM_TUNE_S12_WAVES = 1,      /* wavefronts of 64 angles per S1/S2 block (1, 2, 4; 4) */
   GM_TUNE_PSD_CHUNK = 2,      /* points per split-K chunk, few-PSD contraction (16..2^20; 128) */
   GM_TUNE_MLP_WEIGHT_TERMS = 3, /* precision, not shape: BF16 terms per MLP weight on the matrix
                                    cores, 1 = BF16 weights (default), 2 = hi + lo BF16 pair */
-  GM_TUNE_S12_POINTS = 4        /* points per wavefront of the FP64 S1/S2 kernel (8, 16; 8) */
+  GM_TUNE_S12_POINTS = 4,       /* points per wavefront of the FP64 S1/S2 kernel (8, 16; 8) */
+  GM_TUNE_MIE_WAVES = 5         /* Mie recurrence kernels' register budget: 0 = unconstrained
+                                   (default), 3 or 4 = at least that many resident wavefronts
+                                   per SIMD (fewer registers, a few spills) */
 };
 int gm_ctx_tune(gm_ctx* ctx, int param, int value);
 int gm_ctx_get_tune(gm_ctx* ctx, int param, int* value);
