@@ -181,6 +181,11 @@ int gm_ctx_tune(gm_ctx* ctx, int param, int value) {
         return gm_fail(GM_EINVAL, "gm_ctx_tune: S1/S2 points per wavefront must be 8 or 16");
       ctx->tune_s12_points = value;
       return GM_OK;
+    case GM_TUNE_S12_LAYOUT:
+      if (value != 0 && value != 1)
+        return gm_fail(GM_EINVAL, "gm_ctx_tune: S1/S2 layout must be 0 or 1");
+      ctx->tune_s12_layout = value;
+      return GM_OK;
     case GM_TUNE_MIE_WAVES:
       if (value != 0 && value != 3 && value != 4)
         return gm_fail(GM_EINVAL, "gm_ctx_tune: Mie waves per SIMD must be 0, 3 or 4");
@@ -205,6 +210,7 @@ int gm_ctx_get_tune(gm_ctx* ctx, int param, int* value) {
     case GM_TUNE_MLP_WEIGHT_TERMS: *value = ctx->tune_mlp_wterms; return GM_OK;
     case GM_TUNE_S12_POINTS: *value = ctx->tune_s12_points; return GM_OK;
     case GM_TUNE_MIE_WAVES: *value = ctx->tune_mie_waves; return GM_OK;
+    case GM_TUNE_S12_LAYOUT: *value = ctx->tune_s12_layout; return GM_OK;
     default: return gm_fail(GM_EINVAL, "gm_ctx_get_tune: unknown parameter %d", param);
   }
 }

@@ -69,11 +69,17 @@ __global__ __launch_bounds__(256) void k_s12_f64(
     int64_t n, const int64_t* __restrict__ off, const int32_t* __restrict__ nmax,
     const double* __restrict__ an, const double* __restrict__ bn, int n_ang,
     const double* __restrict__ pi_t, const double* __restrict__ tau_t, int nmax_cap, int mode,
-    double* __restrict__ out0, double* __restrict__ out1) {
+    int wave_points, double* __restrict__ out0, double* __restrict__ out1) {
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
-  const int64_t p0 = (int64_t)blockIdx.x * 8 * MT;
-  const int j0 = (blockIdx.y * (blockDim.x >> 6) + wave) * 64;  // 64 angles per wavefront
+  const int nw = blockDim.x >> 6;
+  // 64 angles x 8 MT points per wavefront.  The waves of a block take
+  // different angles of the same points, or (wave_points) different points of
+  // the same angles, whose angle-table rows they then read through one CU's
+  // cache.
+  const int64_t p0 = wave_points ? ((int64_t)blockIdx.x * nw + wave) * 8 * MT
+                                 : (int64_t)blockIdx.x * 8 * MT;
+  const int j0 = wave_points ? blockIdx.y * 64 : (blockIdx.y * nw + wave) * 64;
   if (j0 >= n_ang) return;  // whole wavefront past the last angle
   const int i = lane & 15;   // A row / B column / D column
   const int kq = lane >> 4;  // k within the 4-step, D row group
@@ -106,34 +112,52 @@ __global__ __launch_bounds__(256) void k_s12_f64(
       accp[mt][t] = (double4_){0.0, 0.0, 0.0, 0.0};
       accm[mt][t] = (double4_){0.0, 0.0, 0.0, 0.0};
     }
-  for (int k0 = 0; k0 < kmax; k0 += 4) {
+  // Operands of step k0 + 4 are loaded while the MFMAs of step k0 run
+  // (register double buffer), so the loads' latency overlaps the products.
+  double ar_n[MT], br_n[MT], pp_n[4], tt_n[4];
+  auto load_step = [&](int k0) {
     const int kk = k0 + kq;
-    double ua[MT], va[MT];
 #pragma unroll
     for (int mt = 0; mt < MT; ++mt) {
-      ua[mt] = 0.0;
-      va[mt] = 0.0;
+      ar_n[mt] = 0.0;
+      br_n[mt] = 0.0;
       if (kk < nm_a[mt]) {
-        const double ar = an[2 * (oa[mt] + kk) + comp];
-        const double br = bn[2 * (oa[mt] + kk) + comp];
-        ua[mt] = ar + br;
-        va[mt] = ar - br;
+        ar_n[mt] = an[2 * (oa[mt] + kk) + comp];
+        br_n[mt] = bn[2 * (oa[mt] + kk) + comp];
       }
     }
     const bool krow = kk < kmax;
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
       const int j = j0 + t * 16 + i;
-      double pp = 0.0, tt = 0.0;
+      pp_n[t] = 0.0;
+      tt_n[t] = 0.0;
       if (krow && j < n_ang) {
-        pp = pi_t[(size_t)kk * n_ang + j];
-        tt = tau_t[(size_t)kk * n_ang + j];
+        pp_n[t] = pi_t[(size_t)kk * n_ang + j];
+        tt_n[t] = tau_t[(size_t)kk * n_ang + j];
       }
-      const double P = pp + tt, M = pp - tt;
+    }
+  };
+  if (kmax > 0) load_step(0);
+  for (int k0 = 0; k0 < kmax; k0 += 4) {
+    double ua[MT], va[MT], P[4], M[4];
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt) {
+      ua[mt] = ar_n[mt] + br_n[mt];
+      va[mt] = ar_n[mt] - br_n[mt];
+    }
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      P[t] = pp_n[t] + tt_n[t];
+      M[t] = pp_n[t] - tt_n[t];
+    }
+    if (k0 + 4 < kmax) load_step(k0 + 4);
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
 #pragma unroll
       for (int mt = 0; mt < MT; ++mt) {
-        accp[mt][t] = __builtin_amdgcn_mfma_f64_16x16x4f64(ua[mt], P, accp[mt][t], 0, 0, 0);
-        accm[mt][t] = __builtin_amdgcn_mfma_f64_16x16x4f64(va[mt], M, accm[mt][t], 0, 0, 0);
+        accp[mt][t] = __builtin_amdgcn_mfma_f64_16x16x4f64(ua[mt], P[t], accp[mt][t], 0, 0, 0);
+        accm[mt][t] = __builtin_amdgcn_mfma_f64_16x16x4f64(va[mt], M[t], accm[mt][t], 0, 0, 0);
       }
     }
   }
@@ -360,17 +384,21 @@ int launch_s12(gm_ctx* ctx, const char* who, int64_t n, const int64_t* off, cons
   const int wpb = ctx->tune_s12_waves;
   dim3 grid((unsigned)((n + 7) / 8), (unsigned)((n_ang + 64 * wpb - 1) / (64 * wpb)));
   dim3 block(64 * wpb);
+  const int wave_points = ctx->tune_s12_layout;   // FP64 kernel only
   size_t slot;
   if (precision == GM_PREC_FP64) {
     ctx->mark_begin(gm::kKS12, s, &slot);
-    if (ctx->tune_s12_points == 16) {
-      dim3 grid16((unsigned)((n + 15) / 16), grid.y);
-      hipLaunchKernelGGL(gm::k_s12_f64<2>, grid16, block, 0, s, n, off, nmax, an, bn, n_ang,
-                         pi_t, tau_t, nmax_cap, mode, out0, out1);
-    } else {
-      hipLaunchKernelGGL(gm::k_s12_f64<1>, grid, block, 0, s, n, off, nmax, an, bn, n_ang, pi_t,
-                         tau_t, nmax_cap, mode, out0, out1);
-    }
+    const int pts = ctx->tune_s12_points == 16 ? 16 : 8;
+    dim3 g = wave_points
+                 ? dim3((unsigned)((n + (int64_t)pts * wpb - 1) / ((int64_t)pts * wpb)),
+                        (unsigned)((n_ang + 63) / 64))
+                 : dim3((unsigned)((n + pts - 1) / pts), grid.y);
+    if (pts == 16)
+      hipLaunchKernelGGL(gm::k_s12_f64<2>, g, block, 0, s, n, off, nmax, an, bn, n_ang, pi_t,
+                         tau_t, nmax_cap, mode, wave_points, out0, out1);
+    else
+      hipLaunchKernelGGL(gm::k_s12_f64<1>, g, block, 0, s, n, off, nmax, an, bn, n_ang, pi_t,
+                         tau_t, nmax_cap, mode, wave_points, out0, out1);
     ctx->mark_end(slot, s);
   } else {
     const int kpad = (nmax_cap + gm::kLpK - 1) / gm::kLpK * gm::kLpK;

@@ -83,7 +83,10 @@ enum gm_tune_param {
   GM_TUNE_S12_POINTS = 4,       /* points per wavefront of the FP64 S1/S2 kernel (8, 16; 8) */
   GM_TUNE_MIE_WAVES = 5         /* Mie recurrence kernels' register budget: 0 = unconstrained
                                    (default), 3 or 4 = at least that many resident wavefronts
-                                   per SIMD (fewer registers, a few spills) */
+                                   per SIMD (fewer registers, a few spills) */,
+  GM_TUNE_S12_LAYOUT = 6        /* FP64 S1/S2 blocks: 0 = the block's wavefronts take different
+                                   angles of the same points (default), 1 = different points
+                                   of the same 64 angles (angle-table rows shared in cache) */
 };
 int gm_ctx_tune(gm_ctx* ctx, int param, int value);
 int gm_ctx_get_tune(gm_ctx* ctx, int param, int* value);

@@ -14,11 +14,13 @@ def test_knobs_do_not_change_results(gpu):
     h = _lib.ctx()
     x = np.logspace(-2, 2.5, 700)
     ref = None
-    for mult, waves, pts in [(1, 4, 8), (2, 4, 8), (4, 4, 8), (1, 1, 8), (1, 2, 16),
-                             (2, 4, 16), (1, 1, 16)]:
+    for mult, waves, pts, lay in [(1, 4, 8, 0), (2, 4, 8, 0), (4, 4, 8, 0), (1, 1, 8, 0),
+                                  (1, 2, 16, 0), (2, 4, 16, 0), (1, 1, 16, 0), (1, 4, 8, 1),
+                                  (1, 2, 8, 1), (1, 4, 16, 1), (1, 1, 16, 1)]:
         autotune.set_knob(h, "mie_grid_mult", mult)
         autotune.set_knob(h, "s12_waves", waves)
         autotune.set_knob(h, "s12_points", pts)
+        autotune.set_knob(h, "s12_layout", lay)
         r = mie_batch(x, m=1.5 + 0.02j, coeffs=True)
         s1, s2 = s12_batch(r["an"][::37], r["bn"][::37],
                            cost=np.cos(np.radians(np.linspace(0, 180, 300))))
@@ -31,6 +33,7 @@ def test_knobs_do_not_change_results(gpu):
     autotune.set_knob(h, "mie_grid_mult", 1)
     autotune.set_knob(h, "s12_waves", 4)
     autotune.set_knob(h, "s12_points", 8)
+    autotune.set_knob(h, "s12_layout", 0)
 
 
 def test_mie_register_variants_agree(gpu):
