@@ -266,19 +266,38 @@ __global__ __launch_bounds__(256) void k_s12_f32(
     accp[t] = (float4_){0.f, 0.f, 0.f, 0.f};
     accm[t] = (float4_){0.f, 0.f, 0.f, 0.f};
   }
-  for (int k0 = 0; k0 < kmax; k0 += 4) {
+  // register double buffer: step k0 + 4's operands load during step k0's MFMAs
+  double ar_n = 0.0, br_n = 0.0;
+  float p_n[4], m_n[4];
+  auto load_step = [&](int k0) {
     const int kk = k0 + kq;
-    float ua = 0.f, va = 0.f;
+    ar_n = 0.0;
+    br_n = 0.0;
     if (kk < nm_a) {
-      const double ar = an[2 * (oa + kk) + comp], br = bn[2 * (oa + kk) + comp];
-      ua = (float)(ar + br);
-      va = (float)(ar - br);
+      ar_n = an[2 * (oa + kk) + comp];
+      br_n = bn[2 * (oa + kk) + comp];
     }
-    const size_t row = (size_t)kk * apad + j0 + i;
+    const size_t row = (size_t)kk * apad + j0 + i;   // rows < kpad: zero padded
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
-      accp[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(ua, Pt[row + 16 * t], accp[t], 0, 0, 0);
-      accm[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(va, Mt[row + 16 * t], accm[t], 0, 0, 0);
+      p_n[t] = Pt[row + 16 * t];
+      m_n[t] = Mt[row + 16 * t];
+    }
+  };
+  if (kmax > 0) load_step(0);
+  for (int k0 = 0; k0 < kmax; k0 += 4) {
+    const float ua = (float)(ar_n + br_n), va = (float)(ar_n - br_n);
+    float P[4], M[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      P[t] = p_n[t];
+      M[t] = m_n[t];
+    }
+    if (k0 + 4 < kmax) load_step(k0 + 4);
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      accp[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(ua, P[t], accp[t], 0, 0, 0);
+      accm[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(va, M[t], accm[t], 0, 0, 0);
     }
   }
 #pragma unroll
@@ -320,29 +339,50 @@ __global__ __launch_bounds__(256) void k_s12_bf16x3(
     accp[t] = (float4_){0.f, 0.f, 0.f, 0.f};
     accm[t] = (float4_){0.f, 0.f, 0.f, 0.f};
   }
+  // register double buffer of the next K-step's table words (the coefficient
+  // loads are issued before the current step's products as well)
+  uint4 tn[4][4];
+  auto load_tab = [&](int k0) {
+    const size_t o4 = (size_t)((k0 + 8 * kq) >> 3) * apad + j0 + i;   // rows < kpad: zero padded
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+#pragma unroll
+      for (int pl4 = 0; pl4 < 4; ++pl4) tn[t][pl4] = tab[pl4 * plane4 + o4 + 16 * t];
+    }
+  };
+  if (kmax > 0) load_tab(0);
   for (int k0 = 0; k0 < kmax; k0 += kLpK) {
     const int kb = k0 + 8 * kq;   // this lane's K-octet
-    union { bf16x8_ v; unsigned short s[8]; } uh, ul, vh, vl;
+    double ar8[8], br8[8];
 #pragma unroll
     for (int t = 0; t < 8; ++t) {
       const int kk = kb + t;
-      double u = 0.0, v = 0.0;
+      ar8[t] = 0.0;
+      br8[t] = 0.0;
       if (kk < nm_a) {
-        const double ar = an[2 * (oa + kk) + comp], br = bn[2 * (oa + kk) + comp];
-        u = ar + br;
-        v = ar - br;
+        ar8[t] = an[2 * (oa + kk) + comp];
+        br8[t] = bn[2 * (oa + kk) + comp];
       }
-      bf16_split(u, uh.s[t], ul.s[t]);
-      bf16_split(v, vh.s[t], vl.s[t]);
     }
-    const size_t o4 = (size_t)(kb >> 3) * apad + j0 + i;
+    uint4 tc[4][4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+#pragma unroll
+      for (int pl4 = 0; pl4 < 4; ++pl4) tc[t][pl4] = tn[t][pl4];
+    if (k0 + kLpK < kmax) load_tab(k0 + kLpK);
+    union { bf16x8_ v; unsigned short s[8]; } uh, ul, vh, vl;
+#pragma unroll
+    for (int t = 0; t < 8; ++t) {
+      bf16_split(ar8[t] + br8[t], uh.s[t], ul.s[t]);
+      bf16_split(ar8[t] - br8[t], vh.s[t], vl.s[t]);
+    }
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
       union { uint4 q; bf16x8_ v; } ph, pl, mh, ml;
-      ph.q = tab[o4 + 16 * t];
-      pl.q = tab[plane4 + o4 + 16 * t];
-      mh.q = tab[2 * plane4 + o4 + 16 * t];
-      ml.q = tab[3 * plane4 + o4 + 16 * t];
+      ph.q = tc[t][0];
+      pl.q = tc[t][1];
+      mh.q = tc[t][2];
+      ml.q = tc[t][3];
       accp[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(uh.v, ph.v, accp[t], 0, 0, 0);
       accp[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(uh.v, pl.v, accp[t], 0, 0, 0);
       accp[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ul.v, ph.v, accp[t], 0, 0, 0);
