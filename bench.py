@@ -366,18 +366,18 @@ def table_build(cpu_sample=True):
     from geosmie_amd.runoptics import build_table
     params, water = oc_like_particle()
     small = dict(params, mList=[params["mList"][0][:, :2]])
-    build_table(small, water=water)               # warm: library, tables, workspaces
+    build_table(small, water=water, n_mom=129)    # warm: library, tables, workspaces
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    tab = build_table(params, water=water)
+    tab = build_table(params, water=water, n_mom=129)
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     n_ent = tab.n_bin * tab.n_lam * tab.n_rh
     from geosmie_amd.particle import initializeXarr
     xx, _ = initializeXarr(params, 0, params["mList"][0][0][0], params["mList"][0][0][-1])
-    out = {"workload": "organic-carbon-like runoptics table: 1 bin, %d size points, 61 "
-                       "wavelengths, 36 rh, 371 angles (%.3g Mie points)" % (
-                           xx.size, xx.size * 61 * 36),
+    out = {"workload": "organic-carbon-like runoptics table with its GSF moments (rungsf, 129 "
+                       "per element): 1 bin, %d size points, 61 wavelengths, 36 rh, 371 angles "
+                       "(%.3g Mie points)" % (xx.size, xx.size * 61 * 36),
            "unit": "table entries/s", "entries": n_ent, "s": dt, "value": n_ent / dt,
            "mie_points_per_s": xx.size * n_ent / dt}
     if cpu_sample:

@@ -150,6 +150,8 @@ def to_dev(a, dtype=np.float64):
     if isinstance(a, t.Tensor):
         return a.to(device=device()).contiguous()
     arr = np.ascontiguousarray(np.asarray(a, dtype=dtype))
+    if not arr.flags.writeable:     # torch.from_numpy wants writable memory
+        arr = arr.copy()
     return t.from_numpy(arr).to(device())
 
 

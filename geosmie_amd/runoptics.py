@@ -101,9 +101,12 @@ def _normalize(pm, ang):
     return d_p.cpu().numpy().reshape(pm.shape), d_b.cpu().numpy().reshape(-1, 6)
 
 
-def build_table(params, water=None, classic=False, precision=0, ang=None):
+def build_table(params, water=None, classic=False, precision=0, ang=None, n_mom=0):
     """An OpticsTable for a particle description (getPPJSON's dict with
-    'mList'); water = getWaterM() array (needed unless rhDep is 'trivial')."""
+    'mList'); water = getWaterM() array (needed unless rhDep is 'trivial').
+    n_mom > 0 adds pmom, the GSF moments of every entry's normalised phase
+    matrix (the rungsf step, src/gsf/convertncdf.py:170-189, 369-394; 129 in
+    the reference), computed on the device for all rh of a wavelength at once."""
     mode = params.get("mode", "mie")
     if mode != "mie":
         raise ValueError("mode %s is not supported (Mie tables only)" % mode)
@@ -126,7 +129,7 @@ def build_table(params, water=None, classic=False, precision=0, ang=None):
         w_mr, w_mi = _interp(water[0], water[1]), _interp(water[0], water[2])
     ang = angle_grid() if ang is None else np.asarray(ang, np.float64)
     rh_table = list(params["rh"])
-    table = OpticsTable(len(radii), rh_table, lams, ang, classic=classic)
+    table = OpticsTable(len(radii), rh_table, lams, ang, classic=classic, n_mom=n_mom)
     rhop00 = params["rhop0"]
     cost = np.cos(np.radians(ang))
     from .s12 import angle_tables_dev
@@ -139,12 +142,12 @@ def build_table(params, water=None, classic=False, precision=0, ang=None):
         fracs = pp["fracs"][radind]
         for li, lam in enumerate(lams):
             _wavelength(table, params, radind, li, lam, xx, dr, fracs, rhop0, dev_pass, ang,
-                        part_mr, part_mi, (w_mr, w_mi) if rhdep != "trivial" else None)
+                        part_mr, part_mi, (w_mr, w_mi) if rhdep != "trivial" else None, n_mom)
     return table
 
 
 def _wavelength(table, params, radind, li, lam, xx, dr, fracs, rhop0, dev_pass, ang, part_mr,
-                part_mi, water):
+                part_mi, water, n_mom=0):
     """All rh entries of one (bin, wavelength) (dointegration.py:811-1031)."""
     nref0 = [complex(f(lam), -g(lam)) for f, g in zip(part_mr, part_mi)]
     nw = complex(water[0](lam), water[1](lam)) if water is not None else complex(1, 0)
@@ -178,6 +181,12 @@ def _wavelength(table, params, radind, li, lam, xx, dr, fracs, rhop0, dev_pass, 
             mixed_p[e] += fracs[k] * pm[r]
             mixed_s[e] += fracs[k] * sc[r]
     p_norm, pback = _normalize(mixed_p, ang)
+    pmom = None
+    if n_mom:
+        from .gsf import pmom_batch
+        # table order p11, p12, p33, p34, p22 (= p11), p44 (= p33)
+        p6 = p_norm[:, [0, 1, 2, 3, 0, 2], :]
+        pmom = pmom_batch(ang, p6, ng=n_mom)
     mass0 = None
     for e, (rhi, mr, mi, gf, rhop, rlo, rup, _) in enumerate(meta):
         s = dict(zip(SCALAR_OUT, mixed_s[e]))
@@ -196,25 +205,29 @@ def _wavelength(table, params, radind, li, lam, xx, dr, fracs, rhop0, dev_pass, 
               "volume": s["volume"] / mass0}
         ret["refreal"] = mr[0]
         ret["refimag"] = -np.abs(mi[0])
-        table.set_entry(radind, li, rhi, ret)
+        table.set_entry(radind, li, rhi, ret, pmom=None if pmom is None else pmom[e])
         table.set_bin_rh(radind, rhi, nl)
     if trivial:   # copyDryValues (:462-490): every rh repeats the dry entry
         for rhi in range(1, len(rh)):
             table.copy_rh(radind, li, 0, rhi)
 
 
-def fun(partID0, oppfx=".", oppclassic=False, data_dir=None, precision=0):
+def fun(partID0, oppfx=".", oppclassic=False, data_dir=None, precision=0, n_mom=0):
     """runoptics for one particle file: builds the table and writes
     optics_<name>.nomom.nc4 (or .nomom.legacy.nc4) under oppfx
-    (dointegration.py:672-1036, 205-213)."""
+    (dointegration.py:672-1036, 205-213).  n_mom > 0 also runs the GSF step
+    and writes optics_<name>.nc4 with pmom instead, the name rungsf gives its
+    output (convertncdf.py:328)."""
     part = os.path.basename(partID0).replace(".json", "")
     src = partID0.replace("-orig", "") if "-orig" in part else partID0
     params = particle.getPPJSON(src, data_dir)
     water = None
     if params["rhDep"]["type"] != "trivial":
         water = particle.getWaterM(data_dir=data_dir)
-    table = build_table(params, water=water, classic=oppclassic, precision=precision)
+    table = build_table(params, water=water, classic=oppclassic, precision=precision, n_mom=n_mom)
     name = "optics_%s.nomom.legacy.nc4" % part if oppclassic else "optics_%s.nomom.nc4" % part
+    if n_mom:
+        name = name.replace("nomom.", "")
     path = os.path.join(oppfx, name)
     table.write(path)
     return path

@@ -88,3 +88,27 @@ def test_fun_from_particle_file(gpu, tmp_path):
     for k in ("p11", "qext", "bbck", "rhop", "pback"):
         np.testing.assert_array_equal(v[k][0], ref.data[k])
     np.testing.assert_allclose(v["wavelength"][0], LAM, rtol=1e-12)
+
+
+@pytest.mark.parametrize("classic", [False, True])
+def test_table_with_gsf_moments(gpu, tmp_path, classic):
+    """n_mom adds pmom (the rungsf step): every entry's moments are the GSF
+    expansion of that entry's normalised phase matrix (oracle/gsf.py, which
+    restates spher_expan.f), in the table's p order, in both layouts."""
+    from geosmie_amd.runoptics import build_table
+    from geosmie_amd.table import read
+    from oracle import gsf as OG
+    tab = build_table(params("ss_maxrh"), water=water(), classic=classic, n_mom=129)
+    d = tab.data
+    for (b, li, rhi) in [(0, 0, 0), (1, 3, 2), (0, 2, 3)]:
+        idx = (b, rhi, li) if classic else (b, li, rhi)
+        ref = OG.pmom_from_table_slab(tab.ang, d["p11"][idx], d["p12"][idx], d["p33"][idx],
+                                      d["p34"][idx], d["p22"][idx], d["p44"][idx])
+        got = d["pmom"][:, :, b, rhi, li] if classic else d["pmom"][b, li, rhi]
+        assert np.abs(got - ref).max() <= 1.01e-10, (b, li, rhi)
+    path = str(tmp_path / "optics_T.nc4")
+    tab.write(path)
+    v = read(path)
+    assert v["pmom"][1] == (("nPol", "nMom", "radius", "rh", "lambda") if classic
+                            else ("bin", "wavelength", "rh", "p", "m"))
+    np.testing.assert_array_equal(v["pmom"][0], d["pmom"])
