@@ -12,7 +12,8 @@ Knobs (include/geosmie.h, enum gm_tune_param):
   s12_points     points per wavefront of the FP64 S1/S2 kernel
   mie_waves      register budget of the Mie recurrence kernels (0 unconstrained,
                  3 or 4 resident wavefronts per SIMD)
-  s12_layout     FP64 S1/S2 blocks split angles (0) or points (1) among their waves
+  s12_layout     FP64 S1/S2 blocks split angles (0) or points (1) among their waves,
+                 or share LDS-staged angle tables among them (2)
 (mlp_weight_terms, the BF16 terms per network weight, is a precision
 setting reached through the same call; it is not tuned for speed.)
 """
@@ -27,7 +28,7 @@ KNOBS = {"mie_grid_mult": (0, (1, 2, 4)),
          "psd_chunk": (2, (64, 128, 256, 512)),
          "s12_points": (4, (8, 16)),
          "mie_waves": (5, (0, 3, 4)),
-         "s12_layout": (6, (0, 1)),
+         "s12_layout": (6, (0, 1, 2)),
          # a precision setting rather than a shape; set explicitly, never tuned for speed
          "mlp_weight_terms": (3, (1, 2))}
 

@@ -182,8 +182,8 @@ int gm_ctx_tune(gm_ctx* ctx, int param, int value) {
       ctx->tune_s12_points = value;
       return GM_OK;
     case GM_TUNE_S12_LAYOUT:
-      if (value != 0 && value != 1)
-        return gm_fail(GM_EINVAL, "gm_ctx_tune: S1/S2 layout must be 0 or 1");
+      if (value < 0 || value > 2)
+        return gm_fail(GM_EINVAL, "gm_ctx_tune: S1/S2 layout must be 0, 1 or 2");
       ctx->tune_s12_layout = value;
       return GM_OK;
     case GM_TUNE_MIE_WAVES:

@@ -178,6 +178,123 @@ __global__ __launch_bounds__(256) void k_s12_f64(
     }
 }
 
+// LDS-staged variant: the kS12LdsWaves wavefronts of a block take 8 points each
+// at the same 64 angles and share the angle tables: every stage of
+// kS12LdsRows K-rows of P = pi' + tau' and M = pi' - tau' (64 angles) is read
+// from global memory once per block into LDS, while the previous stage's
+// MFMAs run (the next stage is held in registers, then stored behind a
+// barrier).  The K loop runs to the block's largest n_max; rows past a
+// point's own n_max multiply zero coefficients.
+constexpr int kS12LdsWaves = 4;
+constexpr int kS12LdsRows = 16;
+constexpr int kS12LdsStride = 64 + 2;   // doubles per LDS row (padding against bank conflicts)
+__global__ __launch_bounds__(64 * kS12LdsWaves) void k_s12_f64_lds(
+    int64_t n, const int64_t* __restrict__ off, const int32_t* __restrict__ nmax,
+    const double* __restrict__ an, const double* __restrict__ bn, int n_ang,
+    const double* __restrict__ pi_t, const double* __restrict__ tau_t, int nmax_cap, int mode,
+    double* __restrict__ out0, double* __restrict__ out1) {
+  __shared__ double Ps[2][kS12LdsRows][kS12LdsStride];
+  __shared__ double Ms[2][kS12LdsRows][kS12LdsStride];
+  __shared__ int kmax_w[kS12LdsWaves];
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int64_t p0 = ((int64_t)blockIdx.x * kS12LdsWaves + wave) * 8;
+  const int j0 = blockIdx.y * 64;
+  const int i = lane & 15;
+  const int kq = lane >> 4;
+  const int comp = (i >> 2) & 1;
+  const int64_t pa = p0 + 2 * (i & 3) + (i >> 3);
+  const bool pa_ok = pa < n;
+  const int nm_a = pa_ok ? nmax[pa] : 0;
+  const int64_t oa = pa_ok ? off[pa] : 0;
+  int kw = 0;
+  for (int q = 0; q < 8; ++q) {
+    const int64_t pq = p0 + q;
+    if (pq < n) kw = max(kw, nmax[pq]);
+  }
+  if (lane == 0) kmax_w[wave] = min(kw, nmax_cap);
+  __syncthreads();
+  int kmax = 0;
+  for (int w = 0; w < kS12LdsWaves; ++w) kmax = max(kmax, kmax_w[w]);
+  // cooperative stage loads: thread tid covers (row, column) pairs tid + 256 e
+  constexpr int kPer = kS12LdsRows * 64 / (64 * kS12LdsWaves);
+  double pn[kPer], mn[kPer];
+  auto load_stage = [&](int s0) {
+#pragma unroll
+    for (int e = 0; e < kPer; ++e) {
+      const int q = tid + 64 * kS12LdsWaves * e;
+      const int r = q >> 6, c = q & 63;
+      const int kk = s0 + r, j = j0 + c;
+      double pp = 0.0, tt = 0.0;
+      if (kk < kmax && j < n_ang) {
+        pp = pi_t[(size_t)kk * n_ang + j];
+        tt = tau_t[(size_t)kk * n_ang + j];
+      }
+      pn[e] = pp + tt;
+      mn[e] = pp - tt;
+    }
+  };
+  auto store_stage = [&](int buf) {
+#pragma unroll
+    for (int e = 0; e < kPer; ++e) {
+      const int q = tid + 64 * kS12LdsWaves * e;
+      const int r = q >> 6, c = q & 63;
+      Ps[buf][r][c] = pn[e];
+      Ms[buf][r][c] = mn[e];
+    }
+  };
+  double4_ accp[4], accm[4];
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    accp[t] = (double4_){0.0, 0.0, 0.0, 0.0};
+    accm[t] = (double4_){0.0, 0.0, 0.0, 0.0};
+  }
+  if (kmax > 0) {
+    load_stage(0);
+    store_stage(0);
+  }
+  __syncthreads();
+  int buf = 0;
+  for (int s0 = 0; s0 < kmax; s0 += kS12LdsRows) {
+    const bool more = s0 + kS12LdsRows < kmax;
+    if (more) load_stage(s0 + kS12LdsRows);     // in flight during this stage's MFMAs
+#pragma unroll
+    for (int ks = 0; ks < kS12LdsRows / 4; ++ks) {
+      const int kk = s0 + 4 * ks + kq;
+      double ua = 0.0, va = 0.0;
+      if (kk < nm_a) {
+        const double ar = an[2 * (oa + kk) + comp];
+        const double br = bn[2 * (oa + kk) + comp];
+        ua = ar + br;
+        va = ar - br;
+      }
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        const double P = Ps[buf][4 * ks + kq][t * 16 + i];
+        const double M = Ms[buf][4 * ks + kq][t * 16 + i];
+        accp[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(ua, P, accp[t], 0, 0, 0);
+        accm[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(va, M, accm[t], 0, 0, 0);
+      }
+    }
+    if (more) store_stage(buf ^ 1);
+    __syncthreads();
+    buf ^= 1;
+  }
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    const int j = j0 + t * 16 + i;
+    if (j >= n_ang) continue;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int64_t p = p0 + 2 * kq + h;
+      if (p >= n) continue;
+      s12_store(mode, p, j, n_ang, cd{accp[t][2 * h], accp[t][2 * h + 1]},
+                cd{accm[t][2 * h], accm[t][2 * h + 1]}, out0, out1);
+    }
+  }
+}
+
 // ---- reduced-precision paths -------------------------------------------------
 
 constexpr int kLpK = 32;     // K padding of the reduced-precision tables
@@ -429,6 +546,16 @@ int launch_s12(gm_ctx* ctx, const char* who, int64_t n, const int64_t* off, cons
   if (precision == GM_PREC_FP64) {
     ctx->mark_begin(gm::kKS12, s, &slot);
     const int pts = ctx->tune_s12_points == 16 ? 16 : 8;
+    if (wave_points == 2) {   // LDS-staged angle tables, 8 points per wavefront
+      dim3 gl((unsigned)((n + 8 * gm::kS12LdsWaves - 1) / (8 * gm::kS12LdsWaves)),
+              (unsigned)((n_ang + 63) / 64));
+      hipLaunchKernelGGL(gm::k_s12_f64_lds, gl, dim3(64 * gm::kS12LdsWaves), 0, s, n, off, nmax,
+                         an, bn, n_ang, pi_t, tau_t, nmax_cap, mode, out0, out1);
+      ctx->mark_end(slot, s);
+      e = hipGetLastError();
+      if (e != hipSuccess) return gm_fail(GM_ERUNTIME, "%s: %s", who, hipGetErrorString(e));
+      return GM_OK;
+    }
     dim3 g = wave_points
                  ? dim3((unsigned)((n + (int64_t)pts * wpb - 1) / ((int64_t)pts * wpb)),
                         (unsigned)((n_ang + 63) / 64))

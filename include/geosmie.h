@@ -86,7 +86,8 @@ enum gm_tune_param {
                                    per SIMD (fewer registers, a few spills) */,
   GM_TUNE_S12_LAYOUT = 6        /* FP64 S1/S2 blocks: 0 = the block's wavefronts take different
                                    angles of the same points (default), 1 = different points
-                                   of the same 64 angles (angle-table rows shared in cache) */
+                                   of the same 64 angles (angle-table rows shared in cache),
+                                   2 = as 1 with the angle tables staged through LDS */
 };
 int gm_ctx_tune(gm_ctx* ctx, int param, int value);
 int gm_ctx_get_tune(gm_ctx* ctx, int param, int* value);

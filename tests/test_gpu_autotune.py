@@ -16,7 +16,8 @@ def test_knobs_do_not_change_results(gpu):
     ref = None
     for mult, waves, pts, lay in [(1, 4, 8, 0), (2, 4, 8, 0), (4, 4, 8, 0), (1, 1, 8, 0),
                                   (1, 2, 16, 0), (2, 4, 16, 0), (1, 1, 16, 0), (1, 4, 8, 1),
-                                  (1, 2, 8, 1), (1, 4, 16, 1), (1, 1, 16, 1)]:
+                                  (1, 2, 8, 1), (1, 4, 16, 1), (1, 1, 16, 1), (1, 4, 8, 2),
+                                  (2, 1, 16, 2)]:
         autotune.set_knob(h, "mie_grid_mult", mult)
         autotune.set_knob(h, "s12_waves", waves)
         autotune.set_knob(h, "s12_points", pts)
