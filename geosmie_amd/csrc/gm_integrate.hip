@@ -55,6 +55,21 @@ __global__ void k_reduce_splits(int n_split, int64_t n_out, const double* __rest
   out[i] = s;
 }
 
+// Many-split variant: one wavefront per output, lane l adds splits l, l + 64,
+// ... in order, then the 64 lane sums combine in a fixed butterfly order
+// (deterministic; used when there are too many splits for one serial pass).
+__global__ __launch_bounds__(64) void k_reduce_splits_wide(int n_split, int64_t n_out,
+                                                           const double* __restrict__ part,
+                                                           double* __restrict__ out) {
+  const int64_t i = blockIdx.x;
+  const int lane = threadIdx.x;
+  double s = 0.0;
+  for (int q = lane; q < n_split; q += 64) s += part[(size_t)q * n_out + i];
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) s += __shfl_xor(s, d);
+  if (lane == 0) out[i] = s;
+}
+
 // Segmented contraction: row r integrates only its own points
 // [start_r, start_r + count_r):  out[r][c] = sum_p w[r][p] v[p][c].  Each
 // row's range is cut into chunks; block (column block, chunk q) writes the
@@ -354,8 +369,12 @@ int gm_psd_contract(gm_ctx* ctx, int n_psd, int64_t n_pts, int n_cols, const dou
     hipLaunchKernelGGL(gm::k_contract_mfma_narrow, dim3((n_psd + 15) / 16, nsplit), dim3(64), 0,
                        s, n_psd, n_pts, n_cols, w, v, part);
     int64_t n_out = (int64_t)n_psd * n_cols;
-    hipLaunchKernelGGL(gm::k_reduce_splits, dim3((unsigned)((n_out + 255) / 256)), dim3(256), 0,
-                       s, nsplit, n_out, part, out);
+    if (nsplit > 64)
+      hipLaunchKernelGGL(gm::k_reduce_splits_wide, dim3((unsigned)n_out), dim3(64), 0, s, nsplit,
+                         n_out, part, out);
+    else
+      hipLaunchKernelGGL(gm::k_reduce_splits, dim3((unsigned)((n_out + 255) / 256)), dim3(256), 0,
+                         s, nsplit, n_out, part, out);
   } else {
     int nsplit =
         (int)std::max<int64_t>(1, (n_pts + gm::kSplitChunkMfma - 1) / gm::kSplitChunkMfma);
