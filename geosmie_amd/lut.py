@@ -87,14 +87,22 @@ def lognorm_psd(rmode, sigma, xxarr, drarr, lam, rmax, rmin):
 class _DevMie(object):
     """Device coefficients of one homogeneous batch (kept resident)."""
 
-    def __init__(self, xx, m):
+    def __init__(self, xx, m, d_eps=None, x_max=None):
+        """xx, m host arrays; or xx a device tensor of size parameters with
+        d_eps (device, interleaved re/im of m^2) and x_max given."""
         t = _lib.torch()
-        xx = np.ascontiguousarray(xx, dtype=np.float64)
-        n = xx.size
+        if hasattr(xx, "data_ptr"):
+            self.d_x, self.d_eps = xx, d_eps
+            n = int(xx.numel())
+            xmax = float(x_max)
+        else:
+            xx = np.ascontiguousarray(xx, dtype=np.float64)
+            n = xx.size
+            self.d_x = _lib.to_dev(xx)
+            self.d_eps = _lib.cplx_dev(np.broadcast_to(np.asarray(m, np.complex128) ** 2, (n,)))
+            xmax = float(xx.max())
         self.n = n
         dev = _lib.device()
-        self.d_x = _lib.to_dev(xx)
-        self.d_eps = _lib.cplx_dev(np.broadcast_to(np.asarray(m, np.complex128) ** 2, (n,)))
         self.d_off = t.empty(n + 1, dtype=t.int64, device=dev)
         L, h, s = _lib.lib(), _lib.ctx(), stream()
         check(L.gm_coef_offsets(h, n, ptr(self.d_x), ptr(self.d_off), s), "gm_coef_offsets")
@@ -104,10 +112,10 @@ class _DevMie(object):
         self.d_props = t.empty(6 * n, dtype=t.float64, device=dev)
         self.d_nmax = t.empty(n, dtype=t.int32, device=dev)
         check(L.gm_mie_batch(h, n, ptr(self.d_x), None, ptr(self.d_eps), None, None,
-                             float(xx.max()), ptr(self.d_off), ptr(self.d_an), ptr(self.d_bn),
+                             xmax, ptr(self.d_off), ptr(self.d_an), ptr(self.d_bn),
                              ptr(self.d_props), ptr(self.d_nmax), _lib.GM_MIE_SORT, s),
               "gm_mie_batch")
-        self.nmax_max = int(np.rint(2 + xx.max() + 4 * xx.max() ** (1. / 3.)))
+        self.nmax_max = int(np.rint(2 + xmax + 4 * xmax ** (1. / 3.)))
 
     def mueller(self, pi_t, tau_t, precision=0):
         t = _lib.torch()

@@ -237,9 +237,11 @@ def getHumidRefractiveIndex(params, radind, rhi, rh, nref0, nrefwater):
     return [v.real for v in n], [v.imag for v in n], gf, rrat
 
 
-def calculatePSD(params, radind, onerh, rh, xxarr, drarr, rrat, lam):
+def calculatePSD(params, radind, onerh, rh, xxarr, drarr, rrat, lam, want_ref=True):
     """Number weights per mode on the size grid, mass-weighted effective radius
-    per mode, and the bin edges (rLow, rUp) (dointegration.py:539-622)."""
+    per mode, and the bin edges (rLow, rUp) (dointegration.py:539-622).
+    want_ref=False skips the effective radii (returned as None), which the
+    table loop only needs at the dry state."""
     pp, kind, rd = params["psd"]["params"], params["psd"]["type"], params["rhDep"]
     rarr = xxarr / (2 * np.pi / lam)
     rr = xxarr * lam / 2. / np.pi
@@ -255,7 +257,8 @@ def calculatePSD(params, radind, onerh, rh, xxarr, drarr, rrat, lam):
             w = w * drarr
             w = w / np.sum(w)
             psd.append(w)
-            ref.append(np.sum(rr ** 4. * w) / np.sum(rr ** 3. * w))
+            if want_ref:
+                ref.append(np.sum(rr ** 4. * w) / np.sum(rr ** 3. * w))
     elif kind == "ss":
         rmin_d, rmax_d = pp["rMinMaj"][radind], pp["rMaxMaj"][radind]
         rLow, rUp = rmin_d, rmax_d
@@ -276,7 +279,8 @@ def calculatePSD(params, radind, onerh, rh, xxarr, drarr, rrat, lam):
         w = dndr * dr
         w = w / np.sum(w)
         psd.append(w)
-        ref.append(np.sum(rr ** 4 * w) / np.sum(rr ** 3 * w))
+        if want_ref:
+            ref.append(np.sum(rr ** 4 * w) / np.sum(rr ** 3 * w))
     else:
         raise ValueError("psd type %s is not supported by the Mie table path" % kind)
-    return psd, ref, rLow, rUp
+    return psd, (ref if want_ref else None), rLow, rUp

@@ -55,12 +55,15 @@ class _WavelengthPass(object):
         t = _lib.torch()
         nx, nb = self.xx.size, len(m_blocks)
         n = nx * nb
-        x = np.tile(self.xx, nb)
-        m = np.repeat(np.asarray(m_blocks, np.complex128), nx)
-        dm = _DevMie(x, m)
+        dev = _lib.device()
+        # the grid repeated once per block, and each block's m^2, formed on the device
+        if getattr(self, "_d_x_nb", None) is None or self._d_x_nb[0] != nb:
+            self._d_x_nb = (nb, _lib.to_dev(self.xx).repeat(nb))
+        eps = np.asarray(m_blocks, np.complex128) ** 2
+        d_eps = _lib.to_dev(eps.view(np.float64).reshape(nb, 2)).repeat_interleave(nx, dim=0)
+        dm = _DevMie(self._d_x_nb[1], None, d_eps=d_eps.reshape(-1), x_max=float(self.xx.max()))
         d_mu = dm.mueller(self.pi_t, self.tau_t, self.precision)
         nr = len(rows)
-        dev = _lib.device()
         # dense [rows][points] weights, zero outside each row's block, assembled
         # on the device from the compact [rows][grid] PSDs
         d_psd = _lib.to_dev(np.stack([r[1] for r in rows]))
@@ -162,7 +165,8 @@ def _wavelength(table, params, radind, li, lam, xx, dr, fracs, rhop0, dev_pass, 
     blocks, rows, meta = [], [], []
     for rhi in todo:
         mr, mi, gf, rrat = particle.getHumidRefractiveIndex(params, radind, rhi, rh, nref0, nw)
-        psd, _, rlo, rup = particle.calculatePSD(params, radind, rh[rhi], rh, xx, dr, rrat, lam)
+        psd, _, rlo, rup = particle.calculatePSD(params, radind, rh[rhi], rh, xx, dr, rrat, lam,
+                                                 want_ref=False)
         rhop = rrat ** 3. * rhop0 + (1. - rrat ** 3.) * RHOW
         b0 = len(blocks)
         blocks.extend(complex(a, b) for a, b in zip(mr, mi))
