@@ -12,8 +12,6 @@ import argparse
 import json
 import time
 
-import numpy as np
-
 
 def main(argv=None):
     ap = argparse.ArgumentParser()

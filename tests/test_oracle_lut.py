@@ -2,7 +2,6 @@
 import os
 
 import numpy as np
-import pytest
 from scipy.integrate import trapezoid
 
 from oracle import lut

@@ -89,7 +89,7 @@ def test_rayleigh_baseline_closed_forms():
     the Rayleigh extinction of the Mie series (oracle Mie to O(x^2)); B
     saturates at 2 for large particles; a dropped factor of the moments
     (exp(6 s^2)) changes B for broad distributions."""
-    from geosmie_amd.surrogate import M_WATER, make_queries, rayleigh_log_b
+    from geosmie_amd.surrogate import make_queries, rayleigh_log_b
     from oracle import mie as OM
     for m in (1.5 + 0.01j, 1.33 + 1e-6j, 1.8 + 0.5j):
         for x in (1e-3, 3e-3):
