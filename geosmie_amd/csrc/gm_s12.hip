@@ -188,6 +188,7 @@ __global__ __launch_bounds__(256) void k_s12_f64(
 constexpr int kS12LdsWaves = 4;
 constexpr int kS12LdsRows = 16;
 constexpr int kS12LdsStride = 64 + 2;   // doubles per LDS row (padding against bank conflicts)
+template <int MT>
 __global__ __launch_bounds__(64 * kS12LdsWaves) void k_s12_f64_lds(
     int64_t n, const int64_t* __restrict__ off, const int32_t* __restrict__ nmax,
     const double* __restrict__ an, const double* __restrict__ bn, int n_ang,
@@ -199,17 +200,22 @@ __global__ __launch_bounds__(64 * kS12LdsWaves) void k_s12_f64_lds(
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wave = tid >> 6;
-  const int64_t p0 = ((int64_t)blockIdx.x * kS12LdsWaves + wave) * 8;
+  const int64_t p0 = ((int64_t)blockIdx.x * kS12LdsWaves + wave) * 8 * MT;
   const int j0 = blockIdx.y * 64;
   const int i = lane & 15;
   const int kq = lane >> 4;
   const int comp = (i >> 2) & 1;
-  const int64_t pa = p0 + 2 * (i & 3) + (i >> 3);
-  const bool pa_ok = pa < n;
-  const int nm_a = pa_ok ? nmax[pa] : 0;
-  const int64_t oa = pa_ok ? off[pa] : 0;
+  int nm_a[MT];
+  int64_t oa[MT];
+#pragma unroll
+  for (int mt = 0; mt < MT; ++mt) {
+    const int64_t pa = p0 + 8 * mt + 2 * (i & 3) + (i >> 3);
+    const bool ok = pa < n;
+    nm_a[mt] = ok ? nmax[pa] : 0;
+    oa[mt] = ok ? off[pa] : 0;
+  }
   int kw = 0;
-  for (int q = 0; q < 8; ++q) {
+  for (int q = 0; q < 8 * MT; ++q) {
     const int64_t pq = p0 + q;
     if (pq < n) kw = max(kw, nmax[pq]);
   }
@@ -244,12 +250,14 @@ __global__ __launch_bounds__(64 * kS12LdsWaves) void k_s12_f64_lds(
       Ms[buf][r][c] = mn[e];
     }
   };
-  double4_ accp[4], accm[4];
+  double4_ accp[MT][4], accm[MT][4];
 #pragma unroll
-  for (int t = 0; t < 4; ++t) {
-    accp[t] = (double4_){0.0, 0.0, 0.0, 0.0};
-    accm[t] = (double4_){0.0, 0.0, 0.0, 0.0};
-  }
+  for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      accp[mt][t] = (double4_){0.0, 0.0, 0.0, 0.0};
+      accm[mt][t] = (double4_){0.0, 0.0, 0.0, 0.0};
+    }
   if (kmax > 0) {
     load_stage(0);
     store_stage(0);
@@ -262,19 +270,27 @@ __global__ __launch_bounds__(64 * kS12LdsWaves) void k_s12_f64_lds(
 #pragma unroll
     for (int ks = 0; ks < kS12LdsRows / 4; ++ks) {
       const int kk = s0 + 4 * ks + kq;
-      double ua = 0.0, va = 0.0;
-      if (kk < nm_a) {
-        const double ar = an[2 * (oa + kk) + comp];
-        const double br = bn[2 * (oa + kk) + comp];
-        ua = ar + br;
-        va = ar - br;
+      double ua[MT], va[MT];
+#pragma unroll
+      for (int mt = 0; mt < MT; ++mt) {
+        ua[mt] = 0.0;
+        va[mt] = 0.0;
+        if (kk < nm_a[mt]) {
+          const double ar = an[2 * (oa[mt] + kk) + comp];
+          const double br = bn[2 * (oa[mt] + kk) + comp];
+          ua[mt] = ar + br;
+          va[mt] = ar - br;
+        }
       }
 #pragma unroll
       for (int t = 0; t < 4; ++t) {
         const double P = Ps[buf][4 * ks + kq][t * 16 + i];
         const double M = Ms[buf][4 * ks + kq][t * 16 + i];
-        accp[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(ua, P, accp[t], 0, 0, 0);
-        accm[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(va, M, accm[t], 0, 0, 0);
+#pragma unroll
+        for (int mt = 0; mt < MT; ++mt) {
+          accp[mt][t] = __builtin_amdgcn_mfma_f64_16x16x4f64(ua[mt], P, accp[mt][t], 0, 0, 0);
+          accm[mt][t] = __builtin_amdgcn_mfma_f64_16x16x4f64(va[mt], M, accm[mt][t], 0, 0, 0);
+        }
       }
     }
     if (more) store_stage(buf ^ 1);
@@ -282,17 +298,19 @@ __global__ __launch_bounds__(64 * kS12LdsWaves) void k_s12_f64_lds(
     buf ^= 1;
   }
 #pragma unroll
-  for (int t = 0; t < 4; ++t) {
-    const int j = j0 + t * 16 + i;
-    if (j >= n_ang) continue;
+  for (int mt = 0; mt < MT; ++mt)
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      const int64_t p = p0 + 2 * kq + h;
-      if (p >= n) continue;
-      s12_store(mode, p, j, n_ang, cd{accp[t][2 * h], accp[t][2 * h + 1]},
-                cd{accm[t][2 * h], accm[t][2 * h + 1]}, out0, out1);
+    for (int t = 0; t < 4; ++t) {
+      const int j = j0 + t * 16 + i;
+      if (j >= n_ang) continue;
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const int64_t p = p0 + 8 * mt + 2 * kq + h;
+        if (p >= n) continue;
+        s12_store(mode, p, j, n_ang, cd{accp[mt][t][2 * h], accp[mt][t][2 * h + 1]},
+                  cd{accm[mt][t][2 * h], accm[mt][t][2 * h + 1]}, out0, out1);
+      }
     }
-  }
 }
 
 // ---- reduced-precision paths -------------------------------------------------
@@ -546,11 +564,15 @@ int launch_s12(gm_ctx* ctx, const char* who, int64_t n, const int64_t* off, cons
   if (precision == GM_PREC_FP64) {
     ctx->mark_begin(gm::kKS12, s, &slot);
     const int pts = ctx->tune_s12_points == 16 ? 16 : 8;
-    if (wave_points == 2) {   // LDS-staged angle tables, 8 points per wavefront
-      dim3 gl((unsigned)((n + 8 * gm::kS12LdsWaves - 1) / (8 * gm::kS12LdsWaves)),
-              (unsigned)((n_ang + 63) / 64));
-      hipLaunchKernelGGL(gm::k_s12_f64_lds, gl, dim3(64 * gm::kS12LdsWaves), 0, s, n, off, nmax,
-                         an, bn, n_ang, pi_t, tau_t, nmax_cap, mode, out0, out1);
+    if (wave_points == 2) {   // LDS-staged angle tables, 8 or 16 points per wavefront
+      const int64_t per_block = (int64_t)pts * gm::kS12LdsWaves;
+      dim3 gl((unsigned)((n + per_block - 1) / per_block), (unsigned)((n_ang + 63) / 64));
+      if (pts == 16)
+        hipLaunchKernelGGL(gm::k_s12_f64_lds<2>, gl, dim3(64 * gm::kS12LdsWaves), 0, s, n, off,
+                           nmax, an, bn, n_ang, pi_t, tau_t, nmax_cap, mode, out0, out1);
+      else
+        hipLaunchKernelGGL(gm::k_s12_f64_lds<1>, gl, dim3(64 * gm::kS12LdsWaves), 0, s, n, off,
+                           nmax, an, bn, n_ang, pi_t, tau_t, nmax_cap, mode, out0, out1);
       ctx->mark_end(slot, s);
       e = hipGetLastError();
       if (e != hipSuccess) return gm_fail(GM_ERUNTIME, "%s: %s", who, hipGetErrorString(e));
