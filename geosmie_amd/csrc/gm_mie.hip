@@ -361,6 +361,7 @@ __global__ __launch_bounds__(64, MINW) void k_mie_homog(
     // behind the previous row's arithmetic)
     cd D_nx = Dp[lane];
     double R_nx = Rp[lane];
+#pragma unroll 2
     for (int n = 1; n <= nmax; ++n) {
       const cd D = D_nx;
       const double R = R_nx;
@@ -538,6 +539,7 @@ __global__ __launch_bounds__(64, MINW) void k_mie_coated_stable(
     const bool store = coef_off != nullptr;
     const int64_t off = store ? coef_off[p] : 0;
     cd du_nx = DU[lane];
+#pragma unroll 2
     for (int n = 1; n <= nmax; ++n) {
       const size_t o = (size_t)(n - 1) * 64 + lane;
       const cd du = du_nx;
