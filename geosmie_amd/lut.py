@@ -87,9 +87,11 @@ def lognorm_psd(rmode, sigma, xxarr, drarr, lam, rmax, rmin):
 class _DevMie(object):
     """Device coefficients of one homogeneous batch (kept resident)."""
 
-    def __init__(self, xx, m, d_eps=None, x_max=None):
+    def __init__(self, xx, m, d_eps=None, x_max=None, offsets=None):
         """xx, m host arrays; or xx a device tensor of size parameters with
-        d_eps (device, interleaved re/im of m^2) and x_max given."""
+        d_eps (device, interleaved re/im of m^2) and x_max given.  offsets =
+        (device coefficient offsets of xx, their total) reuses the offsets of
+        an earlier batch on the same size parameters (no host wait)."""
         t = _lib.torch()
         if hasattr(xx, "data_ptr"):
             self.d_x, self.d_eps = xx, d_eps
@@ -103,10 +105,13 @@ class _DevMie(object):
             xmax = float(xx.max())
         self.n = n
         dev = _lib.device()
-        self.d_off = t.empty(n + 1, dtype=t.int64, device=dev)
         L, h, s = _lib.lib(), _lib.ctx(), stream()
-        check(L.gm_coef_offsets(h, n, ptr(self.d_x), ptr(self.d_off), s), "gm_coef_offsets")
-        total = int(self.d_off[-1].item())
+        if offsets is None:
+            self.d_off = t.empty(n + 1, dtype=t.int64, device=dev)
+            check(L.gm_coef_offsets(h, n, ptr(self.d_x), ptr(self.d_off), s), "gm_coef_offsets")
+            total = int(self.d_off[-1].item())
+        else:
+            self.d_off, total = offsets
         self.d_an = t.empty(2 * max(total, 1), dtype=t.float64, device=dev)
         self.d_bn = t.empty_like(self.d_an)
         self.d_props = t.empty(6 * n, dtype=t.float64, device=dev)

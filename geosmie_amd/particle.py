@@ -209,6 +209,15 @@ def initializeXarr(params, radind, minlam, maxlam):
     elif kind == "ss":
         lo = pp["rMinMaj"][radind] * 2 * np.pi / maxlam
         hi = pp["rMaxMaj"][radind] * 2 * np.pi / minlam * 10
+    elif kind == "du":
+        # the reference raises the log10 values to the tenth power (not ten to
+        # them), so the 1000-point grid is not monotonic: it falls from
+        # log10(minx)**10 to ~0 where the logarithm changes sign and rises to
+        # log10(maxx)**10; kept as written (:456-462).  No widths: the du PSD
+        # takes them from the grid itself.
+        lo = pp["rMinMaj"][radind][0] * 2 * np.pi / maxlam
+        hi = pp["rMaxMaj"][radind][-1] * 2 * np.pi / minlam
+        return (np.linspace(np.log10(lo), np.log10(hi), 1000)) ** 10., None
     else:
         raise ValueError("psd type %s is not supported by the Mie table path" % kind)
     if "numperdec" not in pp:
@@ -281,6 +290,23 @@ def calculatePSD(params, radind, onerh, rh, xxarr, drarr, rrat, lam, want_ref=Tr
         psd.append(w)
         if want_ref:
             ref.append(np.sum(rr ** 4 * w) / np.sum(rr ** 3 * w))
+    elif kind == "du":
+        # r^-4 number distribution per minor sub-bin, with the grid's own
+        # widths; no humidity growth (dointegration.py:626-662)
+        dr = getDR(xxarr)
+        for rmin, rmax in zip(pp["rMinMaj"][radind], pp["rMaxMaj"][radind]):
+            rLow, rUp = rmin, rmax
+            dndr = rarr ** (-4.)
+            dndr[rarr < rmin] = 0.
+            dndr[rarr > rmax] = 0.
+            if np.max(dndr) == 0.:
+                raise ValueError("du sub-bin [%g, %g] m holds no grid radius at wavelength %g m"
+                                 % (rmin, rmax, lam))
+            w = np.copy(dndr) * dr
+            w /= np.sum(w)
+            psd.append(w)
+            if want_ref:
+                ref.append(np.sum(rr ** 4. * w) / np.sum(rr ** 3. * w))
     else:
         raise ValueError("psd type %s is not supported by the Mie table path" % kind)
     return psd, (ref if want_ref else None), rLow, rUp

@@ -18,8 +18,9 @@ and the runoptics command (src/geosmie/runoptics.py):
     (:950-1001);
   * the result goes to an OpticsTable (geosmie_amd.table) in either layout.
 
-Mie mode only ('mode': 'mie', psd types lognorm and ss); GRASP kernel files
-('mode': 'kernel') and the 'du' size grid are not part of this path.
+Mie mode ('mode': 'mie') with psd types lognorm, ss and du (the du grid is
+the reference's non-monotonic one, see particle.initializeXarr); GRASP
+kernel files ('mode': 'kernel') are not part of this path.
 """
 import ctypes
 import os
@@ -47,32 +48,43 @@ class _WavelengthPass(object):
         self.xx, self.pi_t, self.tau_t = xx, pi_t, tau_t
         self.n_ang, self.precision = n_ang, precision
 
-    def run(self, lam, m_blocks, rows, rhop0):
-        """m_blocks: complex index per point block (each block = the whole
-        grid); rows: list of (block, psd [N_x], reff0, rhop).  Returns the
-        un-normalised phase-matrix integrals [rows][4][n_ang] and the scalars
-        [rows][16] (gm_integrate_psd order)."""
+    def launch(self, lam, m_blocks, rows, rhop0):
+        """Enqueue the pass; returns a handle for collect().  m_blocks:
+        complex index per point block (each block = the whole grid); rows:
+        list of (block, psd [N_x], reff0, rhop).  Every host-to-device copy
+        comes first and nothing here waits for a kernel, so the caller can
+        prepare the next wavelength while these kernels run."""
         t = _lib.torch()
         nx, nb = self.xx.size, len(m_blocks)
         n = nx * nb
-        dev = _lib.device()
-        # the grid repeated once per block, and each block's m^2, formed on the device
-        if getattr(self, "_d_x_nb", None) is None or self._d_x_nb[0] != nb:
-            self._d_x_nb = (nb, _lib.to_dev(self.xx).repeat(nb))
-        eps = np.asarray(m_blocks, np.complex128) ** 2
-        d_eps = _lib.to_dev(eps.view(np.float64).reshape(nb, 2)).repeat_interleave(nx, dim=0)
-        dm = _DevMie(self._d_x_nb[1], None, d_eps=d_eps.reshape(-1), x_max=float(self.xx.max()))
-        d_mu = dm.mueller(self.pi_t, self.tau_t, self.precision)
         nr = len(rows)
+        dev = _lib.device()
+        L, h, s = _lib.lib(), _lib.ctx(), stream()
+        # host -> device first (a pageable copy waits for the stream's work)
+        eps = np.asarray(m_blocks, np.complex128) ** 2
+        d_eps = _lib.to_dev(eps.view(np.float64).reshape(nb, 2))
+        d_psd = _lib.to_dev(np.stack([r[1] for r in rows]))
+        d_blk = _lib.to_dev(np.array([r[0] for r in rows], np.int64), dtype=np.int64)
+        d_r0 = _lib.to_dev(np.array([r[2] for r in rows], np.float64))
+        d_rho = _lib.to_dev(np.array([r[3] for r in rows], np.float64))
+        # the grid repeated once per block, and its coefficient offsets: the
+        # same for every wavelength of the bin, so built once per block count
+        if getattr(self, "_grid", None) is None or self._grid[0] != nb:
+            d_x = _lib.to_dev(self.xx).repeat(nb)
+            d_off = t.empty(n + 1, dtype=t.int64, device=dev)
+            check(L.gm_coef_offsets(h, n, ptr(d_x), ptr(d_off), s), "gm_coef_offsets")
+            self._grid = (nb, d_x, d_off, int(d_off[-1].item()))
+        _, d_x, d_off, total = self._grid
+        d_eps = d_eps.repeat_interleave(nx, dim=0).reshape(-1)
+        dm = _DevMie(d_x, None, d_eps=d_eps, x_max=float(self.xx.max()), offsets=(d_off, total))
+        d_mu = dm.mueller(self.pi_t, self.tau_t, self.precision)
         # dense [rows][points] weights, zero outside each row's block, assembled
         # on the device from the compact [rows][grid] PSDs
-        d_psd = _lib.to_dev(np.stack([r[1] for r in rows]))
         d_w = t.zeros((nr, nb, nx), dtype=t.float64, device=dev)
-        d_w[t.arange(nr, device=dev), t.tensor([r[0] for r in rows], device=dev)] = d_psd
+        d_w[t.arange(nr, device=dev), d_blk] = d_psd
         d_w = d_w.view(nr, n)
         n_cols = 4 * self.n_ang
         d_pm = t.empty(nr * n_cols, dtype=t.float64, device=dev)
-        L, h, s = _lib.lib(), _lib.ctx(), stream()
         for r0 in range(0, nr, ROWS_PER_CALL):
             k = min(ROWS_PER_CALL, nr - r0)
             starts = (ctypes.c_int64 * k)(*[rows[r0 + i][0] * nx for i in range(k)])
@@ -82,15 +94,23 @@ class _WavelengthPass(object):
                                          ptr(d_pm[r0 * n_cols:(r0 + k) * n_cols]), s),
                   "gm_psd_contract_rows")
         d_lp = t.full((n,), float(lam), dtype=t.float64, device=dev)
-        d_r0 = _lib.to_dev(np.array([r[2] for r in rows], np.float64))
-        d_rho = _lib.to_dev(np.array([r[3] for r in rows], np.float64))
         d_sc = t.empty(nr * len(SCALAR_OUT), dtype=t.float64, device=dev)
         check(L.gm_integrate_psd_rows(h, nr, n, ptr(dm.d_x), ptr(d_lp), ptr(dm.d_props), ptr(d_w),
                                       ptr(d_r0), float(rhop0), 0.0, ptr(d_rho), ptr(d_sc), s),
               "gm_integrate_psd_rows")
+        # keep every buffer the enqueued kernels use alive until collect()
+        return (nr, d_pm, d_sc, (dm, d_mu, d_w, d_lp, d_r0, d_rho, d_psd, d_blk))
+
+    def collect(self, handle):
+        """The un-normalised phase-matrix integrals [rows][4][n_ang] and the
+        scalars [rows][16] (gm_integrate_psd order) of a launched pass."""
+        nr, d_pm, d_sc, _ = handle
         pm = d_pm.cpu().numpy().reshape(nr, 4, self.n_ang)
         sc = d_sc.cpu().numpy().reshape(nr, len(SCALAR_OUT))
         return pm, sc
+
+    def run(self, lam, m_blocks, rows, rhop0):
+        return self.collect(self.launch(lam, m_blocks, rows, rhop0))
 
 
 def _normalize(pm, ang):
@@ -119,6 +139,8 @@ def build_table(params, water=None, classic=False, precision=0, ang=None, n_mom=
         radii = list(pp["r0"])
     elif kind == "ss":
         radii = list(pp["rMinMaj"])
+    elif kind == "du":       # the first sub-bin's lower edge names the bin (:722-724)
+        radii = [r[0] for r in pp["rMinMaj"]]
     else:
         raise ValueError("psd type %s is not supported by the Mie table path" % kind)
     mlist = params["mList"]
@@ -143,15 +165,24 @@ def build_table(params, water=None, classic=False, precision=0, ang=None, n_mom=
         pi_t, tau_t = angle_tables_dev(cost, nmax_cap)
         dev_pass = _WavelengthPass(xx, pi_t, tau_t, ang.size, precision)
         fracs = pp["fracs"][radind]
+        water = (w_mr, w_mi) if rhdep != "trivial" else None
+
+        def prepare(lam):
+            return _prepare(params, radind, lam, xx, dr, fracs, rhop0, part_mr, part_mi, water)
+        # the host front end of wavelength li+1 (humid indices, PSDs) runs
+        # while the kernels of wavelength li execute
+        prep = prepare(lams[0])
         for li, lam in enumerate(lams):
-            _wavelength(table, params, radind, li, lam, xx, dr, fracs, rhop0, dev_pass, ang,
-                        part_mr, part_mi, (w_mr, w_mi) if rhdep != "trivial" else None, n_mom)
+            handle = dev_pass.launch(lam, prep[0], prep[1], rhop0)
+            nxt = prepare(lams[li + 1]) if li + 1 < len(lams) else None
+            _wavelength(table, radind, li, fracs, rhop0, dev_pass, ang, prep, handle, n_mom)
+            prep = nxt
     return table
 
 
-def _wavelength(table, params, radind, li, lam, xx, dr, fracs, rhop0, dev_pass, ang, part_mr,
-                part_mi, water, n_mom=0):
-    """All rh entries of one (bin, wavelength) (dointegration.py:811-1031)."""
+def _prepare(params, radind, lam, xx, dr, fracs, rhop0, part_mr, part_mi, water):
+    """Host front end of one (bin, wavelength): the point blocks (one wet index
+    each), the weight rows and the per-rh metadata (dointegration.py:811-868)."""
     nref0 = [complex(f(lam), -g(lam)) for f, g in zip(part_mr, part_mi)]
     nw = complex(water[0](lam), water[1](lam)) if water is not None else complex(1, 0)
     rh = list(params["rh"])
@@ -176,7 +207,15 @@ def _wavelength(table, params, radind, li, lam, xx, dr, fracs, rhop0, dev_pass, 
             mode_rows.append(len(rows))
             rows.append((b0 + (k if n_ri > 1 else 0), psd[k], reff0[k], rhop))
         meta.append((rhi, mr, mi, gf, rhop, rlo, rup, mode_rows))
-    pm, sc = dev_pass.run(lam, blocks, rows, rhop0)
+    return blocks, rows, meta, trivial, len(rh)
+
+
+def _wavelength(table, radind, li, fracs, rhop0, dev_pass, ang, prep, handle, n_mom=0):
+    """All rh entries of one (bin, wavelength) (dointegration.py:869-1031):
+    the launched device pass's results, mode mixing, normalisation and the
+    table writes."""
+    blocks, rows, meta, trivial, n_rh = prep
+    pm, sc = dev_pass.collect(handle)
     # mix modes in integratePSD's order, then normalise every entry on the device
     mixed_p = np.zeros((len(meta), 4, ang.size))
     mixed_s = np.zeros((len(meta), len(SCALAR_OUT)))
@@ -212,7 +251,7 @@ def _wavelength(table, params, radind, li, lam, xx, dr, fracs, rhop0, dev_pass, 
         table.set_entry(radind, li, rhi, ret, pmom=None if pmom is None else pmom[e])
         table.set_bin_rh(radind, rhi, nl)
     if trivial:   # copyDryValues (:462-490): every rh repeats the dry entry
-        for rhi in range(1, len(rh)):
+        for rhi in range(1, n_rh):
             table.copy_rh(radind, li, 0, rhi)
 
 

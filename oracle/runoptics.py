@@ -33,6 +33,7 @@ def table(params, water=None, ang=None):
     cost = np.cos(np.radians(ang))
     pp = params["psd"]["params"]
     n_bin = len(pp["r0"]) if params["psd"]["type"] == "lognorm" else len(pp["rMinMaj"])
+    # (du: one bin per rMinMaj list, its sub-bins being the PSD's modes)
     mlist = params["mList"]
     lams = mlist[0][0]
     fmr = [interp1d(m[0], m[1]) for m in mlist]

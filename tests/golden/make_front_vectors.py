@@ -60,13 +60,14 @@ def main():
         os.chdir(cwd)
     water = out["water"]
     cases = []
-    files = ["oc.json", "su.json", "bc.json", "ss.json", "ni.json"]
+    files = ["oc.json", "su.json", "bc.json", "ss.json", "ni.json", "du-mie.json"]
     for fn in files + ["su.json#su"]:
         name, _, variant = fn.partition("#")
         params = json.load(open(os.path.join(CFG, name)))
         if variant == "su":
             params["rhDep"] = {"type": "su", "params": {"temp": 230.0}}
-        params["psd"]["params"]["numperdec"] = [40] * len(params["psd"]["params"]["numperdec"])
+        if "numperdec" in params["psd"]["params"]:
+            params["psd"]["params"]["numperdec"] = [40] * len(params["psd"]["params"]["numperdec"])
         ri = params["ri"]
         os.chdir(SRC)
         try:
@@ -105,7 +106,8 @@ def main():
                     key = "psd_%d" % (len(cases) - 1)
                     out[key] = np.array(psd)
             out["xx_%s_%d" % (fn, radind)] = xx
-            out["dr_%s_%d" % (fn, radind)] = dr
+            if dr is not None:      # the du grid has no widths
+                out["dr_%s_%d" % (fn, radind)] = dr
         cases_params = {fn: params}
         out["params_%s" % fn] = np.array(json.dumps(cases_params[fn]))
     out["cases"] = np.array(json.dumps(cases))

@@ -52,10 +52,20 @@ def case(name):
                     "fracs": [[0.3, 0.7], [0.6, 0.4]]}},
                 "mList": [_mlist(1.53, [0.003, 0.002, 0.004, 0.01]),
                           _mlist(1.45, [0.0, 1e-6, 1e-4, 1e-3])]}
+    if name == "du_trivial":
+        # dust: r^-4 sub-bins on the reference's non-monotonic du grid, which
+        # for the first bin passes through x ~ 1e-30 (log10 x changes sign)
+        return {"rhop0": [2500.0, 2650.0], "rh": rh,
+                "rhDep": {"type": "trivial", "params": {"gf": [1.0]}},
+                "psd": {"type": "du", "params": {
+                    "rMinMaj": [[0.1e-6, 0.3e-6], [1.0e-6]],
+                    "rMaxMaj": [[0.3e-6, 1.0e-6], [2.0e-6]],
+                    "fracs": [[0.3, 0.7], [1.0]]}},
+                "mList": [_mlist(1.53, [0.004, 0.003, 0.002, 0.004])]}
     raise KeyError(name)
 
 
-NAMES = ["lognorm_simple", "lognorm_su", "ss_maxrh", "bimodal_trivial"]
+NAMES = ["lognorm_simple", "lognorm_su", "ss_maxrh", "bimodal_trivial", "du_trivial"]
 
 
 def params(name):

@@ -11,7 +11,7 @@ from tests.runoptics_cases import params, water
 @pytest.fixture(scope="module")
 def tables():
     return {n: ORO.table(params(n), water()) for n in ("lognorm_simple", "ss_maxrh",
-                                                       "bimodal_trivial")}
+                                                       "bimodal_trivial", "du_trivial")}
 
 
 def test_dry_mass_reference(tables):
@@ -29,9 +29,10 @@ def test_dry_mass_reference(tables):
 
 
 def test_trivial_copies_and_maxrh(tables):
-    ent, _ = tables["bimodal_trivial"]
-    for (b, li, rhi), e in ent.items():
-        assert e is ent[b, li, 0]
+    for name in ("bimodal_trivial", "du_trivial"):
+        ent, _ = tables[name]
+        for (b, li, rhi), e in ent.items():
+            assert e is ent[b, li, 0]
     ent, _ = tables["ss_maxrh"]
     for (b, li, rhi), e in ent.items():
         if rhi == 3:      # rh 0.99 is computed at maxrh = 0.95, the entry before it
