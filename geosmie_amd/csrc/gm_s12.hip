@@ -313,6 +313,186 @@ __global__ __launch_bounds__(64 * kS12LdsWaves) void k_s12_f64_lds(
     }
 }
 
+// ---- S1/S2 -> Mueller -> weighted point sums, fused (FP64) ------------------
+//
+// integratePSD integrates each Mueller element of rawMie against a row of PSD
+// weights (dointegration.py:1044-1050, 1164-1166: np.dot(s_ij.T, psd)).  When
+// every weight row covers a contiguous range of points and few rows share a
+// point (the runoptics passes: one block of points per (rh, refractive index),
+// one row per mode), the Mueller planes need not exist in memory: each
+// wavefront forms S1/S2 of 8 points x 64 angles on the matrix cores as
+// k_s12_f64 does, turns them into s11, s12, s33, s34 in registers and adds
+// weight x element into per-row sums.  A block walks one chunk of chunk_pts
+// consecutive points (whole 8-point tiles) of one row group (rows with the
+// same range), its four wavefronts taking 64 angles each; the sums go to a
+// partial buffer [chunk][RM][4][n_ang] that k_s12_rows_reduce adds up per row
+// in chunk order (deterministic).  Memory traffic drops from 2 x 32 B per
+// (point, angle) to the partials (8 / RM times less with one tile per chunk).
+// Chunks of one tile keep the work units as small as k_s12_f64's, so the
+// long K loops of large size parameters do not end up in a few long waves.
+constexpr int kRowsPerCall = 64;   // rows per call (range arrays passed by value)
+constexpr int kRowsWaves = 4;      // wavefronts (64 angles each) per block
+
+struct RowGroups {
+  int n_groups;
+  int64_t start[kRowsPerCall], count[kRowsPerCall];
+  int row0[kRowsPerCall], nrow[kRowsPerCall];
+  int64_t chunk0[kRowsPerCall + 1];   // first chunk of each group; chunk0[n_groups] = total
+};
+
+// RM = most rows of one group this instance accumulates (registers: 16 RM
+// doubles per lane for the sums).
+template <int RM>
+__global__ __launch_bounds__(64 * kRowsWaves) void k_s12_rows_f64(
+    RowGroups g, int chunk_pts, const int64_t* __restrict__ off, const int32_t* __restrict__ nmax,
+    const double* __restrict__ an, const double* __restrict__ bn, int n_ang,
+    const double* __restrict__ pi_t, const double* __restrict__ tau_t, int nmax_cap,
+    const double* __restrict__ w, int64_t ldw, double* __restrict__ part) {
+  const int lane = threadIdx.x & 63;
+  const int64_t chunk = blockIdx.x;
+  const int j0 = (blockIdx.y * kRowsWaves + (threadIdx.x >> 6)) * 64;
+  if (j0 >= n_ang) return;   // whole wavefront past the last angle (no barriers below)
+  int gi = 0;
+  while (gi + 1 < g.n_groups && g.chunk0[gi + 1] <= chunk) ++gi;
+  const int64_t cstart = g.start[gi] + (chunk - g.chunk0[gi]) * chunk_pts;
+  const int64_t cend = min(cstart + (int64_t)chunk_pts, g.start[gi] + g.count[gi]);
+  const int nr = g.nrow[gi];
+  const double* wr = w + (size_t)g.row0[gi] * ldw;
+  const int i = lane & 15;
+  const int kq = lane >> 4;
+  const int comp = (i >> 2) & 1;
+  double sum[RM][4][4];   // [row][angle sub-tile t][element]
+#pragma unroll
+  for (int r = 0; r < RM; ++r)
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) sum[r][t][e] = 0.0;
+  for (int64_t p0 = cstart; p0 < cend; p0 += 8) {
+    const int64_t pa = p0 + 2 * (i & 3) + (i >> 3);
+    const bool ok = pa < cend;
+    const int nm_a = ok ? nmax[pa] : 0;
+    const int64_t oa = ok ? off[pa] : 0;
+    int kmax = 0;
+    for (int q = 0; q < 8; ++q)
+      if (p0 + q < cend) kmax = max(kmax, nmax[p0 + q]);
+    kmax = min(kmax, nmax_cap);
+    double4_ accp[4], accm[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      accp[t] = (double4_){0.0, 0.0, 0.0, 0.0};
+      accm[t] = (double4_){0.0, 0.0, 0.0, 0.0};
+    }
+    double ar_n = 0.0, br_n = 0.0, pp_n[4], tt_n[4];
+    auto load_step = [&](int k0) {
+      const int kk = k0 + kq;
+      ar_n = 0.0;
+      br_n = 0.0;
+      if (kk < nm_a) {
+        ar_n = an[2 * (oa + kk) + comp];
+        br_n = bn[2 * (oa + kk) + comp];
+      }
+      const bool krow = kk < kmax;
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        const int j = j0 + t * 16 + i;
+        pp_n[t] = 0.0;
+        tt_n[t] = 0.0;
+        if (krow && j < n_ang) {
+          pp_n[t] = pi_t[(size_t)kk * n_ang + j];
+          tt_n[t] = tau_t[(size_t)kk * n_ang + j];
+        }
+      }
+    };
+    if (kmax > 0) load_step(0);
+    for (int k0 = 0; k0 < kmax; k0 += 4) {
+      const double ua = ar_n + br_n, va = ar_n - br_n;
+      double P[4], M[4];
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        P[t] = pp_n[t] + tt_n[t];
+        M[t] = pp_n[t] - tt_n[t];
+      }
+      if (k0 + 4 < kmax) load_step(k0 + 4);
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        accp[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(ua, P[t], accp[t], 0, 0, 0);
+        accm[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(va, M[t], accm[t], 0, 0, 0);
+      }
+    }
+    // lane (i, kq) holds angle j0 + 16 t + i of points p0 + 2 kq + h (h = 0, 1)
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int64_t p = p0 + 2 * kq + h;
+      double wv[RM];
+#pragma unroll
+      for (int r = 0; r < RM; ++r) wv[r] = (r < nr && p < cend) ? wr[(size_t)r * ldw + p] : 0.0;
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        const cd sp = cd{accp[t][2 * h], accp[t][2 * h + 1]};
+        const cd sm = cd{accm[t][2 * h], accm[t][2 * h + 1]};
+        const cd s1 = (sp + sm) * 0.5, s2 = (sp - sm) * 0.5;
+        const double a1 = norm2(s1), a2 = norm2(s2);
+        const cd c = s1 * conj(s2);
+        const double el[4] = {0.5 * (a1 + a2), 0.5 * (a2 - a1), c.re, c.im};
+#pragma unroll
+        for (int r = 0; r < RM; ++r)
+#pragma unroll
+          for (int e = 0; e < 4; ++e) sum[r][t][e] = fma(wv[r], el[e], sum[r][t][e]);
+      }
+    }
+  }
+  // add the four point pairs (lanes i + 16 kq) and write; lane group kq
+  // writes element kq
+  double* dst = part + (size_t)chunk * RM * 4 * n_ang;
+#pragma unroll
+  for (int r = 0; r < RM; ++r)
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      double mine = 0.0;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        double v = sum[r][t][e];
+        v += __shfl_xor(v, 16);
+        v += __shfl_xor(v, 32);
+        if (e == kq) mine = v;
+      }
+      const int j = j0 + t * 16 + i;
+      if (r < nr && j < n_ang) dst[((size_t)r * 4 + kq) * n_ang + j] = mine;
+    }
+}
+
+// out[row][e][j] = sum over the row group's chunks of the partials.  A block
+// covers 64 consecutive (e, j) of one row; its 16 wavefronts take every 16th
+// chunk each, and the 16 sums are added in wavefront order (deterministic).
+constexpr int kReduceWaves = 16;
+template <int RM>
+__global__ __launch_bounds__(64 * kReduceWaves) void k_s12_rows_reduce(
+    RowGroups g, int n_ang, const double* __restrict__ part, double* __restrict__ out) {
+  __shared__ double red[kReduceWaves][64];
+  const int lane = threadIdx.x & 63;
+  const int q = threadIdx.x >> 6;
+  const int per_row = 4 * n_ang;
+  const int slabs = (per_row + 63) / 64;
+  const int row = (int)(blockIdx.x / slabs);
+  const int ej = (int)(blockIdx.x % slabs) * 64 + lane;
+  int gi = 0;
+  while (gi + 1 < g.n_groups && g.row0[gi + 1] <= row) ++gi;
+  const int rl = row - g.row0[gi];
+  double s = 0.0;
+  if (ej < per_row)
+    for (int64_t c = g.chunk0[gi] + q; c < g.chunk0[gi + 1]; c += kReduceWaves)
+      s += part[((size_t)c * RM + rl) * per_row + ej];
+  red[q][lane] = s;
+  __syncthreads();
+  if (q == 0 && ej < per_row) {
+    double t = 0.0;
+#pragma unroll
+    for (int k = 0; k < kReduceWaves; ++k) t += red[k][lane];
+    out[(size_t)row * per_row + ej] = t;
+  }
+}
+
 // ---- reduced-precision paths -------------------------------------------------
 
 constexpr int kLpK = 32;     // K padding of the reduced-precision tables
@@ -644,6 +824,94 @@ int gm_s12_mueller(gm_ctx* ctx, int64_t n, const int64_t* coef_off, const int32_
   return launch_s12(ctx, "gm_s12_mueller", n, coef_off, nmax, an, bn, n_ang, pi_t, tau_t,
                     nmax_cap, (int)gm::kOutMueller, mueller, (double*)nullptr, precision,
                     (hipStream_t)stream);
+}
+
+int gm_s12_contract_rows(gm_ctx* ctx, int64_t n, const int64_t* coef_off, const int32_t* nmax,
+                         const double* an, const double* bn, int n_ang, const double* pi_t,
+                         const double* tau_t, int nmax_cap, int n_rows, const int64_t* start_host,
+                         const int64_t* count_host, const double* w, double* out,
+                         gm_stream stream) {
+  if (!ctx || n < 0 || n_ang < 0 || n_rows < 0 || n_rows > gm::kRowsPerCall)
+    return gm_fail(GM_EINVAL, "gm_s12_contract_rows: bad arguments (at most %d rows)",
+                   gm::kRowsPerCall);
+  if (n_rows == 0 || n_ang == 0) return GM_OK;
+  if (!start_host || !count_host || !coef_off || !nmax || !an || !bn || !pi_t || !tau_t || !w ||
+      !out)
+    return gm_fail(GM_EINVAL, "gm_s12_contract_rows: NULL array");
+  if (nmax_cap < 1) return gm_fail(GM_EINVAL, "gm_s12_contract_rows: nmax_cap must be >= 1");
+  // rows with the same range form a group (at most 4 rows; a longer run of
+  // equal ranges is split, its points then contracted once per group).  One
+  // 8-point tile per chunk, two when that still leaves >= 16k wavefronts.
+  int64_t tiles = 0;
+  for (int r = 0; r < n_rows; ++r)
+    if (count_host[r] > 0) tiles += (count_host[r] + 7) / 8;
+  const int64_t ablocks = (n_ang + 64 * gm::kRowsWaves - 1) / (64 * gm::kRowsWaves);
+  const int chunk_pts = (tiles / 2) * ablocks * gm::kRowsWaves >= 16384 ? 16 : 8;
+  gm::RowGroups g;
+  g.n_groups = 0;
+  g.chunk0[0] = 0;
+  int rm = 1;
+  for (int r = 0; r < n_rows; ++r) {
+    if (start_host[r] < 0 || count_host[r] < 1 || start_host[r] + count_host[r] > n)
+      return gm_fail(GM_EINVAL, "gm_s12_contract_rows: row %d range outside [0, n)", r);
+    const int k = g.n_groups - 1;
+    if (k >= 0 && start_host[r] == g.start[k] && count_host[r] == g.count[k] && g.nrow[k] < 4) {
+      rm = max(rm, ++g.nrow[k]);
+      continue;
+    }
+    g.start[g.n_groups] = start_host[r];
+    g.count[g.n_groups] = count_host[r];
+    g.row0[g.n_groups] = r;
+    g.nrow[g.n_groups] = 1;
+    g.chunk0[g.n_groups + 1] = g.chunk0[g.n_groups] + (count_host[r] + chunk_pts - 1) / chunk_pts;
+    ++g.n_groups;
+  }
+  for (int k = g.n_groups; k < gm::kRowsPerCall; ++k) {
+    g.start[k] = g.count[k] = 0;
+    g.row0[k] = n_rows;
+    g.nrow[k] = 0;
+    g.chunk0[k + 1] = g.chunk0[g.n_groups];
+  }
+  const int64_t chunks = g.chunk0[g.n_groups];
+  if (chunks > 0x7fffffffLL) return gm_fail(GM_EINVAL, "gm_s12_contract_rows: too many points");
+  const int RM = rm <= 1 ? 1 : (rm <= 2 ? 2 : 4);
+  hipError_t e = hipSetDevice(ctx->device);
+  if (e != hipSuccess) return gm_fail(GM_ERUNTIME, "hipSetDevice: %s", hipGetErrorString(e));
+  hipStream_t s = (hipStream_t)stream;
+  int rc = ctx->reserve(ctx->gemm_tmp, sizeof(double) * (size_t)chunks * RM * 4 * n_ang);
+  if (rc) return rc;
+  double* part = (double*)ctx->gemm_tmp.ptr;
+  const dim3 grid((unsigned)chunks, (unsigned)ablocks);
+  const dim3 block(64 * gm::kRowsWaves);
+  const dim3 rgrid((unsigned)((int64_t)n_rows * ((4 * (int64_t)n_ang + 63) / 64)));
+  const dim3 rblock(64 * gm::kReduceWaves);
+  size_t slot;
+  ctx->mark_begin(gm::kKS12, s, &slot);
+  if (RM == 1)
+    hipLaunchKernelGGL(gm::k_s12_rows_f64<1>, grid, block, 0, s, g, chunk_pts, coef_off, nmax,
+                       an, bn, n_ang, pi_t, tau_t, nmax_cap, w, n, part);
+  else if (RM == 2)
+    hipLaunchKernelGGL(gm::k_s12_rows_f64<2>, grid, block, 0, s, g, chunk_pts, coef_off, nmax,
+                       an, bn, n_ang, pi_t, tau_t, nmax_cap, w, n, part);
+  else
+    hipLaunchKernelGGL(gm::k_s12_rows_f64<4>, grid, block, 0, s, g, chunk_pts, coef_off, nmax,
+                       an, bn, n_ang, pi_t, tau_t, nmax_cap, w, n, part);
+  ctx->mark_end(slot, s);
+  ctx->mark_begin(gm::kKPsd, s, &slot);
+  if (RM == 1)
+    hipLaunchKernelGGL(gm::k_s12_rows_reduce<1>, rgrid, rblock, 0, s, g, n_ang,
+                       (const double*)part, out);
+  else if (RM == 2)
+    hipLaunchKernelGGL(gm::k_s12_rows_reduce<2>, rgrid, rblock, 0, s, g, n_ang,
+                       (const double*)part, out);
+  else
+    hipLaunchKernelGGL(gm::k_s12_rows_reduce<4>, rgrid, rblock, 0, s, g, n_ang,
+                       (const double*)part, out);
+  ctx->mark_end(slot, s);
+  e = hipGetLastError();
+  if (e != hipSuccess)
+    return gm_fail(GM_ERUNTIME, "gm_s12_contract_rows: %s", hipGetErrorString(e));
+  return GM_OK;
 }
 
 int gm_s12_p11(gm_ctx* ctx, int64_t n, const int64_t* coef_off, const int32_t* nmax,

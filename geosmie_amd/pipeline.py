@@ -8,8 +8,11 @@ runoptics' per-(wavelength, rh) iteration computes
 
   gm_coef_offsets, gm_mie_batch          a_n, b_n and efficiencies (FP64)
   (gm_angle_tables                        pi'_n, tau'_n: once, at construction)
-  gm_s12_mueller                          S1/S2 on MFMA -> s11, s12, s33, s34
-  gm_psd_contract_rows                    phase-matrix integrals of integratePSD
+  gm_s12_contract_rows                    S1/S2 on MFMA -> s11, s12, s33, s34 ->
+                                          phase-matrix integrals of integratePSD,
+                                          in one pass (FP64); at reduced
+                                          precision gm_s12_mueller, then
+                                          gm_psd_contract_rows
   gm_integrate_psd_rows                   the integratePSD scalars of every row
   gm_pmatrix_normalize                    P normalisation and pback
   gm_gsf_expand (optional)                GSF moments of every row
@@ -30,10 +33,12 @@ from ._lib import check, ptr
 class RunopticsChain(object):
 
     def __init__(self, x, m, ang, w, lam_of_row, ranges, precision=0, gsf_ng=0, rhop0=1000.0,
-                 rhop=1000.0):
+                 rhop=1000.0, fused=True):
         """x [n] size parameters, m complex index (scalar or [n]), ang [n_ang]
         degrees, w [rows, n] number weights (row r nonzero only on its points
-        ranges[r] = (start, count)), lam_of_row [rows] wavelengths."""
+        ranges[r] = (start, count)), lam_of_row [rows] wavelengths.  fused:
+        at FP64 (precision 0) S1/S2 and the row integrals in one pass, without
+        the [n][4][n_ang] Mueller planes."""
         t = _lib.torch()
         self.t = t
         x = np.ascontiguousarray(x, dtype=np.float64)
@@ -49,6 +54,7 @@ class RunopticsChain(object):
         if len(self.ranges) != w.shape[0] or not 1 <= w.shape[0] <= 64:
             raise ValueError("one (start, count) range per weight row, 1 to 64 rows")
         self.precision = int(precision)
+        self.fused = bool(fused)
         self.gsf_ng = int(gsf_ng)
         self.rhop0, self.rhop = float(rhop0), float(rhop)
         self.ctx = _lib.private_ctx()
@@ -68,7 +74,7 @@ class RunopticsChain(object):
         self.d_nmax = t.empty(self.n, dtype=t.int32, device=dev)
         self.d_pi = t.empty(self.nmax_cap * self.n_ang, dtype=f64, device=dev)
         self.d_tau = t.empty_like(self.d_pi)
-        self.d_mu = t.empty(self.n * 4 * self.n_ang, dtype=f64, device=dev)
+        self.d_mu = None   # Mueller planes, allocated when a step needs them
         self.d_pm = t.empty(self.rows * 4 * self.n_ang, dtype=f64, device=dev)
         self.d_back = t.empty(self.rows * 6, dtype=f64, device=dev)
         self.d_sc = t.empty(self.rows * 16, dtype=f64, device=dev)
@@ -97,13 +103,24 @@ class RunopticsChain(object):
         check(L.gm_mie_batch(h, self.n, P(self.d_x), None, P(self.d_eps), None, None,
                              self.size_max, P(self.d_off), P(self.d_an), P(self.d_bn),
                              P(self.d_props), P(self.d_nmax), _lib.GM_MIE_SORT, s), "gm_mie_batch")
-        check(L.gm_s12_mueller(h, self.n, P(self.d_off), P(self.d_nmax), P(self.d_an),
-                               P(self.d_bn), self.n_ang, P(self.d_pi), P(self.d_tau),
-                               self.nmax_cap, P(self.d_mu), self.precision, s), "gm_s12_mueller")
-        # one launch integrates every row over its own points
-        check(L.gm_psd_contract_rows(h, self.rows, self._seg_start, self._seg_count, self.n,
-                                     4 * self.n_ang, P(self.d_w), P(self.d_mu), P(self.d_pm), s),
-              "gm_psd_contract_rows")
+        if self.fused and self.precision == 0:
+            check(L.gm_s12_contract_rows(h, self.n, P(self.d_off), P(self.d_nmax), P(self.d_an),
+                                         P(self.d_bn), self.n_ang, P(self.d_pi), P(self.d_tau),
+                                         self.nmax_cap, self.rows, self._seg_start,
+                                         self._seg_count, P(self.d_w), P(self.d_pm), s),
+                  "gm_s12_contract_rows")
+        else:
+            if self.d_mu is None:
+                self.d_mu = self.t.empty(self.n * 4 * self.n_ang, dtype=self.t.float64,
+                                         device=_lib.device())
+            check(L.gm_s12_mueller(h, self.n, P(self.d_off), P(self.d_nmax), P(self.d_an),
+                                   P(self.d_bn), self.n_ang, P(self.d_pi), P(self.d_tau),
+                                   self.nmax_cap, P(self.d_mu), self.precision, s),
+                  "gm_s12_mueller")
+            # one launch integrates every row over its own points
+            check(L.gm_psd_contract_rows(h, self.rows, self._seg_start, self._seg_count, self.n,
+                                         4 * self.n_ang, P(self.d_w), P(self.d_mu), P(self.d_pm),
+                                         s), "gm_psd_contract_rows")
         check(L.gm_integrate_psd_rows(h, self.rows, self.n, P(self.d_x), P(self.d_lam_pts),
                                       P(self.d_props), P(self.d_w), P(self.d_reff0), self.rhop0,
                                       self.rhop, None, P(self.d_sc), s), "gm_integrate_psd_rows")

@@ -77,7 +77,6 @@ class _WavelengthPass(object):
         _, d_x, d_off, total = self._grid
         d_eps = d_eps.repeat_interleave(nx, dim=0).reshape(-1)
         dm = _DevMie(d_x, None, d_eps=d_eps, x_max=float(self.xx.max()), offsets=(d_off, total))
-        d_mu = dm.mueller(self.pi_t, self.tau_t, self.precision)
         # dense [rows][points] weights, zero outside each row's block, assembled
         # on the device from the compact [rows][grid] PSDs
         d_w = t.zeros((nr, nb, nx), dtype=t.float64, device=dev)
@@ -85,14 +84,25 @@ class _WavelengthPass(object):
         d_w = d_w.view(nr, n)
         n_cols = 4 * self.n_ang
         d_pm = t.empty(nr * n_cols, dtype=t.float64, device=dev)
+        # FP64: S1/S2, the Mueller elements and their PSD sums in one pass
+        # (no Mueller planes in memory); reduced precisions go through them
+        d_mu = None if self.precision == 0 else dm.mueller(self.pi_t, self.tau_t, self.precision)
+        cap = int(self.pi_t.shape[0])
         for r0 in range(0, nr, ROWS_PER_CALL):
             k = min(ROWS_PER_CALL, nr - r0)
             starts = (ctypes.c_int64 * k)(*[rows[r0 + i][0] * nx for i in range(k)])
             counts = (ctypes.c_int64 * k)(*([nx] * k))
-            check(L.gm_psd_contract_rows(h, k, starts, counts, n, n_cols,
-                                         ptr(d_w[r0:r0 + k]), ptr(d_mu),
-                                         ptr(d_pm[r0 * n_cols:(r0 + k) * n_cols]), s),
-                  "gm_psd_contract_rows")
+            out = ptr(d_pm[r0 * n_cols:(r0 + k) * n_cols])
+            if d_mu is None:
+                check(L.gm_s12_contract_rows(h, n, ptr(dm.d_off), ptr(dm.d_nmax), ptr(dm.d_an),
+                                             ptr(dm.d_bn), self.n_ang, ptr(self.pi_t),
+                                             ptr(self.tau_t), cap, k, starts, counts,
+                                             ptr(d_w[r0:r0 + k]), out, s),
+                      "gm_s12_contract_rows")
+            else:
+                check(L.gm_psd_contract_rows(h, k, starts, counts, n, n_cols,
+                                             ptr(d_w[r0:r0 + k]), ptr(d_mu), out, s),
+                      "gm_psd_contract_rows")
         d_lp = t.full((n,), float(lam), dtype=t.float64, device=dev)
         d_sc = t.empty(nr * len(SCALAR_OUT), dtype=t.float64, device=dev)
         check(L.gm_integrate_psd_rows(h, nr, n, ptr(dm.d_x), ptr(d_lp), ptr(dm.d_props), ptr(d_w),

@@ -196,6 +196,23 @@ int gm_psd_contract_rows(gm_ctx* ctx, int n_rows, const int64_t* start_host,
                          const int64_t* count_host, int64_t n_pts, int n_cols, const double* w,
                          const double* v, double* out, gm_stream stream);
 
+/* gm_s12_mueller followed by gm_psd_contract_rows, fused (FP64): out[r][e][j]
+ * = sum over p in [start_host[r], start_host[r] + count_host[r]) of w[r][p]
+ * s_e(p, j), e = s11, s12, s33, s34, without materialising the Mueller planes
+ * (calculateScatVals then integratePSD's phase-matrix sums,
+ * src/geosmie/dointegration.py:1044-1050, 1164-1166).  Arguments as
+ * gm_s12_mueller (coefficients of n points) and gm_psd_contract_rows (n_rows
+ * <= 64, ranges on the host, w [n_rows][n] device); out [n_rows][4][n_ang].
+ * Consecutive rows with the same range share one S1/S2 pass (up to 4 rows
+ * each), so it suits rows that overlap little, as runoptics' per-rh rows do;
+ * many rows over the same points are cheaper through gm_s12_mueller and
+ * gm_psd_contract.  Fixed summation order. */
+int gm_s12_contract_rows(gm_ctx* ctx, int64_t n, const int64_t* coef_off, const int32_t* nmax,
+                         const double* an, const double* bn, int n_ang, const double* pi_t,
+                         const double* tau_t, int nmax_cap, int n_rows, const int64_t* start_host,
+                         const int64_t* count_host, const double* w, double* out,
+                         gm_stream stream);
+
 /* integratePSD scalars (src/geosmie/dointegration.py:1064-1209, mixing
  * 'dumix', including its Qsca weighting of g, csca, cext from the aliased
  * thisweight, :1111-1112 and :1154-1157, and the qb back-out, :1133-1169).

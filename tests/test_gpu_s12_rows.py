@@ -1,0 +1,112 @@
+"""gm_s12_contract_rows (S1/S2 -> Mueller elements -> weighted row sums in one
+pass) against the unfused pair gm_s12_mueller + gm_psd_contract_rows on the
+same coefficients, and against a numpy contraction of the Mueller planes.
+Rows cover ragged ranges (single points, the last point, groups of 1, 2, 4
+and 5 rows over the same range, so every accumulator width and the split of a
+long group run), with an angle count that is not a multiple of 64."""
+import ctypes
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _batch(n, seed):
+    from geosmie_amd.lut import _DevMie
+    from geosmie_amd.s12 import angle_tables_dev
+    rng = np.random.default_rng(seed)
+    x = np.sort(10 ** rng.uniform(-2, 2.2, n))
+    dm = _DevMie(x, complex(1.53, 0.008))
+    ang = np.linspace(0, 180, 181)
+    cap = int(np.rint(2 + x.max() + 4 * x.max() ** (1. / 3.)))
+    pi_t, tau_t = angle_tables_dev(np.cos(np.radians(ang)), cap)
+    return dm, ang, cap, pi_t, tau_t, rng
+
+
+def _fused(dm, ang, cap, pi_t, tau_t, starts, counts, W):
+    from geosmie_amd import _lib
+    from geosmie_amd._lib import check, ptr, stream
+    t = _lib.torch()
+    k = len(starts)
+    out = t.empty(k * 4 * ang.size, dtype=t.float64, device=_lib.device())
+    st = (ctypes.c_int64 * k)(*[int(a) for a in starts])
+    ct = (ctypes.c_int64 * k)(*[int(c) for c in counts])
+    check(_lib.lib().gm_s12_contract_rows(_lib.ctx(), dm.n, ptr(dm.d_off), ptr(dm.d_nmax),
+                                          ptr(dm.d_an), ptr(dm.d_bn), ang.size, ptr(pi_t),
+                                          ptr(tau_t), cap, k, st, ct, ptr(_lib.to_dev(W)),
+                                          ptr(out), stream()), "gm_s12_contract_rows")
+    return out.view(k, 4, ang.size).cpu().numpy()
+
+
+@pytest.mark.parametrize("layout", ["mixed", "single", "pairs"])
+def test_fused_rows_match_unfused(gpu, layout):
+    from geosmie_amd import _lib
+    from geosmie_amd._lib import check, ptr, stream
+    t = _lib.torch()
+    n = 3000
+    dm, ang, cap, pi_t, tau_t, rng = _batch(n, 7)
+    if layout == "mixed":
+        ranges = [(0, 700), (700, 1), (700, 1), (701, 799), (701, 799), (701, 799), (701, 799),
+                  (1500, 1499), (1500, 1499), (1500, 1499), (1500, 1499), (1500, 1499),
+                  (2999, 1)]
+    elif layout == "single":
+        ranges = [(0, 3000), (5, 64), (69, 65), (1000, 1), (2999, 1), (100, 2899)]
+    else:
+        ranges = [(0, 1500), (0, 1500), (1500, 1500), (1500, 1500), (17, 9), (17, 9)]
+    starts = np.array([a for a, _ in ranges])
+    counts = np.array([c for _, c in ranges])
+    W = rng.uniform(0, 1, (len(ranges), n))
+    got = _fused(dm, ang, cap, pi_t, tau_t, starts, counts, W)
+    mu = dm.mueller(pi_t, tau_t, 0)
+    ref = t.empty(len(ranges) * 4 * ang.size, dtype=t.float64, device=_lib.device())
+    st = (ctypes.c_int64 * len(ranges))(*starts.tolist())
+    ct = (ctypes.c_int64 * len(ranges))(*counts.tolist())
+    check(_lib.lib().gm_psd_contract_rows(_lib.ctx(), len(ranges), st, ct, n, 4 * ang.size,
+                                          ptr(_lib.to_dev(W)), ptr(mu), ptr(ref), stream()))
+    ref = ref.view(len(ranges), 4, ang.size).cpu().numpy()
+    M = mu.view(n, 4 * ang.size).cpu().numpy()
+    for r, (a, c) in enumerate(ranges):
+        scale = np.abs(ref[r]).max()
+        assert np.abs(got[r] - ref[r]).max() <= 1e-12 * scale, (layout, r)
+        want = (W[r, a:a + c] @ M[a:a + c]).reshape(4, ang.size)
+        assert np.abs(got[r] - want).max() <= 1e-12 * scale, (layout, r)
+
+
+def test_fused_rows_rejects_bad_ranges(gpu):
+    dm, ang, cap, pi_t, tau_t, rng = _batch(200, 3)
+    W = np.ones((1, 200))
+    with pytest.raises(ValueError):
+        _fused(dm, ang, cap, pi_t, tau_t, [150], [51], W)
+    with pytest.raises(ValueError):
+        _fused(dm, ang, cap, pi_t, tau_t, [0], [0], W)
+    W65 = np.ones((65, 200))
+    with pytest.raises(ValueError):
+        _fused(dm, ang, cap, pi_t, tau_t, [0] * 65, [10] * 65, W65)
+
+
+def test_chain_fused_equals_unfused(gpu):
+    """The runoptics chain's FP64 step with and without the fused pass: the
+    row integrals, normalised P, pback, scalars and GSF moments agree."""
+    from geosmie_amd.lut import getLogNormPSD
+    from geosmie_amd.pipeline import RunopticsChain
+    t = __import__("torch")
+    r = np.logspace(np.log10(0.02), np.log10(5.0), 500)
+    lam = np.array([0.35, 0.6, 1.0, 2.2])
+    x = (2 * np.pi * r[None, :] / lam[:, None]).ravel()
+    w = np.zeros((4, x.size))
+    for i, L in enumerate(lam):
+        p = getLogNormPSD(0.4, 1.9, 2 * np.pi * r / L, L, 1e9, 0.0) * np.gradient(r)
+        w[i, i * 500:(i + 1) * 500] = p / p.sum()
+    ang = np.linspace(0.0, 180.0, 201)
+    res = []
+    for fused in (True, False):
+        ch = RunopticsChain(x, 1.5 + 0.02j, ang, w, lam, [(i * 500, 500) for i in range(4)],
+                            gsf_ng=64, fused=fused)
+        ch.step()
+        t.cuda.synchronize()
+        res.append({k: getattr(ch, k).cpu().numpy() for k in ("d_pm", "d_back", "d_sc", "d_pmom")})
+    assert res[0]["d_pm"].size == 4 * 4 * ang.size
+    for k in res[0]:
+        a, b = res[0][k], res[1][k]
+        assert np.abs(a - b).max() <= 1e-12 * np.abs(b).max(), k
