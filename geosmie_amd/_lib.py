@@ -58,6 +58,8 @@ SIGNATURES = {
     "gm_s12_contract_rows": (ctypes.c_int, [c_vp, c_i64, c_vp, c_vp, c_vp, c_vp, ctypes.c_int,
                                             c_vp, c_vp, ctypes.c_int, ctypes.c_int, c_vp, c_vp,
                                             c_vp, c_vp, c_vp]),
+    "gm_lognorm_rows": (ctypes.c_int, [c_vp, c_i64, ctypes.c_int, c_vp, c_vp, c_vp, c_vp, c_vp,
+                                       c_i64, c_vp]),
     "gm_integrate_psd_rows": (ctypes.c_int, [c_vp, c_i64, c_i64, c_vp, c_vp, c_vp, c_vp, c_vp,
                                              c_dbl, c_dbl, c_vp, c_vp, c_vp]),
     "gm_band_average": (ctypes.c_int, [c_vp, c_i64, ctypes.c_int, c_vp, c_vp, ctypes.c_int,

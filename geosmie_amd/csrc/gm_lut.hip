@@ -152,6 +152,46 @@ __global__ __launch_bounds__(256) void k_lut_lognormal(int n_r, const double* __
   for (int p = threadIdx.x; p < n_r; p += 256) wk[p] *= inv;
 }
 
+// Truncated lognormal rows for runoptics: out[k][col0[k] + p] = n(r_p) dr_p /
+// sum, with n zero where r_p >= rmax[k] or r_p <= rmin[k] (getLogNormPSD,
+// particleparams.py:113-127, times the widths and normalised as calculatePSD
+// does, dointegration.py:569-577).  r may be size parameters (then rmode,
+// rmin, rmax are multiplied by 2 pi / lambda).  One block per row; the sum is
+// a fixed-order tree.
+__global__ __launch_bounds__(256) void k_lognorm_rows(int n_r, const double* __restrict__ r,
+                                                      const double* __restrict__ dr,
+                                                      const double* __restrict__ par,
+                                                      const int64_t* __restrict__ col0,
+                                                      double* __restrict__ w, int64_t ldw) {
+  __shared__ double red[256];
+  const int64_t k = blockIdx.x;
+  // par[k] = (rmode, ln sigma, rmin, rmax)
+  const double rm = par[4 * k], ls = par[4 * k + 1], lo = par[4 * k + 2], hi = par[4 * k + 3];
+  const int64_t c0 = col0[k];
+  if (c0 < 0 || c0 > ldw - n_r) return;   // row left untouched (uniform over the block)
+  const double c = 1.0 / (sqrt(2.0 * 3.14159265358979323846) * ls);
+  double* wk = w + k * ldw + c0;
+  double s = 0.0;
+  for (int p = threadIdx.x; p < n_r; p += 256) {
+    const double rp = r[p];
+    double v = 0.0;
+    if (rp < hi && rp > lo) {
+      const double t = log(rp / rm);
+      v = c / rp * exp(-(t * t) / (2.0 * ls * ls)) * dr[p];
+    }
+    wk[p] = v;
+    s += v;
+  }
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if (threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+    __syncthreads();
+  }
+  const double tot = red[0];
+  for (int p = threadIdx.x; p < n_r; p += 256) wk[p] = wk[p] / tot;
+}
+
 }  // namespace gm
 
 extern "C" {
@@ -168,6 +208,23 @@ int gm_lut_lognormal(gm_ctx* ctx, int64_t n_psd, int n_r, const double* r, const
                      n_r, r, dr, rmode, sigma, w);
   e = hipGetLastError();
   if (e != hipSuccess) return gm_fail(GM_ERUNTIME, "k_lut_lognormal: %s", hipGetErrorString(e));
+  return GM_OK;
+}
+
+int gm_lognorm_rows(gm_ctx* ctx, int64_t n_rows, int n_r, const double* r, const double* dr,
+                    const double* par, const int64_t* col0, double* w, int64_t ldw,
+                    gm_stream stream) {
+  if (!ctx || n_rows < 0 || n_r < 1 || ldw < n_r)
+    return gm_fail(GM_EINVAL, "gm_lognorm_rows: bad arguments");
+  if (n_rows == 0) return GM_OK;
+  if (!r || !dr || !par || !col0 || !w) return gm_fail(GM_EINVAL, "gm_lognorm_rows: NULL array");
+  if (n_rows > 0x7fffffffLL) return gm_fail(GM_EINVAL, "gm_lognorm_rows: too many rows");
+  hipError_t e = hipSetDevice(ctx->device);
+  if (e != hipSuccess) return gm_fail(GM_ERUNTIME, "hipSetDevice: %s", hipGetErrorString(e));
+  hipLaunchKernelGGL(gm::k_lognorm_rows, dim3((unsigned)n_rows), dim3(256), 0,
+                     (hipStream_t)stream, n_r, r, dr, par, col0, w, ldw);
+  e = hipGetLastError();
+  if (e != hipSuccess) return gm_fail(GM_ERUNTIME, "k_lognorm_rows: %s", hipGetErrorString(e));
   return GM_OK;
 }
 

@@ -246,6 +246,33 @@ def getHumidRefractiveIndex(params, radind, rhi, rh, nref0, nrefwater):
     return [v.real for v in n], [v.imag for v in n], gf, rrat
 
 
+def lognorm_psd_rows(params, radind, rhs, rh, xxarr, drarr, lam):
+    """calculatePSD's lognormal weights for many rh at once: psd[mode][i] is
+    calculatePSD(params, radind, rhs[i], ...)[0][mode], bit for bit (the same
+    element-wise numpy operations and one row-wise sum per rh), for the table
+    loop, which needs all rh of a wavelength (dointegration.py:549-583)."""
+    pp, rd = params["psd"]["params"], params["rhDep"]
+    if params["psd"]["type"] != "lognorm":
+        raise ValueError("lognorm_psd_rows: psd type must be lognorm")
+    xx = np.asarray(xxarr, dtype=np.float64)
+    k = 2 * np.pi / lam
+    out = []
+    for m, r0 in enumerate(pp["r0"][radind]):
+        rmode = np.array([humidityGrowth(rd, r0, r, rh) for r in rhs], np.float64)
+        rmax = np.array([humidityGrowth(rd, pp["rmax0"][radind][m], r, rh) for r in rhs],
+                        np.float64)
+        rmin = pp["rmin0"][radind][m]
+        ls = np.log(pp["sigma"][radind][m])
+        # getLogNormPSD (particleparams.py:113-127), one row per rh
+        d = 1. / (xx * (2 * np.pi) ** 0.5 * ls) * np.exp(
+            -(np.log(xx / (rmode * k)[:, None]) ** 2) / (2. * ls ** 2))
+        d = np.where(xx >= (rmax * k)[:, None], 0., d)
+        d = np.where(xx <= rmin * k, 0., d)
+        w = d * drarr
+        out.append(w / np.sum(w, axis=1, keepdims=True))
+    return out
+
+
 def calculatePSD(params, radind, onerh, rh, xxarr, drarr, rrat, lam, want_ref=True):
     """Number weights per mode on the size grid, mass-weighted effective radius
     per mode, and the bin edges (rLow, rUp) (dointegration.py:539-622).

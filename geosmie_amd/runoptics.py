@@ -44,14 +44,17 @@ def _interp(xp, fp):
 class _WavelengthPass(object):
     """Device pass over every rh of one (bin, wavelength)."""
 
-    def __init__(self, xx, pi_t, tau_t, n_ang, precision):
-        self.xx, self.pi_t, self.tau_t = xx, pi_t, tau_t
+    def __init__(self, xx, pi_t, tau_t, n_ang, precision, dr=None):
+        self.xx, self.pi_t, self.tau_t, self.dr = xx, pi_t, tau_t, dr
         self.n_ang, self.precision = n_ang, precision
 
     def launch(self, lam, m_blocks, rows, rhop0):
         """Enqueue the pass; returns a handle for collect().  m_blocks:
         complex index per point block (each block = the whole grid); rows:
-        list of (block, psd [N_x], reff0, rhop).  Every host-to-device copy
+        list of (block, psd [N_x], reff0, rhop, spec), spec = (mode radius,
+        ln sigma, lower and upper cut-off, in size-parameter units) of a
+        lognormal row whose weights are formed on the device (psd None), or
+        None when psd holds the weights.  Every host-to-device copy
         comes first and nothing here waits for a kernel, so the caller can
         prepare the next wavelength while these kernels run."""
         t = _lib.torch()
@@ -63,7 +66,11 @@ class _WavelengthPass(object):
         # host -> device first (a pageable copy waits for the stream's work)
         eps = np.asarray(m_blocks, np.complex128) ** 2
         d_eps = _lib.to_dev(eps.view(np.float64).reshape(nb, 2))
-        d_psd = _lib.to_dev(np.stack([r[1] for r in rows]))
+        on_dev = rows[0][4] is not None     # lognormal rows: weights formed on the device
+        if on_dev:
+            d_psd = _lib.to_dev(np.array([r[4] for r in rows], np.float64))
+        else:
+            d_psd = _lib.to_dev(np.stack([r[1] for r in rows]))
         d_blk = _lib.to_dev(np.array([r[0] for r in rows], np.int64), dtype=np.int64)
         d_r0 = _lib.to_dev(np.array([r[2] for r in rows], np.float64))
         d_rho = _lib.to_dev(np.array([r[3] for r in rows], np.float64))
@@ -74,13 +81,21 @@ class _WavelengthPass(object):
             d_off = t.empty(n + 1, dtype=t.int64, device=dev)
             check(L.gm_coef_offsets(h, n, ptr(d_x), ptr(d_off), s), "gm_coef_offsets")
             self._grid = (nb, d_x, d_off, int(d_off[-1].item()))
+            if self.dr is not None:      # one grid and its widths, for gm_lognorm_rows
+                self._d_grid1 = (_lib.to_dev(self.xx), _lib.to_dev(self.dr))
         _, d_x, d_off, total = self._grid
         d_eps = d_eps.repeat_interleave(nx, dim=0).reshape(-1)
         dm = _DevMie(d_x, None, d_eps=d_eps, x_max=float(self.xx.max()), offsets=(d_off, total))
-        # dense [rows][points] weights, zero outside each row's block, assembled
-        # on the device from the compact [rows][grid] PSDs
+        # dense [rows][points] weights, zero outside each row's block: lognormal
+        # rows formed on the device, others scattered from the host PSDs
         d_w = t.zeros((nr, nb, nx), dtype=t.float64, device=dev)
-        d_w[t.arange(nr, device=dev), d_blk] = d_psd
+        d_col = None
+        if on_dev:
+            d_col = d_blk * nx
+            check(L.gm_lognorm_rows(h, nr, nx, ptr(self._d_grid1[0]), ptr(self._d_grid1[1]),
+                                    ptr(d_psd), ptr(d_col), ptr(d_w), n, s), "gm_lognorm_rows")
+        else:
+            d_w[t.arange(nr, device=dev), d_blk] = d_psd
         d_w = d_w.view(nr, n)
         n_cols = 4 * self.n_ang
         d_pm = t.empty(nr * n_cols, dtype=t.float64, device=dev)
@@ -109,7 +124,7 @@ class _WavelengthPass(object):
                                       ptr(d_r0), float(rhop0), 0.0, ptr(d_rho), ptr(d_sc), s),
               "gm_integrate_psd_rows")
         # keep every buffer the enqueued kernels use alive until collect()
-        return (nr, d_pm, d_sc, (dm, d_mu, d_w, d_lp, d_r0, d_rho, d_psd, d_blk))
+        return (nr, d_pm, d_sc, (dm, d_mu, d_w, d_lp, d_r0, d_rho, d_psd, d_blk, d_col))
 
     def collect(self, handle):
         """The un-normalised phase-matrix integrals [rows][4][n_ang] and the
@@ -173,7 +188,7 @@ def build_table(params, water=None, classic=False, precision=0, ang=None, n_mom=
         xx, dr = particle.initializeXarr(params, radind, lams[0], lams[-1])
         nmax_cap = int(np.rint(2 + xx.max() + 4 * xx.max() ** (1. / 3.)))
         pi_t, tau_t = angle_tables_dev(cost, nmax_cap)
-        dev_pass = _WavelengthPass(xx, pi_t, tau_t, ang.size, precision)
+        dev_pass = _WavelengthPass(xx, pi_t, tau_t, ang.size, precision, dr)
         fracs = pp["fracs"][radind]
         water = (w_mr, w_mi) if rhdep != "trivial" else None
 
@@ -203,11 +218,26 @@ def _prepare(params, radind, lam, xx, dr, fracs, rhop0, part_mr, part_mi, water)
     _, _, _, rrat0 = particle.getHumidRefractiveIndex(params, radind, 0, rh, nref0, nw)
     _, reff0, _, _ = particle.calculatePSD(params, radind, 0., rh, xx, dr, rrat0, lam)
     todo = [0] if trivial else list(range(len(rh)))
+    lognorm = params["psd"]["type"] == "lognorm"
+    pp, rd = params["psd"]["params"], params["rhDep"]
+    kx = 2 * np.pi / lam
     blocks, rows, meta = [], [], []
     for rhi in todo:
         mr, mi, gf, rrat = particle.getHumidRefractiveIndex(params, radind, rhi, rh, nref0, nw)
-        psd, _, rlo, rup = particle.calculatePSD(params, radind, rh[rhi], rh, xx, dr, rrat, lam,
-                                                 want_ref=False)
+        if lognorm:
+            # the weights are formed on the device (gm_lognorm_rows): per mode the
+            # grown mode radius and upper cut-off, in size-parameter units, as
+            # calculatePSD passes them to getLogNormPSD (dointegration.py:551-576)
+            spec = [(particle.humidityGrowth(rd, r0, rh[rhi], rh) * kx,
+                     np.log(pp["sigma"][radind][k]), pp["rmin0"][radind][k] * kx,
+                     particle.humidityGrowth(rd, pp["rmax0"][radind][k], rh[rhi], rh) * kx)
+                    for k, r0 in enumerate(pp["r0"][radind])]
+            psd = [None] * len(spec)
+            rlo, rup = pp["rmin0"][radind][0], pp["rmax0"][radind][0]
+        else:
+            psd, _, rlo, rup = particle.calculatePSD(params, radind, rh[rhi], rh, xx, dr, rrat,
+                                                     lam, want_ref=False)
+            spec = [None] * len(psd)
         rhop = rrat ** 3. * rhop0 + (1. - rrat ** 3.) * RHOW
         b0 = len(blocks)
         blocks.extend(complex(a, b) for a, b in zip(mr, mi))
@@ -215,7 +245,7 @@ def _prepare(params, radind, lam, xx, dr, fracs, rhop0, part_mr, part_mi, water)
         mode_rows = []
         for k in range(len(fracs)):
             mode_rows.append(len(rows))
-            rows.append((b0 + (k if n_ri > 1 else 0), psd[k], reff0[k], rhop))
+            rows.append((b0 + (k if n_ri > 1 else 0), psd[k], reff0[k], rhop, spec[k]))
         meta.append((rhi, mr, mi, gf, rhop, rlo, rup, mode_rows))
     return blocks, rows, meta, trivial, len(rh)
 

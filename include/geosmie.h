@@ -287,6 +287,21 @@ int gm_lut_points(gm_ctx* ctx, int64_t n_slab, int n_r, const double* r_dry, con
                   const double* m_dry, const double* gf, double mw_re, double mw_im, double* x,
                   double* eps, gm_stream stream);
 
+/* Truncated lognormal weight rows, as runoptics' calculatePSD builds one
+ * mode (getLogNormPSD, src/geosmie/particleparams.py:113-127, times the
+ * widths dr, normalised to sum 1: src/geosmie/dointegration.py:569-577):
+ * w[k][col0[k] + p] = n(r_p) dr_p / sum_p n(r_p) dr_p for p < n_r, with
+ * n(r) = exp(-ln^2(r / rmode) / (2 ln^2 sigma)) / (r sqrt(2 pi) ln sigma),
+ * zero where r_p >= rmax or r_p <= rmin.  par [n_rows][4] = (rmode,
+ * ln sigma, rmin, rmax) in the units of r (size parameters: radii times
+ * 2 pi / lambda); col0 [n_rows] column offsets.  All arrays on the device;
+ * w has n_rows rows of ldw >= n_r entries, of which each row's n_r from
+ * col0 are written (a row whose offset does not fit is left as it is).
+ * One pass per table wavelength replaces the host PSDs of every rh. */
+int gm_lognorm_rows(gm_ctx* ctx, int64_t n_rows, int n_r, const double* r, const double* dr,
+                    const double* par, const int64_t* col0, double* w, int64_t ldw,
+                    gm_stream stream);
+
 /* Lognormal number weights on a radius grid, one row per distribution:
  * w[k][p] = n(r_p; rmode[k], sigma[k]) dr[p] / sum_p(...) with the density of
  * src/geosmie/particleparams.py:113-127 and the bin widths dr (getDR,

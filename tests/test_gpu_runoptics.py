@@ -112,3 +112,43 @@ def test_table_with_gsf_moments(gpu, tmp_path, classic):
     assert v["pmom"][1] == (("nPol", "nMom", "radius", "rh", "lambda") if classic
                             else ("bin", "wavelength", "rh", "p", "m"))
     np.testing.assert_array_equal(v["pmom"][0], d["pmom"])
+
+
+def test_lognorm_rows_match_calculate_psd(gpu):
+    """gm_lognorm_rows (the table loop's device PSDs) against the front end's
+    calculatePSD for every rh and mode of a bimodal lognormal bin with CARMA
+    growth (grown mode radius and upper cut-off), written at block offsets of
+    a wider weight matrix whose other entries stay untouched."""
+    from geosmie_amd import _lib, particle
+    from geosmie_amd._lib import check, ptr, stream
+    t = _lib.torch()
+    p = params("bimodal_trivial")
+    p["rhDep"] = {"type": "su", "params": {"temp": 230.0}}
+    rh = list(p["rh"])
+    pp = p["psd"]["params"]
+    lam = 0.55e-6
+    kx = 2 * np.pi / lam
+    xx, dr = particle.initializeXarr(p, 0, LAM[0], LAM[-1])
+    spec, want = [], []
+    for rhi, r in enumerate(rh):
+        psd, _, _, _ = particle.calculatePSD(p, 0, r, rh, xx, dr, 1.0, lam, want_ref=False)
+        for k, r0 in enumerate(pp["r0"][0]):
+            spec.append((particle.humidityGrowth(p["rhDep"], r0, r, rh) * kx,
+                         np.log(pp["sigma"][0][k]), pp["rmin0"][0][k] * kx,
+                         particle.humidityGrowth(p["rhDep"], pp["rmax0"][0][k], r, rh) * kx))
+            want.append(psd[k])
+    nr, nx = len(spec), xx.size
+    ldw = nx * 3 + 5
+    col0 = np.array([(i % 3) * nx + (5 if i % 2 else 0) for i in range(nr)], np.int64)
+    d_w = t.full((nr, ldw), -7.0, dtype=t.float64, device=_lib.device())
+    # keep every input tensor referenced until the kernel has run
+    d_x, d_dr = _lib.to_dev(xx), _lib.to_dev(dr)
+    d_par, d_col = _lib.to_dev(np.array(spec)), _lib.to_dev(col0, dtype=np.int64)
+    check(_lib.lib().gm_lognorm_rows(_lib.ctx(), nr, nx, ptr(d_x), ptr(d_dr), ptr(d_par),
+                                     ptr(d_col), ptr(d_w), ldw, stream()), "gm_lognorm_rows")
+    W = d_w.cpu().numpy()
+    for i in range(nr):
+        got = W[i, col0[i]:col0[i] + nx]
+        np.testing.assert_allclose(got, want[i], rtol=1e-13, atol=1e-16 * want[i].max())
+        rest = np.delete(W[i], np.arange(col0[i], col0[i] + nx))
+        assert np.all(rest == -7.0)

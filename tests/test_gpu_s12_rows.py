@@ -32,10 +32,11 @@ def _fused(dm, ang, cap, pi_t, tau_t, starts, counts, W):
     out = t.empty(k * 4 * ang.size, dtype=t.float64, device=_lib.device())
     st = (ctypes.c_int64 * k)(*[int(a) for a in starts])
     ct = (ctypes.c_int64 * k)(*[int(c) for c in counts])
+    d_w = _lib.to_dev(W)
     check(_lib.lib().gm_s12_contract_rows(_lib.ctx(), dm.n, ptr(dm.d_off), ptr(dm.d_nmax),
                                           ptr(dm.d_an), ptr(dm.d_bn), ang.size, ptr(pi_t),
-                                          ptr(tau_t), cap, k, st, ct, ptr(_lib.to_dev(W)),
-                                          ptr(out), stream()), "gm_s12_contract_rows")
+                                          ptr(tau_t), cap, k, st, ct, ptr(d_w), ptr(out),
+                                          stream()), "gm_s12_contract_rows")
     return out.view(k, 4, ang.size).cpu().numpy()
 
 
@@ -62,8 +63,9 @@ def test_fused_rows_match_unfused(gpu, layout):
     ref = t.empty(len(ranges) * 4 * ang.size, dtype=t.float64, device=_lib.device())
     st = (ctypes.c_int64 * len(ranges))(*starts.tolist())
     ct = (ctypes.c_int64 * len(ranges))(*counts.tolist())
+    d_w = _lib.to_dev(W)
     check(_lib.lib().gm_psd_contract_rows(_lib.ctx(), len(ranges), st, ct, n, 4 * ang.size,
-                                          ptr(_lib.to_dev(W)), ptr(mu), ptr(ref), stream()))
+                                          ptr(d_w), ptr(mu), ptr(ref), stream()))
     ref = ref.view(len(ranges), 4, ang.size).cpu().numpy()
     M = mu.view(n, 4 * ang.size).cpu().numpy()
     for r, (a, c) in enumerate(ranges):
