@@ -291,14 +291,18 @@ int gm_mie_batch(gm_ctx* ctx, int64_t n, const double* x, const double* y, const
   int32_t* max_nmax = status + 4;
   int64_t* order = (int64_t*)ctx->order.ptr;
   unsigned gblk = (unsigned)((n + 255) / 256);
-  const unsigned pblk = (unsigned)((n + 256 * gm::kPlanPer - 1) / (256 * gm::kPlanPer));
+  // points per thread of the plan kernels: 1 up to ~256k points (blocks stay
+  // plentiful), growing to kPlanPer for the largest batches (fewer LDS
+  // histogram flushes)
+  const int per = (int)std::min<int64_t>(gm::kPlanPer, std::max<int64_t>(1, n / (256 * 1024)));
+  const unsigned pblk = (unsigned)((n + 256 * (int64_t)per - 1) / (256 * (int64_t)per));
   if (flags & GM_MIE_SORT) {
     hipLaunchKernelGGL(gm::k_zero_i64, dim3((nb + 255) / 256), dim3(256), 0, s, hist, nb);
     hipLaunchKernelGGL(gm::k_plan_keys, dim3(pblk), dim3(256), 0, s, n, x, y, eps, mu, eps2,
-                       (int32_t*)ctx->key.ptr, nmax_out, hist, max_nmax, status);
+                       (int32_t*)ctx->key.ptr, nmax_out, hist, max_nmax, status, per);
     hipLaunchKernelGGL(gm::k_plan_scan, dim3(1), dim3(256), 0, s, hist, cursor, range);
     hipLaunchKernelGGL(gm::k_plan_scatter, dim3(pblk), dim3(256), 0, s, n,
-                       (const int32_t*)ctx->key.ptr, cursor, order);
+                       (const int32_t*)ctx->key.ptr, cursor, order, per);
   } else {
     // input order: each class kernel walks every point and keeps its own
     hipLaunchKernelGGL(gm::k_plan_identity, dim3(gblk), dim3(256), 0, s, n, order, range);

@@ -123,22 +123,24 @@ __global__ void k_zero_i64(int64_t* __restrict__ p, int n) {
 }
 
 // Block-local LDS histogram, then one global atomic per non-empty bucket.
-// Each block covers kPlanPer * 256 points so the histogram's zeroing and flush
-// are amortised.  status bit 2: a coated point with mu != 1 (mie_coeffs.py:57-59).
+// Each block covers per * 256 points (per <= kPlanPer, chosen from n: large
+// batches amortise the histogram's zeroing and flush, small ones keep enough
+// blocks in flight).  status bit 2: a coated point with mu != 1 (mie_coeffs.py:57-59).
 __global__ __launch_bounds__(256) void k_plan_keys(
     int64_t n, const double* __restrict__ x, const double* __restrict__ y,
     const double* __restrict__ eps, const double* __restrict__ mu,
     const double* __restrict__ eps2, int32_t* __restrict__ key, int32_t* __restrict__ nmax_out,
-    int64_t* __restrict__ hist, int32_t* __restrict__ max_nmax, int32_t* __restrict__ status) {
+    int64_t* __restrict__ hist, int32_t* __restrict__ max_nmax, int32_t* __restrict__ status,
+    int per) {
   __shared__ int32_t lh[kClasses * kBuckets];
   __shared__ int32_t lmax;
   for (int i = threadIdx.x; i < kClasses * kBuckets; i += blockDim.x) lh[i] = 0;
   if (threadIdx.x == 0) lmax = 0;
   __syncthreads();
-  const int64_t p0 = (int64_t)blockIdx.x * (256 * kPlanPer);
+  const int64_t p0 = (int64_t)blockIdx.x * (256 * (int64_t)per);
   int tmax = 0;
   bool bad_mu = false;
-  for (int q = 0; q < kPlanPer; ++q) {
+  for (int q = 0; q < per; ++q) {
     const int64_t p = p0 + q * 256 + threadIdx.x;
     if (p >= n) break;
     PointDesc d = describe(p, x, y, eps, mu, eps2);
@@ -162,57 +164,63 @@ __global__ __launch_bounds__(256) void k_plan_keys(
 
 // Exclusive scan of the kClasses*kBuckets histogram (one block of 256).
 // cursor[k] = start of bucket k; range[2c], range[2c+1] = [begin, end) of class c.
+// Each thread sums its run of buckets, the 256 run sums are scanned in LDS
+// (Hillis-Steele, 8 steps), then each thread writes its run's cursors and the
+// class boundaries that fall in it.
 __global__ __launch_bounds__(256) void k_plan_scan(const int64_t* __restrict__ hist,
                                                    int64_t* __restrict__ cursor,
                                                    int64_t* __restrict__ range) {
   __shared__ int64_t part[256];
-  const int per = (kClasses * kBuckets) / 256;
-  int t = threadIdx.x;
+  constexpr int per = (kClasses * kBuckets) / 256;
+  static_assert(per * 256 == kClasses * kBuckets, "histogram must split into 256 runs");
+  const int t = threadIdx.x;
+  int64_t h[per];
   int64_t s = 0;
-  for (int i = 0; i < per; ++i) s += hist[t * per + i];
+#pragma unroll
+  for (int i = 0; i < per; ++i) {
+    h[i] = hist[t * per + i];
+    s += h[i];
+  }
   part[t] = s;
   __syncthreads();
-  if (t == 0) {
-    int64_t acc = 0;
-    for (int i = 0; i < 256; ++i) {
-      int64_t v = part[i];
-      part[i] = acc;
-      acc += v;
-    }
+  for (int off = 1; off < 256; off <<= 1) {
+    const int64_t v = t >= off ? part[t - off] : 0;
+    __syncthreads();
+    part[t] += v;
+    __syncthreads();
   }
-  __syncthreads();
-  int64_t acc = part[t];
+  int64_t acc = t == 0 ? 0 : part[t - 1];
+#pragma unroll
   for (int i = 0; i < per; ++i) {
-    cursor[t * per + i] = acc;
-    acc += hist[t * per + i];
-  }
-  if (t == 0) {
-    int64_t acc2 = 0;
-    for (int c = 0; c < kClasses; ++c) {
-      range[2 * c] = acc2;
-      for (int i = c * kBuckets; i < (c + 1) * kBuckets; ++i) acc2 += hist[i];
-      range[2 * c + 1] = acc2;
+    const int j = t * per + i;
+    cursor[j] = acc;
+    if (j % kBuckets == 0) {
+      const int c = j / kBuckets;
+      range[2 * c] = acc;
+      if (c > 0) range[2 * c - 1] = acc;
     }
+    acc += h[i];
   }
+  if (t == 255) range[2 * kClasses - 1] = acc;
 }
 
 // Scatter with block-local ranks: one global atomic per (block, bucket),
-// kPlanPer * 256 points per block.
+// per * 256 points per block (per <= kPlanPer).
 __global__ __launch_bounds__(256) void k_plan_scatter(int64_t n, const int32_t* __restrict__ key,
                                                       int64_t* __restrict__ cursor,
-                                                      int64_t* __restrict__ order) {
+                                                      int64_t* __restrict__ order, int per) {
   __shared__ int32_t lcnt[kClasses * kBuckets];
   __shared__ int64_t lbase[kClasses * kBuckets];
   for (int i = threadIdx.x; i < kClasses * kBuckets; i += blockDim.x) lcnt[i] = 0;
   __syncthreads();
-  const int64_t p0 = (int64_t)blockIdx.x * (256 * kPlanPer);
+  const int64_t p0 = (int64_t)blockIdx.x * (256 * (int64_t)per);
   int kk[kPlanPer], rank[kPlanPer];
 #pragma unroll
   for (int q = 0; q < kPlanPer; ++q) {
     const int64_t p = p0 + q * 256 + threadIdx.x;
     kk[q] = -1;
     rank[q] = 0;
-    if (p < n) {
+    if (q < per && p < n) {
       kk[q] = key[p];
       rank[q] = atomicAdd(&lcnt[kk[q]], 1);
     }
