@@ -288,7 +288,6 @@ int gm_mie_batch(gm_ctx* ctx, int64_t n, const double* x, const double* y, const
   int64_t* cursor = hist + nb;
   int64_t* range = cursor + nb;
   int32_t* status = (int32_t*)ctx->status.ptr;
-  int32_t* max_nmax = status + 4;
   int64_t* order = (int64_t*)ctx->order.ptr;
   unsigned gblk = (unsigned)((n + 255) / 256);
   // points per thread of the plan kernels: 1 up to ~256k points (blocks stay
@@ -299,7 +298,7 @@ int gm_mie_batch(gm_ctx* ctx, int64_t n, const double* x, const double* y, const
   if (flags & GM_MIE_SORT) {
     hipLaunchKernelGGL(gm::k_zero_i64, dim3((nb + 255) / 256), dim3(256), 0, s, hist, nb);
     hipLaunchKernelGGL(gm::k_plan_keys, dim3(pblk), dim3(256), 0, s, n, x, y, eps, mu, eps2,
-                       (int32_t*)ctx->key.ptr, nmax_out, hist, max_nmax, status, per);
+                       (int32_t*)ctx->key.ptr, nmax_out, hist, status, per);
     hipLaunchKernelGGL(gm::k_plan_scan, dim3(1), dim3(256), 0, s, hist, cursor, range);
     hipLaunchKernelGGL(gm::k_plan_scatter, dim3(pblk), dim3(256), 0, s, n,
                        (const int32_t*)ctx->key.ptr, cursor, order, per);

@@ -31,7 +31,7 @@ struct gm_ctx {
   int tune_mlp_wterms = 1;      // BF16 terms per MLP weight (1 = BF16 weights, 2 = hi + lo)
   int tune_mie_waves = 0;       // Mie register variant: 0 unconstrained, 3 or 4 waves per SIMD
   int tune_s12_layout = 0;      // FP64 S1/S2 block: waves split angles (0) or points (1)
-  Buf status;                                    // int32 flags + int32 max_nmax + scan scratch
+  Buf status;                                    // int32 flags + scan scratch
   Buf order, key, hist, scratch, scan_tmp;
   Buf tab_pi, tab_tau, gemm_tmp, feat_tmp, gsf_tab, tab_lp, lut_tmp, mlp_ws;
   // cached GSF tables: key = (ng, n_ang, checksum of the angle grid)

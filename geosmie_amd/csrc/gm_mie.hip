@@ -100,18 +100,24 @@ __device__ __forceinline__ PointDesc describe(int64_t p, const double* __restric
   return d;
 }
 
-// Work estimate used as the sort key (number of complex recurrence steps).
-__device__ __forceinline__ int point_cost(const PointDesc& d, int* nmax_out) {
-  int nmax = nmax_rule(d.size);
-  *nmax_out = nmax;
+// Work estimate used as the sort key (number of complex recurrence steps):
+// the n_max and nmx rules on the moduli alone (|sqrt(e)| = sqrt|e|), with
+// cbrt for pow(s, 1/3) -- no complex root or FP64 pow; only the bucket
+// (cost >> kBucketShift) matters.
+__device__ __forceinline__ int point_cost(const PointDesc& d) {
+  const double s = d.size;
+  const double nmax = rint(2.0 + s + 4.0 * cbrt(s));
+  double zabs;
   if (d.cls == 0) {
-    cd z = csqrt_(d.eps * d.mu) * d.size;
-    return nmx_rule(nmax, cabs_(z)) + 2 * nmax;
+    const cd em = d.eps * d.mu;
+    zabs = sqrt(hypot(em.re, em.im)) * s;
+  } else {
+    zabs = fmax(sqrt(hypot(d.eps.re, d.eps.im)), sqrt(hypot(d.eps2.re, d.eps2.im))) * s;
   }
-  cd m1 = csqrt_(d.eps), m2 = csqrt_(d.eps2);
-  double mx = fmax(cabs_(m1) * d.size, cabs_(m2) * d.size);
-  if (d.cls == 1) return nmx_rule(nmax, mx) + 4 * nmax;
-  return 3 * nmx_rule(nmax, mx) + 6 * nmax;
+  const double nmx = rint(fmax(nmax, zabs) + 16.0);
+  const double c = d.cls == 0 ? nmx + 2.0 * nmax
+                              : (d.cls == 1 ? nmx + 4.0 * nmax : 3.0 * nmx + 6.0 * nmax);
+  return (int)fmin(c, 1.0e9);
 }
 
 // ---------------------------------------------------------------------------
@@ -130,36 +136,27 @@ __global__ __launch_bounds__(256) void k_plan_keys(
     int64_t n, const double* __restrict__ x, const double* __restrict__ y,
     const double* __restrict__ eps, const double* __restrict__ mu,
     const double* __restrict__ eps2, int32_t* __restrict__ key, int32_t* __restrict__ nmax_out,
-    int64_t* __restrict__ hist, int32_t* __restrict__ max_nmax, int32_t* __restrict__ status,
-    int per) {
+    int64_t* __restrict__ hist, int32_t* __restrict__ status, int per) {
   __shared__ int32_t lh[kClasses * kBuckets];
-  __shared__ int32_t lmax;
   for (int i = threadIdx.x; i < kClasses * kBuckets; i += blockDim.x) lh[i] = 0;
-  if (threadIdx.x == 0) lmax = 0;
   __syncthreads();
   const int64_t p0 = (int64_t)blockIdx.x * (256 * (int64_t)per);
-  int tmax = 0;
   bool bad_mu = false;
   for (int q = 0; q < per; ++q) {
     const int64_t p = p0 + q * 256 + threadIdx.x;
     if (p >= n) break;
     PointDesc d = describe(p, x, y, eps, mu, eps2);
-    int nmax;
-    int cost = point_cost(d, &nmax);
-    int b = min(cost >> kBucketShift, kBuckets - 1);
+    int b = min(max(point_cost(d), 0) >> kBucketShift, kBuckets - 1);
     int k = d.cls * kBuckets + (kBuckets - 1 - b);  // descending cost inside a class
     key[p] = k;
-    if (nmax_out) nmax_out[p] = nmax;
+    if (nmax_out) nmax_out[p] = nmax_rule(d.size);
     atomicAdd(&lh[k], 1);
-    tmax = max(tmax, nmax);
     bad_mu |= (y && mu && (d.mu.re != 1.0 || d.mu.im != 0.0));
   }
-  atomicMax(&lmax, tmax);
   if (bad_mu) atomicOr(status, 2);
   __syncthreads();
   for (int i = threadIdx.x; i < kClasses * kBuckets; i += blockDim.x)
     if (lh[i]) atomicAdd((unsigned long long*)&hist[i], (unsigned long long)lh[i]);
-  if (threadIdx.x == 0) atomicMax(max_nmax, lmax);
 }
 
 // Exclusive scan of the kClasses*kBuckets histogram (one block of 256).
