@@ -256,13 +256,15 @@ def _wavelength(table, radind, li, fracs, rhop0, dev_pass, ang, prep, handle, n_
     table writes."""
     blocks, rows, meta, trivial, n_rh = prep
     pm, sc = dev_pass.collect(handle)
-    # mix modes in integratePSD's order, then normalise every entry on the device
+    # mix modes in integratePSD's order (ret += frac * thisret, :1199-1200),
+    # all entries of the wavelength at once, then normalise on the device
+    rhis = np.array([m[0] for m in meta], dtype=np.intp)
+    mode_rows = np.array([m[7] for m in meta], dtype=np.intp)     # [entry][mode]
     mixed_p = np.zeros((len(meta), 4, ang.size))
     mixed_s = np.zeros((len(meta), len(SCALAR_OUT)))
-    for e, m in enumerate(meta):
-        for k, r in enumerate(m[7]):
-            mixed_p[e] += fracs[k] * pm[r]
-            mixed_s[e] += fracs[k] * sc[r]
+    for k in range(len(fracs)):
+        mixed_p += fracs[k] * pm[mode_rows[:, k]]
+        mixed_s += fracs[k] * sc[mode_rows[:, k]]
     p_norm, pback = _normalize(mixed_p, ang)
     pmom = None
     if n_mom:
@@ -270,26 +272,24 @@ def _wavelength(table, radind, li, fracs, rhop0, dev_pass, ang, prep, handle, n_
         # table order p11, p12, p33, p34, p22 (= p11), p44 (= p33)
         p6 = p_norm[:, [0, 1, 2, 3, 0, 2], :]
         pmom = pmom_batch(ang, p6, ng=n_mom)
-    mass0 = None
-    for e, (rhi, mr, mi, gf, rhop, rlo, rup, _) in enumerate(meta):
-        s = dict(zip(SCALAR_OUT, mixed_s[e]))
-        ret = {"qext": s["qext"], "qsca": s["qsca"], "qabs": s["qabs"], "qb": s["qb"],
-               "g": s["g"], "csca": s["csca"], "cext": s["cext"], "bsca": s["bsca"],
-               "bext": s["bext"], "bbck": s["bbck"]}
-        ret["lidar_ratio"] = s["qext"] / s["qb"] * 4 * np.pi
-        ret["ssa"] = s["qsca"] / s["qext"]
-        p11, p12, p33, p34 = p_norm[e]
-        ret.update(p11=p11, p12=p12, p22=p11.copy(), p33=p33, p34=p34, p44=p33.copy())
-        ret["pback"] = pback[e]
-        if rhi == 0:
-            mass0 = s["volume"] * rhop0
-        nl = {"mass": s["mass"], "rEff": s["rEff"], "rMass": s["rMass"], "rLow": rlo, "rUp": rup,
-              "growth_factor": gf, "rhop": rhop, "area": s["area"] / mass0,
-              "volume": s["volume"] / mass0}
-        ret["refreal"] = mr[0]
-        ret["refimag"] = -np.abs(mi[0])
-        table.set_entry(radind, li, rhi, ret, pmom=None if pmom is None else pmom[e])
-        table.set_bin_rh(radind, rhi, nl)
+    S = {k: mixed_s[:, i] for i, k in enumerate(SCALAR_OUT)}
+    ret = {k: S[k] for k in ("qext", "qsca", "qabs", "qb", "g", "csca", "cext", "bsca", "bext",
+                             "bbck")}
+    ret["lidar_ratio"] = S["qext"] / S["qb"] * 4 * np.pi          # :957-961
+    ret["ssa"] = S["qsca"] / S["qext"]
+    ret.update(p11=p_norm[:, 0], p12=p_norm[:, 1], p22=p_norm[:, 0], p33=p_norm[:, 2],
+               p34=p_norm[:, 3], p44=p_norm[:, 2], pback=pback)
+    ret["refreal"] = np.array([m[1][0] for m in meta])            # :995-996
+    ret["refimag"] = -np.abs(np.array([m[2][0] for m in meta]))
+    # area and volume per dry mass: mass0 from the first rh (:983-988)
+    mass0 = S["volume"][0] * rhop0
+    nl = {"mass": S["mass"], "rEff": S["rEff"], "rMass": S["rMass"],
+          "rLow": np.array([m[5] for m in meta]), "rUp": np.array([m[6] for m in meta]),
+          "growth_factor": np.array([m[3] for m in meta]),
+          "rhop": np.array([m[4] for m in meta]), "area": S["area"] / mass0,
+          "volume": S["volume"] / mass0}
+    table.set_entries(radind, li, rhis, ret, pmom=pmom)
+    table.set_bins_rh(radind, rhis, nl)
     if trivial:   # copyDryValues (:462-490): every rh repeats the dry entry
         for rhi in range(1, n_rh):
             table.copy_rh(radind, li, 0, rhi)

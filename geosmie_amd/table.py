@@ -131,6 +131,40 @@ class OpticsTable(object):
             else:
                 self.data["pmom"][radind, li, rhi] = pm
 
+    def set_entries(self, radind, li, rhis, ret, pmom=None):
+        """set_entry for several rh of one (bin, wavelength) at once: ret maps
+        each key to an array with a leading axis over rhis (p11.. [k, ang],
+        scalars [k], pback [k, 6]); pmom [k, 6, <= n_mom]."""
+        rhis = np.asarray(rhis, dtype=np.intp)
+        for k in SCAT_KEYS + SCALAR_KEYS:
+            if k in ret:
+                v = np.asarray(ret[k], np.float64)
+                if self.classic:
+                    self.data[k][radind, rhis, li] = v
+                else:
+                    self.data[k][radind, li, rhis] = v
+        if "pback" in ret:
+            pb = np.asarray(ret["pback"], np.float64)
+            if self.classic:
+                self.data["pback"][:, radind, rhis, li] = pb.T
+            else:
+                self.data["pback"][radind, li, rhis, :] = pb
+        if pmom is not None:
+            src = np.asarray(pmom, np.float64)[:, :, :self.n_mom]
+            pm = np.zeros((rhis.size, 6, self.n_mom))
+            pm[:, :, :src.shape[2]] = src
+            if self.classic:
+                self.data["pmom"][:, :, radind, rhis, li] = pm.transpose(1, 2, 0)
+            else:
+                self.data["pmom"][radind, li, rhis] = pm
+
+    def set_bins_rh(self, radind, rhis, ret):
+        """set_bin_rh for several rh at once (ret values are arrays over rhis)."""
+        rhis = np.asarray(rhis, dtype=np.intp)
+        for k in NL_SCALAR_KEYS:
+            if k in ret:
+                self.data[k][radind, rhis] = np.asarray(ret[k], np.float64)
+
     def set_bin_rh(self, radind, rhi, ret):
         """Per-(bin, rh) scalars (mass, volume, area, rEff, rMass, rUp, rLow,
         rhop, growth_factor)."""

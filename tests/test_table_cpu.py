@@ -53,3 +53,32 @@ def test_layout_and_round_trip(tmp_path, classic):
         assert np.array_equal(pmw[:, :4], pm) and np.all(pmw[:, 4] == 0)
         if li == 0:
             assert v["rhop"][0][b, ri] == ret["rhop"]
+
+
+@pytest.mark.parametrize("classic", [False, True])
+def test_set_entries_equals_set_entry(classic):
+    """The all-rh writer of the table loop puts every value where the per-entry
+    writer does, in both layouts, pmom zero-padded."""
+    rng = np.random.default_rng(5)
+    rh = [0.0, 0.3, 0.6, 0.9, 0.99]
+    lam = np.array([0.4e-6, 0.8e-6])
+    ang = np.linspace(0, 180, 7)
+    a = OpticsTable(2, rh, lam, ang, classic=classic, n_mom=9)
+    b = OpticsTable(2, rh, lam, ang, classic=classic, n_mom=9)
+    rhis = [4, 1, 2]
+    ret = {k: rng.standard_normal((3, ang.size)) for k in ("p11", "p12", "p22", "p33", "p34",
+                                                            "p44")}
+    for k in ("qsca", "qabs", "qext", "g", "ssa", "qb", "bsca", "bext", "csca", "cext", "bbck",
+              "lidar_ratio", "refreal", "refimag"):
+        ret[k] = rng.standard_normal(3)
+    ret["pback"] = rng.standard_normal((3, 6))
+    pm = rng.standard_normal((3, 6, 7))
+    nl = {k: rng.standard_normal(3) for k in ("mass", "volume", "area", "rEff", "rMass", "rUp",
+                                              "rLow", "rhop", "growth_factor")}
+    a.set_entries(1, 1, rhis, ret, pmom=pm)
+    a.set_bins_rh(1, rhis, nl)
+    for e, rhi in enumerate(rhis):
+        b.set_entry(1, 1, rhi, {k: v[e] for k, v in ret.items()}, pmom=pm[e])
+        b.set_bin_rh(1, rhi, {k: v[e] for k, v in nl.items()})
+    for k in a.data:
+        np.testing.assert_array_equal(a.data[k], b.data[k], err_msg=k)
