@@ -59,7 +59,9 @@ def coated_workload():
 
 
 def coated_flops(x, y, m1, m2):
-    """Algorithmic FP64 FLOP per coated point: 30 nmx + 270 nmax."""
+    """Algorithmic FP64 FLOP per coated point: 30 nmx + 270 nmax (three D_n
+    chains at 10 FLOP per step; shell combination, coefficients and series
+    sums at 270 FLOP per term of the reference algorithm)."""
     nmax = np.rint(2 + y + 4 * y ** (1.0 / 3.0))
     mx = np.maximum(np.abs(m1) * y, np.abs(m2) * y)
     nmx = np.rint(np.maximum(nmax, mx) + 16)
@@ -432,7 +434,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; --gpus %d is not supported "
-              "(the path shards by independent table slabs, see DESIGN.md)" % args.gpus,
+              "(the path shards by independent table slabs: one process per GPU)" % args.gpus,
               file=sys.stderr)
         sys.exit(2)
 

@@ -20,7 +20,8 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 # sources whose results must match numpy bit for bit: no fused multiply-add
 NO_CONTRACT = {"gm_band.hip"}
 
-# -packed-fp32-ops: no v_pk_*_f32 from the compiler (build notes)
+# -packed-fp32-ops: no packed FP32 instructions from the compiler (some of their
+# forms have given wrong results on these GPUs without an error)
 FLAGS = ["-O3", "-std=c++17", "--offload-arch=gfx950", "-fPIC", "-shared",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
          "-ffp-contract=fast", "-Wno-unused-result", "-I" + INCLUDE, "-I" + CSRC]
