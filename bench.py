@@ -8,7 +8,7 @@ parameter up to 209 -- through gm_mie_batch (plan + length-bucketed coated
 kernel): a_n, b_n and Qext, Qsca, Qabs, Qb, g, qratio per point, FP64.
 Inputs are resident in HBM before the timed region.
 
-Prints one JSON line.  --gpus N > 1 exits: this build runs on one GPU.
+Prints one JSON line.  --gpus N > 1 exits: the benchmark runs on one GPU.
 """
 import argparse
 import ctypes
@@ -433,7 +433,7 @@ def main():
                     help="skip the homogeneous-LUT and large-x secondary measurements")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py: this build runs on one GPU; --gpus %d is not supported "
+        print("bench.py: runs on one GPU; --gpus %d is not supported "
               "(the path shards by independent table slabs: one process per GPU)" % args.gpus,
               file=sys.stderr)
         sys.exit(2)

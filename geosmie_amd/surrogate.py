@@ -264,7 +264,7 @@ class Surrogate(object):
         z = np.load(path, allow_pickle=False)
         ver = int(z["features"]) if "features" in z.files else 1
         if ver not in (2, 3):
-            raise ValueError("%s was trained with feature transform %d, this build reads 2 and "
+            raise ValueError("%s was trained with feature transform %d, this library reads 2 and "
                              "3: retrain with python -m geosmie_amd.train_surrogate" % (path, ver))
         info = dict(zip([str(k) for k in z["info_keys"]], z["info_vals"].tolist()))
         return cls(z["params"], z["feat_mean"], z["feat_std"], z["tgt_mean"], z["tgt_std"],

@@ -2,8 +2,9 @@
 
 Runs oracle/_ref/spher_expan.x (compiled from src/gsf/spher_expan.f by
 oracle/Makefile) on synthetic phase matrices in the 371-angle runoptics grid
-(dointegration.py:739-742), written with np.savetxt exactly as the driver
-does (convertncdf.py:177-189), and stores the '.expan_coeff' rows.
+(dointegration.py:739-742), written with np.savetxt exactly as the
+reference's convertncdf.py does (:177-189), and stores the '.expan_coeff'
+rows.
 
 Inputs: Rayleigh, three Henyey-Greenstein-like matrices, and four Mie
 size-distribution matrices computed by the numpy oracle.
