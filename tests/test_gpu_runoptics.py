@@ -152,3 +152,17 @@ def test_lognorm_rows_match_calculate_psd(gpu):
         np.testing.assert_allclose(got, want[i], rtol=1e-13, atol=1e-16 * want[i].max())
         rest = np.delete(W[i], np.arange(col0[i], col0[i] + nx))
         assert np.all(rest == -7.0)
+
+
+def test_table_reduced_precision_path(gpu):
+    """build_table at FP32 S1/S2 (Mueller planes + row contraction instead of
+    the fused FP64 pass): P within FP32 rounding of the FP64 table, the
+    efficiencies (FP64 series) unchanged."""
+    from geosmie_amd.runoptics import build_table
+    p64 = build_table(params("lognorm_simple"), water=water()).data
+    p32 = build_table(params("lognorm_simple"), water=water(), precision=1).data
+    for k in ("p11", "p12", "p33", "p34"):
+        scale = np.abs(p64["p11"]).max()
+        assert np.abs(p32[k] - p64[k]).max() <= 1e-5 * scale, k
+    for k in ("qext", "qsca", "g", "bext"):
+        np.testing.assert_allclose(p32[k], p64[k], rtol=1e-12)
