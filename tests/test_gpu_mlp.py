@@ -131,7 +131,8 @@ def _queries(n, seed, spread=1.0):
     from geosmie_amd.surrogate import make_queries
     r = np.random.default_rng(seed)
     c = lambda lo, hi: (lo + hi) / 2 + spread * (r.uniform(lo, hi, n) - (lo + hi) / 2)
-    gf = (1 - c(0.0, 1 - 2.16 ** -3)) ** (-1 / 3)
+    with np.errstate(invalid="ignore"):    # spread > 1 puts some growth factors out of range
+        gf = (1 - c(0.0, 1 - 2.16 ** -3)) ** (-1 / 3)
     return make_queries(np.exp(c(np.log(0.01), np.log(5.0))), c(1.4, 2.6), c(1.3, 1.8),
                         np.exp(c(np.log(1e-8), 0.0)), np.exp(c(np.log(0.2), np.log(40.0))), gf)
 
