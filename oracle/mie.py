@@ -210,11 +210,14 @@ def mie_pt(u, nmax):
 
     pi_1 = 1, pi_2 = 3u, pi_k = (2k-1)/(k-1) u pi_{k-1} - k/(k-1) pi_{k-2};
     tau_1 = u, tau_2 = 6u^2 - 3, tau_k = k u pi_k - (k+1) pi_{k-1}.
-    Handles nmax >= 1 (the reference indexes p[1] unconditionally).
+    Handles nmax >= 1 (the reference indexes p[1] unconditionally).  u may be
+    a scalar (rows [nmax]) or an array of cosines (rows [len(u), nmax]); the
+    recurrence is the same, carried out for all cosines at once.
     """
-    u = float(u)
-    p = np.zeros(nmax)
-    t = np.zeros(nmax)
+    scalar = np.ndim(u) == 0
+    u = np.atleast_1d(np.asarray(u, dtype=np.float64))
+    p = np.zeros((nmax, u.size))
+    t = np.zeros((nmax, u.size))
     p[0] = 1.0
     t[0] = u
     if nmax > 1:
@@ -225,17 +228,19 @@ def mie_pt(u, nmax):
         p[ni] = (2 * nf + 1) / nf * p[ni - 1] * u - (nf + 1) / nf * p[ni - 2]
         t[ni] = (nf + 1) * u * p[ni] - (nf + 2) * p[ni - 1]
     k = np.arange(1, nmax + 1, dtype=np.float64)
-    n2 = (2 * k + 1) / (k * (k + 1))
-    return p * n2, t * n2
+    n2 = ((2 * k + 1) / (k * (k + 1)))[:, None]
+    p = (p * n2).T
+    t = (t * n2).T
+    if scalar:
+        return p[0].copy(), t[0].copy()
+    return np.ascontiguousarray(p), np.ascontiguousarray(t)
 
 
 def pt_table(costarr, nmax):
     """[N_ang, nmax] pi'/tau' tables, mie_coated.py:160-179."""
-    P = np.zeros((len(costarr), nmax))
-    T = np.zeros((len(costarr), nmax))
-    for i, c in enumerate(costarr):
-        P[i], T[i] = mie_pt(c, nmax)
-    return P, T
+    if len(costarr) == 0:
+        return np.zeros((0, nmax)), np.zeros((0, nmax))
+    return mie_pt(np.asarray(costarr, dtype=np.float64), nmax)
 
 
 def s12_from_pt(an, bn, pin, tin):

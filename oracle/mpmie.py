@@ -127,3 +127,35 @@ def coated(m1, m2, x, y, nmax=None, start="converged"):
             an[n - 1] = complex((py[n] * a1 - py[n - 1]) / (gsy * a1 - gs1y))
             bn[n - 1] = complex((py[n] * b1 - py[n - 1]) / (gsy * b1 - gs1y))
     return an, bn, nmax
+
+
+def s12(an, bn, cost, dps=40):
+    """S1, S2 at each cosine, summed in mpmath (Bohren & Huffman eq. 4.74).
+
+    pi_n, tau_n by the upward recurrences of B&H eq. 4.47 (pi_0 = 0, pi_1 = 1,
+    pi_n = (2n-1)/(n-1) mu pi_{n-1} - n/(n-1) pi_{n-2}, tau_n = n mu pi_n -
+    (n+1) pi_{n-1}), stable for every |mu| <= 1; the sums
+    S1 = sum (2n+1)/(n(n+1)) (a_n pi_n + b_n tau_n), S2 likewise with pi and
+    tau exchanged, the definition src/pymiecoated/pymiecoated/mie_props.py:133-149
+    evaluates.  an, bn are complex128 (or mpc) arrays; returns complex128
+    arrays [len(cost)].
+    """
+    s1 = np.zeros(len(cost), dtype=np.complex128)
+    s2 = np.zeros(len(cost), dtype=np.complex128)
+    with mp.workdps(dps):
+        A = [mp.mpc(complex(a)) for a in an]
+        B = [mp.mpc(complex(b)) for b in bn]
+        for i, c in enumerate(cost):
+            mu = mp.mpf(float(c))
+            pm1, p = mp.mpf(0), mp.mpf(1)
+            t1, t2 = mp.mpc(0), mp.mpc(0)
+            for n in range(1, len(A) + 1):
+                if n > 1:
+                    pm1, p = p, ((2 * n - 1) * mu * p - n * pm1) / (n - 1)
+                tau = n * mu * p - (n + 1) * pm1
+                f = mp.mpf(2 * n + 1) / (n * (n + 1))
+                t1 += f * (A[n - 1] * p + B[n - 1] * tau)
+                t2 += f * (A[n - 1] * tau + B[n - 1] * p)
+            s1[i] = complex(t1)
+            s2[i] = complex(t2)
+    return s1, s2

@@ -47,3 +47,52 @@ def test_threads_deterministic():
     a = cmie.batch(x, 1.5 + 0.01j, threads=1)
     b = cmie.batch(x, 1.5 + 0.01j, threads=4)
     assert np.array_equal(a, b)
+
+
+MPL = np.load(os.path.join(HERE, "golden", "mp_largex_vectors.npz"))
+
+
+def _coated_sets():
+    """Every coated mpmath vector with the reference's start: the dense-core
+    set (m1 = 1.95+0.79i) and the dense-shell set (|m2| >= |m1|), whose
+    start lies close enough to |m2 x| or |m2 y| to need the start correction."""
+    return (np.concatenate([MP["c_x"], MPL["cd_x"]]), np.concatenate([MP["c_y"], MPL["cd_y"]]),
+            np.concatenate([MP["c_m1"], MPL["cd_m1"]]), np.concatenate([MP["c_m2"], MPL["cd_m2"]]),
+            np.concatenate([MP["c_props_ref"], MPL["cd_props"]]))
+
+
+def _start_correction_on(x, y, m1, m2):
+    """Whether the stable form's dD terms are nonzero (cmie.c converged_start)."""
+    nmax = np.rint(2 + y + 4 * y ** (1.0 / 3.0))
+    nmx = np.rint(np.maximum(nmax, np.maximum(np.abs(m1 * y), np.abs(m2 * y))) + 16)
+    av, aw = np.abs(m2 * x), np.abs(m2 * y)
+    return (nmx - av < 8 * np.cbrt(av)) | (nmx - aw < 8 * np.cbrt(aw))
+
+
+def test_coated_stable_form_vs_mp_every_vector():
+    """cmie.batch(stable=True), the checker of the GPU's coated kernels, against
+    every coated mpmath vector (reference start).  Real and weakly absorbing
+    shells: Qext, Qsca 1e-11, g 1e-11 absolute; the absorbing-shell points
+    (|Im m2| y > 1) use the literal form in both and keep its 1e-6."""
+    x, y, m1, m2, ref = _coated_sets()
+    on = _start_correction_on(x, y, m1, m2)
+    assert on.sum() >= 90          # the dD branch is exercised, not only the converged case
+    r = cmie.batch(x, m1, y, m2, stable=True)
+    rel = np.abs(r[:2].T - ref[:, :2]) / np.abs(ref[:, :2])
+    weak = np.abs(m2.imag) * y <= 1.0
+    assert rel[weak].max() < 1e-11, rel[weak].max()
+    assert np.abs(r[4] - ref[:, 4])[weak].max() < 1e-11
+    assert rel[~weak].max() < 1e-6
+    # the start correction matters: the converged coefficients differ from the
+    # reference's by far more than the tolerance on these points
+    conv = MPL["cd_props_conv"][:, :2]
+    assert (np.abs(conv - MPL["cd_props"][:, :2]) / np.abs(MPL["cd_props"][:, :2])).max() > 1e-4
+
+
+def test_coated_literal_form_vs_mp_dense_shell():
+    """The literal restatement of mie_coeffs.py:230-249 on the dense-shell set:
+    within 1e-9 (its FP64 conditioning on these points is ~2.5e-10)."""
+    r = cmie.batch(MPL["cd_x"], MPL["cd_m1"], MPL["cd_y"], MPL["cd_m2"], stable=False)
+    ref = MPL["cd_props"]
+    rel = np.abs(r[:2].T - ref[:, :2]) / np.abs(ref[:, :2])
+    assert rel.max() < 1e-9, rel.max()
