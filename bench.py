@@ -5,8 +5,17 @@ m = 1.95+0.79i, shells 1.33 + (1.45-1.33)/gf^3 for 12 growth factors
 gf = 1.0..2.5, core/shell radius ratio 0.3..0.9 (7), 5000 shell radii
 log-spaced 0.01-10 um, 20 wavelengths 0.3-3.0 um: 8.4 M points, size
 parameter up to 209 -- through gm_mie_batch (plan + length-bucketed coated
-kernel): a_n, b_n and Qext, Qsca, Qabs, Qb, g, qratio per point, FP64.
-Inputs are resident in HBM before the timed region.
+kernel): the coefficient recurrences in FP64 with Qext, Qsca, Qabs, Qb, g
+and qratio written per point (the a_n, b_n are formed in registers and not
+stored; the secondary line "coated_with_coefficients" also stores them).
+Inputs are resident in HBM before the timed region; the step is replayed as
+a captured HIP graph.
+
+Also reported (JSON keys): the plain homogeneous runoptics chain and the
+large-x chain (secondary), the LUT sweep and the surrogate with its plain
+fallback (lut), a whole runoptics table (table), and host-CPU baselines of
+the coated batch and of the homogeneous chain on every host core the
+process may use (cpu_baseline, secondary[0].cpu_chain).
 
 Prints one JSON line.  --gpus N > 1 exits: the benchmark runs on one GPU.
 """
@@ -26,8 +35,8 @@ METRIC = "(r,λ,m) Mie points/s + bulk LUT entries/s vs plain GPU at ≤1e-3 rel
 PEAK_FP64_TFLOPS = 78.6  # MI355X FP64 (vector = matrix on gfx950)
 
 
-TRAFFIC_FILE = os.path.join(ROOT, "profiles",
-                            "k_mie_coated_stable__coated8.4M__pmc_fetch_write.txt")
+TRAFFIC_FILE_REL = "profiles/k_mie_coated_stable__coated8.4M__pmc_fetch_write.txt"
+TRAFFIC_FILE = os.path.join(ROOT, TRAFFIC_FILE_REL)
 
 
 def measured_traffic():
@@ -99,10 +108,17 @@ def largex_workload():
     return x, m, ang
 
 
+def host_cores():
+    """Cores this process may run on (its affinity mask)."""
+    return len(os.sched_getaffinity(0))
+
+
 def cpu_baseline(x, y, m1, m2, target_s=12.0):
+    """The C/OpenMP restatement (oracle/cmie.c, literal coated formula) on
+    every core of the process's affinity mask, on an evenly strided sample of
+    the coated batch sized to about target_s seconds."""
     from oracle import cmie
-    cores = len(os.sched_getaffinity(0))
-    cores = min(cores, 16)
+    cores = host_cores()
     probe = slice(0, None, 997)
     t = time.perf_counter()
     cmie.batch(x[probe], m1, y[probe], m2[probe], threads=cores)
@@ -115,8 +131,38 @@ def cpu_baseline(x, y, m1, m2, target_s=12.0):
     dt = time.perf_counter() - t
     n = x[sel].size
     return {"value": n / dt, "unit": "points/s", "cores": cores, "kind": "port",
+            "per_core": n / dt / cores,
+            "omp_num_threads_env": os.environ.get("OMP_NUM_THREADS"),
             "sample": "every %d-th point of the 8.4M-point coated batch (%d points, %.1f s), "
-                      "C OpenMP restatement oracle/cmie.c" % (stride, n, dt)}
+                      "C OpenMP restatement oracle/cmie.c on %d threads" % (stride, n, dt, cores)}
+
+
+def cpu_chain_baseline(target_s=8.0):
+    """The plain homogeneous runoptics chain on the host: oracle/cmie.c
+    cmie_chain_homog (coefficients, S1/S2 at 181 angles, Mueller elements,
+    weighted row sums) on every core of the affinity mask, on a strided
+    sample of the 2000-radius x 10-wavelength grid."""
+    from oracle import cmie
+    x, lam, ang, w, m = homog_lut_workload()
+    r = np.tile(np.logspace(np.log10(0.005), np.log10(20.0), 2000), lam.size)
+    row = np.repeat(np.arange(lam.size), 2000)
+    wp = w[row, np.arange(x.size)]
+    cost = np.cos(np.radians(ang))
+    cores = host_cores()
+    t = time.perf_counter()
+    cmie.chain_homog(x[::50], m, r[::50], row[::50], wp[::50], lam.size, cost, threads=cores)
+    est = (time.perf_counter() - t) * 50
+    stride = max(1, int(np.ceil(est / target_s)))
+    sel = slice(0, None, stride)
+    t = time.perf_counter()
+    cmie.chain_homog(x[sel], m, r[sel], row[sel], wp[sel], lam.size, cost, threads=cores)
+    dt = time.perf_counter() - t
+    n = x[sel].size
+    return {"value": n / dt, "unit": "points/s", "cores": cores, "kind": "port",
+            "per_core": n / dt / cores,
+            "sample": "%s points of the 20000-point grid (every %d-th), %.2f s: coefficients, "
+                      "S1/S2 at 181 angles, Mueller elements and weighted row sums, "
+                      "oracle/cmie.c cmie_chain_homog on %d threads" % (n, stride, dt, cores)}
 
 
 def time_pipeline(pipe, steps, warmup):
@@ -156,18 +202,18 @@ def time_pipeline(pipe, steps, warmup):
     return (t1 - t0) * 1e3 / steps, kt, (t3 - t2) * 1e3 / steps
 
 
-def secondary(steps=5, warmup=1, which=("homog", "largex")):
+def secondary(steps=5, warmup=1, which=("homog", "largex"), cpu=True):
     from geosmie_amd import autotune
     from geosmie_amd.pipeline import RunopticsChain
     out = []
     if "homog" in which:
-        out.append(_secondary_homog(steps, warmup, autotune, RunopticsChain))
+        out.append(_secondary_homog(steps, warmup, autotune, RunopticsChain, cpu))
     if "largex" in which:
         out.append(_secondary_largex(steps, warmup, autotune, RunopticsChain))
     return out
 
 
-def _secondary_homog(steps, warmup, autotune, RunopticsChain):
+def _secondary_homog(steps, warmup, autotune, RunopticsChain, cpu=True):
     x, lam, ang, w, m = homog_lut_workload()
     pipe = RunopticsChain(x, m, ang, w, lam, [(i * 2000, 2000) for i in range(10)])
     # the FP64 step fuses S1/S2 with the row integrals: only the Mie kernel's
@@ -175,14 +221,39 @@ def _secondary_homog(steps, warmup, autotune, RunopticsChain):
     tuned = autotune.tune(pipe.ctx, pipe.step, knobs=("mie_waves",), reps=3)
     ms_eager, kt, ms = time_pipeline(pipe, steps, warmup)
     s12_flop = 16.0 * pipe.sum_nmax * ang.size
-    return ({"workload": "plain homogeneous FP64 Mie, 2000 radii 0.005-20 um x 10 wavelengths "
-                            "0.3-2.5 um, m=1.53+0.005i, 181 angles, lognormal r_mode 0.5 um "
-                            "sigma_g 2.0 integrated per wavelength (Qext/Qsca/g, P11/P12/P33/P34)",
-                "points": int(x.size), "ms_per_step": ms, "value": x.size / (ms * 1e-3),
-                "mode": "HIP graph replay", "ms_per_step_eager": ms_eager,
-                "autotune": {k: v["best"] for k, v in tuned.items()},
-                "unit": "points/s", "kernel_ms": kt,
-                "s12_tflops": s12_flop / (kt.get("s12", ms) * 1e-3) / 1e12})
+    out = {"workload": "plain homogeneous FP64 Mie, 2000 radii 0.005-20 um x 10 wavelengths "
+                       "0.3-2.5 um, m=1.53+0.005i, 181 angles, lognormal r_mode 0.5 um "
+                       "sigma_g 2.0 integrated per wavelength (Qext/Qsca/g, P11/P12/P33/P34)",
+           "points": int(x.size), "ms_per_step": ms, "value": x.size / (ms * 1e-3),
+           "mode": "HIP graph replay", "ms_per_step_eager": ms_eager,
+           "autotune": {k: v["best"] for k, v in tuned.items()},
+           "unit": "points/s", "kernel_ms": kt,
+           "s12_tflops": s12_flop / (kt.get("s12", ms) * 1e-3) / 1e12,
+           "k_mie_homog_roofline": homog_roofline(x, m, kt.get("k_mie_homog"))}
+    if cpu:
+        c = cpu_chain_baseline()
+        out["cpu_chain"] = c
+        out["speedup_vs_cpu_chain"] = out["value"] / c["value"]
+    return out
+
+
+def homog_flops(x, m):
+    """Algorithmic FLOP of the homogeneous recurrence kernel: the per-point
+    convention 10 nmx + 40 nmax + 30 nmax (D_n step, coefficient pair,
+    efficiency sums), nmax and nmx by the reference's rules."""
+    x = np.asarray(x, dtype=np.float64)
+    nmax = np.rint(2 + x + 4 * x ** (1.0 / 3.0))
+    nmx = np.rint(np.maximum(nmax, np.abs(np.asarray(m)) * x) + 16)
+    return float(np.sum(10 * nmx + 70 * nmax))
+
+
+def homog_roofline(x, m, kernel_ms):
+    if not kernel_ms:
+        return None
+    f = homog_flops(x, m)
+    a = f / (kernel_ms * 1e-3) / 1e12
+    return {"flop_per_launch": f, "kernel_ms": kernel_ms, "achieved": a, "unit": "TFLOP/s",
+            "peak": PEAK_FP64_TFLOPS, "frac": a / PEAK_FP64_TFLOPS}
 
 
 def _secondary_largex(steps, warmup, autotune, RunopticsChain):
@@ -232,7 +303,8 @@ def _secondary_largex(steps, warmup, autotune, RunopticsChain):
             }
         modes.append(ent)
     fp64 = modes[0]
-    return ({"workload": "large-x homogeneous Mie, 5000 radii 0.1-100 um x 8 wavelengths "
+    return ({"k_mie_homog_roofline": homog_roofline(x, m, fp64["kernel_ms"].get("k_mie_homog")),
+             "workload": "large-x homogeneous Mie, 5000 radii 0.1-100 um x 8 wavelengths "
                             "0.3-1.0 um (x <= 2094), m_r U(1.33,1.60), m_i 10^U(-6,-2), 1801 "
                             "angles, S1/S2 on MFMA (FP64; FP32 and split-BF16 below), lognormal "
                             "integration (r_mode 5 um, sigma_g 2.0), 300 GSF moments",
@@ -246,10 +318,11 @@ def _secondary_largex(steps, warmup, autotune, RunopticsChain):
 SURROGATE_FILE = os.path.join(ROOT, "geosmie_amd", "data", "surrogate_lut.npz")
 
 
-def lut_surrogate(n_queries=1_000_000, n_sample=4096, seed=20261202):
+def lut_surrogate(n_queries=1_000_000, n_sample=4096, seed=20261202, full=True):
     """Bulk LUT entries/s: plain GPU Mie (grid sweep and per-query) against the
     trained surrogate with its plain-Mie fallback, on queries drawn uniformly
-    in the sweep's ranges (ln r_mode, sigma_g, m_r, ln m_i, ln lambda, f_w)."""
+    in the sweep's ranges (ln r_mode, sigma_g, m_r, ln m_i, ln lambda, f_w).
+    full: also time the whole 56.16M-entry grid at FP64."""
     import torch
     from geosmie_amd.lut_sweep import LutSweep, full_grid
     g = full_grid()
@@ -271,6 +344,25 @@ def lut_surrogate(n_queries=1_000_000, n_sample=4096, seed=20261202):
                                    "(2400 slabs of the 1.4M-slab grid), FP64",
                          "full_grid_s_est": dt * 1_404_000 / (n_ent / 40)}
     del tab
+    if full:
+        last = [time.perf_counter()]
+
+        def progress(done, total):
+            if time.perf_counter() - last[0] > 20:
+                torch.cuda.synchronize()
+                print("lut sweep: %d of %d slabs" % (done, total), file=sys.stderr, flush=True)
+                last[0] = time.perf_counter()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        tab = sw.sweep(g["mr"], g["mi"], g["lam"], g["gf"], progress=progress)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        n_ent = tab.numel() // tab.shape[-1]
+        out["plain_full_grid"] = {"entries": int(n_ent), "s": dt, "value": n_ent / dt,
+                                  "finite": bool(torch.isfinite(tab).all().item()),
+                                  "precision": "fp64 (coefficients and S1/S2)"}
+        del tab
+        torch.cuda.empty_cache()
     if not os.path.exists(SURROGATE_FILE):
         out["surrogate"] = None
         return out
@@ -306,7 +398,7 @@ def lut_surrogate(n_queries=1_000_000, n_sample=4096, seed=20261202):
     served = f[pick] == 0
     net = rows[torch.from_numpy(pick).to(rows.device)].cpu().numpy()
     err = rel_errors(net, plain.cpu().numpy())
-    t_total = t_net + n_fb * per_query
+    t_est = t_net + n_fb * per_query
     # end to end on the sample: network + gate + plain Mie re-solve of the flagged queries
     qs = qd[torch.from_numpy(pick).to(qd.device)]
     sur.query(qs[:256])
@@ -315,15 +407,27 @@ def lut_surrogate(n_queries=1_000_000, n_sample=4096, seed=20261202):
     _, nf_s = sur.query(qs)
     torch.cuda.synchronize()
     t_e2e = time.perf_counter() - t0
+    # end to end on all queries: network, gate and the plain re-solve of every
+    # flagged query, as a caller gets them
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    _, nf_all = sur.query(qd)
+    torch.cuda.synchronize()
+    t_all = time.perf_counter() - t0
+    grid_rate = out.get("plain_full_grid", out["plain_grid"])["value"]
     out["surrogate"] = {
         "queries": n, "served_by_network": n - n_fb, "fallback": n_fb,
         "network_s": t_net, "network_entries_per_s": n / t_net,
         "plain_per_query_entries_per_s": 1.0 / per_query,
-        "value": n / t_total,
-        "value_note": "network time for all queries + fallback count x measured plain-Mie "
-                      "time per query (plain timed on %d random queries)" % pick.size,
-        "speedup_vs_plain_per_query": (n / t_total) * per_query,
-        "speedup_vs_plain_grid": (n / t_total) / out["plain_grid"]["value"],
+        "value": n / t_all,
+        "value_note": "measured end to end on all %d queries: network, gate and the plain-Mie "
+                      "re-solve of the %d flagged queries" % (n, nf_all),
+        "end_to_end_s": t_all,
+        "speedup_vs_plain_per_query": (n / t_all) * per_query,
+        "speedup_vs_plain_grid": (n / t_all) / grid_rate,
+        "estimate_from_parts": {"value": n / t_est, "note": "network time + fallback count x "
+                                "plain time per query (plain timed on %d random queries)"
+                                % pick.size},
         "sample_end_to_end": {"queries": int(pick.size), "fallback": int(nf_s), "s": t_e2e,
                               "plain_s": t_plain, "speedup_vs_plain": t_plain / t_e2e},
         "served_err_vs_plain": None if not served.any() else {
@@ -419,6 +523,21 @@ def table_build(cpu_sample=True):
     return out
 
 
+def coated_conventions(x, y, m1, m2):
+    """Algorithmic FLOP per launch of the coated kernel under three counts.
+
+    survey: the per-point step costs 10 per D_n step, 40 per coefficient pair,
+    30 per efficiency term, applied to the reference's coated algorithm, which
+    runs three D_n recurrences of nmx steps (mie_coeffs.py:214-218):
+    30 nmx + 70 nmax.  single_chain: the same costs with one D_n recurrence,
+    10 nmx + 70 nmax.  full_term: 270 FLOP per series term for the shell
+    combination, coefficients and sums of the reference's formula
+    (mie_coeffs.py:230-249) and 30 nmx for the three chains."""
+    full, sum_nmax, sum_nmx = coated_flops(x, y, m1, m2)
+    return {"survey": 30 * sum_nmx + 70 * sum_nmax, "single_chain": 10 * sum_nmx + 70 * sum_nmax,
+            "full_term": full}, sum_nmax, sum_nmx
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -430,7 +549,9 @@ def main():
     ap.add_argument("--secondary-only", choices=("homog", "largex", "lut", "table"), default=None,
                     help="run only this secondary measurement (for profiling) and print it")
     ap.add_argument("--no-secondary", action="store_true",
-                    help="skip the homogeneous-LUT and large-x secondary measurements")
+                    help="skip the secondary measurements (chains, LUT, table)")
+    ap.add_argument("--no-full-lut", action="store_true",
+                    help="time the LUT sweep on its 2400-slab sample only")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: runs on one GPU; --gpus %d is not supported "
@@ -445,38 +566,41 @@ def main():
     torch.cuda.set_device(0)
     if args.secondary_only:
         if args.secondary_only == "lut":
-            res = lut_surrogate()
+            res = lut_surrogate(full=not args.no_full_lut)
         elif args.secondary_only == "table":
             res = table_build()
         else:
-            res = secondary(which=(args.secondary_only,))[0]
+            res = secondary(which=(args.secondary_only,), cpu=not args.no_cpu)[0]
         print(json.dumps({"secondary_only": args.secondary_only, "result": res}))
         return
     x, y, m1, m2 = coated_workload()
     n = x.size
-    flops, sum_nmax, sum_nmx = coated_flops(x, y, m1, m2)
+    conv, sum_nmax, sum_nmx = coated_conventions(x, y, m1, m2)
     d_x = _lib.to_dev(x)
     d_y = _lib.to_dev(y)
     d_eps = _lib.cplx_dev(np.full(n, m1 * m1))
     d_eps2 = _lib.cplx_dev(m2 * m2)
     d_props = torch.empty(6 * n, dtype=torch.float64, device="cuda")
     L = _lib.lib()
-    h = _lib.ctx()
-    s = _lib.stream()
+    pc = _lib.private_ctx()      # owned workspaces: the step is captured as a graph
+    h = pc
     size_max = float(y.max())
 
     def step():
         check(L.gm_mie_batch(h, n, ptr(d_x), ptr(d_y), ptr(d_eps), None, ptr(d_eps2), size_max,
-                             None, None, None, ptr(d_props), None, _lib.GM_MIE_SORT, s),
-              "gm_mie_batch")
+                             None, None, None, ptr(d_props), None, _lib.GM_MIE_SORT,
+                             _lib.stream()), "gm_mie_batch")
 
     from geosmie_amd import autotune
     tuned = autotune.tune(h, step, knobs=("mie_waves", "mie_grid_mult"), reps=2, warmup=1)
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
-    flags = _lib.check_flags()
-    assert flags == 0, flags
+    flags = ctypes.c_int32(0)
+    check(L.gm_ctx_check(h, _lib.stream(), ctypes.byref(flags)), "gm_ctx_check")
+    assert flags.value == 0, flags.value
+    # eager pass with the library's own HIP-event pairs around the coated
+    # kernel, on the stream it is launched on
     L.gm_ctx_set_timing(h, 1)
     ms_k = ctypes.c_double(0)
     cnt = ctypes.c_int64(0)
@@ -486,14 +610,32 @@ def main():
     for _ in range(args.steps):
         step()
     torch.cuda.synchronize()
-    t1 = time.perf_counter()
+    ms_eager = (time.perf_counter() - t0) * 1e3 / args.steps
     check(L.gm_ctx_kernel_time(h, 1, ctypes.byref(ms_k), ctypes.byref(cnt)), "kernel_time")
     L.gm_ctx_set_timing(h, 0)
-    ms_step = (t1 - t0) * 1e3 / args.steps
     kern_ms = ms_k.value / max(cnt.value, 1)
+    # the timed region: the step captured once as a HIP graph, replayed
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        step()
+    torch.cuda.current_stream().wait_stream(side)
+    torch.cuda.synchronize()
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph):
+        step()
+    graph.replay()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        graph.replay()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ms_step = (t1 - t0) * 1e3 / args.steps
     props = d_props.view(6, n)
     finite = bool(torch.isfinite(props).all().item())
-    achieved = flops / (kern_ms * 1e-3) / 1e12
+    frac = {k: v / (kern_ms * 1e-3) / 1e12 / PEAK_FP64_TFLOPS for k, v in conv.items()}
+    achieved = conv["survey"] / (kern_ms * 1e-3) / 1e12
     out = {
         "metric": METRIC,
         "value": n / (ms_step * 1e-3),
@@ -509,20 +651,30 @@ def main():
         "data": "synthetic",
         "config": {"workload": "coated-sphere batch: core 1.95+0.79i, 12 water-mixed shells "
                                "(gf 1.0-2.5), 7 core/shell ratios 0.3-0.9, 5000 radii 0.01-10 um, "
-                               "20 wavelengths 0.3-3.0 um; 8.4M points, y <= 209; "
-                               "coefficients + Qext/Qsca/Qabs/Qb/g",
+                               "20 wavelengths 0.3-3.0 um; 8.4M points, y <= 209; FP64 "
+                               "recurrences, Qext/Qsca/Qabs/Qb/g/qratio written per point "
+                               "(a_n, b_n not stored: see secondary coated_with_coefficients)",
                    "points": n, "sum_nmax": sum_nmax, "sum_nmx": sum_nmx,
+                   "mode": "HIP graph replay", "ms_per_step_eager": ms_eager,
                    "outputs_finite": finite},
-        "roofline": {"bound": "mfma", "pipe": "FP64 VALU (gfx950 FP64 vector peak = FP64 "
-                                              "matrix peak = 78.6 TFLOP/s)",
-                     "kernel": "k_mie_coated", "achieved": achieved, "peak": PEAK_FP64_TFLOPS,
-                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS,
-                     "traffic": measured_traffic(), "kernel_ms": kern_ms,
-                     "flop_per_launch": flops,
+        "roofline": {"bound": "mfma",
+                     "pipe": "FP64 vector ALU: the kernel is compute-bound on dependent FP64 "
+                             "chains and uses no matrix instructions; its roof is gfx950's FP64 "
+                             "vector peak, equal to the FP64 MFMA peak (78.6 TFLOP/s)",
+                     "kernel": "k_mie_coated_stable", "achieved": achieved,
+                     "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s", "frac": frac["survey"],
+                     "flop_convention": "30 nmx + 70 nmax per point (10 per D_n step x 3 "
+                                        "recurrences, 40 per coefficient pair, 30 per "
+                                        "efficiency term)",
+                     "frac_other_conventions": {"single_chain_10nmx_70nmax": frac["single_chain"],
+                                                "full_term_30nmx_270nmax": frac["full_term"]},
+                     "traffic": measured_traffic(), "traffic_source": TRAFFIC_FILE_REL,
+                     "kernel_ms": kern_ms, "flop_per_launch": conv["survey"],
                      "algorithmic_bytes_per_launch": 104.0 * n},
         "autotune": {k: {"best": v["best"], "ms": {str(a): b for a, b in v["ms"].items()}}
                      for k, v in tuned.items()},
     }
+    del graph
     if args.copy_calib:
         a = torch.ones(64 << 20, dtype=torch.float64, device="cuda")   # 512 MiB
         b = torch.empty_like(a)
@@ -531,13 +683,48 @@ def main():
         torch.cuda.synchronize()
         del a, b
     if not args.no_secondary:
+        sec = [coated_with_coefficients(L, h, n, d_x, d_y, d_eps, d_eps2, size_max, sum_nmax,
+                                        args.steps)]
         del d_x, d_y, d_eps, d_eps2, d_props
-        out["secondary"] = secondary()
-        out["lut"] = lut_surrogate()
+        torch.cuda.empty_cache()
+        sec += secondary(cpu=not args.no_cpu)
+        out["secondary"] = sec
+        out["lut"] = lut_surrogate(full=not args.no_full_lut)
         out["table"] = table_build()
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(x, y, m1, m2)
+        out["speedup_vs_cpu_baseline"] = out["value"] / out["cpu_baseline"]["value"]
     print(json.dumps(out))
+
+
+def coated_with_coefficients(L, h, n, d_x, d_y, d_eps, d_eps2, size_max, sum_nmax, steps):
+    """The headline batch with every a_n, b_n stored (ragged, Sum nmax complex
+    pairs: gm_coef_offsets + gm_mie_batch), eager."""
+    import torch
+    from geosmie_amd import _lib
+    from geosmie_amd._lib import check, ptr
+    d_off = torch.empty(n + 1, dtype=torch.int64, device="cuda")
+    d_an = torch.empty(2 * int(sum_nmax), dtype=torch.float64, device="cuda")
+    d_bn = torch.empty_like(d_an)
+    d_props = torch.empty(6 * n, dtype=torch.float64, device="cuda")
+
+    def step():
+        s = _lib.stream()
+        check(L.gm_coef_offsets(h, n, ptr(d_y), ptr(d_off), s), "gm_coef_offsets")
+        check(L.gm_mie_batch(h, n, ptr(d_x), ptr(d_y), ptr(d_eps), None, ptr(d_eps2), size_max,
+                             ptr(d_off), ptr(d_an), ptr(d_bn), ptr(d_props), None,
+                             _lib.GM_MIE_SORT, s), "gm_mie_batch")
+    step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    ok = bool(torch.isfinite(d_an).all().item())
+    return {"name": "coated_with_coefficients", "points": n, "ms_per_step": ms,
+            "value": n / (ms * 1e-3), "unit": "points/s", "mode": "eager",
+            "coefficient_bytes_written": 32.0 * sum_nmax, "coefficients_finite": ok}
 
 
 if __name__ == "__main__":

@@ -106,9 +106,11 @@ def lib():
                 raise GMError("libgeosmie_hip.so not built (run python -m geosmie_amd.build): %s"
                               % LIB_PATH)
             L = ctypes.CDLL(LIB_PATH)
+            missing = [name for name in SIGNATURES if not hasattr(L, name)]
+            if missing:
+                raise GMError("%s lacks declared entry points %s: rebuild it "
+                              "(python -m geosmie_amd.build)" % (LIB_PATH, ", ".join(missing)))
             for name, (res, args) in SIGNATURES.items():
-                if not hasattr(L, name):
-                    continue
                 f = getattr(L, name)
                 f.restype = res
                 f.argtypes = args

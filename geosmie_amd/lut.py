@@ -87,12 +87,15 @@ def lognorm_psd(rmode, sigma, xxarr, drarr, lam, rmax, rmin):
 class _DevMie(object):
     """Device coefficients of one homogeneous batch (kept resident)."""
 
-    def __init__(self, xx, m, d_eps=None, x_max=None, offsets=None):
+    def __init__(self, xx, m, d_eps=None, x_max=None, offsets=None, yy=None, m2=None):
         """xx, m host arrays; or xx a device tensor of size parameters with
         d_eps (device, interleaved re/im of m^2) and x_max given.  offsets =
         (device coefficient offsets of xx, their total) reuses the offsets of
-        an earlier batch on the same size parameters (no host wait)."""
+        an earlier batch on the same size parameters (no host wait).  yy, m2
+        (host arrays): coated spheres with core sizes xx, shell sizes yy and
+        shell index m2; the series length and efficiencies follow yy."""
         t = _lib.torch()
+        self.d_y = self.d_eps2 = None
         if hasattr(xx, "data_ptr"):
             self.d_x, self.d_eps = xx, d_eps
             n = int(xx.numel())
@@ -103,12 +106,21 @@ class _DevMie(object):
             self.d_x = _lib.to_dev(xx)
             self.d_eps = _lib.cplx_dev(np.broadcast_to(np.asarray(m, np.complex128) ** 2, (n,)))
             xmax = float(xx.max())
+            if yy is not None:
+                yy = np.ascontiguousarray(yy, dtype=np.float64)
+                if yy.shape != xx.shape:
+                    raise ValueError("core and shell size arrays differ in length")
+                self.d_y = _lib.to_dev(yy)
+                self.d_eps2 = _lib.cplx_dev(
+                    np.broadcast_to(np.asarray(m2, np.complex128) ** 2, (n,)))
+                xmax = float(yy.max())
         self.n = n
         dev = _lib.device()
         L, h, s = _lib.lib(), _lib.ctx(), stream()
+        size = self.d_y if self.d_y is not None else self.d_x
         if offsets is None:
             self.d_off = t.empty(n + 1, dtype=t.int64, device=dev)
-            check(L.gm_coef_offsets(h, n, ptr(self.d_x), ptr(self.d_off), s), "gm_coef_offsets")
+            check(L.gm_coef_offsets(h, n, ptr(size), ptr(self.d_off), s), "gm_coef_offsets")
             total = int(self.d_off[-1].item())
         else:
             self.d_off, total = offsets
@@ -116,10 +128,13 @@ class _DevMie(object):
         self.d_bn = t.empty_like(self.d_an)
         self.d_props = t.empty(6 * n, dtype=t.float64, device=dev)
         self.d_nmax = t.empty(n, dtype=t.int32, device=dev)
-        check(L.gm_mie_batch(h, n, ptr(self.d_x), None, ptr(self.d_eps), None, None,
-                             xmax, ptr(self.d_off), ptr(self.d_an), ptr(self.d_bn),
-                             ptr(self.d_props), ptr(self.d_nmax), _lib.GM_MIE_SORT, s),
-              "gm_mie_batch")
+        check(L.gm_mie_batch(h, n, ptr(self.d_x), ptr(self.d_y), ptr(self.d_eps), None,
+                             ptr(self.d_eps2), xmax, ptr(self.d_off), ptr(self.d_an),
+                             ptr(self.d_bn), ptr(self.d_props), ptr(self.d_nmax),
+                             _lib.GM_MIE_SORT, s), "gm_mie_batch")
+        if self.d_y is not None and _lib.check_flags() & 2:
+            raise ValueError("Multilayer calculations for magnetic particles "
+                             "are not currently supported.")
         self.nmax_max = int(np.rint(2 + xmax + 4 * xmax ** (1. / 3.)))
 
     def mueller(self, pi_t, tau_t, precision=0):
@@ -150,8 +165,13 @@ class MultipleMie(object):
     The reference precomputes jv/yv per size and pi/tau per n_max on the
     host; here the Riccati-Bessel functions are computed inside the
     coefficient kernel and one device pi/tau table serves every size.
-    Coated size ranges (yArr) are not reachable in the reference (its branch
-    calls an undefined name, mie_coated.py:80); here yArr is honoured.
+
+    Coated size ranges are not reachable in the reference: its branch calls an
+    undefined name (mie_coated.py:80) and has no argument for the shell's
+    index.  Here a coated range is xArr (core sizes) with yArr (shell sizes);
+    calculateS12SizeRange(mr, mi, mr2, mi2) then takes the core index
+    (mr, mi) and the shell index (mr2, mi2), and the efficiencies are
+    normalised by the shell size y, as Mie(x=, y=) does (mie_coated.py:196).
     """
 
     def __init__(self, xArr, yArr, costarr):
@@ -166,14 +186,18 @@ class MultipleMie(object):
         nmax = int(np.rint(2 + size.max() + 4 * size.max() ** (1. / 3.)))
         self._tab = angle_tables_dev(self.costarr, nmax)
 
-    def size_range_arrays(self, mr, mi, want="s12"):
-        """Arrays: 's1','s2' [N_x, N_ang] (or 'mueller' [N_x, 4, N_ang]) and efficiencies."""
+    def size_range_arrays(self, mr, mi, want="s12", mr2=None, mi2=None):
+        """Arrays: 's1','s2' [N_x, N_ang] (or 'mueller' [N_x, 4, N_ang]) and
+        efficiencies.  (mr2, mi2): shell index of a coated range (yArr given)."""
         if self._tab is None:
             self.preCalculate()
         m = complex(mr, mi)
         if self.yArr is not None:
-            raise NotImplementedError("coated size ranges: use geosmie_amd.mie.mie_batch")
-        dm = _DevMie(self.xArr, m)
+            if mr2 is None:
+                raise ValueError("Must specify both y and m2 for coated particles.")
+            dm = _DevMie(self.xArr, m, yy=self.yArr, m2=complex(mr2, 0.0 if mi2 is None else mi2))
+        else:
+            dm = _DevMie(self.xArr, m)
         out = {}
         n_ang = self.costarr.size
         if want == "mueller":
@@ -187,9 +211,10 @@ class MultipleMie(object):
             out[k] = props[i]
         return out
 
-    def calculateS12SizeRange(self, mr, mi):
-        """Same return structure as the reference (mie_coated.py:61-89)."""
-        a = self.size_range_arrays(mr, mi)
+    def calculateS12SizeRange(self, mr, mi, mr2=None, mi2=None):
+        """Same return structure as the reference (mie_coated.py:61-89);
+        (mr2, mi2) is the shell index of a coated range."""
+        a = self.size_range_arrays(mr, mi, mr2=mr2, mi2=mi2)
         ret = {"s12": [list(zip(a["s1"][i].tolist(), a["s2"][i].tolist()))
                        for i in range(self.xArr.size)]}
         for k in ("qext", "qsca", "qabs", "asy", "qb", "qratio"):

@@ -299,3 +299,85 @@ int cmie_batch(int64_t n, const double* x, const double* y, const double* m, con
   }
   return 0;
 }
+
+/*
+ * Homogeneous runoptics chain on the host (the CPU baseline of the plain
+ * homogeneous workload): per point the coefficients (homog above), S1/S2 at
+ * every angle from shared pi'/tau' tables (mie_props.py:133-149 with the
+ * tables of mie_coated.py:160-179), the Mueller elements s11, s12, s33, s34
+ * (dointegration.py:1045-1050) and their number-weighted sums per row
+ * (dointegration.py:1164-1166), plus the area-weighted efficiency sums
+ * sum w pi r^2, sum w pi r^2 Qext, sum w pi r^2 Qsca, sum w pi r^2 Qsca g
+ * (dointegration.py:1178-1190).  Each point belongs to one row (row[i]) with
+ * weight w[i] and radius r[i].  OpenMP over points with per-thread row
+ * accumulators added in thread order at the end.
+ *   pt[2 * nmax_cap * n_ang]: pi' then tau', [n][angle]
+ *   pm[n_rows * 4 * n_ang], sc[n_rows * 4] (zeroed here)
+ */
+int cmie_chain_homog(int64_t n, const double* x, const double* m, const double* r,
+                     const int32_t* row, const double* w, int n_rows, int n_ang,
+                     const double* pt, int nmax_cap, double* pm, double* sc, int threads) {
+#ifdef _OPENMP
+  if (threads > 0) omp_set_num_threads(threads);
+#endif
+  const double* pi_t = pt;
+  const double* tau_t = pt + (size_t)nmax_cap * n_ang;
+  for (int64_t k = 0; k < (int64_t)n_rows * 4 * n_ang; ++k) pm[k] = 0.0;
+  for (int k = 0; k < n_rows * 4; ++k) sc[k] = 0.0;
+  int bad = 0;
+#pragma omp parallel
+  {
+    double* acc = (double*)calloc((size_t)n_rows * 4 * n_ang, sizeof(double));
+    double* asc = (double*)calloc((size_t)n_rows * 4, sizeof(double));
+    cplx* work = (cplx*)malloc(sizeof(cplx) * (size_t)(16 * nmax_cap + 64 + ratio_start(nmax_cap, 4.0 * nmax_cap)));
+    cplx* an = (cplx*)malloc(sizeof(cplx) * (size_t)nmax_cap);
+    cplx* bn = (cplx*)malloc(sizeof(cplx) * (size_t)nmax_cap);
+    cplx* s1 = (cplx*)malloc(sizeof(cplx) * (size_t)n_ang);
+    cplx* s2 = (cplx*)malloc(sizeof(cplx) * (size_t)n_ang);
+#pragma omp for schedule(dynamic, 16)
+    for (int64_t i = 0; i < n; ++i) {
+      if (nmax_rule(x[i]) > nmax_cap) { bad = 1; continue; }
+      cplx mi = CMPLX(m[2 * i], m[2 * i + 1]);
+      int k = homog(x[i], mi, an, bn, work);
+      double out[6];
+      props_of(an, bn, k, x[i], out);
+      for (int j = 0; j < n_ang; ++j) { s1[j] = 0.0; s2[j] = 0.0; }
+      for (int q = 0; q < k; ++q) {
+        const double* p = pi_t + (size_t)q * n_ang;
+        const double* t = tau_t + (size_t)q * n_ang;
+        cplx a = an[q], b = bn[q];
+        for (int j = 0; j < n_ang; ++j) {
+          s1[j] += a * p[j] + b * t[j];
+          s2[j] += a * t[j] + b * p[j];
+        }
+      }
+      double wi = w[i];
+      double* o = acc + (size_t)row[i] * 4 * n_ang;
+      for (int j = 0; j < n_ang; ++j) {
+        double a1 = creal(s1[j]) * creal(s1[j]) + cimag(s1[j]) * cimag(s1[j]);
+        double a2 = creal(s2[j]) * creal(s2[j]) + cimag(s2[j]) * cimag(s2[j]);
+        cplx c = s1[j] * conj(s2[j]);
+        o[j] += wi * 0.5 * (a1 + a2);
+        o[n_ang + j] += wi * 0.5 * (a2 - a1);
+        o[2 * n_ang + j] += wi * creal(c);
+        o[3 * n_ang + j] += wi * cimag(c);
+      }
+      double area = wi * M_PI * r[i] * r[i];
+      double* so = asc + (size_t)row[i] * 4;
+      so[0] += area;
+      so[1] += area * out[0];
+      so[2] += area * out[1];
+      so[3] += area * out[1] * out[4];
+    }
+#pragma omp for ordered schedule(static, 1)
+    for (int tid = 0; tid < omp_get_num_threads(); ++tid) {
+#pragma omp ordered
+      {
+        for (int64_t q = 0; q < (int64_t)n_rows * 4 * n_ang; ++q) pm[q] += acc[q];
+        for (int q = 0; q < n_rows * 4; ++q) sc[q] += asc[q];
+      }
+    }
+    free(acc); free(asc); free(work); free(an); free(bn); free(s1); free(s2);
+  }
+  return bad ? -1 : 0;
+}

@@ -70,6 +70,29 @@ def test_size_range_structure(gpu):
     assert np.allclose(got["qext"], ref["qext"], rtol=1e-12)
 
 
+def test_size_range_coated(gpu):
+    """A coated size range (core sizes xArr, shell sizes yArr, shell index
+    (mr2, mi2)): S1/S2 and efficiencies against the mpmath coated vectors with
+    the reference's start (dense core 1.95+0.79i, real shell 1.45)."""
+    from geosmie_amd.lut import MultipleMie
+    MP = np.load(os.path.join(HERE, "golden", "mp_vectors.npz"))
+    sel = np.flatnonzero((MP["c_m2"] == 1.45 + 0j))
+    x, y = MP["c_x"][sel], MP["c_y"][sel]
+    cost = np.cos(np.radians(MP["theta"]))
+    mm = MultipleMie(x, y, cost)
+    with pytest.raises(ValueError):
+        mm.calculateS12SizeRange(1.95, 0.79)
+    got = mm.calculateS12SizeRange(1.95, 0.79, 1.45, 0.0)
+    for i, k in enumerate(sel):
+        s1 = np.array([v[0] for v in got["s12"][i]])
+        s2 = np.array([v[1] for v in got["s12"][i]])
+        scale = max(np.abs(MP["c_s1_ref"][k]).max(), np.abs(MP["c_s2_ref"][k]).max())
+        assert np.abs(s1 - MP["c_s1_ref"][k]).max() < 1e-9 * scale
+        assert np.abs(s2 - MP["c_s2_ref"][k]).max() < 1e-9 * scale
+        assert abs(got["qext"][i] - MP["c_props_ref"][k][0]) < 1e-9 * MP["c_props_ref"][k][0]
+        assert abs(got["qsca"][i] - MP["c_props_ref"][k][1]) < 1e-9 * MP["c_props_ref"][k][1]
+
+
 def test_post_integration_vs_oracle(gpu):
     from geosmie_amd.lut import post_integration
     raw = _raw()

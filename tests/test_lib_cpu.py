@@ -64,3 +64,16 @@ def test_surrogate_prep_rejects_cell_table_size_mismatch():
     rc = L.gm_surrogate_prep(None, 10, None, None, None, None, None, fine, None, 829440, 1.33,
                              None, None, None)
     assert rc < 0 and b"bad arguments" in L.gm_last_error()
+
+
+def test_missing_declared_symbol_raises_at_load(monkeypatch):
+    """A library that lacks an entry point the binding declares is refused when
+    it is loaded, not when the missing function is first called."""
+    import pytest
+    from geosmie_amd import _lib
+    sig = dict(_lib.SIGNATURES)
+    sig["gm_not_an_export"] = (ctypes.c_int, [])
+    monkeypatch.setattr(_lib, "SIGNATURES", sig)
+    monkeypatch.setattr(_lib, "_lib", None)
+    with pytest.raises(_lib.GMError, match="gm_not_an_export"):
+        _lib.lib()
