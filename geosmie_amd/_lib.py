@@ -42,6 +42,8 @@ SIGNATURES = {
     "gm_coef_offsets": (ctypes.c_int, [c_vp, c_i64, c_vp, c_vp, c_vp]),
     "gm_mie_batch": (ctypes.c_int, [c_vp, c_i64, c_vp, c_vp, c_vp, c_vp, c_vp, c_dbl, c_vp,
                                     c_vp, c_vp, c_vp, c_vp, c_u32, c_vp]),
+    "gm_mie_batch_noalloc": (ctypes.c_int, [c_vp, c_i64, c_vp, c_vp, c_vp, c_vp, c_vp, c_dbl,
+                                            c_vp, c_vp, c_vp, c_vp, c_vp, c_u32, c_vp, c_vp]),
     "gm_angle_tables": (ctypes.c_int, [c_vp, ctypes.c_int, c_vp, ctypes.c_int, c_vp, c_vp,
                                        c_vp]),
     "gm_mie_s12": (ctypes.c_int, [c_vp, c_i64, c_vp, c_vp, c_vp, c_vp, ctypes.c_int, c_vp, c_vp,

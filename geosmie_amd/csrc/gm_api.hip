@@ -87,6 +87,149 @@ void gm_ctx::mark_end(size_t slot, hipStream_t s) {
   hipEventRecord(pairs[slot].b, s);
 }
 
+namespace {
+
+// Static scratch of gm_mie_batch_noalloc: one copy per GPU, held until the
+// process ends.  Host code never uses the name as an address: its device
+// address is taken once per device (hipGetSymbolAddress in gm_ctx_create,
+// with that device current) and kept in g_noalloc[device].
+constexpr size_t kNoAllocBytes = (size_t)128 << 20;
+static __device__ __align__(256) double scratch[kNoAllocBytes / sizeof(double)];
+constexpr int kMaxDevices = 64;
+std::mutex g_noalloc_mu;
+void* g_noalloc[kMaxDevices] = {};
+
+// Take this device's address of the static scratch (the device is current).
+int noalloc_init(int device) {
+  if (device >= kMaxDevices) return GM_OK;   // such devices only lack the no-alloc entry
+  std::lock_guard<std::mutex> lk(g_noalloc_mu);
+  if (g_noalloc[device]) return GM_OK;
+  void* p = nullptr;
+  hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p)
+    return gm_fail(GM_ERUNTIME, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+  g_noalloc[device] = p;
+  return GM_OK;
+}
+
+void* noalloc_scratch(int device) {
+  if (device < 0 || device >= kMaxDevices) return nullptr;
+  std::lock_guard<std::mutex> lk(g_noalloc_mu);
+  return g_noalloc[device];
+}
+
+// Buffers of one gm_mie_batch call.
+struct MieWs {
+  int64_t* order = nullptr;
+  int32_t* key = nullptr;
+  int64_t* hist = nullptr;     // histogram [nb], cursors [nb], class ranges [2 kClasses]
+  int32_t* status = nullptr;
+  double* slots = nullptr;     // recurrence slots of the persistent kernels
+};
+
+// Persistent grid of each Mie kernel: one wavefront per resident slot
+// (occupancy measured per kernel at context creation) times the tuned
+// multiple, capped by the waves of work and by the slots the budget holds.
+struct MieGrids {
+  std::pair<int64_t, size_t> homog, stable, literal;
+};
+
+MieGrids mie_grids(const gm_ctx* ctx, int64_t n, int rows_cap, size_t budget) {
+  const int64_t waves = (n + 63) / 64;
+  auto grid_for = [&](int occ, int row_doubles) {
+    const size_t slot_bytes = (size_t)rows_cap * 64 * row_doubles * sizeof(double);
+    int64_t g = std::min<int64_t>(
+        waves, (int64_t)ctx->n_cu * std::max(occ, 1) * std::max(ctx->tune_mie_grid_mult, 1));
+    g = std::min<int64_t>(g, std::max<int64_t>(1, (int64_t)(budget / slot_bytes)));
+    return std::make_pair(g, (size_t)g * slot_bytes);
+  };
+  const int var = ctx->tune_mie_waves == 4 ? 2 : ctx->tune_mie_waves == 3 ? 1 : 0;
+  MieGrids g;
+  g.homog = grid_for(ctx->occ_homog[var], gm::kHomogRowDoubles);
+  g.stable = grid_for(ctx->occ_stable[var], gm::kCoatedRowDoubles);
+  g.literal = grid_for(ctx->occ_literal, gm::kCoatedRowDoubles);
+  return g;
+}
+
+int mie_batch_check(const gm_ctx* ctx, int64_t n, const double* x, const double* y,
+                    const double* eps, const double* eps2, double size_max,
+                    const int64_t* coef_off, const double* an, const double* bn,
+                    const char* who) {
+  if (!ctx) return gm_fail(GM_EINVAL, "%s: ctx is NULL", who);
+  if (n < 0) return gm_fail(GM_EINVAL, "%s: n < 0", who);
+  if (n == 0) return GM_OK;
+  if (!x || !eps) return gm_fail(GM_EINVAL, "Must specify x and either eps or m.");
+  if ((y != nullptr) != (eps2 != nullptr))
+    return gm_fail(GM_EINVAL, "Must specify both y and m2 for coated particles.");
+  if (coef_off && (!an || !bn)) return gm_fail(GM_EINVAL, "%s: an/bn NULL", who);
+  if (!(size_max >= 0.0) || !isfinite(size_max))
+    return gm_fail(GM_EINVAL, "%s: size_max must be finite and >= 0", who);
+  return GM_OK;
+}
+
+// Plan (classify, sort by cost) and the three class kernels of gm_mie_batch.
+int mie_batch_run(gm_ctx* ctx, int64_t n, const double* x, const double* y, const double* eps,
+                  const double* mu, const double* eps2, int rows_cap, const int64_t* coef_off,
+                  double* an, double* bn, double* props, int32_t* nmax_out, uint32_t flags,
+                  const MieWs& ws, const MieGrids& g, hipStream_t s) {
+  const int nb = gm::kClasses * gm::kBuckets;
+  int64_t* hist = ws.hist;
+  int64_t* cursor = hist + nb;
+  int64_t* range = cursor + nb;
+  unsigned gblk = (unsigned)((n + 255) / 256);
+  // points per thread of the plan kernels: 1 up to ~256k points (blocks stay
+  // plentiful), growing to kPlanPer for the largest batches (fewer LDS
+  // histogram flushes)
+  const int per = (int)std::min<int64_t>(gm::kPlanPer, std::max<int64_t>(1, n / (256 * 1024)));
+  const unsigned pblk = (unsigned)((n + 256 * (int64_t)per - 1) / (256 * (int64_t)per));
+  if (flags & GM_MIE_SORT) {
+    hipLaunchKernelGGL(gm::k_zero_i64, dim3((nb + 255) / 256), dim3(256), 0, s, hist, nb);
+    hipLaunchKernelGGL(gm::k_plan_keys, dim3(pblk), dim3(256), 0, s, n, x, y, eps, mu, eps2,
+                       ws.key, nmax_out, hist, ws.status, per);
+    hipLaunchKernelGGL(gm::k_plan_scan, dim3(1), dim3(256), 0, s, hist, cursor, range);
+    hipLaunchKernelGGL(gm::k_plan_scatter, dim3(pblk), dim3(256), 0, s, n,
+                       (const int32_t*)ws.key, cursor, ws.order, per);
+  } else {
+    // input order: each class kernel walks every point and keeps its own
+    hipLaunchKernelGGL(gm::k_plan_identity, dim3(gblk), dim3(256), 0, s, n, ws.order, range);
+    if (nmax_out) {
+      hipLaunchKernelGGL(gm::k_nmax_i32, dim3(gblk), dim3(256), 0, s, n, y ? y : x, nmax_out);
+    }
+  }
+  HIP_OK(hipGetLastError());
+  const int dn_extra = (flags & GM_MIE_DN_CONVERGED) ? -1 : 0;
+  const int var = ctx->tune_mie_waves == 4 ? 2 : ctx->tune_mie_waves == 3 ? 1 : 0;
+  const int64_t* order = ws.order;
+  size_t slot;
+  ctx->mark_begin(gm::kKMieHomog, s, &slot);
+  auto homog = var == 2 ? gm::k_mie_homog<4> : gm::k_mie_homog<1>;
+  hipLaunchKernelGGL(homog, dim3((unsigned)g.homog.first), dim3(64), 0, s, order, range, n, x, y,
+                     eps, mu, eps2, coef_off, an, bn, props, ws.status, ws.slots, rows_cap,
+                     dn_extra);
+  ctx->mark_end(slot, s);
+  HIP_OK(hipGetLastError());
+  if (y) {
+    ctx->mark_begin(gm::kKMieCoated, s, &slot);
+    auto stable = var == 2   ? gm::k_mie_coated_stable<4>
+                  : var == 1 ? gm::k_mie_coated_stable<3>
+                             : gm::k_mie_coated_stable<1>;
+    hipLaunchKernelGGL(stable, dim3((unsigned)g.stable.first), dim3(64), 0, s, order, range, n, x,
+                       y, eps, mu, eps2, coef_off, an, bn, props, ws.status, ws.slots, rows_cap,
+                       dn_extra);
+    ctx->mark_end(slot, s);
+    HIP_OK(hipGetLastError());
+    ctx->mark_begin(gm::kKMieCoatedLit, s, &slot);
+    hipLaunchKernelGGL(gm::k_mie_coated_literal, dim3((unsigned)g.literal.first), dim3(64), 0, s,
+                       order, range, n, x, y, eps, mu, eps2, coef_off, an, bn, props, ws.status,
+                       ws.slots, rows_cap, dn_extra);
+    ctx->mark_end(slot, s);
+    HIP_OK(hipGetLastError());
+  }
+  return GM_OK;
+}
+
+}  // namespace
+
 extern "C" {
 
 const char* gm_version(void) { return "geosmie_amd 0.1 (gfx950)"; }
@@ -126,7 +269,8 @@ int gm_ctx_create(int device, gm_ctx** out) {
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, gm::k_mie_coated_literal, 64, 0) ==
       hipSuccess)
     c->occ_literal = occ;
-  int rc = c->reserve(c->status, 64);
+  int rc = noalloc_init(device);
+  if (!rc) rc = c->reserve(c->status, 64);
   if (rc) {
     delete c;
     return rc;
@@ -268,94 +412,71 @@ int gm_mie_batch(gm_ctx* ctx, int64_t n, const double* x, const double* y, const
                  const double* mu, const double* eps2, double size_max, const int64_t* coef_off,
                  double* an, double* bn, double* props, int32_t* nmax_out, uint32_t flags,
                  gm_stream stream) {
-  if (!ctx) return gm_fail(GM_EINVAL, "gm_mie_batch: ctx is NULL");
-  if (n < 0) return gm_fail(GM_EINVAL, "gm_mie_batch: n < 0");
-  if (n == 0) return GM_OK;
-  if (!x || !eps) return gm_fail(GM_EINVAL, "Must specify x and either eps or m.");
-  if ((y != nullptr) != (eps2 != nullptr))
-    return gm_fail(GM_EINVAL, "Must specify both y and m2 for coated particles.");
-  if (coef_off && (!an || !bn)) return gm_fail(GM_EINVAL, "gm_mie_batch: an/bn NULL");
-  if (!(size_max >= 0.0) || !isfinite(size_max))
-    return gm_fail(GM_EINVAL, "gm_mie_batch: size_max must be finite and >= 0");
+  int rc = mie_batch_check(ctx, n, x, y, eps, eps2, size_max, coef_off, an, bn, "gm_mie_batch");
+  if (rc || n == 0) return rc;
   HIP_OK(hipSetDevice(ctx->device));
-  hipStream_t s = (hipStream_t)stream;
-  int rc;
   const int nb = gm::kClasses * gm::kBuckets;
   if ((rc = ctx->reserve(ctx->order, sizeof(int64_t) * n))) return rc;
   if ((rc = ctx->reserve(ctx->key, sizeof(int32_t) * n))) return rc;
   if ((rc = ctx->reserve(ctx->hist, sizeof(int64_t) * (2 * nb + 2 * gm::kClasses)))) return rc;
-  int64_t* hist = (int64_t*)ctx->hist.ptr;
-  int64_t* cursor = hist + nb;
-  int64_t* range = cursor + nb;
-  int32_t* status = (int32_t*)ctx->status.ptr;
-  int64_t* order = (int64_t*)ctx->order.ptr;
-  unsigned gblk = (unsigned)((n + 255) / 256);
-  // points per thread of the plan kernels: 1 up to ~256k points (blocks stay
-  // plentiful), growing to kPlanPer for the largest batches (fewer LDS
-  // histogram flushes)
-  const int per = (int)std::min<int64_t>(gm::kPlanPer, std::max<int64_t>(1, n / (256 * 1024)));
-  const unsigned pblk = (unsigned)((n + 256 * (int64_t)per - 1) / (256 * (int64_t)per));
-  if (flags & GM_MIE_SORT) {
-    hipLaunchKernelGGL(gm::k_zero_i64, dim3((nb + 255) / 256), dim3(256), 0, s, hist, nb);
-    hipLaunchKernelGGL(gm::k_plan_keys, dim3(pblk), dim3(256), 0, s, n, x, y, eps, mu, eps2,
-                       (int32_t*)ctx->key.ptr, nmax_out, hist, status, per);
-    hipLaunchKernelGGL(gm::k_plan_scan, dim3(1), dim3(256), 0, s, hist, cursor, range);
-    hipLaunchKernelGGL(gm::k_plan_scatter, dim3(pblk), dim3(256), 0, s, n,
-                       (const int32_t*)ctx->key.ptr, cursor, order, per);
-  } else {
-    // input order: each class kernel walks every point and keeps its own
-    hipLaunchKernelGGL(gm::k_plan_identity, dim3(gblk), dim3(256), 0, s, n, order, range);
-    if (nmax_out) {
-      hipLaunchKernelGGL(gm::k_nmax_i32, dim3(gblk), dim3(256), 0, s, n, y ? y : x, nmax_out);
-    }
-  }
-  HIP_OK(hipGetLastError());
-
+  MieWs ws;
+  ws.order = (int64_t*)ctx->order.ptr;
+  ws.key = (int32_t*)ctx->key.ptr;
+  ws.hist = (int64_t*)ctx->hist.ptr;
+  ws.status = (int32_t*)ctx->status.ptr;
   const int rows_cap = std::max(1, gm_nmax(size_max));
-  const int dn_extra = (flags & GM_MIE_DN_CONVERGED) ? -1 : 0;
-  // Persistent grids: one wavefront per resident slot (occupancy measured per
-  // kernel at context creation), each owning a reused scratch slot.
-  const int64_t waves = (n + 63) / 64;
-  auto grid_for = [&](int occ, int row_doubles) {
-    const size_t slot_bytes = (size_t)rows_cap * 64 * row_doubles * sizeof(double);
-    int64_t g = std::min<int64_t>(
-        waves, (int64_t)ctx->n_cu * std::max(occ, 1) * std::max(ctx->tune_mie_grid_mult, 1));
-    g = std::min<int64_t>(g, std::max<int64_t>(1, (int64_t)(ctx->scratch_budget / slot_bytes)));
-    return std::make_pair(g, (size_t)g * slot_bytes);
-  };
-  const int var = ctx->tune_mie_waves == 4 ? 2 : ctx->tune_mie_waves == 3 ? 1 : 0;
-  auto gh = grid_for(ctx->occ_homog[var], gm::kHomogRowDoubles);
-  auto gs = grid_for(ctx->occ_stable[var], gm::kCoatedRowDoubles);
-  auto gl = grid_for(ctx->occ_literal, gm::kCoatedRowDoubles);
-  size_t need = gh.second;
-  if (y) need = std::max(need, std::max(gs.second, gl.second));
+  MieGrids g = mie_grids(ctx, n, rows_cap, ctx->scratch_budget);
+  size_t need = g.homog.second;
+  if (y) need = std::max(need, std::max(g.stable.second, g.literal.second));
   if ((rc = ctx->reserve(ctx->scratch, need))) return rc;
-  double* scratch = (double*)ctx->scratch.ptr;
-  size_t slot;
-  ctx->mark_begin(gm::kKMieHomog, s, &slot);
-  auto homog = var == 2 ? gm::k_mie_homog<4> : gm::k_mie_homog<1>;
-  hipLaunchKernelGGL(homog, dim3((unsigned)gh.first), dim3(64), 0, s, order, range, n, x, y, eps,
-                     mu, eps2, coef_off, an, bn, props, status, scratch, rows_cap, dn_extra);
-  ctx->mark_end(slot, s);
+  ws.slots = (double*)ctx->scratch.ptr;
+  return mie_batch_run(ctx, n, x, y, eps, mu, eps2, rows_cap, coef_off, an, bn, props, nmax_out,
+                       flags, ws, g, (hipStream_t)stream);
+}
+
+int gm_mie_batch_noalloc(gm_ctx* ctx, int64_t n, const double* x, const double* y,
+                         const double* eps, const double* mu, const double* eps2,
+                         double size_max, const int64_t* coef_off, double* an, double* bn,
+                         double* props, int32_t* nmax_out, uint32_t flags, int32_t* status,
+                         gm_stream stream) {
+  int rc = mie_batch_check(ctx, n, x, y, eps, eps2, size_max, coef_off, an, bn,
+                           "gm_mie_batch_noalloc");
+  if (rc) return rc;
+  if (!status) return gm_fail(GM_EINVAL, "gm_mie_batch_noalloc: status is NULL");
+  HIP_OK(hipSetDevice(ctx->device));
+  hipStream_t s = (hipStream_t)stream;
+  char* base = (char*)noalloc_scratch(ctx->device);
+  if (!base)
+    return gm_fail(GM_ERUNTIME, "gm_mie_batch_noalloc: no static scratch on device %d",
+                   ctx->device);
+  // the call's flags start at zero (a fill kernel: no memset node if captured)
+  hipLaunchKernelGGL(gm::k_zero_i32, dim3(1), dim3(64), 0, s, status, 1);
   HIP_OK(hipGetLastError());
-  if (y) {
-    ctx->mark_begin(gm::kKMieCoated, s, &slot);
-    auto stable = var == 2   ? gm::k_mie_coated_stable<4>
-                  : var == 1 ? gm::k_mie_coated_stable<3>
-                             : gm::k_mie_coated_stable<1>;
-    hipLaunchKernelGGL(stable, dim3((unsigned)gs.first), dim3(64), 0, s, order, range, n, x, y,
-                       eps, mu, eps2, coef_off, an, bn, props, status, scratch, rows_cap,
-                       dn_extra);
-    ctx->mark_end(slot, s);
-    HIP_OK(hipGetLastError());
-    ctx->mark_begin(gm::kKMieCoatedLit, s, &slot);
-    hipLaunchKernelGGL(gm::k_mie_coated_literal, dim3((unsigned)gl.first), dim3(64), 0, s, order,
-                       range, n, x, y, eps, mu, eps2, coef_off, an, bn, props, status, scratch,
-                       rows_cap, dn_extra);
-    ctx->mark_end(slot, s);
-    HIP_OK(hipGetLastError());
-  }
-  return GM_OK;
+  if (n == 0) return GM_OK;
+  // layout of the static array: plan histogram/cursors/ranges, sort keys,
+  // sorted order, then the recurrence slots in what is left
+  const int nb = gm::kClasses * gm::kBuckets;
+  const size_t hist_bytes = sizeof(int64_t) * (2 * nb + 2 * gm::kClasses);
+  auto up = [](size_t v) { return (v + 255) / 256 * 256; };
+  const size_t plan_bytes = up(hist_bytes) + up(sizeof(int32_t) * (size_t)n) +
+                            up(sizeof(int64_t) * (size_t)n);
+  const int rows_cap = std::max(1, gm_nmax(size_max));
+  const int row_doubles = y ? gm::kCoatedRowDoubles : gm::kHomogRowDoubles;
+  const size_t slot_bytes = (size_t)rows_cap * 64 * row_doubles * sizeof(double);
+  if (plan_bytes + slot_bytes > kNoAllocBytes)
+    return gm_fail(GM_EINVAL,
+                   "gm_mie_batch_noalloc: %lld points with series length %d need %zu bytes of "
+                   "scratch for the plan and one recurrence slot; the static scratch holds %zu",
+                   (long long)n, rows_cap, plan_bytes + slot_bytes, kNoAllocBytes);
+  MieWs ws;
+  ws.hist = (int64_t*)base;
+  ws.key = (int32_t*)(base + up(hist_bytes));
+  ws.order = (int64_t*)(base + up(hist_bytes) + up(sizeof(int32_t) * (size_t)n));
+  ws.status = status;
+  ws.slots = (double*)(base + plan_bytes);
+  MieGrids g = mie_grids(ctx, n, rows_cap, kNoAllocBytes - plan_bytes);
+  return mie_batch_run(ctx, n, x, y, eps, mu, eps2, rows_cap, coef_off, an, bn, props, nmax_out,
+                       flags, ws, g, s);
 }
 
 }  // extern "C"

@@ -15,6 +15,7 @@ constexpr int kClasses = 3;
 constexpr int kPlanPer = 16;
 
 __global__ void k_zero_i64(int64_t* p, int n);
+__global__ void k_zero_i32(int32_t* p, int n);
 __global__ void k_plan_keys(int64_t n, const double* x, const double* y, const double* eps,
                             const double* mu, const double* eps2, int32_t* key,
                             int32_t* nmax_out, int64_t* hist, int32_t* status, int per);

@@ -127,6 +127,10 @@ __global__ void k_zero_i64(int64_t* __restrict__ p, int n) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = 0;
 }
+__global__ void k_zero_i32(int32_t* __restrict__ p, int n) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
 
 // Block-local LDS histogram, then one global atomic per non-empty bucket.
 // Each block covers per * 256 points (per <= kPlanPer, chosen from n: large

@@ -139,6 +139,29 @@ int gm_mie_batch(gm_ctx* ctx, int64_t n, const double* x, const double* y, const
                  double* an, double* bn, double* props, int32_t* nmax_out, uint32_t flags,
                  gm_stream stream);
 
+/* gm_mie_batch without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ *
+ * Same arguments, outputs and kernels as gm_mie_batch, plus `status`: a
+ * caller-owned device int32 that the call clears and then ORs the status bits
+ * of gm_ctx_check into (bit 0: a point exceeded size_max; bit 1: a coated
+ * point with mu != 1).  The plan buffers and the recurrence slots live in one
+ * static device array of 128 MiB per GPU, held until the process ends; the
+ * call itself allocates nothing (ctx must exist: gm_ctx_create, which may
+ * allocate, takes the array's address on its device).  Batches whose plan
+ * (12 bytes per point plus 49 KiB) and one recurrence slot (n_max(size_max)
+ * x 64 lanes x 24 bytes, or 88 bytes when coated) exceed that array are
+ * rejected with GM_EINVAL; smaller slot budgets only shorten the persistent
+ * grid.  Every call on a GPU shares that GPU's array, which keeps the last
+ * call's contents: calls that overlap on one GPU (other streams, threads or
+ * contexts) race and must be serialised by the caller. */
+int gm_mie_batch_noalloc(gm_ctx* ctx, int64_t n, const double* x, const double* y,
+                         const double* eps, const double* mu, const double* eps2,
+                         double size_max, const int64_t* coef_off, double* an, double* bn,
+                         double* props, int32_t* nmax_out, uint32_t flags, int32_t* status,
+                         gm_stream stream);
+
 /* Efficiencies of given coefficient sets: qext, qsca, qabs, qb, asy, qratio
  * planes in props[6*n].  Replaces mie_props / mie_props_raw
  * (src/pymiecoated/pymiecoated/mie_props.py:28-75).  size[n] is the size
