@@ -14,6 +14,8 @@ Knobs (include/geosmie.h, enum gm_tune_param):
                  3 or 4 resident wavefronts per SIMD)
   s12_layout     FP64 S1/S2 blocks split angles (0) or points (1) among their waves,
                  or share LDS-staged angle tables among them (2)
+  mie_ckpt       coated kernel: downward-chain states kept every 4 or 8 rows and
+                 regenerated per block in LDS, or every row kept in scratch (0)
 (mlp_weight_terms, the BF16 terms per network weight, is a precision
 setting reached through the same call; it is not tuned for speed.)
 """
@@ -29,6 +31,7 @@ KNOBS = {"mie_grid_mult": (0, (1, 2, 4)),
          "s12_points": (4, (8, 16)),
          "mie_waves": (5, (0, 3, 4)),
          "s12_layout": (6, (0, 1, 2)),
+         "mie_ckpt": (7, (4, 8, 0)),
          # a precision setting rather than a shape; set explicitly, never tuned for speed
          "mlp_weight_terms": (3, (1, 2))}
 

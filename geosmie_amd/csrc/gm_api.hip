@@ -146,8 +146,21 @@ MieGrids mie_grids(const gm_ctx* ctx, int64_t n, int rows_cap, size_t budget) {
   const int var = ctx->tune_mie_waves == 4 ? 2 : ctx->tune_mie_waves == 3 ? 1 : 0;
   MieGrids g;
   g.homog = grid_for(ctx->occ_homog[var], gm::kHomogRowDoubles);
-  g.stable = grid_for(ctx->occ_stable[var], gm::kCoatedRowDoubles);
   g.literal = grid_for(ctx->occ_literal, gm::kCoatedRowDoubles);
+  const int ck = ctx->tune_mie_ckpt;
+  if (ck == 0) {
+    g.stable = grid_for(ctx->occ_stable[var], gm::kCoatedRowDoubles);
+  } else {
+    // a checkpointed slot: coated_ckpt_lane_doubles per lane (no per-row planes
+    // beyond the start-correction ones)
+    const size_t slot_bytes =
+        (size_t)64 * gm::coated_ckpt_lane_doubles(rows_cap, ck) * sizeof(double);
+    const int occ = ck == 8 ? ctx->occ_ckpt8 : ctx->occ_ckpt4[var ? 1 : 0];
+    int64_t gg = std::min<int64_t>(
+        waves, (int64_t)ctx->n_cu * std::max(occ, 1) * std::max(ctx->tune_mie_grid_mult, 1));
+    gg = std::min<int64_t>(gg, std::max<int64_t>(1, (int64_t)(budget / slot_bytes)));
+    g.stable = std::make_pair(gg, (size_t)gg * slot_bytes);
+  }
   return g;
 }
 
@@ -213,6 +226,10 @@ int mie_batch_run(gm_ctx* ctx, int64_t n, const double* x, const double* y, cons
     auto stable = var == 2   ? gm::k_mie_coated_stable<4>
                   : var == 1 ? gm::k_mie_coated_stable<3>
                              : gm::k_mie_coated_stable<1>;
+    if (ctx->tune_mie_ckpt == 4)
+      stable = var ? gm::k_mie_coated_ckpt<3, 4> : gm::k_mie_coated_ckpt<1, 4>;
+    else if (ctx->tune_mie_ckpt == 8)
+      stable = gm::k_mie_coated_ckpt<1, 8>;
     hipLaunchKernelGGL(stable, dim3((unsigned)g.stable.first), dim3(64), 0, s, order, range, n, x,
                        y, eps, mu, eps2, coef_off, an, bn, props, ws.status, ws.slots, rows_cap,
                        dn_extra);
@@ -269,6 +286,15 @@ int gm_ctx_create(int device, gm_ctx** out) {
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, gm::k_mie_coated_literal, 64, 0) ==
       hipSuccess)
     c->occ_literal = occ;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, gm::k_mie_coated_ckpt<1, 4>, 64, 0) ==
+      hipSuccess)
+    c->occ_ckpt4[0] = occ;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, gm::k_mie_coated_ckpt<3, 4>, 64, 0) ==
+      hipSuccess)
+    c->occ_ckpt4[1] = occ;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, gm::k_mie_coated_ckpt<1, 8>, 64, 0) ==
+      hipSuccess)
+    c->occ_ckpt8 = occ;
   int rc = noalloc_init(device);
   if (!rc) rc = c->reserve(c->status, 64);
   if (rc) {
@@ -335,6 +361,11 @@ int gm_ctx_tune(gm_ctx* ctx, int param, int value) {
         return gm_fail(GM_EINVAL, "gm_ctx_tune: Mie waves per SIMD must be 0, 3 or 4");
       ctx->tune_mie_waves = value;
       return GM_OK;
+    case GM_TUNE_MIE_CKPT:
+      if (value != 0 && value != 4 && value != 8)
+        return gm_fail(GM_EINVAL, "gm_ctx_tune: rows per checkpoint must be 0, 4 or 8");
+      ctx->tune_mie_ckpt = value;
+      return GM_OK;
     case GM_TUNE_MLP_WEIGHT_TERMS:
       if (value != 1 && value != 2)
         return gm_fail(GM_EINVAL, "gm_ctx_tune: MLP weight terms must be 1 or 2");
@@ -355,6 +386,7 @@ int gm_ctx_get_tune(gm_ctx* ctx, int param, int* value) {
     case GM_TUNE_S12_POINTS: *value = ctx->tune_s12_points; return GM_OK;
     case GM_TUNE_MIE_WAVES: *value = ctx->tune_mie_waves; return GM_OK;
     case GM_TUNE_S12_LAYOUT: *value = ctx->tune_s12_layout; return GM_OK;
+    case GM_TUNE_MIE_CKPT: *value = ctx->tune_mie_ckpt; return GM_OK;
     default: return gm_fail(GM_EINVAL, "gm_ctx_get_tune: unknown parameter %d", param);
   }
 }

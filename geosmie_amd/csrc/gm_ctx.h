@@ -22,6 +22,9 @@ struct gm_ctx {
   // resident Mie wavefronts per CU, per register-allocation variant (index
   // 0: unconstrained, 1: >= 3 waves per SIMD, 2: >= 4 waves per SIMD)
   int occ_homog[3] = {8, 8, 8}, occ_stable[3] = {8, 8, 8}, occ_literal = 4;
+  // resident wavefronts per CU of the checkpointed coated kernel: K = 4
+  // (unconstrained, >= 3 waves per SIMD) and K = 8
+  int occ_ckpt4[2] = {8, 8}, occ_ckpt8 = 5;
   size_t scratch_budget = (size_t)3 << 30;       // bytes of recurrence scratch
   // launch shapes chosen by gm_ctx_tune (an autotuner's knobs)
   int tune_mie_grid_mult = 1;   // persistent Mie grid = multiple of resident wavefronts
@@ -31,6 +34,7 @@ struct gm_ctx {
   int tune_mlp_wterms = 1;      // BF16 terms per MLP weight (1 = BF16 weights, 2 = hi + lo)
   int tune_mie_waves = 0;       // Mie register variant: 0 unconstrained, 3 or 4 waves per SIMD
   int tune_s12_layout = 0;      // FP64 S1/S2 block: waves split angles (0) or points (1)
+  int tune_mie_ckpt = 4;        // coated kernel: rows per checkpoint (4, 8) or 0 = every row stored
   Buf status;                                    // int32 flags + scan scratch
   Buf order, key, hist, scratch, scan_tmp;
   Buf tab_pi, tab_tau, gemm_tmp, feat_tmp, gsf_tab, tab_lp, lut_tmp, mlp_ws;

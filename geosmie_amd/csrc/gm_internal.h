@@ -35,6 +35,12 @@ __global__ void k_mie_coated_stable(const int64_t* order, const int64_t* range, 
                                     const double* mu, const double* eps2, const int64_t* coef_off,
                                     double* an, double* bn, double* props, int32_t* status,
                                     double* scratch, int rows_cap, int dn_extra);
+template <int MINW, int K>
+__global__ void k_mie_coated_ckpt(const int64_t* order, const int64_t* range, int64_t n_all,
+                                  const double* x, const double* y, const double* eps,
+                                  const double* mu, const double* eps2, const int64_t* coef_off,
+                                  double* an, double* bn, double* props, int32_t* status,
+                                  double* scratch, int rows_cap, int dn_extra);
 __global__ void k_mie_coated_literal(const int64_t* order, const int64_t* range, int64_t n_all,
                                      const double* x, const double* y, const double* eps,
                                      const double* mu, const double* eps2, const int64_t* coef_off,
@@ -45,5 +51,10 @@ __global__ void k_nmax_of(int64_t n, const double* size, int64_t* cnt);
 // Doubles of scratch per row per lane.
 constexpr int kHomogRowDoubles = 3;
 constexpr int kCoatedRowDoubles = 11;
+// Doubles per lane of a checkpointed coated slot: 7 per K-row block + the
+// two complex start-correction planes.
+inline int64_t coated_ckpt_lane_doubles(int rows_cap, int k) {
+  return 7 * (int64_t)((rows_cap + k - 1) / k) + 4 * (int64_t)rows_cap;
+}
 
 }  // namespace gm

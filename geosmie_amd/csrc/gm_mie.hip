@@ -603,6 +603,219 @@ __global__ __launch_bounds__(64, MINW) void k_mie_coated_stable(
 }
 
 // ---------------------------------------------------------------------------
+// Coated sphere, real or weakly absorbing shell, checkpointed downward chains
+// ---------------------------------------------------------------------------
+// The arithmetic of k_mie_coated_stable, with a different memory schedule.
+// Pass 1 runs the four downward chains (D_n(u) and the psi ratios of v, w,
+// y) over every row but stores only their states every K rows: checkpoint b
+// is the state after row bK+K+1, from which rows bK+K .. bK+1 follow.  Pass 2
+// walks the blocks upward: it reloads block b's checkpoint (the next one is
+// already in flight), regenerates the block's K rows into LDS with exactly
+// the operations of pass 1 (so every row is bit-identical to the value pass 1
+// computed), and consumes them in the K upward steps.  Scratch traffic per
+// point drops from up to 7 doubles per row written and read to 7 doubles per
+// K rows, and the upward pass no longer waits on scratch rows.
+// Slot layout: checkpoints [b][7][lane] (du.re, du.im, dv.re, dv.im, dw.re,
+// dw.im, dy), then the start-correction planes DDV, DDW [row][lane] (complex),
+// written only when the reference's start is not converged for v or w.
+template <int MINW, int K>
+__global__ __launch_bounds__(64, MINW) void k_mie_coated_ckpt(
+    const int64_t* __restrict__ order, const int64_t* __restrict__ range, int64_t n_all,
+    const double* __restrict__ x, const double* __restrict__ y, const double* __restrict__ eps,
+    const double* __restrict__ mu, const double* __restrict__ eps2,
+    const int64_t* __restrict__ coef_off, double* __restrict__ an, double* __restrict__ bn,
+    double* __restrict__ props, int32_t* __restrict__ status, double* __restrict__ scratch,
+    int rows_cap, int dn_extra) {
+  __shared__ double rows[K][7][64];
+  const int lane = threadIdx.x;
+  const int64_t begin = range[2], end = range[3];
+  const int bcap = (rows_cap + K - 1) / K;
+  double* slot = scratch + (size_t)blockIdx.x * (size_t)64 * (7 * (size_t)bcap + 4 * rows_cap);
+  double* CK = slot;                                          // [b][7][lane]
+  cd* DDV = reinterpret_cast<cd*>(slot + (size_t)7 * bcap * 64);
+  cd* DDW = DDV + (size_t)rows_cap * 64;
+
+  for (int64_t base = begin + (int64_t)blockIdx.x * 64; base < end;
+       base += (int64_t)gridDim.x * 64) {
+    const int64_t idx = base + lane;
+    if (idx >= end) continue;
+    const int64_t p = order[idx];
+    PointDesc d = describe(p, x, y, eps, mu, eps2);
+    if (d.cls != 1) continue;
+    const double yy = d.size, xx = d.xcore;
+    const int nmax = nmax_rule(yy);
+    if (nmax > rows_cap) {
+      atomicOr(status, 1);
+      continue;
+    }
+    const cd m1 = csqrt_(d.eps), m2 = csqrt_(d.eps2);
+    const cd m = cdiv(m2, m1);
+    const cd u = m1 * xx, v = m2 * xx, w = m2 * yy;
+    const double mxv = fmax(cabs_(m1 * yy), cabs_(w));
+    const int nmx =
+        nmx_rule(nmax, mxv) + (dn_extra < 0 ? 16 + 10 * (int)ceil(cbrt(mxv)) : dn_extra);
+    const double av = cabs_(v), aw = cabs_(w);
+    const int nsw_v = min((int)av, nmax), nsw_w = min((int)aw, nmax), nsw_y = min((int)yy, nmax);
+    const cdd ui = crcp_dd(u), vi = crcp_dd(v), wi = crcp_dd(w);
+    const ddr yi = rcp_dd(yy);
+    const int sv = nsw_v < nmax ? ratio_start(nmax, av) : 0;   // ratio rows (nsw, s]
+    const int sw = nsw_w < nmax ? ratio_start(nmax, aw) : 0;
+    const int sy = nsw_y < nmax ? ratio_start(nmax, yy) : 0;
+    const int nblk = (nmax + K - 1) / K;
+    // ---- pass 1: the four downward chains, checkpointed every K rows
+    {
+      const int top = max(max(nmx - 1, sv), max(sw, sy));
+      cd du = cd{0.0, 0.0}, dv = cd{0.0, 0.0}, dw = cd{0.0, 0.0};
+      double dy = 0.0;
+      for (int k = top; k >= 1; --k) {
+        if (k <= nmx - 1) {   // D_k(u) = (k+1)/u - 1/(D_{k+1} + (k+1)/u), D_nmx = 0
+          const cd r = mul_dd((double)(k + 1), ui);
+          du = r - crcp_fast(du + r);
+        }
+        if (k <= sv && k > nsw_v) {
+          const cd r = mul_dd((double)k, vi);
+          dv = r - crcp_fast(dv + r);
+        }
+        if (k <= sw && k > nsw_w) {
+          const cd r = mul_dd((double)k, wi);
+          dw = r - crcp_fast(dw + r);
+        }
+        if (k <= sy && k > nsw_y) {
+          const double r = mul_dd((double)k, yi);
+          dy = r - rcp_nr(dy + r);
+        }
+        const int b = (k - 1) / K - 1;      // state after row bK+K+1
+        if ((k - 1) % K == 0 && b >= 0 && b < nblk) {
+          double* c = CK + (size_t)b * 7 * 64 + lane;
+          c[0] = du.re; c[64] = du.im; c[128] = dv.re; c[192] = dv.im;
+          c[256] = dw.re; c[320] = dw.im; c[384] = dy;
+        }
+      }
+    }
+    const bool ddv_on = !converged_start(nmx, av), ddw_on = !converged_start(nmx, aw);
+    if (ddv_on) {
+      dn_chain(vi, nmx, nmax, DDV, lane, false);
+      dn_chain(vi, nmx + 16 + 8 * (int)ceil(cbrt(av)), nmax, DDV, lane, true);
+    }
+    if (ddw_on) {
+      dn_chain(wi, nmx, nmax, DDW, lane, false);
+      dn_chain(wi, nmx + 16 + 8 * (int)ceil(cbrt(aw)), nmax, DDW, lane, true);
+    }
+    // ---- pass 2: per block, regenerate K rows into LDS, then K upward steps
+    const cdd vinv = vi, winv = wi, minv = crcp_dd(m), m2inv = crcp_dd(m2);
+    const ddr yinv = yi;
+    cd pv_pp = ccos_(v), pv_p = csin_(v);   // psi_{-1}(v), psi_0(v)
+    cd cv_pp = -pv_p, cv_p = pv_pp;         // chi_{-1}(v), chi_0(v)
+    cd pw_pp = ccos_(w), pw_p = csin_(w);
+    cd cw_pp = -pw_p, cw_p = pw_pp;
+    double sn, cs;
+    sincos(yy, &sn, &cs);
+    double py_pp = cs, py_p = sn, cy_pp = -sn, cy_p = cs;
+    Props pr;
+    pr.init();
+    const bool store = coef_off != nullptr;
+    const int64_t off = store ? coef_off[p] : 0;
+    double ck[7], ckn[7];
+#pragma unroll
+    for (int q = 0; q < 7; ++q) ckn[q] = CK[q * 64 + lane];
+    for (int b = 0; b < nblk; ++b) {
+#pragma unroll
+      for (int q = 0; q < 7; ++q) ck[q] = ckn[q];
+      if (b + 1 < nblk) {
+#pragma unroll
+        for (int q = 0; q < 7; ++q) ckn[q] = CK[((size_t)(b + 1) * 7 + q) * 64 + lane];
+      }
+      {
+        cd du = cd{ck[0], ck[1]}, dv = cd{ck[2], ck[3]}, dw = cd{ck[4], ck[5]};
+        double dy = ck[6];
+#pragma unroll
+        for (int kk = K - 1; kk >= 0; --kk) {
+          const int k = b * K + kk + 1;
+          if (k <= nmx - 1) {
+            const cd r = mul_dd((double)(k + 1), ui);
+            du = r - crcp_fast(du + r);
+          }
+          rows[kk][0][lane] = du.re;
+          rows[kk][1][lane] = du.im;
+          if (k <= sv && k > nsw_v) {
+            const cd r = mul_dd((double)k, vi);
+            const cd it = crcp_fast(dv + r);
+            rows[kk][2][lane] = it.re;
+            rows[kk][3][lane] = it.im;
+            dv = r - it;
+          }
+          if (k <= sw && k > nsw_w) {
+            const cd r = mul_dd((double)k, wi);
+            const cd it = crcp_fast(dw + r);
+            rows[kk][4][lane] = it.re;
+            rows[kk][5][lane] = it.im;
+            dw = r - it;
+          }
+          if (k <= sy && k > nsw_y) {
+            const double r = mul_dd((double)k, yi);
+            const double it = rcp_nr(dy + r);
+            rows[kk][6][lane] = it;
+            dy = r - it;
+          }
+        }
+      }
+      const int kend = min(K, nmax - b * K);
+      for (int kk = 0; kk < kend; ++kk) {
+        const int n = b * K + kk + 1;
+        const size_t o = (size_t)(n - 1) * 64 + lane;
+        const cd du = cd{rows[kk][0][lane], rows[kk][1][lane]};
+        const double fn = (double)(2 * n - 1), dn = (double)n;
+        const cd fv = mul_dd(fn, vinv), fw = mul_dd(fn, winv);
+        const cd pv = (n <= nsw_v) ? fv * pv_p - pv_pp
+                                   : pv_p * cd{rows[kk][2][lane], rows[kk][3][lane]};
+        const cd chv = fv * cv_p - cv_pp;
+        const cd pw = (n <= nsw_w) ? fw * pw_p - pw_pp
+                                   : pw_p * cd{rows[kk][4][lane], rows[kk][5][lane]};
+        const cd chw = fw * cw_p - cw_pp;
+        const double f = mul_dd(fn, yinv);
+        const double py = (n <= nsw_y) ? fma(f, py_p, -py_pp) : py_p * rows[kk][6][lane];
+        const double chy = fma(f, cy_p, -cy_pp);
+        const cd dpv = pv_p - mul_dd(dn, vinv) * pv;
+        const cd nw = mul_dd(dn, winv);
+        const cd dpw = pw_p - nw * pw, dcw = cw_p - nw * chw;
+        const cd t = du * pv;
+        cd U = m * t - dpv, V = mul_dd(t, minv) - dpv;
+        if (ddv_on) {
+          const cd c = DDV[o] * pv;
+          U = U - c;
+          V = V - c;
+        }
+        const cd A = pw * chv - chw * pv, B = dpw * chv - dcw * pv;
+        cd dns = cdiv(dpw + U * B, pw + U * A);
+        cd gns = cdiv(dpw + V * B, pw + V * A);
+        if (ddw_on) {
+          const cd c = DDW[o];
+          dns = dns + c;
+          gns = gns + c;
+        }
+        const double nrat = mul_dd(dn, yinv);
+        const cd a1 = mul_dd(dns, m2inv) + nrat;
+        const cd b1 = m2 * gns + nrat;
+        const cd gsy = cd{py, -chy}, gs1y = cd{py_p, -cy_p};
+        const cd a = cdiv(a1 * py - py_p, gsy * a1 - gs1y);
+        const cd bb = cdiv(b1 * py - py_p, gsy * b1 - gs1y);
+        pr.add(n, a, bb);
+        if (store) {
+          an[2 * (off + n - 1)] = a.re;
+          an[2 * (off + n - 1) + 1] = a.im;
+          bn[2 * (off + n - 1)] = bb.re;
+          bn[2 * (off + n - 1) + 1] = bb.im;
+        }
+        pv_pp = pv_p; pv_p = pv; cv_pp = cv_p; cv_p = chv;
+        pw_pp = pw_p; pw_p = pw; cw_pp = cw_p; cw_p = chw;
+        py_pp = py_p; py_p = py; cy_pp = cy_p; cy_p = chy;
+      }
+    }
+    if (props) pr.finish(yy, p, n_all, props);
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Coated sphere, mie_coeffs.py:183-251 + mie_props.py:28-70 (size = y), literal
 // form for absorbing shells (|Im m2| y > 1)
 // ---------------------------------------------------------------------------
@@ -788,6 +1001,9 @@ template __global__ void k_mie_homog<4>(GM_MIE_ARGS);
 template __global__ void k_mie_coated_stable<1>(GM_MIE_ARGS);
 template __global__ void k_mie_coated_stable<3>(GM_MIE_ARGS);
 template __global__ void k_mie_coated_stable<4>(GM_MIE_ARGS);
+template __global__ void k_mie_coated_ckpt<1, 4>(GM_MIE_ARGS);
+template __global__ void k_mie_coated_ckpt<3, 4>(GM_MIE_ARGS);
+template __global__ void k_mie_coated_ckpt<1, 8>(GM_MIE_ARGS);
 #undef GM_MIE_ARGS
 
 __global__ void k_nmax_of(int64_t n, const double* __restrict__ size, int64_t* __restrict__ cnt) {

@@ -87,7 +87,11 @@ enum gm_tune_param {
   GM_TUNE_S12_LAYOUT = 6        /* FP64 S1/S2 blocks: 0 = the block's wavefronts take different
                                    angles of the same points (default), 1 = different points
                                    of the same 64 angles (angle-table rows shared in cache),
-                                   2 = as 1 with the angle tables staged through LDS */
+                                   2 = as 1 with the angle tables staged through LDS */,
+  GM_TUNE_MIE_CKPT = 7          /* coated kernel (real / weakly absorbing shells): the downward
+                                   chains' states kept every 4 or 8 rows and each block of rows
+                                   regenerated in LDS before use (4, the default), or 0 = every
+                                   row kept in scratch; results are bitwise the same */
 };
 int gm_ctx_tune(gm_ctx* ctx, int param, int value);
 int gm_ctx_get_tune(gm_ctx* ctx, int param, int* value);

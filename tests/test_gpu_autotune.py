@@ -86,3 +86,37 @@ def test_tune_picks_a_candidate(gpu):
         assert autotune.get(ch.ctx, name) == r_["best"]
     with pytest.raises(ValueError):
         autotune.set_knob(ch.ctx, "s12_waves", 3)
+
+
+def test_coated_checkpoint_variants_agree(gpu):
+    """The checkpointed coated kernel (chain states kept every 4 or 8 rows,
+    each block of rows regenerated in LDS) against the kernel that keeps every
+    row in scratch: the regenerated rows repeat pass 1's operations from the
+    same states, so the results agree to the last bits (asserted at 1e-12, and
+    the bitwise outcome printed), including points that need the start
+    correction (|m2| >= |m1|) and stored coefficients."""
+    from geosmie_amd import _lib, autotune
+    from geosmie_amd.mie import mie_batch
+    h = _lib.ctx()
+    r = np.random.default_rng(5)
+    y = np.exp(r.uniform(np.log(0.05), np.log(209.0), 4000))
+    x = y * r.uniform(0.3, 0.95, y.size)
+    m1 = np.where(r.uniform(size=y.size) < 0.5, 1.95 + 0.79j, 1.33 + 1e-4j)
+    m2 = np.where(m1.real > 1.5, 1.45 + 0j, 1.5 + 1e-3j)
+    res = {}
+    try:
+        for ck in (0, 4, 8):
+            autotune.set_knob(h, "mie_ckpt", ck)
+            res[ck] = mie_batch(x, m=m1, y=y, m2=m2, coeffs=True)
+    finally:
+        autotune.set_knob(h, "mie_ckpt", 4)
+    for ck in (4, 8):
+        same = True
+        for k in ("qext", "qsca", "asy", "qb"):
+            a, b = res[0][k], res[ck][k]
+            same &= bool(np.array_equal(a, b))
+            assert np.all(np.abs(a - b) <= 1e-12 * np.abs(a)), (ck, k)
+        for a, b in zip(res[0]["an"], res[ck]["an"]):
+            same &= bool(np.array_equal(a, b))
+            assert np.all(np.abs(a - b) <= 1e-12 * np.abs(a).max()), ck
+        print("checkpoint every %d rows: bitwise identical to the full-row kernel: %s" % (ck, same))
