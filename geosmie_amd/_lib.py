@@ -44,6 +44,10 @@ SIGNATURES = {
                                     c_vp, c_vp, c_vp, c_vp, c_u32, c_vp]),
     "gm_mie_batch_noalloc": (ctypes.c_int, [c_vp, c_i64, c_vp, c_vp, c_vp, c_vp, c_vp, c_dbl,
                                             c_vp, c_vp, c_vp, c_vp, c_vp, c_u32, c_vp, c_vp]),
+    "gm_shape_average": (ctypes.c_int, [c_vp, ctypes.c_int, c_i64, c_vp, c_vp, c_vp, c_vp]),
+    "gm_ri_bilinear": (ctypes.c_int, [c_vp, ctypes.c_int, ctypes.c_int, c_i64, c_vp, c_i64, c_vp,
+                                      c_vp, c_vp, c_vp]),
+    "gm_pmatrix_normalize6": (ctypes.c_int, [c_vp, c_i64, ctypes.c_int, c_vp, c_vp, c_vp, c_vp]),
     "gm_angle_tables": (ctypes.c_int, [c_vp, ctypes.c_int, c_vp, ctypes.c_int, c_vp, c_vp,
                                        c_vp]),
     "gm_mie_s12": (ctypes.c_int, [c_vp, c_i64, c_vp, c_vp, c_vp, c_vp, ctypes.c_int, c_vp, c_vp,

@@ -13,12 +13,12 @@ INCLUDE = os.path.join(os.path.dirname(HERE), "include")
 
 SOURCES = ["gm_api.hip", "gm_util.hip", "gm_mie.hip", "gm_angles.hip", "gm_s12.hip",
            "gm_integrate.hip", "gm_gsf.hip", "gm_lut.hip", "gm_mlp.hip",
-           "gm_surrogate.hip", "gm_band.hip"]
+           "gm_surrogate.hip", "gm_band.hip", "gm_grasp.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 # sources whose results must match numpy bit for bit: no fused multiply-add
-NO_CONTRACT = {"gm_band.hip"}
+NO_CONTRACT = {"gm_band.hip", "gm_grasp.hip"}
 
 # -packed-fp32-ops: no packed FP32 instructions from the compiler (some of their
 # forms have given wrong results on these GPUs without an error)

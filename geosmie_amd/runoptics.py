@@ -20,7 +20,8 @@ and the runoptics command (src/geosmie/runoptics.py):
 
 Mie mode ('mode': 'mie') with psd types lognorm, ss and du (the du grid is
 the reference's non-monotonic one, see particle.initializeXarr); GRASP
-kernel files ('mode': 'kernel') are not part of this path.
+kernel mode ('mode': 'kernel') is geosmie_amd.grasp.build_kernel_table, which
+build_table dispatches to.
 """
 import ctypes
 import os
@@ -149,15 +150,23 @@ def _normalize(pm, ang):
     return d_p.cpu().numpy().reshape(pm.shape), d_b.cpu().numpy().reshape(-1, 6)
 
 
-def build_table(params, water=None, classic=False, precision=0, ang=None, n_mom=0):
+def build_table(params, water=None, classic=False, precision=0, ang=None, n_mom=0,
+                kernels=None):
     """An OpticsTable for a particle description (getPPJSON's dict with
     'mList'); water = getWaterM() array (needed unless rhDep is 'trivial').
     n_mom > 0 adds pmom, the GSF moments of every entry's normalised phase
     matrix (the rungsf step, src/gsf/convertncdf.py:170-189, 369-394; 129 in
-    the reference), computed on the device for all rh of a wavelength at once."""
+    the reference), computed on the device for all rh of a wavelength at once.
+    'mode': 'kernel' builds the table from GRASP kernel tables instead
+    (geosmie_amd.grasp; kernels = a grasp.KernelTables, or read from
+    params['kernel_params'])."""
     mode = params.get("mode", "mie")
+    if mode == "kernel":
+        from .grasp import build_kernel_table
+        return build_kernel_table(params, water=water, classic=classic, n_mom=n_mom,
+                                  kernels=kernels, ang=ang)
     if mode != "mie":
-        raise ValueError("mode %s is not supported (Mie tables only)" % mode)
+        raise ValueError("mode %s is not supported (mie or kernel)" % mode)
     kind = params["psd"]["type"]
     pp = params["psd"]["params"]
     if kind == "lognorm":

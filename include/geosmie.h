@@ -416,6 +416,34 @@ int gm_surrogate_targets(gm_ctx* ctx, int64_t n, int n_out, const float* q, doub
                          const float* rows, const float* tgt_mean, const float* tgt_std, float* y,
                          gm_stream stream);
 
+/* ---- GRASP kernel mode (runoptics 'mode': 'kernel') ------------------- */
+
+/* Shape averaging of kernel tables: out[j] = sum_r w[r] data[r][j] over
+ * n_shape shapes (r ascending), data [n_shape][n_elem], w [n_shape] device
+ * weights.  Replaces integrateShapes (src/geosmie/dointegration.py:74-91),
+ * whose weights are the shape fractions normalised to sum 1. */
+int gm_shape_average(gm_ctx* ctx, int n_shape, int64_t n_elem, const double* w,
+                     const double* data, double* out, gm_stream stream);
+
+/* Bilinear interpolation in refractive index of a table [n_mr][n_mi][n_elem]
+ * at n_q queries: idx[2q], idx[2q+1] = the bracketing (m_r, m_i) indices,
+ * w[4q..4q+3] = the weights of the corners (i, j), (i+1, j), (i+1, j+1),
+ * (i, j+1); out[q][k] = ((w0 f(i,j) + w1 f(i+1,j)) + w3 f(i,j+1)) + w2 f(i+1,j+1),
+ * each product rounded, as numpy evaluates bilinear_interpolation
+ * (dointegration.py:29-38).  Indices and weights come from get_interpolated's
+ * rules (:40-72: bisect, clamped indices, clipped index), formed on the host.
+ * idx, w, table, out on the device; n_q <= 65535. */
+int gm_ri_bilinear(gm_ctx* ctx, int n_mr, int n_mi, int64_t n_elem, const double* table,
+                   int64_t n_q, const int32_t* idx, const double* w, double* out,
+                   gm_stream stream);
+
+/* P normalisation of six independent elements, pmat [e][6][n_ang] in the
+ * order p11, p12, p22, p33, p34, p44 (kernel mode: p22 != p11, p44 != p33),
+ * in place, and pback [e][6] in the table's order p11, p12, p33, p34, p22,
+ * p44.  Replaces dointegration.py:978-988 for the kernel path. */
+int gm_pmatrix_normalize6(gm_ctx* ctx, int64_t n_entries, int n_ang, const double* theta_rad,
+                          double* pmat, double* pback, gm_stream stream);
+
 #ifdef __cplusplus
 }
 #endif
