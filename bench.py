@@ -218,7 +218,7 @@ def _secondary_homog(steps, warmup, autotune, RunopticsChain, cpu=True):
     pipe = RunopticsChain(x, m, ang, w, lam, [(i * 2000, 2000) for i in range(10)])
     # the FP64 step fuses S1/S2 with the row integrals: only the Mie kernel's
     # register budget is a knob of it
-    tuned = autotune.tune(pipe.ctx, pipe.step, knobs=("mie_waves",), reps=3)
+    tuned = autotune.tune(pipe.ctx, pipe.step, knobs=("mie_lanes", "mie_waves"), reps=3)
     ms_eager, kt, ms = time_pipeline(pipe, steps, warmup)
     s12_flop = 16.0 * pipe.sum_nmax * ang.size
     out = {"workload": "plain homogeneous FP64 Mie, 2000 radii 0.005-20 um x 10 wavelengths "
@@ -267,7 +267,7 @@ def _secondary_largex(steps, warmup, autotune, RunopticsChain):
         w[i, i * 5000:(i + 1) * 5000] = p / p.sum()
     pipe = RunopticsChain(x, m, ang, w, lam_row, [(i * 5000, 5000) for i in range(8)],
                           precision=0, gsf_ng=300)
-    tuned3 = autotune.tune(pipe.ctx, pipe.step, knobs=("mie_waves",), reps=2)
+    tuned3 = autotune.tune(pipe.ctx, pipe.step, knobs=("mie_lanes", "mie_waves"), reps=2)
     s12_flop = 16.0 * pipe.sum_nmax * ang.size
     modes = []
     ref = None

@@ -16,6 +16,8 @@ Knobs (include/geosmie.h, enum gm_tune_param):
                  or share LDS-staged angle tables among them (2)
   mie_ckpt       coated kernel: downward-chain states kept every 4 or 8 rows and
                  regenerated per block in LDS, or every row kept in scratch (0)
+  mie_lanes      homogeneous kernel: lanes per point (1, or 16/32/64 with the serial
+                 recurrences joined by parallel scans; 0 = chosen by batch)
 (mlp_weight_terms, the BF16 terms per network weight, is a precision
 setting reached through the same call; it is not tuned for speed.)
 """
@@ -32,6 +34,7 @@ KNOBS = {"mie_grid_mult": (0, (1, 2, 4)),
          "mie_waves": (5, (0, 3, 4)),
          "s12_layout": (6, (0, 1, 2)),
          "mie_ckpt": (7, (4, 8, 0)),
+         "mie_lanes": (8, (0, 1, 16, 32, 64)),
          # a precision setting rather than a shape; set explicitly, never tuned for speed
          "mlp_weight_terms": (3, (1, 2))}
 

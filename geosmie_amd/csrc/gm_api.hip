@@ -132,7 +132,18 @@ struct MieWs {
 // multiple, capped by the waves of work and by the slots the budget holds.
 struct MieGrids {
   std::pair<int64_t, size_t> homog, stable, literal;
+  int lanes = 1;      // homogeneous kernel: lanes per point
+  int lcap = 0;       // rows per lane segment (lanes > 1)
 };
+
+// Lanes per point of the homogeneous kernel: the tuned value, or by batch:
+// one point per lane once the batch fills the GPU (64 points per wavefront,
+// four per SIMD), otherwise P lanes per point, more for longer series.
+int mie_lanes(const gm_ctx* ctx, int64_t n, int rows_cap) {
+  if (ctx->tune_mie_lanes) return ctx->tune_mie_lanes;
+  if (n >= (int64_t)64 * ctx->n_cu * 4 * 4) return 1;
+  return rows_cap <= 300 ? 16 : rows_cap <= 1200 ? 32 : 64;
+}
 
 MieGrids mie_grids(const gm_ctx* ctx, int64_t n, int rows_cap, size_t budget) {
   const int64_t waves = (n + 63) / 64;
@@ -145,7 +156,19 @@ MieGrids mie_grids(const gm_ctx* ctx, int64_t n, int rows_cap, size_t budget) {
   };
   const int var = ctx->tune_mie_waves == 4 ? 2 : ctx->tune_mie_waves == 3 ? 1 : 0;
   MieGrids g;
-  g.homog = grid_for(ctx->occ_homog[var], gm::kHomogRowDoubles);
+  g.lanes = mie_lanes(ctx, n, rows_cap);
+  if (g.lanes == 1) {
+    g.homog = grid_for(ctx->occ_homog[var], gm::kHomogRowDoubles);
+  } else {
+    g.lcap = (rows_cap + g.lanes - 1) / g.lanes;
+    const size_t slot_bytes = (size_t)g.lcap * 64 * gm::kHomogRowDoubles * sizeof(double);
+    const int64_t per_wave = 64 / g.lanes;
+    int64_t gg = std::min<int64_t>((n + per_wave - 1) / per_wave,
+                                   (int64_t)ctx->n_cu * std::max(ctx->occ_coop, 1) *
+                                       std::max(ctx->tune_mie_grid_mult, 1));
+    gg = std::min<int64_t>(gg, std::max<int64_t>(1, (int64_t)(budget / slot_bytes)));
+    g.homog = std::make_pair(gg, (size_t)gg * slot_bytes);
+  }
   g.literal = grid_for(ctx->occ_literal, gm::kCoatedRowDoubles);
   const int ck = ctx->tune_mie_ckpt;
   if (ck == 0) {
@@ -216,8 +239,14 @@ int mie_batch_run(gm_ctx* ctx, int64_t n, const double* x, const double* y, cons
   size_t slot;
   ctx->mark_begin(gm::kKMieHomog, s, &slot);
   auto homog = var == 2 ? gm::k_mie_homog<4> : gm::k_mie_homog<1>;
+  int cap_arg = rows_cap;
+  if (g.lanes > 1) {
+    homog = g.lanes == 16 ? gm::k_mie_homog_coop<16>
+            : g.lanes == 32 ? gm::k_mie_homog_coop<32> : gm::k_mie_homog_coop<64>;
+    cap_arg = g.lcap;
+  }
   hipLaunchKernelGGL(homog, dim3((unsigned)g.homog.first), dim3(64), 0, s, order, range, n, x, y,
-                     eps, mu, eps2, coef_off, an, bn, props, ws.status, ws.slots, rows_cap,
+                     eps, mu, eps2, coef_off, an, bn, props, ws.status, ws.slots, cap_arg,
                      dn_extra);
   ctx->mark_end(slot, s);
   HIP_OK(hipGetLastError());
@@ -295,6 +324,9 @@ int gm_ctx_create(int device, gm_ctx** out) {
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, gm::k_mie_coated_ckpt<1, 8>, 64, 0) ==
       hipSuccess)
     c->occ_ckpt8 = occ;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, gm::k_mie_homog_coop<16>, 64, 0) ==
+      hipSuccess)
+    c->occ_coop = occ;
   int rc = noalloc_init(device);
   if (!rc) rc = c->reserve(c->status, 64);
   if (rc) {
@@ -366,6 +398,11 @@ int gm_ctx_tune(gm_ctx* ctx, int param, int value) {
         return gm_fail(GM_EINVAL, "gm_ctx_tune: rows per checkpoint must be 0, 4 or 8");
       ctx->tune_mie_ckpt = value;
       return GM_OK;
+    case GM_TUNE_MIE_LANES:
+      if (value != 0 && value != 1 && value != 16 && value != 32 && value != 64)
+        return gm_fail(GM_EINVAL, "gm_ctx_tune: lanes per point must be 0, 1, 16, 32 or 64");
+      ctx->tune_mie_lanes = value;
+      return GM_OK;
     case GM_TUNE_MLP_WEIGHT_TERMS:
       if (value != 1 && value != 2)
         return gm_fail(GM_EINVAL, "gm_ctx_tune: MLP weight terms must be 1 or 2");
@@ -387,6 +424,7 @@ int gm_ctx_get_tune(gm_ctx* ctx, int param, int* value) {
     case GM_TUNE_MIE_WAVES: *value = ctx->tune_mie_waves; return GM_OK;
     case GM_TUNE_S12_LAYOUT: *value = ctx->tune_s12_layout; return GM_OK;
     case GM_TUNE_MIE_CKPT: *value = ctx->tune_mie_ckpt; return GM_OK;
+    case GM_TUNE_MIE_LANES: *value = ctx->tune_mie_lanes; return GM_OK;
     default: return gm_fail(GM_EINVAL, "gm_ctx_get_tune: unknown parameter %d", param);
   }
 }

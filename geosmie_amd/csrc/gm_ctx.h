@@ -25,6 +25,8 @@ struct gm_ctx {
   // resident wavefronts per CU of the checkpointed coated kernel: K = 4
   // (unconstrained, >= 3 waves per SIMD) and K = 8
   int occ_ckpt4[2] = {8, 8}, occ_ckpt8 = 5;
+  int occ_coop = 8;             // the homogeneous kernel with P lanes per point
+
   size_t scratch_budget = (size_t)3 << 30;       // bytes of recurrence scratch
   // launch shapes chosen by gm_ctx_tune (an autotuner's knobs)
   int tune_mie_grid_mult = 1;   // persistent Mie grid = multiple of resident wavefronts
@@ -35,6 +37,7 @@ struct gm_ctx {
   int tune_mie_waves = 0;       // Mie register variant: 0 unconstrained, 3 or 4 waves per SIMD
   int tune_s12_layout = 0;      // FP64 S1/S2 block: waves split angles (0) or points (1)
   int tune_mie_ckpt = 4;        // coated kernel: rows per checkpoint (4, 8) or 0 = every row stored
+  int tune_mie_lanes = 0;       // homogeneous kernel: lanes per point (1, 16, 32, 64; 0 = by batch)
   Buf status;                                    // int32 flags + scan scratch
   Buf order, key, hist, scratch, scan_tmp;
   Buf tab_pi, tab_tau, gemm_tmp, feat_tmp, gsf_tab, tab_lp, lut_tmp, mlp_ws;

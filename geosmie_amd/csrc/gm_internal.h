@@ -35,6 +35,12 @@ __global__ void k_mie_coated_stable(const int64_t* order, const int64_t* range, 
                                     const double* mu, const double* eps2, const int64_t* coef_off,
                                     double* an, double* bn, double* props, int32_t* status,
                                     double* scratch, int rows_cap, int dn_extra);
+template <int P>
+__global__ void k_mie_homog_coop(const int64_t* order, const int64_t* range, int64_t n_all,
+                                 const double* x, const double* y, const double* eps,
+                                 const double* mu, const double* eps2, const int64_t* coef_off,
+                                 double* an, double* bn, double* props, int32_t* status,
+                                 double* scratch, int lcap, int dn_extra);
 template <int MINW, int K>
 __global__ void k_mie_coated_ckpt(const int64_t* order, const int64_t* range, int64_t n_all,
                                   const double* x, const double* y, const double* eps,

@@ -405,6 +405,367 @@ __global__ __launch_bounds__(64, MINW) void k_mie_homog(
 }
 
 // ---------------------------------------------------------------------------
+// Homogeneous sphere, P lanes per point (small batches)
+// ---------------------------------------------------------------------------
+// The lane-per-point kernel above needs thousands of points to fill the GPU,
+// and its time is then the serial chain of the longest point.  Here each
+// point's rows 1..nmax are split into P contiguous segments, one per lane of
+// a P-lane group (64/P points per wavefront), and every serial chain of the
+// algorithm is joined across the segments by a parallel scan of its
+// transfer matrices:
+//   * D_n downward (mie_coeffs.py:118-121): D_{k} = r - 1/(D_{k+1} + r),
+//     r = (k+1)/z, is the Moebius map [[r, r^2-1], [1, r]] on (num, den);
+//     the rows above nmax (down from the reference's start nmx, D = 0) are a
+//     head split over the lanes as well.  A suffix scan of the segment
+//     matrices gives each lane the D entering its segment from above, from
+//     which it runs the recurrence over its own rows (the same operations as
+//     the serial kernel) into its scratch rows.
+//   * the psi ratio chain above n > x, the same form with r = k/x (real).
+//   * psi_n (n <= x) and chi_n upward, psi_{n} = f psi_{n-1} - psi_{n-2},
+//     f = (2n-1)/x: a prefix scan of [[f, -1], [1, 0]] gives each lane the
+//     pair entering its segment.  Above x, psi_n = psi_x * prod R_k: the
+//     psi at x comes from the lane that holds row x, the products of R from
+//     a prefix-product scan.
+// Each lane then forms a_n, b_n for its rows and partial efficiency sums,
+// reduced over the group in a fixed order.  Matrix products are rescaled by
+// powers of two (exact) to stay in range.  Against the serial kernel the
+// results differ only by rounding (segment boundaries are reached by matrix
+// products instead of the chain), checked against mpmath to x = 2094.
+struct cm2 {
+  cd a, b, c, d;   // [[a, b], [c, d]]
+};
+struct rm2 {
+  double a, b, c, d;
+};
+__device__ __forceinline__ cm2 operator*(const cm2& u, const cm2& v) {
+  return cm2{u.a * v.a + u.b * v.c, u.a * v.b + u.b * v.d, u.c * v.a + u.d * v.c,
+             u.c * v.b + u.d * v.d};
+}
+__device__ __forceinline__ rm2 operator*(const rm2& u, const rm2& v) {
+  return rm2{fma(u.a, v.a, u.b * v.c), fma(u.a, v.b, u.b * v.d), fma(u.c, v.a, u.d * v.c),
+             fma(u.c, v.b, u.d * v.d)};
+}
+__device__ __forceinline__ cm2 cm2_eye() {
+  return cm2{cd{1.0, 0.0}, cd{0.0, 0.0}, cd{0.0, 0.0}, cd{1.0, 0.0}};
+}
+__device__ __forceinline__ rm2 rm2_eye() { return rm2{1.0, 0.0, 0.0, 1.0}; }
+// scale by a power of two so the largest component is in [0.5, 1) (exact)
+__device__ __forceinline__ void renorm(cm2& m) {
+  double mx = fmax(fmax(fmax(fabs(m.a.re), fabs(m.a.im)), fmax(fabs(m.b.re), fabs(m.b.im))),
+                   fmax(fmax(fabs(m.c.re), fabs(m.c.im)), fmax(fabs(m.d.re), fabs(m.d.im))));
+  if (mx == 0.0) return;
+  const int e = __builtin_amdgcn_frexp_exp(mx);
+  m.a = cd{ldexp(m.a.re, -e), ldexp(m.a.im, -e)};
+  m.b = cd{ldexp(m.b.re, -e), ldexp(m.b.im, -e)};
+  m.c = cd{ldexp(m.c.re, -e), ldexp(m.c.im, -e)};
+  m.d = cd{ldexp(m.d.re, -e), ldexp(m.d.im, -e)};
+}
+__device__ __forceinline__ void renorm(rm2& m) {
+  double mx = fmax(fmax(fabs(m.a), fabs(m.b)), fmax(fabs(m.c), fabs(m.d)));
+  if (mx == 0.0) return;
+  const int e = __builtin_amdgcn_frexp_exp(mx);
+  m.a = ldexp(m.a, -e);
+  m.b = ldexp(m.b, -e);
+  m.c = ldexp(m.c, -e);
+  m.d = ldexp(m.d, -e);
+}
+// shuffles within a group of P lanes (source lane index relative to the group)
+template <int P>
+__device__ __forceinline__ double grp_down(double v, int d) { return __shfl_down(v, d, P); }
+template <int P>
+__device__ __forceinline__ double grp_up(double v, int d) { return __shfl_up(v, d, P); }
+template <int P>
+__device__ __forceinline__ double grp_get(double v, int src) { return __shfl(v, src, P); }
+template <int P>
+__device__ __forceinline__ cd grp_down(cd v, int d) {
+  return cd{grp_down<P>(v.re, d), grp_down<P>(v.im, d)};
+}
+template <int P>
+__device__ __forceinline__ cm2 grp_down(const cm2& m, int d) {
+  return cm2{grp_down<P>(m.a, d), grp_down<P>(m.b, d), grp_down<P>(m.c, d), grp_down<P>(m.d, d)};
+}
+template <int P>
+__device__ __forceinline__ cm2 grp_get(const cm2& m, int src) {
+  return cm2{cd{grp_get<P>(m.a.re, src), grp_get<P>(m.a.im, src)},
+             cd{grp_get<P>(m.b.re, src), grp_get<P>(m.b.im, src)},
+             cd{grp_get<P>(m.c.re, src), grp_get<P>(m.c.im, src)},
+             cd{grp_get<P>(m.d.re, src), grp_get<P>(m.d.im, src)}};
+}
+template <int P>
+__device__ __forceinline__ rm2 grp_down(const rm2& m, int d) {
+  return rm2{grp_down<P>(m.a, d), grp_down<P>(m.b, d), grp_down<P>(m.c, d), grp_down<P>(m.d, d)};
+}
+template <int P>
+__device__ __forceinline__ rm2 grp_up(const rm2& m, int d) {
+  return rm2{grp_up<P>(m.a, d), grp_up<P>(m.b, d), grp_up<P>(m.c, d), grp_up<P>(m.d, d)};
+}
+template <int P>
+__device__ __forceinline__ rm2 grp_get(const rm2& m, int src) {
+  return rm2{grp_get<P>(m.a, src), grp_get<P>(m.b, src), grp_get<P>(m.c, src),
+             grp_get<P>(m.d, src)};
+}
+// Inclusive suffix product over the group: lane j gets T_j T_{j+1} ... T_{P-1}.
+template <int P, class M>
+__device__ __forceinline__ M suffix_scan(M t, int j) {
+#pragma unroll
+  for (int d = 1; d < P; d <<= 1) {
+    M o = grp_down<P>(t, d);
+    if (j + d < P) {
+      t = t * o;
+      renorm(t);
+    }
+  }
+  return t;
+}
+// Inclusive prefix product over the group: lane j gets U_j U_{j-1} ... U_0.
+template <int P>
+__device__ __forceinline__ rm2 prefix_scan(rm2 t, int j) {
+#pragma unroll
+  for (int d = 1; d < P; d <<= 1) {
+    rm2 o = grp_up<P>(t, d);
+    if (j >= d) t = t * o;
+  }
+  return t;
+}
+// sum over the group, the same butterfly order on every lane
+template <int P>
+__device__ __forceinline__ double grp_sum(double v) {
+#pragma unroll
+  for (int d = P / 2; d >= 1; d >>= 1) v += __shfl_xor(v, d, P);
+  return v;
+}
+
+template <int P>
+__global__ __launch_bounds__(64) void k_mie_homog_coop(
+    const int64_t* __restrict__ order, const int64_t* __restrict__ range, int64_t n_all,
+    const double* __restrict__ x, const double* __restrict__ y, const double* __restrict__ eps,
+    const double* __restrict__ mu, const double* __restrict__ eps2,
+    const int64_t* __restrict__ coef_off, double* __restrict__ an, double* __restrict__ bn,
+    double* __restrict__ props, int32_t* __restrict__ status, double* __restrict__ scratch,
+    int lcap, int dn_extra) {
+  constexpr int W = 64 / P;
+  const int lane = threadIdx.x;
+  const int j = lane % P, grp = lane / P;
+  const int64_t begin = range[0], end = range[1];
+  double* slot = scratch + (size_t)blockIdx.x * (size_t)lcap * 64 * 3;
+  cd* Ds = reinterpret_cast<cd*>(slot);            // [row in segment][lane]
+  double* Rs = slot + (size_t)lcap * 64 * 2;       // [row in segment][lane]
+
+  for (int64_t base = begin + (int64_t)blockIdx.x * W; base < end;
+       base += (int64_t)gridDim.x * W) {
+    const int64_t idx = base + grp;
+    if (idx >= end) continue;                      // the whole group: no shuffle partners lost
+    const int64_t p = order[idx];
+    PointDesc dsc = describe(p, x, y, eps, mu, eps2);
+    if (dsc.cls != 0) continue;
+    const double s = dsc.size;
+    const int nmax = nmax_rule(s);
+    const int L = (nmax + P - 1) / P;
+    if (L > lcap) {
+      if (j == 0) atomicOr(status, 1);
+      continue;
+    }
+    const cd z = csqrt_(dsc.eps * dsc.mu) * s;
+    const cd m = csqrt_(cdiv(dsc.eps, dsc.mu));
+    const cdd minv = crcp_dd(m);
+    const double zabs = cabs_(z);
+    const int nmx =
+        nmx_rule(nmax, zabs) + (dn_extra < 0 ? 16 + 10 * (int)ceil(cbrt(zabs)) : dn_extra);
+    const cdd zinv = crcp_dd(z);
+    const ddr xinv = rcp_dd(s);
+    const int nsw = min((int)s, nmax);
+    const int n0 = j * L + 1, n1 = min((j + 1) * L, nmax);   // this lane's rows (may be empty)
+
+    // ---- D_n: head rows nmax+1 .. nmx-1, then this lane's rows
+    cd d_in;
+    {
+      const int H = nmx - 1 - nmax;
+      const int HL = (H + P - 1) / P;
+      const int h0 = nmax + 1 + j * HL, h1 = min(nmax + (j + 1) * HL, nmx - 1);
+      cm2 t = cm2_eye();
+      int cnt = 0;
+      for (int k = h1; k >= h0; --k) {
+        const cd r = mul_dd((double)(k + 1), zinv);
+        const cm2 mk{r, r * r - 1.0, cd{1.0, 0.0}, r};
+        t = mk * t;
+        if ((++cnt & 3) == 0) renorm(t);
+      }
+      renorm(t);
+      const cm2 head = grp_get<P>(suffix_scan<P>(t, j), 0);
+      const cd vh_a = head.b, vh_b = head.d;     // head (0, 1)
+      t = cm2_eye();
+      cnt = 0;
+      for (int k = n1; k >= n0; --k) {
+        const cd r = mul_dd((double)(k + 1), zinv);
+        const cm2 mk{r, r * r - 1.0, cd{1.0, 0.0}, r};
+        t = mk * t;
+        if ((++cnt & 3) == 0) renorm(t);
+      }
+      renorm(t);
+      cm2 sfx = grp_down<P>(suffix_scan<P>(t, j), 1);   // T_{j+1} ... T_{P-1}
+      if (j == P - 1) sfx = cm2_eye();
+      const cd va = sfx.a * vh_a + sfx.b * vh_b, vb = sfx.c * vh_a + sfx.d * vh_b;
+      d_in = cdiv(va, vb);
+    }
+    {
+      cd dd = d_in;
+      for (int k = n1; k >= n0; --k) {
+        const cd r = mul_dd((double)(k + 1), zinv);
+        dd = r - crcp_fast(dd + r);
+        Ds[(size_t)(k - n0) * 64 + lane] = dd;
+      }
+    }
+    // ---- psi ratios above nsw: head nmax+1 .. sr, then this lane's rows > nsw
+    double seg_prod = 1.0;
+    if (nsw < nmax) {
+      const int sr = ratio_start(nmax, s);
+      const int H = sr - nmax;
+      const int HL = (H + P - 1) / P;
+      const int h0 = nmax + 1 + j * HL, h1 = min(nmax + (j + 1) * HL, sr);
+      rm2 t = rm2_eye();
+      for (int k = h1; k >= h0; --k) {
+        const double r = mul_dd((double)k, xinv);
+        t = rm2{r, fma(r, r, -1.0), 1.0, r} * t;
+        renorm(t);
+      }
+      const rm2 head = grp_get<P>(suffix_scan<P>(t, j), 0);
+      const double vh_a = head.b, vh_b = head.d;
+      t = rm2_eye();
+      const int lo = max(n0, nsw + 1);
+      for (int k = n1; k >= lo; --k) {
+        const double r = mul_dd((double)k, xinv);
+        t = rm2{r, fma(r, r, -1.0), 1.0, r} * t;
+        renorm(t);
+      }
+      rm2 sfx = grp_down<P>(suffix_scan<P>(t, j), 1);
+      if (j == P - 1) sfx = rm2_eye();
+      double dr = fma(sfx.a, vh_a, sfx.b * vh_b) / fma(sfx.c, vh_a, sfx.d * vh_b);
+      for (int k = n1; k >= lo; --k) {
+        const double r = mul_dd((double)k, xinv);
+        const double it = rcp_nr(dr + r);
+        Rs[(size_t)(k - n0) * 64 + lane] = it;
+        seg_prod *= it;
+        dr = r - it;
+      }
+    }
+    // ---- psi (n <= nsw) and chi upward: the pair entering this lane's rows
+    double sn, cs;
+    sincos(s, &sn, &cs);
+    double psi_p, psi_pp, chi_p, chi_pp;
+    {
+      rm2 u = rm2_eye();
+      for (int n = n0; n <= n1; ++n) {
+        const double f = mul_dd((double)(2 * n - 1), xinv);
+        u = rm2{f, -1.0, 1.0, 0.0} * u;
+      }
+      rm2 e = prefix_scan<P>(u, j);
+      rm2 ex = grp_up<P>(e, 1);                    // U_{j-1} ... U_0
+      if (j == 0) ex = rm2_eye();
+      psi_p = fma(ex.a, sn, ex.b * cs);            // psi_{n0-1}
+      psi_pp = fma(ex.c, sn, ex.d * cs);
+      chi_p = fma(ex.a, cs, -(ex.b * sn));
+      chi_pp = fma(ex.c, cs, -(ex.d * sn));
+    }
+    if (nsw < nmax) {
+      // psi at nsw, from the lane whose rows hold it, upward from its pair
+      double pn = sn;
+      const int js = nsw >= 1 ? (nsw - 1) / L : 0;
+      if (nsw >= 1 && j == js) {
+        double a = psi_p, b = psi_pp;
+        for (int n = n0; n <= nsw; ++n) {
+          const double f = mul_dd((double)(2 * n - 1), xinv);
+          const double c = fma(f, a, -b);
+          b = a;
+          a = c;
+        }
+        pn = a;
+      }
+      if (nsw >= 1) pn = __shfl(pn, js, P);
+      // prefix product of the ratios below this lane's rows
+      double pre = seg_prod;
+#pragma unroll
+      for (int d = 1; d < P; d <<= 1) {
+        const double o = __shfl_up(pre, d, P);
+        if (j >= d) pre *= o;
+      }
+      pre = __shfl_up(pre, 1, P);
+      if (j == 0) pre = 1.0;
+      if (n0 - 1 >= nsw) psi_p = pn * pre;
+    }
+    // ---- a_n, b_n and partial efficiency sums over this lane's rows
+    const bool store = coef_off != nullptr;
+    const int64_t off = store ? coef_off[p] : 0;
+    double sext = 0.0, ssca = 0.0, asy1 = 0.0, asy2 = 0.0;
+    cd qb = cd{0.0, 0.0}, a_first = cd{0.0, 0.0}, b_first = cd{0.0, 0.0};
+    cd ap = cd{0.0, 0.0}, bp = cd{0.0, 0.0};
+    for (int n = n0; n <= n1; ++n) {
+      const cd D = Ds[(size_t)(n - n0) * 64 + lane];
+      const double f = mul_dd((double)(2 * n - 1), xinv);
+      const double psi = (n <= nsw) ? fma(f, psi_p, -psi_pp)
+                                    : psi_p * Rs[(size_t)(n - n0) * 64 + lane];
+      const double chi = fma(f, chi_p, -chi_pp);
+      const double nx = mul_dd((double)n, xinv);
+      const cd da = mul_dd(D, minv) + nx;
+      const cd db = D * m + nx;
+      const cd xi = cd{psi, -chi};
+      const cd xi1 = cd{psi_p, -chi_p};
+      const cd a = cdiv(da * psi - psi_p, da * xi - xi1);
+      const cd b = cdiv(db * psi - psi_p, db * xi - xi1);
+      const double dn = (double)n, cn = 2.0 * dn + 1.0;
+      sext = fma(cn, a.re + b.re, sext);
+      ssca = fma(cn, norm2(a) + norm2(b), ssca);
+      qb = qb + (a - b) * ((n & 1) ? -cn : cn);
+      if (n > n0) {
+        const double dm = dn - 1.0;
+        asy1 = fma(dm * (dm + 2.0) * rcp_nr(dm + 1.0),
+                   ap.re * a.re + ap.im * a.im + bp.re * b.re + bp.im * b.im, asy1);
+      } else {
+        a_first = a;
+        b_first = b;
+      }
+      asy2 = fma(cn * rcp_nr(dn * (dn + 1.0)), a.re * b.re + a.im * b.im, asy2);
+      ap = a;
+      bp = b;
+      if (store) {
+        an[2 * (off + n - 1)] = a.re;
+        an[2 * (off + n - 1) + 1] = a.im;
+        bn[2 * (off + n - 1)] = b.re;
+        bn[2 * (off + n - 1) + 1] = b.im;
+      }
+      psi_pp = psi_p;
+      psi_p = psi;
+      chi_pp = chi_p;
+      chi_p = chi;
+    }
+    // the pair (n1, n1 + 1) across the segment boundary
+    {
+      const cd an1 = grp_down<P>(a_first, 1), bn1 = grp_down<P>(b_first, 1);
+      if (j < P - 1 && n1 >= n0 && n1 + 1 <= nmax) {
+        const double dm = (double)n1;
+        asy1 = fma(dm * (dm + 2.0) * rcp_nr(dm + 1.0),
+                   ap.re * an1.re + ap.im * an1.im + bp.re * bn1.re + bp.im * bn1.im, asy1);
+      }
+    }
+    sext = grp_sum<P>(sext);
+    ssca = grp_sum<P>(ssca);
+    asy1 = grp_sum<P>(asy1);
+    asy2 = grp_sum<P>(asy2);
+    qb = cd{grp_sum<P>(qb.re), grp_sum<P>(qb.im)};
+    if (j == 0 && props) {
+      const double y2 = s * s;
+      const double qext = 2.0 * sext / y2, qsca = 2.0 * ssca / y2;
+      const double qbv = (qb.re * qb.re + qb.im * qb.im) / y2;
+      props[0 * n_all + p] = qext;
+      props[1 * n_all + p] = qsca;
+      props[2 * n_all + p] = qext - qsca;
+      props[3 * n_all + p] = qbv;
+      props[4 * n_all + p] = 4.0 / y2 * (asy1 + asy2) / qsca;
+      props[5 * n_all + p] = qbv / qsca;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Recurrence helpers (one lane, rows [n-1][lane] of a scratch plane)
 // ---------------------------------------------------------------------------
 // D_1..D_nmax of z from D_nstart = 0 (mie_coeffs.py:118-121); subtract=true
@@ -1004,6 +1365,9 @@ template __global__ void k_mie_coated_stable<4>(GM_MIE_ARGS);
 template __global__ void k_mie_coated_ckpt<1, 4>(GM_MIE_ARGS);
 template __global__ void k_mie_coated_ckpt<3, 4>(GM_MIE_ARGS);
 template __global__ void k_mie_coated_ckpt<1, 8>(GM_MIE_ARGS);
+template __global__ void k_mie_homog_coop<16>(GM_MIE_ARGS);
+template __global__ void k_mie_homog_coop<32>(GM_MIE_ARGS);
+template __global__ void k_mie_homog_coop<64>(GM_MIE_ARGS);
 #undef GM_MIE_ARGS
 
 __global__ void k_nmax_of(int64_t n, const double* __restrict__ size, int64_t* __restrict__ cnt) {

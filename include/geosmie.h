@@ -91,7 +91,12 @@ enum gm_tune_param {
   GM_TUNE_MIE_CKPT = 7          /* coated kernel (real / weakly absorbing shells): the downward
                                    chains' states kept every 4 or 8 rows and each block of rows
                                    regenerated in LDS before use (4, the default), or 0 = every
-                                   row kept in scratch; results are bitwise the same */
+                                   row kept in scratch; results agree to rounding */,
+  GM_TUNE_MIE_LANES = 8         /* homogeneous kernel: lanes per point.  1 = one point per lane
+                                   (large batches); 16, 32, 64 = each point's rows split over that
+                                   many lanes, its serial recurrences joined by parallel scans
+                                   (batches too small to fill the GPU); 0 (default) = chosen from
+                                   the batch size and series length.  Results agree to rounding */
 };
 int gm_ctx_tune(gm_ctx* ctx, int param, int value);
 int gm_ctx_get_tune(gm_ctx* ctx, int param, int* value);

@@ -120,3 +120,37 @@ def test_coated_checkpoint_variants_agree(gpu):
             same &= bool(np.array_equal(a, b))
             assert np.all(np.abs(a - b) <= 1e-12 * np.abs(a).max()), ck
         print("checkpoint every %d rows: bitwise identical to the full-row kernel: %s" % (ck, same))
+
+
+def test_homog_lane_variants_agree(gpu):
+    """The homogeneous kernel with one point per lane and with 16, 32 or 64
+    lanes per point (segments joined by parallel scans) agree to rounding
+    level: efficiencies within 1e-10 relative (g 1e-10 absolute), stored
+    coefficients within 1e-10 of each point's largest, on sizes from the
+    Rayleigh limit to x = 2094 and indices from weakly to strongly absorbing."""
+    from geosmie_amd import _lib, autotune
+    from geosmie_amd.mie import mie_batch
+    h = _lib.ctx()
+    r = np.random.default_rng(11)
+    x = np.concatenate([np.logspace(-3, np.log10(2094.0), 900), [0.5, 1.0, 1.7, 3.0]])
+    m = (r.uniform(1.01, 1.9, x.size) + 1j * 10 ** r.uniform(-8, 0.3, x.size))
+    res = {}
+    try:
+        for lanes in (1, 16, 32, 64):
+            autotune.set_knob(h, "mie_lanes", lanes)
+            res[lanes] = mie_batch(x, m=m, coeffs=True)
+    finally:
+        autotune.set_knob(h, "mie_lanes", 0)
+    ref = res[1]
+    for lanes in (16, 32, 64):
+        got = res[lanes]
+        for k in ("qext", "qsca", "qb"):
+            rel = np.abs(got[k] - ref[k]) / np.abs(ref[k])
+            assert rel.max() < 1e-10, (lanes, k, rel.max(), x[rel.argmax()], m[rel.argmax()])
+        assert np.abs(got["asy"] - ref["asy"]).max() < 1e-10, lanes
+        # each point's coefficients against its largest |a_n|, |b_n| (in the
+        # Rayleigh limit b_1 ~ x^5 is formed by cancellation in both kernels)
+        for a, b, ga, gb in zip(ref["an"], ref["bn"], got["an"], got["bn"]):
+            scale = max(np.abs(a).max(), np.abs(b).max())
+            assert np.abs(a - ga).max() <= 1e-10 * scale, lanes
+            assert np.abs(b - gb).max() <= 1e-10 * scale, lanes
