@@ -134,6 +134,7 @@ struct MieGrids {
   std::pair<int64_t, size_t> homog, stable, literal;
   int lanes = 1;      // homogeneous kernel: lanes per point
   int lcap = 0;       // rows per lane segment (lanes > 1)
+  int64_t small_grid = 0;   // grid of the 4-lane kernel for the short series (lanes > 1)
 };
 
 // Lanes per point of the homogeneous kernel: the tuned value, or by batch:
@@ -167,7 +168,12 @@ MieGrids mie_grids(const gm_ctx* ctx, int64_t n, int rows_cap, size_t budget) {
                                    (int64_t)ctx->n_cu * std::max(ctx->occ_coop, 1) *
                                        std::max(ctx->tune_mie_grid_mult, 1));
     gg = std::min<int64_t>(gg, std::max<int64_t>(1, (int64_t)(budget / slot_bytes)));
-    g.homog = std::make_pair(gg, (size_t)gg * slot_bytes);
+    // the 4-lane kernel of the short series runs after it on the same slots
+    const size_t small_slot = (size_t)gm::kSmallLcap * 64 * gm::kHomogRowDoubles * sizeof(double);
+    int64_t gsm = std::min<int64_t>((n + 15) / 16, (int64_t)ctx->n_cu * std::max(ctx->occ_coop, 1));
+    gsm = std::max<int64_t>(1, std::min<int64_t>(gsm, (int64_t)(budget / small_slot)));
+    g.small_grid = gsm;
+    g.homog = std::make_pair(gg, std::max((size_t)gg * slot_bytes, (size_t)gsm * small_slot));
   }
   g.literal = grid_for(ctx->occ_literal, gm::kCoatedRowDoubles);
   const int ck = ctx->tune_mie_ckpt;
@@ -212,6 +218,7 @@ int mie_batch_run(gm_ctx* ctx, int64_t n, const double* x, const double* y, cons
   int64_t* hist = ws.hist;
   int64_t* cursor = hist + nb;
   int64_t* range = cursor + nb;
+  int64_t* ab = range + 2 * gm::kClasses;   // homogeneous class: many-lane part, few-lane part
   unsigned gblk = (unsigned)((n + 255) / 256);
   // points per thread of the plan kernels: 1 up to ~256k points (blocks stay
   // plentiful), growing to kPlanPer for the largest batches (fewer LDS
@@ -222,12 +229,13 @@ int mie_batch_run(gm_ctx* ctx, int64_t n, const double* x, const double* y, cons
     hipLaunchKernelGGL(gm::k_zero_i64, dim3((nb + 255) / 256), dim3(256), 0, s, hist, nb);
     hipLaunchKernelGGL(gm::k_plan_keys, dim3(pblk), dim3(256), 0, s, n, x, y, eps, mu, eps2,
                        ws.key, nmax_out, hist, ws.status, per);
-    hipLaunchKernelGGL(gm::k_plan_scan, dim3(1), dim3(256), 0, s, hist, cursor, range);
+    hipLaunchKernelGGL(gm::k_plan_scan, dim3(1), dim3(256), 0, s, hist, cursor, range,
+                       gm::kSmallCost >> gm::kBucketShift, ab);
     hipLaunchKernelGGL(gm::k_plan_scatter, dim3(pblk), dim3(256), 0, s, n,
                        (const int32_t*)ws.key, cursor, ws.order, per);
   } else {
     // input order: each class kernel walks every point and keeps its own
-    hipLaunchKernelGGL(gm::k_plan_identity, dim3(gblk), dim3(256), 0, s, n, ws.order, range);
+    hipLaunchKernelGGL(gm::k_plan_identity, dim3(gblk), dim3(256), 0, s, n, ws.order, range, ab);
     if (nmax_out) {
       hipLaunchKernelGGL(gm::k_nmax_i32, dim3(gblk), dim3(256), 0, s, n, y ? y : x, nmax_out);
     }
@@ -240,14 +248,22 @@ int mie_batch_run(gm_ctx* ctx, int64_t n, const double* x, const double* y, cons
   ctx->mark_begin(gm::kKMieHomog, s, &slot);
   auto homog = var == 2 ? gm::k_mie_homog<4> : gm::k_mie_homog<1>;
   int cap_arg = rows_cap;
+  const int64_t* hrange = range;
   if (g.lanes > 1) {
     homog = g.lanes == 16 ? gm::k_mie_homog_coop<16>
             : g.lanes == 32 ? gm::k_mie_homog_coop<32> : gm::k_mie_homog_coop<64>;
     cap_arg = g.lcap;
+    hrange = ab;
   }
-  hipLaunchKernelGGL(homog, dim3((unsigned)g.homog.first), dim3(64), 0, s, order, range, n, x, y,
+  hipLaunchKernelGGL(homog, dim3((unsigned)g.homog.first), dim3(64), 0, s, order, hrange, n, x, y,
                      eps, mu, eps2, coef_off, an, bn, props, ws.status, ws.slots, cap_arg,
                      dn_extra);
+  if (g.lanes > 1 && (flags & GM_MIE_SORT)) {
+    // the short series (sorted last): 4 lanes per point, 16 points per wavefront
+    hipLaunchKernelGGL(gm::k_mie_homog_coop<4>, dim3((unsigned)g.small_grid), dim3(64), 0, s,
+                       order, ab + 2, n, x, y, eps, mu, eps2, coef_off, an, bn, props, ws.status,
+                       ws.slots, gm::kSmallLcap, dn_extra);
+  }
   ctx->mark_end(slot, s);
   HIP_OK(hipGetLastError());
   if (y) {
@@ -488,7 +504,7 @@ int gm_mie_batch(gm_ctx* ctx, int64_t n, const double* x, const double* y, const
   const int nb = gm::kClasses * gm::kBuckets;
   if ((rc = ctx->reserve(ctx->order, sizeof(int64_t) * n))) return rc;
   if ((rc = ctx->reserve(ctx->key, sizeof(int32_t) * n))) return rc;
-  if ((rc = ctx->reserve(ctx->hist, sizeof(int64_t) * (2 * nb + 2 * gm::kClasses)))) return rc;
+  if ((rc = ctx->reserve(ctx->hist, sizeof(int64_t) * (2 * nb + 2 * gm::kClasses + 4)))) return rc;
   MieWs ws;
   ws.order = (int64_t*)ctx->order.ptr;
   ws.key = (int32_t*)ctx->key.ptr;
@@ -526,13 +542,16 @@ int gm_mie_batch_noalloc(gm_ctx* ctx, int64_t n, const double* x, const double* 
   // layout of the static array: plan histogram/cursors/ranges, sort keys,
   // sorted order, then the recurrence slots in what is left
   const int nb = gm::kClasses * gm::kBuckets;
-  const size_t hist_bytes = sizeof(int64_t) * (2 * nb + 2 * gm::kClasses);
+  const size_t hist_bytes = sizeof(int64_t) * (2 * nb + 2 * gm::kClasses + 4);
   auto up = [](size_t v) { return (v + 255) / 256 * 256; };
   const size_t plan_bytes = up(hist_bytes) + up(sizeof(int32_t) * (size_t)n) +
                             up(sizeof(int64_t) * (size_t)n);
   const int rows_cap = std::max(1, gm_nmax(size_max));
   const int row_doubles = y ? gm::kCoatedRowDoubles : gm::kHomogRowDoubles;
-  const size_t slot_bytes = (size_t)rows_cap * 64 * row_doubles * sizeof(double);
+  // one recurrence slot of the largest kind any of the kernels may take
+  const size_t slot_bytes =
+      std::max((size_t)rows_cap * 64 * row_doubles * sizeof(double),
+               (size_t)gm::kSmallLcap * 64 * gm::kHomogRowDoubles * sizeof(double));
   if (plan_bytes + slot_bytes > kNoAllocBytes)
     return gm_fail(GM_EINVAL,
                    "gm_mie_batch_noalloc: %lld points with series length %d need %zu bytes of "

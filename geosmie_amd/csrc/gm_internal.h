@@ -19,10 +19,15 @@ __global__ void k_zero_i32(int32_t* p, int n);
 __global__ void k_plan_keys(int64_t n, const double* x, const double* y, const double* eps,
                             const double* mu, const double* eps2, int32_t* key,
                             int32_t* nmax_out, int64_t* hist, int32_t* status, int per);
-__global__ void k_plan_scan(const int64_t* hist, int64_t* cursor, int64_t* range);
+__global__ void k_plan_scan(const int64_t* hist, int64_t* cursor, int64_t* range, int small_b,
+                            int64_t* ab);
 __global__ void k_plan_scatter(int64_t n, const int32_t* key, int64_t* cursor, int64_t* order,
                                int per);
-__global__ void k_plan_identity(int64_t n, int64_t* order, int64_t* range);
+__global__ void k_plan_identity(int64_t n, int64_t* order, int64_t* range, int64_t* ab);
+// homogeneous points of cost below this (bucketed) run 4 lanes per point: then
+// 3 n_max + 16 <= cost < 160, so n_max < 48 and at most 12 rows per lane
+constexpr int kSmallCost = 160;
+constexpr int kSmallLcap = 12;
 template <int MINW>
 __global__ void k_mie_homog(const int64_t* order, const int64_t* range, int64_t n_all,
                             const double* x, const double* y, const double* eps, const double* mu,

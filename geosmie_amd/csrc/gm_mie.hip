@@ -168,9 +168,13 @@ __global__ __launch_bounds__(256) void k_plan_keys(
 // Each thread sums its run of buckets, the 256 run sums are scanned in LDS
 // (Hillis-Steele, 8 steps), then each thread writes its run's cursors and the
 // class boundaries that fall in it.
+// Also splits the homogeneous class at the first bucket of cost below
+// small_b (buckets descend along the order): ab = {begin, split, split, end},
+// the points run with many lanes per point and those run with few.
 __global__ __launch_bounds__(256) void k_plan_scan(const int64_t* __restrict__ hist,
                                                    int64_t* __restrict__ cursor,
-                                                   int64_t* __restrict__ range) {
+                                                   int64_t* __restrict__ range, int small_b,
+                                                   int64_t* __restrict__ ab) {
   __shared__ int64_t part[256];
   constexpr int per = (kClasses * kBuckets) / 256;
   static_assert(per * 256 == kClasses * kBuckets, "histogram must split into 256 runs");
@@ -203,6 +207,15 @@ __global__ __launch_bounds__(256) void k_plan_scan(const int64_t* __restrict__ h
     acc += h[i];
   }
   if (t == 255) range[2 * kClasses - 1] = acc;
+  __syncthreads();
+  if (t == 0 && ab) {
+    const int key = kBuckets - min(max(small_b, 0), kBuckets);   // class 0's first small key
+    const int64_t split = key < kBuckets ? cursor[key] : range[1];
+    ab[0] = range[0];
+    ab[1] = split;
+    ab[2] = split;
+    ab[3] = range[1];
+  }
 }
 
 // Scatter with block-local ranks: one global atomic per (block, bucket),
@@ -239,14 +252,21 @@ __global__ __launch_bounds__(256) void k_plan_scatter(int64_t n, const int32_t* 
 // Identity order (no sort): every class kernel walks all points and keeps
 // those of its own class.
 __global__ void k_plan_identity(int64_t n, int64_t* __restrict__ order,
-                                int64_t* __restrict__ range) {
+                                int64_t* __restrict__ range, int64_t* __restrict__ ab) {
   int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (p < n) order[p] = p;
-  if (p == 0)
+  if (p == 0) {
     for (int c = 0; c < kClasses; ++c) {
       range[2 * c] = 0;
       range[2 * c + 1] = n;
     }
+    if (ab) {   // unsorted: every point with many lanes per point
+      ab[0] = 0;
+      ab[1] = n;
+      ab[2] = n;
+      ab[3] = n;
+    }
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1365,6 +1385,7 @@ template __global__ void k_mie_coated_stable<4>(GM_MIE_ARGS);
 template __global__ void k_mie_coated_ckpt<1, 4>(GM_MIE_ARGS);
 template __global__ void k_mie_coated_ckpt<3, 4>(GM_MIE_ARGS);
 template __global__ void k_mie_coated_ckpt<1, 8>(GM_MIE_ARGS);
+template __global__ void k_mie_homog_coop<4>(GM_MIE_ARGS);
 template __global__ void k_mie_homog_coop<16>(GM_MIE_ARGS);
 template __global__ void k_mie_homog_coop<32>(GM_MIE_ARGS);
 template __global__ void k_mie_homog_coop<64>(GM_MIE_ARGS);
