@@ -48,6 +48,8 @@ SIGNATURES = {
     "gm_ri_bilinear": (ctypes.c_int, [c_vp, ctypes.c_int, ctypes.c_int, c_i64, c_vp, c_i64, c_vp,
                                       c_vp, c_vp, c_vp]),
     "gm_pmatrix_normalize6": (ctypes.c_int, [c_vp, c_i64, ctypes.c_int, c_vp, c_vp, c_vp, c_vp]),
+    "gm_gsf_matr": (ctypes.c_int, [c_vp, c_i64, ctypes.c_int, c_vp, ctypes.c_int, c_vp, c_vp,
+                                   c_vp]),
     "gm_angle_tables": (ctypes.c_int, [c_vp, ctypes.c_int, c_vp, ctypes.c_int, c_vp, c_vp,
                                        c_vp]),
     "gm_mie_s12": (ctypes.c_int, [c_vp, c_i64, c_vp, c_vp, c_vp, c_vp, ctypes.c_int, c_vp, c_vp,

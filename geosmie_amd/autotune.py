@@ -36,7 +36,10 @@ KNOBS = {"mie_grid_mult": (0, (1, 2, 4)),
          "mie_ckpt": (7, (4, 8, 0)),
          "mie_lanes": (8, (0, 1, 16, 32, 64)),
          # a precision setting rather than a shape; set explicitly, never tuned for speed
-         "mlp_weight_terms": (3, (1, 2))}
+         "mlp_weight_terms": (3, (1, 2)),
+         # a behaviour setting: the integratePSD quirks reproduced (bit 0: the aliased
+         # Qsca weighting of csca and cext); set explicitly, never tuned
+         "psd_quirks": (9, (1, 0))}
 
 
 def get(ctx, name):

@@ -419,6 +419,11 @@ int gm_ctx_tune(gm_ctx* ctx, int param, int value) {
         return gm_fail(GM_EINVAL, "gm_ctx_tune: lanes per point must be 0, 1, 16, 32 or 64");
       ctx->tune_mie_lanes = value;
       return GM_OK;
+    case GM_TUNE_PSD_QUIRKS:
+      if (value & ~(int)GM_PSD_QUIRK_ALIAS)
+        return gm_fail(GM_EINVAL, "gm_ctx_tune: unknown integratePSD quirk bits %d", value);
+      ctx->psd_quirks = value;
+      return GM_OK;
     case GM_TUNE_MLP_WEIGHT_TERMS:
       if (value != 1 && value != 2)
         return gm_fail(GM_EINVAL, "gm_ctx_tune: MLP weight terms must be 1 or 2");
@@ -441,6 +446,7 @@ int gm_ctx_get_tune(gm_ctx* ctx, int param, int* value) {
     case GM_TUNE_S12_LAYOUT: *value = ctx->tune_s12_layout; return GM_OK;
     case GM_TUNE_MIE_CKPT: *value = ctx->tune_mie_ckpt; return GM_OK;
     case GM_TUNE_MIE_LANES: *value = ctx->tune_mie_lanes; return GM_OK;
+    case GM_TUNE_PSD_QUIRKS: *value = ctx->psd_quirks; return GM_OK;
     default: return gm_fail(GM_EINVAL, "gm_ctx_get_tune: unknown parameter %d", param);
   }
 }

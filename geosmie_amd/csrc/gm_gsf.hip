@@ -190,12 +190,79 @@ __global__ __launch_bounds__(256) void k_gsf_normalize(int ng, const double* __r
   const int64_t e = blockIdx.x;
   const double* r = raw + (size_t)e * 6 * ng;
   double* o = out + (size_t)e * 6 * ng;
-  const double cnorm = 1.0 / r[0];
+  const double cnorm = (quantize & 2) ? 1.0 : 1.0 / r[0];
   for (int i = threadIdx.x; i < 6 * ng; i += blockDim.x) {
-    double v = r[i] * cnorm;
-    if (quantize) v = rint(v * 1e10) * 1e-10;
+    double v = (quantize & 2) ? r[i] : r[i] * cnorm;
+    if (quantize & 1) v = rint(v * 1e10) * 1e-10;
     o[i] = v;
   }
+}
+
+// MATR (spher_expan.f:419-517): the phase matrix at angles theta from its
+// expansion coefficients, one thread per (entry, angle), by the program's
+// upward recurrences of P^0_0, P^2_2, P^2_-2 and P^0_2 over l.
+//   coef [e][6][l1max] in gm_gsf_expand's order (alpha1, beta1, alpha3, beta2,
+//   alpha2, alpha4); out [e][6][n_ang] in the table order p11, p12, p33, p34,
+//   p22, p44.
+__global__ __launch_bounds__(64) void k_gsf_matr(int l1max, const double* __restrict__ coef,
+                                                 int n_ang, const double* __restrict__ theta,
+                                                 double* __restrict__ out) {
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t e = blockIdx.y;
+  if (a >= n_ang) return;
+  const double* c = coef + (size_t)e * 6 * l1max;
+  const double* A1 = c;
+  const double* B1 = c + l1max;
+  const double* A3 = c + 2 * l1max;
+  const double* B2 = c + 3 * l1max;
+  const double* A2 = c + 4 * l1max;
+  const double* A4 = c + 5 * l1max;
+  const int lmax = l1max - 1;
+  const double d6 = sqrt(6.0) * 0.25;
+  const double u = cos(theta[a]);
+  double f11 = 0.0, f2 = 0.0, f3 = 0.0, f44 = 0.0, f12 = 0.0, f34 = 0.0;
+  double p1 = 0.0, p2 = 0.0, p3 = 0.0, p4 = 0.0;
+  double pp1 = 1.0;
+  double pp2 = 0.25 * (1.0 + u) * (1.0 + u);
+  double pp3 = 0.25 * (1.0 - u) * (1.0 - u);
+  double pp4 = d6 * (u * u - 1.0);
+  for (int l1 = 1; l1 <= l1max; ++l1) {
+    const int l = l1 - 1;
+    const double dl = (double)l, dl1 = (double)l1;
+    f11 = f11 + A1[l] * pp1;
+    f44 = f44 + A4[l] * pp1;
+    const double pl1 = (double)(2 * l + 1);
+    if (l != lmax) {
+      const double pn = (pl1 * u * pp1 - dl * p1) / dl1;
+      p1 = pp1;
+      pp1 = pn;
+    }
+    if (l < 2) continue;
+    f2 = f2 + (A2[l] + A3[l]) * pp2;
+    f3 = f3 + (A2[l] - A3[l]) * pp3;
+    f12 = f12 + B1[l] * pp4;
+    f34 = f34 + B2[l] * pp4;
+    if (l == lmax) continue;
+    const double pl2 = dl * dl1 * u;
+    const double pl3 = dl1 * (dl * dl - 4.0);
+    const double pl4 = 1.0 / (dl * (dl1 * dl1 - 4.0));
+    double pn = (pl1 * (pl2 - 4.0) * pp2 - pl3 * p2) * pl4;
+    p2 = pp2;
+    pp2 = pn;
+    pn = (pl1 * (pl2 + 4.0) * pp3 - pl3 * p3) * pl4;
+    p3 = pp3;
+    pp3 = pn;
+    pn = (pl1 * u * pp4 - sqrt(dl * dl - 4.0) * p4) / sqrt(dl1 * dl1 - 4.0);
+    p4 = pp4;
+    pp4 = pn;
+  }
+  double* o = out + (size_t)e * 6 * n_ang;
+  o[a] = f11;
+  o[n_ang + a] = f12;
+  o[2 * n_ang + a] = (f2 - f3) * 0.5;   // F33
+  o[3 * n_ang + a] = f34;
+  o[4 * n_ang + a] = (f2 + f3) * 0.5;   // F22
+  o[5 * n_ang + a] = f44;
 }
 
 // Legendre series sum_l c[l] P_l(x) (convertncdf.py:32-43), one thread per x.
@@ -270,6 +337,21 @@ int gm_gsf_expand(gm_ctx* ctx, int64_t n_entries, int n_ang, const double* ang_d
   ctx->mark_end(slot, s);
   e = hipGetLastError();
   if (e != hipSuccess) return gm_fail(1, "k_gsf_expand: %s", hipGetErrorString(e));
+  return 0;
+}
+
+int gm_gsf_matr(gm_ctx* ctx, int64_t n_entries, int l1max, const double* coef, int n_ang,
+                const double* theta_rad, double* out, gm_stream stream) {
+  if (!ctx || n_entries < 0 || n_entries > 65535 || l1max < 3 || n_ang < 1 || !coef ||
+      !theta_rad || !out)
+    return gm_fail(-1, "gm_gsf_matr: bad arguments");
+  if (n_entries == 0) return 0;
+  hipError_t e = hipSetDevice(ctx->device);
+  if (e != hipSuccess) return gm_fail(1, "hipSetDevice: %s", hipGetErrorString(e));
+  hipLaunchKernelGGL(gm::k_gsf_matr, dim3((unsigned)((n_ang + 63) / 64), (unsigned)n_entries),
+                     dim3(64), 0, (hipStream_t)stream, l1max, coef, n_ang, theta_rad, out);
+  e = hipGetLastError();
+  if (e != hipSuccess) return gm_fail(1, "gm_gsf_matr: %s", hipGetErrorString(e));
   return 0;
 }
 

@@ -197,7 +197,7 @@ __global__ __launch_bounds__(64) void k_contract_mfma_narrow(int n_psd, int64_t 
 //   0 1   1 r^2   2 r^3   3 r^4   4 qext r^2   5 qsca r^2   6 qabs r^2
 //   7 qb r^2   8 g qsca r^2   9 csca qsca r^2   10 cext qsca r^2
 // (csca = qsca pi r^2, cext = qext pi r^2, dointegration.py:1219-1220)
-constexpr int kNFeat = 11;
+constexpr int kNFeat = 13;
 // lam_pts (optional): a wavelength per point, for batches of several rows.
 __global__ void k_scalar_features(int64_t n, const double* __restrict__ xx, double lam,
                                   const double* __restrict__ lam_pts,
@@ -221,6 +221,9 @@ __global__ void k_scalar_features(int64_t n, const double* __restrict__ xx, doub
   o[8] = g * qsca * r2;
   o[9] = (qsca * pi * r2) * qsca * r2;
   o[10] = (qext * pi * r2) * qsca * r2;
+  // csca, cext area-weighted only (the weighting without the aliased Qsca)
+  o[11] = (qsca * pi * r2) * r2;
+  o[12] = (qext * pi * r2) * r2;
 }
 
 // Finish integratePSD for n_modes PSDs (sums[k][feature]) and mix them with
@@ -228,9 +231,12 @@ __global__ void k_scalar_features(int64_t n, const double* __restrict__ xx, doub
 // order.  out (kNOut): num, area, volume, mass, rEff, rMass, qext, qsca, qabs,
 // qb, g, csca, cext, bsca, bext, bbck.
 constexpr int kNOut = 16;
+// quirks bit 0 (GM_PSD_QUIRK_ALIAS): csca and cext carry the Qsca weight they
+// inherit from the aliased thisweight; without it they are area-weighted like
+// the other efficiencies.
 __device__ void finish_modes(int n_modes, const double* __restrict__ sums,
                              const double* __restrict__ fracs, const double* __restrict__ reff0,
-                             double rhop0, double rhop, double* __restrict__ out) {
+                             double rhop0, double rhop, int quirks, double* __restrict__ out) {
   const double pi = 3.14159265358979323846;
   double acc[kNOut];
   for (int q = 0; q < kNOut; ++q) acc[q] = 0.0;
@@ -254,8 +260,13 @@ __device__ void finish_modes(int n_modes, const double* __restrict__ sums,
     th[9] = (4.0 * pi * s[7] / s[5]) * th[7] / (4.0 * pi);
     // g and every key after it carry the extra Qsca weight (:1111-1112, 1154-1157)
     th[10] = s[8] / s[5];                     // g
-    th[11] = s[9] / s[5];                     // csca
-    th[12] = s[10] / s[5];                    // cext
+    if (quirks & 1) {
+      th[11] = s[9] / s[5];                   // csca
+      th[12] = s[10] / s[5];                  // cext
+    } else {
+      th[11] = s[11] / S2;
+      th[12] = s[12] / S2;
+    }
     double mc = 1.0 / rhop / th[4] * th[5] / rMass0;   // (:1193)
     th[13] = 0.75 * th[7] * mc;               // bsca
     th[14] = 0.75 * th[6] * mc;               // bext
@@ -268,21 +279,21 @@ __device__ void finish_modes(int n_modes, const double* __restrict__ sums,
 __global__ void k_scalar_finish(int n_modes, const double* __restrict__ sums,
                                 const double* __restrict__ fracs,
                                 const double* __restrict__ reff0, double rhop0, double rhop,
-                                double* __restrict__ out) {
+                                int quirks, double* __restrict__ out) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  finish_modes(n_modes, sums, fracs, reff0, rhop0, rhop, out);
+  finish_modes(n_modes, sums, fracs, reff0, rhop0, rhop, quirks, out);
 }
 
 // One row per thread: rows are independent single-mode integrations.
 // rhop_rows (or nullptr: rhop for every row) is each row's wet density.
 __global__ void k_scalar_finish_rows(int64_t n_rows, const double* __restrict__ sums,
                                      const double* __restrict__ reff0, double rhop0, double rhop,
-                                     const double* __restrict__ rhop_rows,
+                                     const double* __restrict__ rhop_rows, int quirks,
                                      double* __restrict__ out) {
   const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= n_rows) return;
   finish_modes(1, sums + r * kNFeat, nullptr, reff0 + r, rhop0, rhop_rows ? rhop_rows[r] : rhop,
-               out + r * kNOut);
+               quirks, out + r * kNOut);
 }
 
 // Phase-matrix normalisation, dointegration.py:978-988, per entry:
@@ -414,7 +425,7 @@ int gm_integrate_psd(gm_ctx* ctx, int64_t n_pts, int n_modes, const double* xx, 
   rc = gm_psd_contract(ctx, n_modes, n_pts, gm::kNFeat, psd, v, sums, stream);
   if (rc) return rc;
   hipLaunchKernelGGL(gm::k_scalar_finish, dim3(1), dim3(64), 0, s, n_modes, sums, fracs, reff0,
-                     rhop0, rhop, out);
+                     rhop0, rhop, ctx->psd_quirks, out);
   e = hipGetLastError();
   if (e != hipSuccess) return gm_fail(1, "gm_integrate_psd: %s", hipGetErrorString(e));
   return 0;
@@ -455,7 +466,7 @@ int gm_integrate_psd_rows(gm_ctx* ctx, int64_t n_rows, int64_t n_pts, const doub
   rc = gm_psd_contract(ctx, (int)n_rows, n_pts, gm::kNFeat, psd, v, sums, stream);
   if (rc) return rc;
   hipLaunchKernelGGL(gm::k_scalar_finish_rows, dim3((unsigned)((n_rows + 63) / 64)), dim3(64), 0, s,
-                     n_rows, sums, reff0, rhop0, rhop, rhop_rows, out);
+                     n_rows, sums, reff0, rhop0, rhop, rhop_rows, ctx->psd_quirks, out);
   e = hipGetLastError();
   if (e != hipSuccess) return gm_fail(1, "gm_integrate_psd_rows: %s", hipGetErrorString(e));
   return 0;

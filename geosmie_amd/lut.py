@@ -264,7 +264,7 @@ def _contract(w, v):
     return d_o.cpu().numpy().reshape(w.shape[0], v.shape[1])
 
 
-def _integrate_scalars(xxarr, props, psd, fracs, lam, reff0, rhop0, rhop):
+def _integrate_scalars(xxarr, props, psd, fracs, lam, reff0, rhop0, rhop, alias_quirk=True):
     t = _lib.torch()
     psd = np.ascontiguousarray(np.atleast_2d(psd), dtype=np.float64)
     d_out = t.empty(len(SCALAR_OUT), dtype=t.float64, device=_lib.device())
@@ -272,18 +272,28 @@ def _integrate_scalars(xxarr, props, psd, fracs, lam, reff0, rhop0, rhop):
     d_x, d_p, d_w = _lib.to_dev(xxarr), _lib.to_dev(props), _lib.to_dev(psd)
     d_f = _lib.to_dev(np.asarray(fracs, np.float64))
     d_r = _lib.to_dev(np.asarray(reff0, np.float64))
-    check(_lib.lib().gm_integrate_psd(
-        _lib.ctx(), psd.shape[1], psd.shape[0], ptr(d_x), ptr(d_p), ptr(d_w), ptr(d_f), ptr(d_r),
-        float(lam), float(rhop0), float(rhop), ptr(d_out), stream()), "gm_integrate_psd")
+    h = _lib.ctx()
+    from .autotune import get, set_knob
+    old = get(h, "psd_quirks")
+    set_knob(h, "psd_quirks", 1 if alias_quirk else 0)
+    try:
+        check(_lib.lib().gm_integrate_psd(
+            h, psd.shape[1], psd.shape[0], ptr(d_x), ptr(d_p), ptr(d_w), ptr(d_f), ptr(d_r),
+            float(lam), float(rhop0), float(rhop), ptr(d_out), stream()), "gm_integrate_psd")
+    finally:
+        set_knob(h, "psd_quirks", old)
     return dict(zip(SCALAR_OUT, d_out.cpu().numpy().tolist()))
 
 
-def integratePSD(xxarr, rawret, psd, fracs, lam, reff0, rhop0, rhop):
+def integratePSD(xxarr, rawret, psd, fracs, lam, reff0, rhop0, rhop, alias_quirk=True):
     """integratePSD (dointegration.py:1064-1209, mixing 'dumix') on the GPU.
 
     rawret: list (one per mode) of rawMie dicts; psd: list of weight arrays.
     Each mode's phase-matrix planes and scalars are integrated with its own
     rawret entry and weighted by fracs, as the reference loops over modes.
+    alias_quirk (default True, the reference's behaviour): csca and cext
+    carry the Qsca weight the aliased thisweight gives every key after 'g'
+    (:1111-1112, 1154-1157); False weights them by area only.
     """
     fracs = list(fracs)
     ret = {}
@@ -293,7 +303,8 @@ def integratePSD(xxarr, rawret, psd, fracs, lam, reff0, rhop0, rhop):
         integ = _contract(psd[fi], planes).reshape(4, -1)
         props = np.stack([rr["qext"], rr["qsca"], rr["qabs"], rr["qb"], rr["g"],
                           np.zeros_like(rr["qext"])])
-        sc = _integrate_scalars(xxarr, props, [psd[fi]], [1.0], lam, [reff0[fi]], rhop0, rhop)
+        sc = _integrate_scalars(xxarr, props, [psd[fi]], [1.0], lam, [reff0[fi]], rhop0, rhop,
+                                alias_quirk)
         th = dict(sc)
         th["p11"], th["p12"], th["p33"], th["p34"] = integ
         th["p22"], th["p44"] = integ[0].copy(), integ[2].copy()

@@ -48,6 +48,15 @@ enum gm_mie_flags {
                                   reference's start exactly */
 };
 
+/* integratePSD quirks (GM_TUNE_PSD_QUIRKS bits) */
+enum gm_psd_quirks {
+  GM_PSD_QUIRK_ALIAS = 1u   /* csca and cext weighted by area x Qsca, as the reference's aliased
+                               thisweight leaves them after key 'g' (dointegration.py:1111-1112,
+                               1154-1157); clear for area weighting like qext, qsca, qabs.  (The
+                               qb back-out, :1133-1169, equals the area-weighted average of Qb
+                               in exact arithmetic and needs no switch.) */
+};
+
 /* S1/S2 contraction precision */
 enum gm_precision {
   GM_PREC_FP64 = 0,     /* FP64 MFMA */
@@ -96,7 +105,10 @@ enum gm_tune_param {
                                    (large batches); 16, 32, 64 = each point's rows split over that
                                    many lanes, its serial recurrences joined by parallel scans
                                    (batches too small to fill the GPU); 0 (default) = chosen from
-                                   the batch size and series length.  Results agree to rounding */
+                                   the batch size and series length.  Results agree to rounding */,
+  GM_TUNE_PSD_QUIRKS = 9        /* a behaviour setting, not tuned for speed: the integratePSD
+                                   quirks gm_integrate_psd[_rows] reproduce (GM_PSD_QUIRK_* bits;
+                                   default GM_PSD_QUIRK_ALIAS) */
 };
 int gm_ctx_tune(gm_ctx* ctx, int param, int value);
 int gm_ctx_get_tune(gm_ctx* ctx, int param, int* value);
@@ -283,11 +295,22 @@ int gm_pmatrix_normalize(gm_ctx* ctx, int64_t n_entries, int n_ang, const double
  *   F [n_entries][6][n_ang] elements in table order p11, p12, p33, p34, p22, p44;
  *   ng  number of Gauss nodes = number of coefficients (NSPHER = 129 in the
  *       reference, params.h:14);
- *   quantize  nonzero: round to 10 decimals like the F17.10 text output;
+ *   quantize  bit 0: round to 10 decimals like the F17.10 text output;
+ *             bit 1 (GM_GSF_RAW): the coefficients before the 1/alpha1(0)
+ *             normalisation (the AL1..BET2 the program's MATR reads);
  *   out [n_entries][6][ng] normalised coefficients, same element order
  *       (alpha1, beta1, alpha3, beta2, alpha2, alpha4), i.e. pmom[p][m]. */
 int gm_gsf_expand(gm_ctx* ctx, int64_t n_entries, int n_ang, const double* ang_deg_host,
                   const double* F, int ng, int quantize, double* out, gm_stream stream);
+
+/* The fitted phase matrix back from expansion coefficients: MATR
+ * (src/gsf/spher_expan.f:419-517), which the program evaluates at the input
+ * angles for '<file>.expan_matr' (:85-91) and for its fit error (ERREVAL,
+ * :635-687, here left to the caller).  coef [n_entries][6][l1max] in
+ * gm_gsf_expand's order; theta_rad [n_ang] (device); out [n_entries][6][n_ang]
+ * in the table order p11, p12, p33, p34, p22, p44.  n_entries <= 65535. */
+int gm_gsf_matr(gm_ctx* ctx, int64_t n_entries, int l1max, const double* coef, int n_ang,
+                const double* theta_rad, double* out, gm_stream stream);
 
 /* Band means of table spectra (bandaverage.doAverage, src/geosmie/
  * bandaverage.py:18-50): out[s][b] = mean over the n_sub (<= 128) sample

@@ -85,6 +85,11 @@ def linterpol(xx, yy, x):
     i = 1
     while i < nn and not (xx[i] > x):
         i += 1
+    if i == nn:
+        # x equal to the last node: the program's loop runs past the grid and
+        # reads the zeros beyond it in its COMMON arrays, so the slope term is
+        # multiplied by x - xx[nn-1] = 0 and the last value comes back
+        return yy[nn - 1]
     return (yy[i] - yy[i - 1]) / (xx[i] - xx[i - 1]) * (x - xx[i - 1]) + yy[i - 1]
 
 
@@ -210,6 +215,39 @@ def matr(al, ang_rad):
             p4, pp4 = pp4, p
         out[:, j] = [f11, (f2 + f3) * 0.5, (f2 - f3) * 0.5, f44, f12, f34]
     return out
+
+
+def alt_angles(ang_deg):
+    """The program's second check grid (READMATRIX, spher_expan.f:240-247):
+    first and last angles kept, midpoints of each angle with its lower
+    neighbour in the first half and its upper neighbour in the second."""
+    a = np.asarray(ang_deg, dtype=np.float64)
+    n = a.size
+    alt = a.copy()
+    for i in range(2, n // 2 + 1):            # Fortran i = 2 .. NANG/2
+        alt[i - 1] = 0.5 * (a[i - 2] + a[i - 1])
+    for i in range(n // 2 + 1, n):            # Fortran i = NANG/2+1 .. NANG-1
+        alt[i - 1] = 0.5 * (a[i] + a[i - 1])
+    return alt
+
+
+def fit(ang_deg, f11, f22, f33, f44, f12, f34, ng=NSPHER):
+    """one_calc's fit check (spher_expan.f:157-177) for one matrix: the
+    unnormalised coefficients' MATR at the input angles (the '.expan_matr'
+    rows F11..F34) and the fit error max(MAXABS at the input angles, MAXABS at
+    the midpoint grid against the linearly interpolated F11) (ERREVAL,
+    :635-687, ERRTYP = MAXABS, 0-180 degrees).  Returns (matr [6, N], fiterr)."""
+    angl = np.asarray(ang_deg, dtype=np.float64) * D2R
+    x, w = gauss(ng)
+    F = [np.asarray(v, dtype=np.float64) for v in (f11, f22, f33, f44, f12, f34)]
+    Fn = [np.array([linterpol(angl, Fi, math.acos(xi)) for xi in x]) for Fi in F]
+    al = spher_expan(x, w, *Fn, l1max=ng)
+    alt = alt_angles(ang_deg) * D2R
+    f11alt = np.array([linterpol(angl, F[0], t) for t in alt])
+    m_alt = matr(al, alt)
+    m = matr(al, angl)
+    err = max(np.max(np.abs(F[0] - m[0])), np.max(np.abs(f11alt - m_alt[0])))
+    return m, err
 
 
 def hbleg(x, pmoms):

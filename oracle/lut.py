@@ -114,13 +114,14 @@ def raw_mie(xxarr, lam, size_range_out):
     return ret
 
 
-def integrate_psd(xxarr, rawret, psd, fracs, lam, reff0, rhop0, rhop):
+def integrate_psd(xxarr, rawret, psd, fracs, lam, reff0, rhop0, rhop, alias=True):
     """integratePSD with mixing = 'dumix' (dointegration.py:1064-1209).
 
     rawret: list (one per mode) of dicts with SCATKEYS planes [N_x, N_ang]
     and SCALARKEYS arrays [N_x], keys in SCATKEYS + SCALARKEYS order (the
     order rawMie builds, :1245, which fixes where the Qsca weighting of key
-    'g' starts to leak into later keys).
+    'g' starts to leak into later keys).  alias=False weights csca and cext
+    by area only (the switch the device path offers).
     """
     rrarr = np.asarray(xxarr) * lam / (2. * np.pi)
     retkeys = SCATKEYS + SCALARKEYS
@@ -153,6 +154,9 @@ def integrate_psd(xxarr, rawret, psd, fracs, lam, reff0, rhop0, rhop):
                 p11back = np.dot(thisp11.T, w) / beta
             if k == 'g':
                 weight = weight * np.asarray(rr['qsca'])   # persists for later keys
+            elif not alias and k in ('csca', 'cext'):
+                # the weighting without the alias: area only, as for qext, qsca
+                weight = np.pi * rrarr ** 2.
             sumarea = np.sum(w * weight)
             if k in SCATKEYS:
                 th[k] = np.dot(rr[k].T, w)

@@ -4,7 +4,8 @@ Runs oracle/_ref/spher_expan.x (compiled from src/gsf/spher_expan.f by
 oracle/Makefile) on synthetic phase matrices in the 371-angle runoptics grid
 (dointegration.py:739-742), written with np.savetxt exactly as the
 reference's convertncdf.py does (:177-189), and stores the '.expan_coeff'
-rows.
+rows, the '.expan_matr' rows (the fitted matrix back at the input angles,
+E15.5 / F11.5 text) and the fit error the program prints (ERREVAL, MAXABS).
 
 Inputs: Rayleigh, three Henyey-Greenstein-like matrices, and four Mie
 size-distribution matrices computed by the numpy oracle.
@@ -26,14 +27,19 @@ EXE = os.path.join(ROOT, "oracle", "_ref", "spher_expan.x")
 
 
 def run_ref(ang, f):
-    """f: [6, N] in program order F11, F22, F33, F44, F12, F34."""
+    """f: [6, N] in program order F11, F22, F33, F44, F12, F34.  Returns the
+    '.expan_coeff' rows, CNORM, the '.expan_matr' rows (angle, F11..F34 back
+    from the coefficients) and the fit error the program prints."""
     with tempfile.TemporaryDirectory() as d:
         fn = os.path.join(d, "m.txt")
         np.savetxt(fn, np.vstack([ang, f]).T)
-        subprocess.run([EXE, fn], stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL, check=True)
+        r = subprocess.run([EXE, fn], stdout=subprocess.PIPE, stderr=subprocess.DEVNULL,
+                           check=True, text=True)
         rows = np.loadtxt(fn + ".expan_coeff", skiprows=1, unpack=True)
         head = open(fn + ".expan_coeff").readline().split()
-    return rows[1:], float(head[1])
+        matr = np.loadtxt(fn + ".expan_matr", unpack=True)
+    fit = [ln for ln in r.stdout.splitlines() if "FINAL NG" in ln][0].split()
+    return rows[1:], float(head[1]), matr, float(fit[-1])
 
 
 def matrices():
@@ -63,16 +69,19 @@ def matrices():
 
 def main():
     ang, mats = matrices()
-    names, F, A, C = [], [], [], []
+    names, F, A, C, MT, FE = [], [], [], [], [], []
     for name, f in mats:
-        al, cn = run_ref(ang, f)
+        al, cn, mt, fe = run_ref(ang, f)
         names.append(name)
         F.append(f)
         A.append(al)
         C.append(cn)
-        print(name, cn, al[0, :4])
+        MT.append(mt)
+        FE.append(fe)
+        print(name, cn, al[0, :4], fe)
     np.savez_compressed(os.path.join(HERE, "gsf_vectors.npz"), names=np.array(names), ang=ang,
-                        F=np.array(F), coeffs=np.array(A), cnorm_text=np.array(C))
+                        F=np.array(F), coeffs=np.array(A), cnorm_text=np.array(C),
+                        matr=np.array(MT), fiterr=np.array(FE))
 
 
 if __name__ == "__main__":

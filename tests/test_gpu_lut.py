@@ -36,6 +36,25 @@ def test_integrate_psd_vs_reference_vectors(gpu):
         assert np.allclose(out[k], ref, rtol=1e-12, atol=0), (k, out[k], ref)
 
 
+def test_integrate_psd_alias_quirk_switch(gpu):
+    """With the alias quirk off, csca and cext are area-weighted (the oracle's
+    alias=False); every other key is unchanged, and csca, cext move by the
+    Qsca weighting the quirk adds."""
+    from geosmie_amd.lut import integratePSD
+    raw = _raw()
+    args = (LV["xx"], [raw, raw], list(LV["psd"]), list(LV["fracs"]), float(LV["lam"]),
+            list(LV["reff0"]), float(LV["rhop0"]), float(LV["rhop"]))
+    on = integratePSD(*args)
+    off = integratePSD(*args, alias_quirk=False)
+    ref = OL.integrate_psd(*args, alias=False)
+    for k in ["qext", "qsca", "qabs", "qb", "g", "csca", "cext", "bsca", "bext", "rEff"]:
+        assert np.allclose(off[k], ref[k], rtol=1e-12, atol=0), (k, off[k], ref[k])
+    for k in ["qext", "qsca", "qabs", "qb", "g", "bsca", "bext", "bbck"]:
+        assert off[k] == on[k], k
+    for k in ["csca", "cext"]:
+        assert abs(off[k] - on[k]) > 1e-6 * abs(on[k]), k
+
+
 def test_rawmie_vs_oracle(gpu):
     from geosmie_amd.lut import MultipleMie, rawMie, scatkeys, scalarkeys
     xx = LV["xx"][::3]

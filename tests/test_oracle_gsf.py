@@ -88,3 +88,22 @@ def test_oracle_vs_reference_program_live(tmp_path):
     ref = np.loadtxt(str(fn) + ".expan_coeff", skiprows=1, unpack=True)[1:]
     al, _ = gsf.expand(ang, *F)
     assert np.array_equal(al, ref)
+
+
+def test_fit_check_vs_reference_program():
+    """MATR of the unnormalised coefficients at the input angles against the
+    reference program's '.expan_matr' (printed E15.5 / F11.5), and the fit
+    error against the value it prints (MAXABS over both check grids)."""
+    for i, name in enumerate(GV["names"]):
+        m, err = gsf.fit(GV["ang"], *GV["F"][i])
+        ref = GV["matr"][i]
+        np.testing.assert_allclose(ref[0], GV["ang"], atol=0.006)
+        # F11..F44 printed E15.5 (mantissa 0.ddddd: half a unit is <= 5e-5 relative;
+        # at a zero crossing the printed value is rounding noise of the fit),
+        # F12 and F34 F11.5 (5e-6 absolute)
+        for k in range(4):
+            assert np.all(np.abs(m[k] - ref[k + 1]) <= 5.01e-5 * np.abs(ref[k + 1]) +
+                          1e-12 * np.abs(ref[k + 1]).max()), (name, k)
+        for k in (4, 5):
+            assert np.all(np.abs(m[k] - ref[k + 1]) <= 5.01e-6), (name, k)
+        assert abs(err - GV["fiterr"][i]) <= 1e-9 * GV["fiterr"][i] + 1e-15, (name, err)

@@ -101,3 +101,24 @@ def test_angle_grid():
     a = lut.angle_grid()
     assert a.size == 371 and a[0] == 0.0 and a[-1] == 180.0
     assert np.all(np.diff(a) > 0)
+
+
+def test_integrate_psd_alias_switch_closed_form():
+    """alias=False: csca = sum w (pi r^2)^2 Qsca / sum w pi r^2 (area weighted);
+    alias=True keeps the reference's extra Qsca factor on csca, cext."""
+    xx = np.logspace(-1, 1, 40)
+    lam = 0.5
+    rr = xx * lam / (2 * np.pi)
+    rng = np.random.default_rng(2)
+    q = rng.uniform(0.5, 2.0, (3, xx.size))
+    from oracle import lut as OL
+    raw = {k: np.ones((xx.size, 3)) for k in OL.SCATKEYS}
+    raw.update(qext=q[0] + q[1], qsca=q[1], qabs=q[0], qb=q[2], g=np.full(xx.size, 0.6),
+               csca=q[1] * np.pi * rr ** 2, cext=(q[0] + q[1]) * np.pi * rr ** 2)
+    w = rng.uniform(0.1, 1.0, xx.size)
+    off = OL.integrate_psd(xx, [raw], [w], [1.0], lam, [rr.mean()], 1000., 1000., alias=False)
+    on = OL.integrate_psd(xx, [raw], [w], [1.0], lam, [rr.mean()], 1000., 1000.)
+    a = w * np.pi * rr ** 2
+    assert np.isclose(off["csca"], np.sum(a * raw["csca"]) / a.sum(), rtol=1e-13)
+    assert np.isclose(on["csca"], np.sum(a * q[1] * raw["csca"]) / np.sum(a * q[1]), rtol=1e-13)
+    assert off["qext"] == on["qext"] and off["g"] == on["g"]
