@@ -3,7 +3,8 @@
 Times the caller's own workload (a callable that enqueues one pass on the
 current stream) under each candidate value of each knob -- coordinate
 search, one knob at a time, the others held at their best so far -- and
-leaves the fastest setting on the context.
+leaves the fastest setting on the context.  Choices can be kept in a JSON
+cache per (device, workload name) and reused without timing.
 
 Knobs (include/geosmie.h, enum gm_tune_param):
   mie_grid_mult  persistent Mie grid as a multiple of the resident wavefronts
@@ -22,6 +23,8 @@ Knobs (include/geosmie.h, enum gm_tune_param):
 setting reached through the same call; it is not tuned for speed.)
 """
 import ctypes
+import json
+import os
 import time
 
 from . import _lib
@@ -52,12 +55,59 @@ def set_knob(ctx, name, value):
     check(_lib.lib().gm_ctx_tune(ctx, KNOBS[name][0], int(value)), "gm_ctx_tune")
 
 
+def cache_path():
+    """The tuning cache file: $GM_AUTOTUNE_CACHE, else
+    ~/.cache/geosmie_amd/autotune.json."""
+    return os.environ.get("GM_AUTOTUNE_CACHE") or os.path.join(
+        os.path.expanduser("~"), ".cache", "geosmie_amd", "autotune.json")
+
+
+def _device_tag():
+    t = _lib.torch()
+    p = t.cuda.get_device_properties(t.cuda.current_device())
+    return "%s/%s" % (getattr(p, "gcnArchName", "") or p.name, p.multi_processor_count)
+
+
+def load_cache(path):
+    try:
+        with open(path) as f:
+            d = json.load(f)
+        return d if isinstance(d, dict) else {}
+    except (OSError, ValueError):
+        return {}
+
+
+def store_cache(path, entry_key, choices):
+    """Merge {knob: value} under entry_key into the JSON file at path (written
+    to a temporary file, then renamed)."""
+    d = load_cache(path)
+    d.setdefault(entry_key, {}).update({k: int(v) for k, v in choices.items()})
+    os.makedirs(os.path.dirname(os.path.abspath(path)), exist_ok=True)
+    tmp = path + ".tmp%d" % os.getpid()
+    with open(tmp, "w") as f:
+        json.dump(d, f, indent=1, sort_keys=True)
+    os.replace(tmp, path)
+
+
 def tune(ctx, workload, knobs=("mie_grid_mult", "s12_points", "s12_waves", "psd_chunk"), reps=3,
-         warmup=1, candidates=None):
+         warmup=1, candidates=None, cache_key=None, cache=None):
     """Returns {knob: {"best": value, "ms": {value: ms per pass}}}; the context
-    keeps the best values."""
+    keeps the best values.  With cache_key (a name for the workload), choices
+    already in the cache file for this device and workload are applied without
+    timing (reported with "cached": True), and new ones are saved there; cache
+    is the file (default cache_path())."""
     t = _lib.torch()
     out = {}
+    path = cache or cache_path()
+    entry = None
+    if cache_key is not None:
+        entry = "%s|%s" % (_device_tag(), cache_key)
+        known = load_cache(path).get(entry, {})
+        for name in list(knobs):
+            if name in known and known[name] in KNOBS[name][1]:
+                set_knob(ctx, name, known[name])
+                out[name] = {"best": known[name], "ms": {}, "cached": True}
+        knobs = [k for k in knobs if k not in out]
     for name in knobs:
         cands = (candidates or {}).get(name, KNOBS[name][1])
         ms = {}
@@ -74,4 +124,6 @@ def tune(ctx, workload, knobs=("mie_grid_mult", "s12_points", "s12_waves", "psd_
         best = min(ms, key=ms.get)
         set_knob(ctx, name, best)
         out[name] = {"best": best, "ms": ms}
+    if entry is not None and knobs:
+        store_cache(path, entry, {k: out[k]["best"] for k in knobs})
     return out

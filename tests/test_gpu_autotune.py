@@ -154,3 +154,22 @@ def test_homog_lane_variants_agree(gpu):
             scale = max(np.abs(a).max(), np.abs(b).max())
             assert np.abs(a - ga).max() <= 1e-10 * scale, lanes
             assert np.abs(b - gb).max() <= 1e-10 * scale, lanes
+
+
+def test_tune_cache_reuses_choices(gpu, tmp_path):
+    """A second tune() with the same workload name reads the cached choices
+    (no timing) and leaves them on a fresh context."""
+    from geosmie_amd import _lib, autotune
+    from geosmie_amd.mie import mie_batch
+    x = np.logspace(-1, 2, 300)
+    path = str(tmp_path / "tune.json")
+    h = _lib.ctx()
+    first = autotune.tune(h, lambda: mie_batch(x, m=1.5 + 0.01j), knobs=("mie_lanes",), reps=1,
+                          cache_key="homog300", cache=path)
+    assert not first["mie_lanes"].get("cached")
+    h2 = _lib.private_ctx()
+    again = autotune.tune(h2, lambda: None, knobs=("mie_lanes",), cache_key="homog300",
+                          cache=path)
+    assert again["mie_lanes"]["cached"] and again["mie_lanes"]["best"] == first["mie_lanes"]["best"]
+    assert autotune.get(h2, "mie_lanes") == first["mie_lanes"]["best"]
+    autotune.set_knob(h, "mie_lanes", 0)

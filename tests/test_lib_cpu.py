@@ -77,3 +77,20 @@ def test_missing_declared_symbol_raises_at_load(monkeypatch):
     monkeypatch.setattr(_lib, "_lib", None)
     with pytest.raises(_lib.GMError, match="gm_not_an_export"):
         _lib.lib()
+
+
+def test_autotune_cache_file(tmp_path):
+    """The tuning cache: entries merge per (device, workload) key, survive a
+    reload, and a damaged file reads as empty."""
+    from geosmie_amd import autotune
+    path = str(tmp_path / "sub" / "tune.json")
+    assert autotune.load_cache(path) == {}
+    autotune.store_cache(path, "gfx950/256|coated", {"mie_waves": 0})
+    autotune.store_cache(path, "gfx950/256|coated", {"mie_grid_mult": 2})
+    autotune.store_cache(path, "gfx950/256|homog", {"mie_lanes": 32})
+    d = autotune.load_cache(path)
+    assert d["gfx950/256|coated"] == {"mie_waves": 0, "mie_grid_mult": 2}
+    assert d["gfx950/256|homog"] == {"mie_lanes": 32}
+    with open(path, "w") as f:
+        f.write("{not json")
+    assert autotune.load_cache(path) == {}

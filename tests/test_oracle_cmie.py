@@ -1,6 +1,7 @@
 """Pins of the C restatement (oracle/cmie.c) against the scipy-based oracle and
 the arbitrary-precision golden vectors."""
 import os
+import subprocess
 
 import numpy as np
 import pytest
@@ -126,3 +127,22 @@ def test_chain_homog_vs_numpy_oracle():
     a, _ = cmie.chain_homog(x, m, r, row, w, 2, cost, threads=1)
     b, _ = cmie.chain_homog(x, m, r, row, w, 2, cost, threads=5)
     np.testing.assert_allclose(a, b, rtol=1e-13)
+
+
+def test_sanitizer_build_runs_clean():
+    """The C restatement built with AddressSanitizer and UndefinedBehaviorSanitizer
+    (make -C oracle sanitize) runs its driver (every coated and homogeneous path,
+    the chain, several OpenMP threads) without a report and with finite output."""
+    import shutil
+    if shutil.which("gcc") is None:
+        pytest.skip("gcc not available")
+    oracle_dir = os.path.join(os.path.dirname(HERE), "oracle")
+    subprocess.run(["make", "-s", "-C", oracle_dir, "sanitize"], check=True)
+    # verify_asan_link_order=0: the run may start with other libraries preloaded
+    env = dict(os.environ,
+               ASAN_OPTIONS="detect_leaks=1:abort_on_error=1:verify_asan_link_order=0",
+               UBSAN_OPTIONS="print_stacktrace=1:halt_on_error=1")
+    r = subprocess.run([os.path.join(oracle_dir, "_san", "cmie_check")], env=env,
+                       stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr[-2000:]
+    assert "ok" in r.stdout
