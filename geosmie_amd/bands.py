@@ -146,11 +146,16 @@ def average_table(variables, lam, mode, classic=False, noIR=False):
     return out, mean, lo, up, nb
 
 
-def fun(path, part, opfn, mode, useSolar=False, noIR=False):
+def fun(path, part, opfn, mode, useSolar=False, noIR=False, format="NETCDF4",
+        band_up_quirk=True):
     """Band-average one table file written by geosmie_amd.table (either
-    layout) into opfn (bandaverage.py:126-294; NetCDF-3, see table.py)."""
-    from scipy.io import netcdf_file
+    layout, NetCDF-4 or NetCDF-3) into opfn (bandaverage.py:126-294).
 
+    The output is NetCDF-4 as the reference writes it (:222-290: the averaged
+    variables compressed with zlib, the others not), or NetCDF-3 with
+    format="NETCDF3_64BIT_OFFSET".  band_up_quirk keeps the reference's
+    bandUp, which holds the lower band edges (:266-268 assign lBandLow to
+    it); False writes the upper edges."""
     from .table import read
     v = read(path)
     classic = "wavelength" not in v
@@ -166,35 +171,48 @@ def fun(path, part, opfn, mode, useSolar=False, noIR=False):
         lo, up = up_m[::-1], lo_m[::-1]
     lo = np.asarray(lo) * 1e6
     up = np.asarray(up) * 1e6
+    names = np.zeros((nbin, 80), "S1")
+    for i in range(nbin):
+        s = ("%s%03d" % (part, i + 1)).encode()
+        names[i, :len(s)] = [bytes([ch]) for ch in s]
+    edge_lo = lo[:nb]
+    edge_up = edge_lo if band_up_quirk else up[:nb]
+    scal = (rad_nm, "rh", lam_nm) if classic else (rad_nm, lam_nm, "rh")
+    # (name, type, dimensions, compressed, data, attributes) in the reference's order
+    plan = [(k, "f8", scal, True, out[k], {}) for k in VARS]
+    plan += [(lam_nm, "f8", (lam_nm,), False, np.asarray(mean), {}),
+             ("rh", "f8", ("rh",), False, rh, {}),
+             ("qname", "c", (rad_nm, "nchar"), False, names, {}),
+             ("bandLow", "f8", (lam_nm,), False, edge_lo,
+              {"long_name": "Lower edges of the bands", "units": "micrometers"}),
+             ("bandUp", "f8", (lam_nm,), False, edge_up,
+              {"long_name": "Upper edges of the bands", "units": "micrometers"}),
+             (rad_nm, "f8", (rad_nm,), False, radius.astype(np.float64), {})]
+    plan += [(k, "f8", (rad_nm,), False, v[k][0][:, 0], {}) for k in ("rLow", "rUp")]
+    plan += [(k, "f8", (rad_nm, "rh"), False, v[k][0], {}) for k in ("rEff", "rMass")]
+    dims = [("rh", rh.size), (lam_nm, nb), (rad_nm, nbin), ("nchar", 80)]
+    if format == "NETCDF4":
+        from . import nc4
+        with nc4.Writer(opfn) as w:
+            for d, n in dims:
+                w.dim(d, n)
+            for name, typ, dd, z, _, at in plan:
+                w.var(name, typ, dd, zlib=z, attrs=at)
+            for name, _, _, _, data, _ in plan:
+                w.put(name, data)
+        return opfn
+    if format != "NETCDF3_64BIT_OFFSET":
+        raise ValueError("format must be NETCDF4 or NETCDF3_64BIT_OFFSET, not %r" % (format,))
+    from scipy.io import netcdf_file
     f = netcdf_file(opfn, "w", version=2)
     try:
-        f.createDimension("rh", rh.size)
-        f.createDimension(lam_nm, nb)
-        f.createDimension(rad_nm, nbin)
-        f.createDimension("nchar", 80)
-        for k in VARS:
-            var = f.createVariable(k, "d", (rad_nm, "rh", lam_nm) if classic else
-                                   (rad_nm, lam_nm, "rh"))
-            var[:] = out[k]
-        f.createVariable(lam_nm, "d", (lam_nm,))[:] = np.asarray(mean)
-        f.createVariable("rh", "d", ("rh",))[:] = rh
-        q = f.createVariable("qname", "c", (rad_nm, "nchar"))
-        names = np.zeros((nbin, 80), "S1")
-        for i in range(nbin):
-            s = ("%s%03d" % (part, i + 1)).encode()
-            names[i, :len(s)] = [bytes([ch]) for ch in s]
-        q[:] = names
-        bl = f.createVariable("bandLow", "d", (lam_nm,))
-        bl[:] = lo[:nb]
-        bl.long_name, bl.units = "Lower edges of the bands", "micrometers"
-        bu = f.createVariable("bandUp", "d", (lam_nm,))
-        bu[:] = up[:nb]
-        bu.long_name, bu.units = "Upper edges of the bands", "micrometers"
-        f.createVariable(rad_nm, "d", (rad_nm,))[:] = radius.astype(np.float64)
-        for k in ("rLow", "rUp"):
-            f.createVariable(k, "d", (rad_nm,))[:] = v[k][0][:, 0]
-        for k in ("rEff", "rMass"):
-            f.createVariable(k, "d", (rad_nm, "rh"))[:] = v[k][0]
+        for d, n in dims:
+            f.createDimension(d, n)
+        for name, typ, dd, _, data, at in plan:
+            var = f.createVariable(name, "c" if typ == "c" else "d", dd)
+            var[:] = data
+            for k, t in at.items():
+                setattr(var, k, t)
     finally:
         f.close()
     return opfn

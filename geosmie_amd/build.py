@@ -16,6 +16,8 @@ SOURCES = ["gm_api.hip", "gm_util.hip", "gm_mie.hip", "gm_angles.hip", "gm_s12.h
            "gm_surrogate.hip", "gm_band.hip", "gm_grasp.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+CC = os.environ.get("CC", "gcc")
+NC4_LIB = os.path.join(HERE, "libgeosmie_nc4.so")
 
 # sources whose results must match numpy bit for bit: no fused multiply-add
 NO_CONTRACT = {"gm_band.hip", "gm_grasp.hip"}
@@ -59,7 +61,43 @@ def build(verbose=False):
         sys.stderr.write(r.stdout.decode(errors="replace"))
         raise RuntimeError("link failed")
     os.replace(tmp_lib, LIB)
+    build_nc4(verbose)
     return LIB
+
+
+def hdf5_dir():
+    """Prefix holding the HDF5 C library and headers (include/hdf5.h,
+    lib/libhdf5.so, lib/libhdf5_hl.so), or None."""
+    cands = [os.environ.get("GM_HDF5_DIR", ""), "/opt/conda", "/usr/local", "/usr"]
+    for c in cands:
+        if c and os.path.exists(os.path.join(c, "include", "hdf5.h")) and \
+                os.path.exists(os.path.join(c, "lib", "libhdf5.so")) and \
+                os.path.exists(os.path.join(c, "lib", "libhdf5_hl.so")):
+            return c
+    return None
+
+
+def build_nc4(verbose=False):
+    """The NetCDF-4 table writer/reader (plain C on HDF5); None when no HDF5
+    is found (tables can then be written only as NetCDF-3)."""
+    h5 = hdf5_dir()
+    if h5 is None:
+        sys.stderr.write("geosmie_amd.build: no HDF5 C library found (set GM_HDF5_DIR); "
+                         "libgeosmie_nc4.so not built\n")
+        return None
+    tmp_lib = NC4_LIB + ".tmp"
+    cmd = [CC, "-O2", "-std=c11", "-D_GNU_SOURCE", "-Wall", "-fPIC", "-shared",
+           "-I" + os.path.join(h5, "include"), "-I" + INCLUDE,
+           os.path.join(CSRC, "gm_nc4.c"), "-L" + os.path.join(h5, "lib"),
+           "-Wl,-rpath," + os.path.join(h5, "lib"), "-lhdf5_hl", "-lhdf5", "-o", tmp_lib]
+    if verbose:
+        print(" ".join(cmd))
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout.decode(errors="replace"))
+        raise RuntimeError("gcc failed on gm_nc4.c")
+    os.replace(tmp_lib, NC4_LIB)
+    return NC4_LIB
 
 
 if __name__ == "__main__":

@@ -28,8 +28,9 @@ negative; the reference flips the sign, :893), x [n_x] (a geometric grid), the
 scalar efficiencies qext, qabs, qsca, qb, g [n_shape, n_mr, n_mi, n_x] and
 scama [n_shape, n_mr, n_mi, n_x, 6, n_ang] with elements in the order p11,
 p22, p33, p44, p12, p34 (:899).  The kernel files of GRASP are NetCDF-4 and
-are not shipped with the reference; KernelTables.read takes NetCDF-3 files
-(scipy) or .npz files with the same variable names.
+are not shipped with the reference; KernelTables.read takes NetCDF-4 files
+(geosmie_amd.nc4), NetCDF-3 files (scipy) or .npz files with the same
+variable names.
 """
 import bisect
 
@@ -123,18 +124,23 @@ class KernelTables(object):
 
     @classmethod
     def read(cls, path, shape_dist):
-        """From a kernel file (NetCDF-3 or .npz with the file's variable names)
-        and a shape distribution file (first column, as np.loadtxt reads it at
-        dointegration.py:765)."""
+        """From a kernel file (NetCDF-4, NetCDF-3 or .npz with the file's
+        variable names) and a shape distribution file (first column, as
+        np.loadtxt reads it at dointegration.py:765)."""
         fracs = np.loadtxt(shape_dist, usecols=[0], unpack=True, ndmin=1)
         names = ("mr", "mi", "x", "scama") + SCALAR_KEYS
         if str(path).endswith(".npz"):
             with np.load(path, allow_pickle=False) as f:
                 var = {k: np.array(f[k]) for k in names}
         else:
-            from scipy.io import netcdf_file
-            with netcdf_file(path, "r", mmap=False) as f:
-                var = {k: np.array(f.variables[k][:]) for k in names}
+            from . import nc4
+            if nc4.is_hdf5(path):
+                v = nc4.read(path, attrs=())
+                var = {k: np.array(v[k][0]) for k in names}
+            else:
+                from scipy.io import netcdf_file
+                with netcdf_file(path, "r", mmap=False) as f:
+                    var = {k: np.array(f.variables[k][:]) for k in names}
         return cls(var, fracs)
 
     def queries(self, mr, mi):

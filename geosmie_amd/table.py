@@ -12,11 +12,13 @@ the GSF step adds (src/gsf/convertncdf.py:70-79, 381-394, zero-padded to m):
                (legacy: radius, rh, lambda order; pback (nPol, radius, rh, lambda);
                pmom (nPol, nMom, radius, rh, lambda))
 
-Differences from the reference files, forced by this image: netCDF4/HDF5 is
-not installed, so the file is NetCDF-3 (64-bit offset, scipy.io.netcdf_file),
-without zlib compression, and the integer index variables are int32 (NetCDF-3
-has no int64).  Names, dimension orders, units and long names are the
-reference's.
+The file is NetCDF-4 (HDF5) as the reference writes it: netCDF's HDF5
+conventions, every variable compressed with shuffle + deflate level 4
+(netCDF4-python's compression='zlib') and int64 bin and p indices, written by
+libgeosmie_nc4.so (geosmie_amd.nc4).  format="NETCDF3_64BIT_OFFSET" writes
+the NetCDF-3 form instead (scipy.io.netcdf_file: no compression, int32
+indices, since NetCDF-3 has no int64).  Names, dimension orders, units and
+long names are the reference's.
 """
 import numpy as np
 
@@ -195,7 +197,25 @@ class OpticsTable(object):
         else:
             self.data[key][radind, li, rhi] = value
 
-    def write(self, path):
+    def _coords(self):
+        r, lm, p = self.radius, self.lamname, self.pname
+        # dointegration.py:336-354: rh, wavelength, bin (i8), p (i8), ang
+        return [("rh", self.rh, "f8", ("fraction", "relative humidity")),
+                (lm, self.lam, "f8", ("m", "wavelength")),
+                (r, np.arange(1, self.n_bin + 1), "i8",
+                 ("dimensionless", "radius bin index (1-indexed)")),
+                (p, np.array(P_ORDER), "i8",
+                 ("dimensionless", "Scattering matrix element index, ordered as P11, P12, P33, "
+                                   "P34, P22, P44")),
+                ("ang", self.ang, "f8", ("degrees", "scattering angle"))]
+
+    def write(self, path, format="NETCDF4"):
+        """Write the table: NetCDF-4 with zlib as the reference does
+        (dointegration.py:205-354, 418), or format="NETCDF3_64BIT_OFFSET"."""
+        if format == "NETCDF4":
+            return self._write_nc4(path)
+        if format != "NETCDF3_64BIT_OFFSET":
+            raise ValueError("format must be NETCDF4 or NETCDF3_64BIT_OFFSET, not %r" % (format,))
         from scipy.io import netcdf_file
         f = netcdf_file(path, "w", version=2)
         try:
@@ -204,17 +224,9 @@ class OpticsTable(object):
                 f.createDimension(d, self.size[d])
             if self.n_mom:
                 f.createDimension(self.mname, self.n_mom)
-            coords = {r: (np.arange(1, self.n_bin + 1, dtype=np.int32), "i",
-                          ("dimensionless", "radius bin index (1-indexed)")),
-                      "rh": (self.rh, "d", ("fraction", "relative humidity")),
-                      lm: (self.lam, "d", ("m", "wavelength")),
-                      "ang": (self.ang, "d", ("degrees", "scattering angle")),
-                      p: (np.array(P_ORDER, dtype=np.int32), "i",
-                          ("dimensionless", "Scattering matrix element index, ordered as P11, "
-                                            "P12, P33, P34, P22, P44"))}
-            for name, (vals, typ, (units, lname)) in coords.items():
-                v = f.createVariable(name, typ, (name,))
-                v[:] = vals
+            for name, vals, typ, (units, lname) in self._coords():
+                v = f.createVariable(name, "i" if typ == "i8" else "d", (name,))
+                v[:] = np.asarray(vals, np.int32 if typ == "i8" else np.float64)
                 v.units = units
                 v.long_name = lname
             for k, dims in self.dims.items():
@@ -226,9 +238,35 @@ class OpticsTable(object):
         finally:
             f.close()
 
+    def _write_nc4(self, path):
+        from . import nc4
+        r, lm, p = self.radius, self.lamname, self.pname
+        with nc4.Writer(path) as w:
+            for d in ("rh", lm, r, p, "ang"):
+                w.dim(d, self.size[d])
+            if self.n_mom:
+                w.dim(self.mname, self.n_mom)   # convertncdf.py:70-79
+            coords = self._coords()
+            for name, _, typ, (units, lname) in coords:
+                w.var(name, typ, (name,), attrs={"long_name": lname, "units": units})
+            for k, dims in self.dims.items():
+                units, lname = META[k]
+                at = {"long_name": lname}
+                if units is not None:
+                    at["units"] = units
+                w.var(k, "f8", dims, attrs=at)
+            for name, vals, _, _ in coords:
+                w.put(name, vals)
+            for k in self.dims:
+                w.put(k, self.data[k])
+
 
 def read(path):
-    """Variables of a written table as numpy arrays (with their dimension names)."""
+    """Variables of a written table as numpy arrays, {name: (array, dimension
+    names, units)}; NetCDF-4 or NetCDF-3 by the file's signature."""
+    from . import nc4
+    if nc4.is_hdf5(path):
+        return {k: (a, dims, at.get("units", "")) for k, (a, dims, at) in nc4.read(path).items()}
     from scipy.io import netcdf_file
     f = netcdf_file(path, "r", mmap=False)
     try:

@@ -88,5 +88,15 @@ def test_runbands_file(gpu, tmp_path):
     np.testing.assert_array_equal(v["qext"][0], got["qext"])
     assert v["qext"][1] == ("bin", "wavelength", "rh")
     np.testing.assert_array_equal(v["wavelength"][0], np.asarray(mean))
-    assert np.all(v["bandUp"][0] > v["bandLow"][0])
+    # the reference's bandUp holds the lower edges (bandaverage.py:266-268)
+    np.testing.assert_array_equal(v["bandUp"][0], v["bandLow"][0])
+    assert v["bandUp"][2] == "micrometers"
     assert b"".join(v["qname"][0][0]).rstrip(b"\0") == b"T001"
+    out3 = B.fun(src, "T", str(tmp_path / "bands3.nc"), "RRTMG", format="NETCDF3_64BIT_OFFSET",
+                 band_up_quirk=False)
+    v3 = read(out3)
+    assert np.all(v3["bandUp"][0] > v3["bandLow"][0])
+    np.testing.assert_array_equal(v3["bandLow"][0], v["bandLow"][0])
+    for k in B.VARS + ["rEff", "rMass", "rLow", "rUp", "bin", "wavelength", "rh", "qname"]:
+        np.testing.assert_array_equal(v3[k][0], v[k][0], err_msg=k)
+        assert v3[k][1] == v[k][1], k

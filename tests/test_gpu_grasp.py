@@ -123,8 +123,10 @@ def test_kernel_table_vs_oracle(gpu, kernel_oracle, name):
 
 
 def test_kernel_files_npz_and_netcdf(gpu, kernel_oracle, tmp_path):
-    """KernelTables.read on .npz and NetCDF-3 kernel files with a shape
-    distribution file, through params['kernel_params'] as fun() sees them."""
+    """KernelTables.read on .npz, NetCDF-3 and NetCDF-4 kernel files with a
+    shape distribution file, through params['kernel_params'] as fun() sees
+    them."""
+    from geosmie_amd import nc4
     from scipy.io import netcdf_file
     from geosmie_amd.runoptics import build_table
     v, oracle = kernel_oracle
@@ -143,8 +145,19 @@ def test_kernel_files_npz_and_netcdf(gpu, kernel_oracle, tmp_path):
         for k in ("qext", "qabs", "qsca", "qb", "g"):
             f.createVariable(k, "d", ("shape", "mr", "mi", "x"))[:] = v[k]
         f.createVariable("scama", "d", ("shape", "mr", "mi", "x", "elem", "ang"))[:] = v["scama"]
+    nc4f = tmp_path / "k.nc4"
+    with nc4.Writer(str(nc4f)) as w:
+        for k, n in dims.items():
+            w.dim(k, n)
+        for k in ("mr", "mi", "x"):
+            w.var(k, "f8", (k,))
+        for k in ("qext", "qabs", "qsca", "qb", "g"):
+            w.var(k, "f8", ("shape", "mr", "mi", "x"))
+        w.var("scama", "f8", ("shape", "mr", "mi", "x", "elem", "ang"))
+        for k in ("mr", "mi", "x", "qext", "qabs", "qsca", "qb", "g", "scama"):
+            w.put(k, v[k])
     ent, nl = oracle["lognorm_simple"]
-    for path in (npz, nc):
+    for path in (npz, nc, nc4f):
         p = kernel_params(params("lognorm_simple"))
         p["kernel_params"] = {"path": str(path), "shape_dist": str(dist)}
         tab = build_table(p, water=water())

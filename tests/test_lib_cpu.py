@@ -29,6 +29,17 @@ def test_library_exports_every_declared_symbol():
     assert not unbound, unbound
 
 
+def test_nc4_library_exports_every_declared_symbol():
+    """libgeosmie_nc4.so (NetCDF-4 tables) against include/geosmie_nc4.h."""
+    from geosmie_amd import nc4
+    src = open(os.path.join(ROOT, "include", "geosmie_nc4.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    names = sorted(set(re.findall(r"\b(gmnc_[a-z0-9_]+)\s*\(", src)))
+    assert "gmnc_create" in names and "gmnc_get_var" in names
+    L = nc4.lib()
+    assert not [n for n in names if not hasattr(L, n)]
+
+
 def test_host_only_entry_points():
     from geosmie_amd import _lib
     L = _lib.lib()

@@ -15,8 +15,9 @@ def _ret(seed, n_ang):
     return ret
 
 
+@pytest.mark.parametrize("fmt", ["NETCDF4", "NETCDF3_64BIT_OFFSET"])
 @pytest.mark.parametrize("classic", [False, True])
-def test_layout_and_round_trip(tmp_path, classic):
+def test_layout_and_round_trip(tmp_path, classic, fmt):
     rh = np.array([0.0, 0.5, 0.9])
     lam = np.array([0.44e-6, 0.55e-6])
     ang = np.linspace(0, 180, 7)
@@ -32,7 +33,7 @@ def test_layout_and_round_trip(tmp_path, classic):
                     t.set_bin_rh(b, ri, ret)
                 placed[b, li, ri] = (ret, pm)
     path = str(tmp_path / "optics_TEST.nomom.nc")
-    t.write(path)
+    t.write(path, format=fmt)
     v = read(path)
     r, lm, p = ("radius", "lambda", "nPol") if classic else ("bin", "wavelength", "p")
     assert list(v[r][0]) == [1, 2, 3, 4] and list(v[p][0]) == P_ORDER
