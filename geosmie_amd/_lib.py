@@ -65,7 +65,8 @@ SIGNATURES = {
                                             c_vp, c_vp, c_vp, c_vp]),
     "gm_s12_contract_rows": (ctypes.c_int, [c_vp, c_i64, c_vp, c_vp, c_vp, c_vp, ctypes.c_int,
                                             c_vp, c_vp, ctypes.c_int, ctypes.c_int, c_vp, c_vp,
-                                            c_vp, c_vp, c_vp]),
+                                            c_vp, c_vp, ctypes.c_int, c_vp, c_vp]),
+    "gm_s12_split_coef": (ctypes.c_int, [c_vp, c_i64, c_vp, c_vp, ctypes.c_int, c_vp, c_vp]),
     "gm_lognorm_rows": (ctypes.c_int, [c_vp, c_i64, ctypes.c_int, c_vp, c_vp, c_vp, c_vp, c_vp,
                                        c_i64, c_vp]),
     "gm_integrate_psd_rows": (ctypes.c_int, [c_vp, c_i64, c_i64, c_vp, c_vp, c_vp, c_vp, c_vp,

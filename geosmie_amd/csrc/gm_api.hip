@@ -419,6 +419,11 @@ int gm_ctx_tune(gm_ctx* ctx, int param, int value) {
         return gm_fail(GM_EINVAL, "gm_ctx_tune: lanes per point must be 0, 1, 16, 32 or 64");
       ctx->tune_mie_lanes = value;
       return GM_OK;
+    case GM_TUNE_S12_LP_TILES:
+      if (value < 0 || value > 2)
+        return gm_fail(GM_EINVAL, "gm_ctx_tune: S1/S2 tiles per wavefront must be 0, 1 or 2");
+      ctx->tune_s12_lp_tiles = value;
+      return GM_OK;
     case GM_TUNE_PSD_QUIRKS:
       if (value & ~(int)GM_PSD_QUIRK_ALIAS)
         return gm_fail(GM_EINVAL, "gm_ctx_tune: unknown integratePSD quirk bits %d", value);
@@ -447,6 +452,7 @@ int gm_ctx_get_tune(gm_ctx* ctx, int param, int* value) {
     case GM_TUNE_MIE_CKPT: *value = ctx->tune_mie_ckpt; return GM_OK;
     case GM_TUNE_MIE_LANES: *value = ctx->tune_mie_lanes; return GM_OK;
     case GM_TUNE_PSD_QUIRKS: *value = ctx->psd_quirks; return GM_OK;
+    case GM_TUNE_S12_LP_TILES: *value = ctx->tune_s12_lp_tiles; return GM_OK;
     default: return gm_fail(GM_EINVAL, "gm_ctx_get_tune: unknown parameter %d", param);
   }
 }

@@ -720,6 +720,241 @@ __global__ __launch_bounds__(256) void k_s12_bf16x3(
   }
 }
 
+// ---- reduced precision, fused with the row sums ------------------------------
+//
+// The coefficients enter the MFMAs of every angle wavefront of a point; they
+// are converted once per call into 16 bytes per coefficient by
+// k_s12_split_coef, in the index order of a_n, b_n (coef_off + k):
+//   FP32   : float4 {U.re, V.re, U.im, V.im}  (float2 (U_c, V_c) at 2 c + comp)
+//   BF16X3 : uint2 per component {U_hi | U_lo << 16, V_hi | V_lo << 16}
+// with U = a + b, V = a - b formed in FP64 and split as k_s12_bf16x3 does, so
+// the products are bit for bit those of the unfused kernels.
+__global__ void k_s12_split_coef(int64_t n_coef, const double* __restrict__ an,
+                                 const double* __restrict__ bn, int precision,
+                                 uint4* __restrict__ out) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n_coef) return;
+  const double ar = an[2 * c], ai = an[2 * c + 1], br = bn[2 * c], bi = bn[2 * c + 1];
+  const double ur = ar + br, ui = ai + bi, vr = ar - br, vi = ai - bi;
+  uint4 o;
+  if (precision == GM_PREC_FP32) {
+    o.x = __float_as_uint((float)ur);
+    o.y = __float_as_uint((float)vr);
+    o.z = __float_as_uint((float)ui);
+    o.w = __float_as_uint((float)vi);
+  } else {
+    unsigned short h, l;
+    bf16_split(ur, h, l);
+    o.x = h | ((unsigned)l << 16);
+    bf16_split(vr, h, l);
+    o.y = h | ((unsigned)l << 16);
+    bf16_split(ui, h, l);
+    o.z = h | ((unsigned)l << 16);
+    bf16_split(vi, h, l);
+    o.w = h | ((unsigned)l << 16);
+  }
+  out[c] = o;
+}
+
+// S1/S2 of 8 MT points x 64 angles per wavefront on the FP32 or split-BF16
+// matrix cores (operands as in k_s12_f32 / k_s12_bf16x3, coefficients from
+// k_s12_split_coef), then the Mueller elements and their weighted row sums
+// in FP64 as k_s12_rows_f64 forms them; same partial buffer and reduction
+// (k_s12_rows_reduce).  A chunk is one group of MT 8-point tiles: each
+// angle-table word fetched feeds the products of all MT tiles, and the row
+// sums are formed once, after the K loop, so they hold no registers while
+// it runs.
+template <int PREC, int MT, int RM>
+__global__ __launch_bounds__(64 * kRowsWaves) void k_s12_rows_lp(
+    RowGroups g, const int64_t* __restrict__ off, const int32_t* __restrict__ nmax,
+    const uint4* __restrict__ coef, int n_ang, int apad, int kpad, const void* __restrict__ tab,
+    int nmax_cap, const double* __restrict__ w, int64_t ldw, double* __restrict__ part) {
+  const int lane = threadIdx.x & 63;
+  const int64_t chunk = blockIdx.x;
+  const int j0 = (blockIdx.y * kRowsWaves + (threadIdx.x >> 6)) * 64;
+  if (j0 >= n_ang) return;   // whole wavefront past the last angle (no barriers below)
+  int gi = 0;
+  while (gi + 1 < g.n_groups && g.chunk0[gi + 1] <= chunk) ++gi;
+  const int64_t p0 = g.start[gi] + (chunk - g.chunk0[gi]) * 8 * MT;
+  const int64_t cend = min(p0 + 8 * MT, g.start[gi] + g.count[gi]);
+  const int nr = g.nrow[gi];
+  const double* wr = w + (size_t)g.row0[gi] * ldw;
+  const int i = lane & 15;
+  const int kq = lane >> 4;
+  const int comp = i & 1;   // A row i of tile mt: point p0 + 8 mt + (i >> 1), component i & 1
+  constexpr int kStep = PREC == GM_PREC_FP32 ? 4 : kLpK;
+  int nm_a[MT];
+  int64_t oa[MT];
+#pragma unroll
+  for (int mt = 0; mt < MT; ++mt) {
+    const int64_t pa = p0 + 8 * mt + (i >> 1);
+    const bool ok = pa < cend;
+    nm_a[mt] = ok ? nmax[pa] : 0;
+    oa[mt] = ok ? off[pa] : 0;
+  }
+  int kmax = 0;
+  for (int q = 0; q < 8 * MT; ++q)
+    if (p0 + q < cend) kmax = max(kmax, nmax[p0 + q]);
+  kmax = (min(kmax, nmax_cap) + kStep - 1) / kStep * kStep;   // <= kpad
+  float4_ accp[MT][4], accm[MT][4];
+#pragma unroll
+  for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      accp[mt][t] = (float4_){0.f, 0.f, 0.f, 0.f};
+      accm[mt][t] = (float4_){0.f, 0.f, 0.f, 0.f};
+    }
+  if constexpr (PREC == GM_PREC_FP32) {
+    const float2* cf = (const float2*)coef;
+    const float* Pt = (const float*)tab;
+    const float* Mt = Pt + (size_t)kpad * apad;
+    float2 uv_n[MT];
+    float p_n[4], m_n[4];
+    auto load_step = [&](int k0) {
+      const int kk = k0 + kq;
+#pragma unroll
+      for (int mt = 0; mt < MT; ++mt)
+        uv_n[mt] = kk < nm_a[mt] ? cf[2 * (oa[mt] + kk) + comp] : make_float2(0.f, 0.f);
+      const size_t row = (size_t)kk * apad + j0 + i;
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        p_n[t] = Pt[row + 16 * t];
+        m_n[t] = Mt[row + 16 * t];
+      }
+    };
+    if (kmax > 0) load_step(0);
+    for (int k0 = 0; k0 < kmax; k0 += 4) {
+      float ua[MT], va[MT], P[4], M[4];
+#pragma unroll
+      for (int mt = 0; mt < MT; ++mt) {
+        ua[mt] = uv_n[mt].x;
+        va[mt] = uv_n[mt].y;
+      }
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        P[t] = p_n[t];
+        M[t] = m_n[t];
+      }
+      if (k0 + 4 < kmax) load_step(k0 + 4);
+#pragma unroll
+      for (int t = 0; t < 4; ++t)
+#pragma unroll
+        for (int mt = 0; mt < MT; ++mt) {
+          accp[mt][t] = __builtin_amdgcn_mfma_f32_16x16x4f32(ua[mt], P[t], accp[mt][t], 0, 0, 0);
+          accm[mt][t] = __builtin_amdgcn_mfma_f32_16x16x4f32(va[mt], M[t], accm[mt][t], 0, 0, 0);
+        }
+    }
+  } else {
+    const uint2* cb = (const uint2*)coef;
+    const uint4* tb = (const uint4*)tab;   // 8 BF16 (one K-octet) per uint4
+    const size_t plane4 = (size_t)kpad * apad / 8;
+    // next step's operands: each 16-angle sub-tile's table words are reloaded
+    // as soon as that sub-tile's products are issued
+    uint4 tn[4][4];
+    uint2 cn[MT][8];
+    auto load_coef = [&](int k0) {
+      const int kb = k0 + 8 * kq;
+#pragma unroll
+      for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+        for (int t = 0; t < 8; ++t)
+          cn[mt][t] = kb + t < nm_a[mt] ? cb[2 * (oa[mt] + kb + t) + comp] : make_uint2(0u, 0u);
+    };
+    auto load_tab = [&](int k0, int t) {
+      const size_t o4 = (size_t)((k0 + 8 * kq) >> 3) * apad + j0 + i + 16 * t;
+#pragma unroll
+      for (int pl = 0; pl < 4; ++pl) tn[t][pl] = tb[pl * plane4 + o4];
+    };
+    if (kmax > 0) {
+      load_coef(0);
+#pragma unroll
+      for (int t = 0; t < 4; ++t) load_tab(0, t);
+    }
+    for (int k0 = 0; k0 < kmax; k0 += kLpK) {
+      union { bf16x8_ v; unsigned u[4]; } uh[MT], ul[MT], vh[MT], vl[MT];
+#pragma unroll
+      for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+          const uint2 a = cn[mt][2 * t], b = cn[mt][2 * t + 1];
+          uh[mt].u[t] = (a.x & 0xffffu) | (b.x << 16);
+          ul[mt].u[t] = (a.x >> 16) | (b.x & 0xffff0000u);
+          vh[mt].u[t] = (a.y & 0xffffu) | (b.y << 16);
+          vl[mt].u[t] = (a.y >> 16) | (b.y & 0xffff0000u);
+        }
+      const bool more = k0 + kLpK < kmax;
+      if (more) load_coef(k0 + kLpK);
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        union { uint4 q; bf16x8_ v; } ph, pl, mh, ml;
+        ph.q = tn[t][0];
+        pl.q = tn[t][1];
+        mh.q = tn[t][2];
+        ml.q = tn[t][3];
+        if (more) load_tab(k0 + kLpK, t);
+#pragma unroll
+        for (int mt = 0; mt < MT; ++mt) {
+          accp[mt][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(uh[mt].v, ph.v, accp[mt][t], 0, 0, 0);
+          accp[mt][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(uh[mt].v, pl.v, accp[mt][t], 0, 0, 0);
+          accp[mt][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ul[mt].v, ph.v, accp[mt][t], 0, 0, 0);
+          accm[mt][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(vh[mt].v, mh.v, accm[mt][t], 0, 0, 0);
+          accm[mt][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(vh[mt].v, ml.v, accm[mt][t], 0, 0, 0);
+          accm[mt][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(vl[mt].v, mh.v, accm[mt][t], 0, 0, 0);
+        }
+      }
+    }
+  }
+  // D[4 kq + r][i] of tile mt: elements (0, 1) = point p0 + 8 mt + 2 kq
+  // (re, im), (2, 3) = the next; the row sums over the chunk's points in
+  // point order, then over the lane groups
+  double* dst = part + (size_t)chunk * RM * 4 * n_ang;
+  double wv[MT][2][RM];
+#pragma unroll
+  for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int64_t p = p0 + 8 * mt + 2 * kq + h;
+#pragma unroll
+      for (int r = 0; r < RM; ++r)
+        wv[mt][h][r] = (r < nr && p < cend) ? wr[(size_t)r * ldw + p] : 0.0;
+    }
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    double el[MT][2][4];
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const cd sp = cd{(double)accp[mt][t][2 * h], (double)accp[mt][t][2 * h + 1]};
+        const cd sm = cd{(double)accm[mt][t][2 * h], (double)accm[mt][t][2 * h + 1]};
+        const cd s1 = (sp + sm) * 0.5, s2 = (sp - sm) * 0.5;
+        const double a1 = norm2(s1), a2 = norm2(s2);
+        const cd c = s1 * conj(s2);
+        el[mt][h][0] = 0.5 * (a1 + a2);
+        el[mt][h][1] = 0.5 * (a2 - a1);
+        el[mt][h][2] = c.re;
+        el[mt][h][3] = c.im;
+      }
+    const int j = j0 + t * 16 + i;
+#pragma unroll
+    for (int r = 0; r < RM; ++r) {
+      double mine = 0.0;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        double v = 0.0;
+#pragma unroll
+        for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+          for (int h = 0; h < 2; ++h) v = fma(wv[mt][h][r], el[mt][h][e], v);
+        v += __shfl_xor(v, 16);
+        v += __shfl_xor(v, 32);
+        if (e == kq) mine = v;
+      }
+      if (r < nr && j < n_ang) dst[((size_t)r * 4 + kq) * n_ang + j] = mine;
+    }
+  }
+}
+
 }  // namespace gm
 
 namespace {
@@ -826,18 +1061,42 @@ int gm_s12_mueller(gm_ctx* ctx, int64_t n, const int64_t* coef_off, const int32_
                     (hipStream_t)stream);
 }
 
+int gm_s12_split_coef(gm_ctx* ctx, int64_t n_coef, const double* an, const double* bn,
+                      int precision, void* coef_lp, gm_stream stream) {
+  if (!ctx || n_coef < 0 || (precision != GM_PREC_FP32 && precision != GM_PREC_BF16X3))
+    return gm_fail(GM_EINVAL, "gm_s12_split_coef: bad arguments (precision FP32 or BF16X3)");
+  if (n_coef == 0) return GM_OK;
+  if (!an || !bn || !coef_lp) return gm_fail(GM_EINVAL, "gm_s12_split_coef: NULL array");
+  hipError_t e = hipSetDevice(ctx->device);
+  if (e != hipSuccess) return gm_fail(GM_ERUNTIME, "hipSetDevice: %s", hipGetErrorString(e));
+  size_t slot;
+  ctx->mark_begin(gm::kKS12Prep, (hipStream_t)stream, &slot);
+  hipLaunchKernelGGL(gm::k_s12_split_coef, dim3((unsigned)((n_coef + 255) / 256)), dim3(256), 0,
+                     (hipStream_t)stream, n_coef, an, bn, precision, (uint4*)coef_lp);
+  ctx->mark_end(slot, (hipStream_t)stream);
+  e = hipGetLastError();
+  if (e != hipSuccess) return gm_fail(GM_ERUNTIME, "gm_s12_split_coef: %s", hipGetErrorString(e));
+  return GM_OK;
+}
+
 int gm_s12_contract_rows(gm_ctx* ctx, int64_t n, const int64_t* coef_off, const int32_t* nmax,
                          const double* an, const double* bn, int n_ang, const double* pi_t,
                          const double* tau_t, int nmax_cap, int n_rows, const int64_t* start_host,
-                         const int64_t* count_host, const double* w, double* out,
-                         gm_stream stream) {
+                         const int64_t* count_host, const double* w, double* out, int precision,
+                         const void* coef_lp, gm_stream stream) {
   if (!ctx || n < 0 || n_ang < 0 || n_rows < 0 || n_rows > gm::kRowsPerCall)
     return gm_fail(GM_EINVAL, "gm_s12_contract_rows: bad arguments (at most %d rows)",
                    gm::kRowsPerCall);
+  if (precision != GM_PREC_FP64 && precision != GM_PREC_FP32 && precision != GM_PREC_BF16X3)
+    return gm_fail(GM_EINVAL, "gm_s12_contract_rows: unknown precision %d", precision);
   if (n_rows == 0 || n_ang == 0) return GM_OK;
-  if (!start_host || !count_host || !coef_off || !nmax || !an || !bn || !pi_t || !tau_t || !w ||
-      !out)
+  if (!start_host || !count_host || !coef_off || !nmax || !pi_t || !tau_t || !w || !out)
     return gm_fail(GM_EINVAL, "gm_s12_contract_rows: NULL array");
+  if (precision == GM_PREC_FP64 ? (!an || !bn) : !coef_lp)
+    return gm_fail(GM_EINVAL, precision == GM_PREC_FP64
+                                  ? "gm_s12_contract_rows: NULL a_n / b_n"
+                                  : "gm_s12_contract_rows: reduced precision needs the "
+                                    "coefficients of gm_s12_split_coef (coef_lp)");
   if (nmax_cap < 1) return gm_fail(GM_EINVAL, "gm_s12_contract_rows: nmax_cap must be >= 1");
   // rows with the same range form a group (at most 4 rows; a longer run of
   // equal ranges is split, its points then contracted once per group).  One
@@ -846,7 +1105,14 @@ int gm_s12_contract_rows(gm_ctx* ctx, int64_t n, const int64_t* coef_off, const 
   for (int r = 0; r < n_rows; ++r)
     if (count_host[r] > 0) tiles += (count_host[r] + 7) / 8;
   const int64_t ablocks = (n_ang + 64 * gm::kRowsWaves - 1) / (64 * gm::kRowsWaves);
-  const int chunk_pts = (tiles / 2) * ablocks * gm::kRowsWaves >= 16384 ? 16 : 8;
+  // FP64: one 8-point tile per chunk, two when that still leaves >= 16k
+  // wavefronts; reduced precision: the chunk is the kernel's MT tiles (two
+  // when that leaves >= 8k wavefronts, so each table word feeds both)
+  const bool wide = (tiles / 2) * ablocks * gm::kRowsWaves >= 16384;
+  const int lp_mt = ctx->tune_s12_lp_tiles
+                        ? ctx->tune_s12_lp_tiles
+                        : ((tiles / 2) * ablocks * gm::kRowsWaves >= 8192 ? 2 : 1);
+  const int chunk_pts = precision == GM_PREC_FP64 ? (wide ? 16 : 8) : 8 * lp_mt;
   gm::RowGroups g;
   g.n_groups = 0;
   g.chunk0[0] = 0;
@@ -886,17 +1152,51 @@ int gm_s12_contract_rows(gm_ctx* ctx, int64_t n, const int64_t* coef_off, const 
   const dim3 rgrid((unsigned)((int64_t)n_rows * ((4 * (int64_t)n_ang + 63) / 64)));
   const dim3 rblock(64 * gm::kReduceWaves);
   size_t slot;
-  ctx->mark_begin(gm::kKS12, s, &slot);
-  if (RM == 1)
-    hipLaunchKernelGGL(gm::k_s12_rows_f64<1>, grid, block, 0, s, g, chunk_pts, coef_off, nmax,
-                       an, bn, n_ang, pi_t, tau_t, nmax_cap, w, n, part);
-  else if (RM == 2)
-    hipLaunchKernelGGL(gm::k_s12_rows_f64<2>, grid, block, 0, s, g, chunk_pts, coef_off, nmax,
-                       an, bn, n_ang, pi_t, tau_t, nmax_cap, w, n, part);
-  else
-    hipLaunchKernelGGL(gm::k_s12_rows_f64<4>, grid, block, 0, s, g, chunk_pts, coef_off, nmax,
-                       an, bn, n_ang, pi_t, tau_t, nmax_cap, w, n, part);
-  ctx->mark_end(slot, s);
+  if (precision == GM_PREC_FP64) {
+    ctx->mark_begin(gm::kKS12, s, &slot);
+    if (RM == 1)
+      hipLaunchKernelGGL(gm::k_s12_rows_f64<1>, grid, block, 0, s, g, chunk_pts, coef_off, nmax,
+                         an, bn, n_ang, pi_t, tau_t, nmax_cap, w, n, part);
+    else if (RM == 2)
+      hipLaunchKernelGGL(gm::k_s12_rows_f64<2>, grid, block, 0, s, g, chunk_pts, coef_off, nmax,
+                         an, bn, n_ang, pi_t, tau_t, nmax_cap, w, n, part);
+    else
+      hipLaunchKernelGGL(gm::k_s12_rows_f64<4>, grid, block, 0, s, g, chunk_pts, coef_off, nmax,
+                         an, bn, n_ang, pi_t, tau_t, nmax_cap, w, n, part);
+    ctx->mark_end(slot, s);
+  } else {
+    // P, M tables of this precision (padded to whole K-steps and 256 angles)
+    const int kpad = (nmax_cap + gm::kLpK - 1) / gm::kLpK * gm::kLpK;
+    const int apad = (n_ang + 255) / 256 * 256;
+    const size_t elems = (size_t)kpad * apad;
+    const size_t bytes = precision == GM_PREC_FP32 ? 2 * elems * sizeof(float)
+                                                   : 4 * elems * sizeof(unsigned short);
+    rc = ctx->reserve(ctx->tab_lp, bytes);
+    if (rc) return rc;
+    ctx->mark_begin(gm::kKS12, s, &slot);
+    hipLaunchKernelGGL(gm::k_s12_prep_tables, dim3((unsigned)((elems + 255) / 256)), dim3(256), 0,
+                       s, n_ang, nmax_cap, kpad, apad, pi_t, tau_t, precision,
+                       (float*)ctx->tab_lp.ptr, (unsigned short*)ctx->tab_lp.ptr);
+    const uint4* cf = (const uint4*)coef_lp;
+    const void* tb = ctx->tab_lp.ptr;
+#define GM_ROWS_LP(P_, M_, R_)                                                                \
+  hipLaunchKernelGGL((gm::k_s12_rows_lp<P_, M_, R_>), grid, block, 0, s, g, coef_off, nmax, cf,  \
+                     n_ang, apad, kpad, tb, nmax_cap, w, n, part)
+#define GM_ROWS_LP_RM(P_, M_)        \
+  if (RM == 1) GM_ROWS_LP(P_, M_, 1); \
+  else if (RM == 2) GM_ROWS_LP(P_, M_, 2); \
+  else GM_ROWS_LP(P_, M_, 4)
+    if (precision == GM_PREC_FP32) {
+      if (lp_mt == 2) { GM_ROWS_LP_RM(GM_PREC_FP32, 2); }
+      else { GM_ROWS_LP_RM(GM_PREC_FP32, 1); }
+    } else {
+      if (lp_mt == 2) { GM_ROWS_LP_RM(GM_PREC_BF16X3, 2); }
+      else { GM_ROWS_LP_RM(GM_PREC_BF16X3, 1); }
+    }
+#undef GM_ROWS_LP_RM
+#undef GM_ROWS_LP
+    ctx->mark_end(slot, s);
+  }
   ctx->mark_begin(gm::kKPsd, s, &slot);
   if (RM == 1)
     hipLaunchKernelGGL(gm::k_s12_rows_reduce<1>, rgrid, rblock, 0, s, g, n_ang,

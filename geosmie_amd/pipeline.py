@@ -8,11 +8,12 @@ runoptics' per-(wavelength, rh) iteration computes
 
   gm_coef_offsets, gm_mie_batch          a_n, b_n and efficiencies (FP64)
   (gm_angle_tables                        pi'_n, tau'_n: once, at construction)
+  (gm_s12_split_coef                       FP32 / split-BF16: U = a + b, V = a - b
+                                          converted once)
   gm_s12_contract_rows                    S1/S2 on MFMA -> s11, s12, s33, s34 ->
                                           phase-matrix integrals of integratePSD,
-                                          in one pass (FP64); at reduced
-                                          precision gm_s12_mueller, then
-                                          gm_psd_contract_rows
+                                          in one pass (fused=False: gm_s12_mueller,
+                                          then gm_psd_contract_rows)
   gm_integrate_psd_rows                   the integratePSD scalars of every row
   gm_pmatrix_normalize                    P normalisation and pback
   gm_gsf_expand (optional)                GSF moments of every row
@@ -37,8 +38,9 @@ class RunopticsChain(object):
         """x [n] size parameters, m complex index (scalar or [n]), ang [n_ang]
         degrees, w [rows, n] number weights (row r nonzero only on its points
         ranges[r] = (start, count)), lam_of_row [rows] wavelengths.  fused:
-        at FP64 (precision 0) S1/S2 and the row integrals in one pass, without
-        the [n][4][n_ang] Mueller planes."""
+        S1/S2 and the row integrals in one pass, without the [n][4][n_ang]
+        Mueller planes (at FP32 / split-BF16 on coefficients converted once
+        per step by gm_s12_split_coef)."""
         t = _lib.torch()
         self.t = t
         x = np.ascontiguousarray(x, dtype=np.float64)
@@ -75,6 +77,7 @@ class RunopticsChain(object):
         self.d_pi = t.empty(self.nmax_cap * self.n_ang, dtype=f64, device=dev)
         self.d_tau = t.empty_like(self.d_pi)
         self.d_mu = None   # Mueller planes, allocated when a step needs them
+        self.d_cl = None   # converted coefficients (16 B per pair), reduced precision
         self.d_pm = t.empty(self.rows * 4 * self.n_ang, dtype=f64, device=dev)
         self.d_back = t.empty(self.rows * 6, dtype=f64, device=dev)
         self.d_sc = t.empty(self.rows * 16, dtype=f64, device=dev)
@@ -103,12 +106,20 @@ class RunopticsChain(object):
         check(L.gm_mie_batch(h, self.n, P(self.d_x), None, P(self.d_eps), None, None,
                              self.size_max, P(self.d_off), P(self.d_an), P(self.d_bn),
                              P(self.d_props), P(self.d_nmax), _lib.GM_MIE_SORT, s), "gm_mie_batch")
-        if self.fused and self.precision == 0:
+        if self.fused:
+            cl = None
+            if self.precision:
+                if self.d_cl is None:
+                    self.d_cl = self.t.empty(2 * self.sum_nmax, dtype=self.t.float64,
+                                             device=_lib.device())
+                check(L.gm_s12_split_coef(h, self.sum_nmax, P(self.d_an), P(self.d_bn),
+                                          self.precision, P(self.d_cl), s), "gm_s12_split_coef")
+                cl = P(self.d_cl)
             check(L.gm_s12_contract_rows(h, self.n, P(self.d_off), P(self.d_nmax), P(self.d_an),
                                          P(self.d_bn), self.n_ang, P(self.d_pi), P(self.d_tau),
                                          self.nmax_cap, self.rows, self._seg_start,
-                                         self._seg_count, P(self.d_w), P(self.d_pm), s),
-                  "gm_s12_contract_rows")
+                                         self._seg_count, P(self.d_w), P(self.d_pm),
+                                         self.precision, cl, s), "gm_s12_contract_rows")
         else:
             if self.d_mu is None:
                 self.d_mu = self.t.empty(self.n * 4 * self.n_ang, dtype=self.t.float64,

@@ -113,7 +113,7 @@ class _WavelengthPass(object):
                 check(L.gm_s12_contract_rows(h, n, ptr(dm.d_off), ptr(dm.d_nmax), ptr(dm.d_an),
                                              ptr(dm.d_bn), self.n_ang, ptr(self.pi_t),
                                              ptr(self.tau_t), cap, k, starts, counts,
-                                             ptr(d_w[r0:r0 + k]), out, s),
+                                             ptr(d_w[r0:r0 + k]), out, 0, None, s),
                       "gm_s12_contract_rows")
             else:
                 check(L.gm_psd_contract_rows(h, k, starts, counts, n, n_cols,

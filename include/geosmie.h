@@ -106,9 +106,12 @@ enum gm_tune_param {
                                    many lanes, its serial recurrences joined by parallel scans
                                    (batches too small to fill the GPU); 0 (default) = chosen from
                                    the batch size and series length.  Results agree to rounding */,
-  GM_TUNE_PSD_QUIRKS = 9        /* a behaviour setting, not tuned for speed: the integratePSD
+  GM_TUNE_PSD_QUIRKS = 9,       /* a behaviour setting, not tuned for speed: the integratePSD
                                    quirks gm_integrate_psd[_rows] reproduce (GM_PSD_QUIRK_* bits;
                                    default GM_PSD_QUIRK_ALIAS) */
+  GM_TUNE_S12_LP_TILES = 10     /* FP32 / split-BF16 gm_s12_contract_rows: 8-point tiles per
+                                   wavefront (1, 2; 0 = by batch size, the default); results
+                                   agree to rounding */
 };
 int gm_ctx_tune(gm_ctx* ctx, int param, int value);
 int gm_ctx_get_tune(gm_ctx* ctx, int param, int* value);
@@ -119,7 +122,8 @@ int gm_ctx_get_tune(gm_ctx* ctx, int param, int* value);
  * synchronises those events and returns the summed milliseconds and launch
  * count of `kernel` (0 homogeneous Mie, 1 coated Mie, 2 S1/S2 contraction,
  * 3 PSD contraction, 4 GSF expansion, 5 MLP, 6 coated Mie with absorbing
- * shells, 7 LUT bulk reduction), then forgets them. */
+ * shells, 7 LUT bulk reduction, 8 S1/S2 operand conversion), then forgets
+ * them. */
 int gm_ctx_set_timing(gm_ctx* ctx, int enable);
 int gm_ctx_kernel_time(gm_ctx* ctx, int kernel, double* ms_total, int64_t* launches);
 
@@ -240,22 +244,35 @@ int gm_psd_contract_rows(gm_ctx* ctx, int n_rows, const int64_t* start_host,
                          const int64_t* count_host, int64_t n_pts, int n_cols, const double* w,
                          const double* v, double* out, gm_stream stream);
 
-/* gm_s12_mueller followed by gm_psd_contract_rows, fused (FP64): out[r][e][j]
+/* gm_s12_mueller followed by gm_psd_contract_rows, fused: out[r][e][j]
  * = sum over p in [start_host[r], start_host[r] + count_host[r]) of w[r][p]
  * s_e(p, j), e = s11, s12, s33, s34, without materialising the Mueller planes
  * (calculateScatVals then integratePSD's phase-matrix sums,
  * src/geosmie/dointegration.py:1044-1050, 1164-1166).  Arguments as
  * gm_s12_mueller (coefficients of n points) and gm_psd_contract_rows (n_rows
  * <= 64, ranges on the host, w [n_rows][n] device); out [n_rows][4][n_ang].
- * Consecutive rows with the same range share one S1/S2 pass (up to 4 rows
- * each), so it suits rows that overlap little, as runoptics' per-rh rows do;
- * many rows over the same points are cheaper through gm_s12_mueller and
- * gm_psd_contract.  Fixed summation order. */
+ * precision: GM_PREC_FP64 reads an, bn; GM_PREC_FP32 and GM_PREC_BF16X3 read
+ * coef_lp, the same coefficients converted by gm_s12_split_coef (an, bn may
+ * then be NULL); the Mueller elements and the sums are FP64 at every
+ * precision.  Consecutive rows with the same range share one S1/S2 pass (up
+ * to 4 rows each), so it suits rows that overlap little, as runoptics'
+ * per-rh rows do; many rows over the same points are cheaper through
+ * gm_s12_mueller and gm_psd_contract.  Fixed summation order. */
 int gm_s12_contract_rows(gm_ctx* ctx, int64_t n, const int64_t* coef_off, const int32_t* nmax,
                          const double* an, const double* bn, int n_ang, const double* pi_t,
                          const double* tau_t, int nmax_cap, int n_rows, const int64_t* start_host,
-                         const int64_t* count_host, const double* w, double* out,
-                         gm_stream stream);
+                         const int64_t* count_host, const double* w, double* out, int precision,
+                         const void* coef_lp, gm_stream stream);
+
+/* The coefficients of n_coef (a_n, b_n) pairs (an, bn: 2 n_coef doubles,
+ * indexed as gm_mie_batch writes them) in the form the reduced-precision
+ * S1/S2 passes multiply: U = a + b, V = a - b, formed in FP64, then as FP32
+ * (precision GM_PREC_FP32) or as BF16 hi + lo pairs (GM_PREC_BF16X3); 16
+ * bytes per pair into coef_lp (device, 16 n_coef bytes).  Converting once
+ * per call rather than in every angle wavefront is what this saves
+ * (mie_props.py:133-149 forms the same sums per angle). */
+int gm_s12_split_coef(gm_ctx* ctx, int64_t n_coef, const double* an, const double* bn,
+                      int precision, void* coef_lp, gm_stream stream);
 
 /* integratePSD scalars (src/geosmie/dointegration.py:1064-1209, mixing
  * 'dumix', including its Qsca weighting of g, csca, cext from the aliased

@@ -129,7 +129,7 @@ def test_fused_rows_large_x_vs_mp(gpu):
     _lib.check(_lib.lib().gm_s12_contract_rows(
         _lib.ctx(), n, _lib.ptr(d_off), _lib.ptr(d_nmax), _lib.ptr(d_an), _lib.ptr(d_bn), n_ang,
         _lib.ptr(d_pi), _lib.ptr(d_tau), nmax_cap, n, start, count, _lib.ptr(d_w),
-        _lib.ptr(d_out), _lib.stream()), "gm_s12_contract_rows")
+        _lib.ptr(d_out), 0, None, _lib.stream()), "gm_s12_contract_rows")
     got = d_out.view(n, 4, n_ang).cpu().numpy()
     sv = OL.scat_vals(MPL["hx_s1"], MPL["hx_s2"])      # s11, s12, s22, s33, s34, s44
     want = np.stack([sv[0], sv[1], sv[3], sv[4]], axis=1)

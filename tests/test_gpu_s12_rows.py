@@ -24,7 +24,7 @@ def _batch(n, seed):
     return dm, ang, cap, pi_t, tau_t, rng
 
 
-def _fused(dm, ang, cap, pi_t, tau_t, starts, counts, W):
+def _fused(dm, ang, cap, pi_t, tau_t, starts, counts, W, precision=0):
     from geosmie_amd import _lib
     from geosmie_amd._lib import check, ptr, stream
     t = _lib.torch()
@@ -33,15 +33,28 @@ def _fused(dm, ang, cap, pi_t, tau_t, starts, counts, W):
     st = (ctypes.c_int64 * k)(*[int(a) for a in starts])
     ct = (ctypes.c_int64 * k)(*[int(c) for c in counts])
     d_w = _lib.to_dev(W)
+    cl = None
+    if precision:
+        n_coef = dm.d_an.numel() // 2
+        d_cl = t.empty(2 * n_coef, dtype=t.float64, device=_lib.device())   # 16 B per pair
+        check(_lib.lib().gm_s12_split_coef(_lib.ctx(), n_coef, ptr(dm.d_an), ptr(dm.d_bn),
+                                           precision, ptr(d_cl), stream()), "gm_s12_split_coef")
+        cl = ptr(d_cl)
     check(_lib.lib().gm_s12_contract_rows(_lib.ctx(), dm.n, ptr(dm.d_off), ptr(dm.d_nmax),
-                                          ptr(dm.d_an), ptr(dm.d_bn), ang.size, ptr(pi_t),
-                                          ptr(tau_t), cap, k, st, ct, ptr(d_w), ptr(out),
-                                          stream()), "gm_s12_contract_rows")
+                                          None if precision else ptr(dm.d_an),
+                                          None if precision else ptr(dm.d_bn), ang.size,
+                                          ptr(pi_t), ptr(tau_t), cap, k, st, ct, ptr(d_w),
+                                          ptr(out), precision, cl, stream()),
+          "gm_s12_contract_rows")
     return out.view(k, 4, ang.size).cpu().numpy()
 
 
+@pytest.mark.parametrize("precision", [0, 1, 2])
 @pytest.mark.parametrize("layout", ["mixed", "single", "pairs"])
-def test_fused_rows_match_unfused(gpu, layout):
+def test_fused_rows_match_unfused(gpu, layout, precision):
+    """At FP32 and split-BF16 the fused pass multiplies the same operands in
+    the same order as gm_s12_mueller at that precision, so only the order of
+    the FP64 row sums differs from the unfused pair."""
     from geosmie_amd import _lib
     from geosmie_amd._lib import check, ptr, stream
     t = _lib.torch()
@@ -58,8 +71,16 @@ def test_fused_rows_match_unfused(gpu, layout):
     starts = np.array([a for a, _ in ranges])
     counts = np.array([c for _, c in ranges])
     W = rng.uniform(0, 1, (len(ranges), n))
-    got = _fused(dm, ang, cap, pi_t, tau_t, starts, counts, W)
-    mu = dm.mueller(pi_t, tau_t, 0)
+    from geosmie_amd import autotune
+    gots = []
+    for tiles in ((0,) if precision == 0 else (1, 2)):    # 8-point tiles per wavefront
+        print("fused rows", layout, "precision", precision, "tiles", tiles, flush=True)
+        autotune.set_knob(_lib.ctx(), "s12_lp_tiles", tiles)
+        try:
+            gots.append(_fused(dm, ang, cap, pi_t, tau_t, starts, counts, W, precision))
+        finally:
+            autotune.set_knob(_lib.ctx(), "s12_lp_tiles", 0)
+    mu = dm.mueller(pi_t, tau_t, precision)
     ref = t.empty(len(ranges) * 4 * ang.size, dtype=t.float64, device=_lib.device())
     st = (ctypes.c_int64 * len(ranges))(*starts.tolist())
     ct = (ctypes.c_int64 * len(ranges))(*counts.tolist())
@@ -68,11 +89,12 @@ def test_fused_rows_match_unfused(gpu, layout):
                                           ptr(d_w), ptr(mu), ptr(ref), stream()))
     ref = ref.view(len(ranges), 4, ang.size).cpu().numpy()
     M = mu.view(n, 4 * ang.size).cpu().numpy()
-    for r, (a, c) in enumerate(ranges):
-        scale = np.abs(ref[r]).max()
-        assert np.abs(got[r] - ref[r]).max() <= 1e-12 * scale, (layout, r)
-        want = (W[r, a:a + c] @ M[a:a + c]).reshape(4, ang.size)
-        assert np.abs(got[r] - want).max() <= 1e-12 * scale, (layout, r)
+    for got in gots:
+        for r, (a, c) in enumerate(ranges):
+            scale = np.abs(ref[r]).max()
+            assert np.abs(got[r] - ref[r]).max() <= 1e-12 * scale, (layout, r)
+            want = (W[r, a:a + c] @ M[a:a + c]).reshape(4, ang.size)
+            assert np.abs(got[r] - want).max() <= 1e-12 * scale, (layout, r)
 
 
 def test_fused_rows_rejects_bad_ranges(gpu):
@@ -85,11 +107,24 @@ def test_fused_rows_rejects_bad_ranges(gpu):
     W65 = np.ones((65, 200))
     with pytest.raises(ValueError):
         _fused(dm, ang, cap, pi_t, tau_t, [0] * 65, [10] * 65, W65)
+    # reduced precision without the converted coefficients
+    from geosmie_amd import _lib
+    t = _lib.torch()
+    out = t.empty(4 * ang.size, dtype=t.float64, device=_lib.device())
+    st, ct = (ctypes.c_int64 * 1)(0), (ctypes.c_int64 * 1)(200)
+    d_w = _lib.to_dev(W)
+    with pytest.raises(ValueError, match="gm_s12_split_coef"):
+        _lib.check(_lib.lib().gm_s12_contract_rows(
+            _lib.ctx(), dm.n, _lib.ptr(dm.d_off), _lib.ptr(dm.d_nmax), None, None, ang.size,
+            _lib.ptr(pi_t), _lib.ptr(tau_t), cap, 1, st, ct, _lib.ptr(d_w), _lib.ptr(out), 2,
+            None, _lib.stream()), "gm_s12_contract_rows")
 
 
-def test_chain_fused_equals_unfused(gpu):
-    """The runoptics chain's FP64 step with and without the fused pass: the
-    row integrals, normalised P, pback, scalars and GSF moments agree."""
+@pytest.mark.parametrize("precision", [0, 1, 2])
+def test_chain_fused_equals_unfused(gpu, precision):
+    """The runoptics chain's step with and without the fused pass, at each
+    precision: the row integrals, normalised P, pback, scalars and GSF
+    moments agree."""
     from geosmie_amd.lut import getLogNormPSD
     from geosmie_amd.pipeline import RunopticsChain
     t = __import__("torch")
@@ -104,7 +139,7 @@ def test_chain_fused_equals_unfused(gpu):
     res = []
     for fused in (True, False):
         ch = RunopticsChain(x, 1.5 + 0.02j, ang, w, lam, [(i * 500, 500) for i in range(4)],
-                            gsf_ng=64, fused=fused)
+                            gsf_ng=64, fused=fused, precision=precision)
         ch.step()
         t.cuda.synchronize()
         res.append({k: getattr(ch, k).cpu().numpy() for k in ("d_pm", "d_back", "d_sc", "d_pmom")})
