@@ -206,6 +206,12 @@ int mie_batch_check(const gm_ctx* ctx, int64_t n, const double* x, const double*
   if (coef_off && (!an || !bn)) return gm_fail(GM_EINVAL, "%s: an/bn NULL", who);
   if (!(size_max >= 0.0) || !isfinite(size_max))
     return gm_fail(GM_EINVAL, "%s: size_max must be finite and >= 0", who);
+  // the coated kernels address a wavefront's scratch slot with 32-bit byte
+  // offsets (range-checked buffer loads): one slot must stay below 2 GiB
+  if (y && (double)std::max(1, gm_nmax(size_max)) * 64.0 * gm::kCoatedRowDoubles *
+                   sizeof(double) >= 2147483648.0)
+    return gm_fail(GM_EINVAL, "%s: coated size parameter %.6g too large (series > 2 GiB of "
+                              "scratch per wavefront)", who, size_max);
   return GM_OK;
 }
 
@@ -419,10 +425,10 @@ int gm_ctx_tune(gm_ctx* ctx, int param, int value) {
         return gm_fail(GM_EINVAL, "gm_ctx_tune: lanes per point must be 0, 1, 16, 32 or 64");
       ctx->tune_mie_lanes = value;
       return GM_OK;
-    case GM_TUNE_S12_LP_TILES:
+    case GM_TUNE_S12_ROW_TILES:
       if (value < 0 || value > 2)
         return gm_fail(GM_EINVAL, "gm_ctx_tune: S1/S2 tiles per wavefront must be 0, 1 or 2");
-      ctx->tune_s12_lp_tiles = value;
+      ctx->tune_s12_row_tiles = value;
       return GM_OK;
     case GM_TUNE_PSD_QUIRKS:
       if (value & ~(int)GM_PSD_QUIRK_ALIAS)
@@ -452,7 +458,7 @@ int gm_ctx_get_tune(gm_ctx* ctx, int param, int* value) {
     case GM_TUNE_MIE_CKPT: *value = ctx->tune_mie_ckpt; return GM_OK;
     case GM_TUNE_MIE_LANES: *value = ctx->tune_mie_lanes; return GM_OK;
     case GM_TUNE_PSD_QUIRKS: *value = ctx->psd_quirks; return GM_OK;
-    case GM_TUNE_S12_LP_TILES: *value = ctx->tune_s12_lp_tiles; return GM_OK;
+    case GM_TUNE_S12_ROW_TILES: *value = ctx->tune_s12_row_tiles; return GM_OK;
     default: return gm_fail(GM_EINVAL, "gm_ctx_get_tune: unknown parameter %d", param);
   }
 }

@@ -288,7 +288,8 @@ struct Props {
     ssca = fma(cn, norm2(a) + norm2(b), ssca);
     double sg = (n & 1) ? -cn : cn;
     qb = qb + (a - b) * sg;
-    if (n > 1) {
+    {
+      // at n = 1, c1 = 0 and a_0 = b_0 = 0 leave asy1 unchanged (no branch)
       double dm = dn - 1.0;  // previous index
       double c1 = dm * (dm + 2.0) * rcp_nr(dm + 1.0);
       asy1 = fma(c1, ap.re * a.re + ap.im * a.im + bp.re * b.re + bp.im * b.im, asy1);
@@ -803,6 +804,28 @@ __device__ __forceinline__ void dn_chain(cdd zi, int nstart, int nmax, cd* __res
   }
 }
 // ---------------------------------------------------------------------------
+// Range-checked loads from a block's scratch slot
+// ---------------------------------------------------------------------------
+// A raw buffer load whose offset lies past the resource's byte range returns
+// zeros without touching memory.  A row that only some lanes need (a ratio
+// row, a start-index correction, the next row past nmax) is thus loaded
+// without a branch: the other lanes pass kNoRow.  Slots are < 2 GiB (checked
+// on the host), so kNoRow is always out of range.
+constexpr uint32_t kNoRow = 0x80000000u;
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t slot_rsrc(double* base, uint32_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(base, (short)0, (int)bytes, 0x00020000);
+}
+__device__ __forceinline__ cd ld_cd(__amdgpu_buffer_rsrc_t r, uint32_t off, bool ok) {
+  const auto v = __builtin_amdgcn_raw_buffer_load_b128(r, ok ? off : kNoRow, 0, 0);
+  return cd{__builtin_bit_cast(double, (unsigned long long)v[0] | ((unsigned long long)v[1] << 32)),
+            __builtin_bit_cast(double, (unsigned long long)v[2] | ((unsigned long long)v[3] << 32))};
+}
+__device__ __forceinline__ double ld_d(__amdgpu_buffer_rsrc_t r, uint32_t off, bool ok) {
+  const auto v = __builtin_amdgcn_raw_buffer_load_b64(r, ok ? off : kNoRow, 0, 0);
+  return __builtin_bit_cast(double, (unsigned long long)v[0] | ((unsigned long long)v[1] << 32));
+}
+
+// ---------------------------------------------------------------------------
 // Coated sphere, real or weakly absorbing shell (|Im m2| y <= 1)
 // ---------------------------------------------------------------------------
 // The coefficients of mie_coeffs.py:183-251, with the shell combination
@@ -820,6 +843,107 @@ __device__ __forceinline__ void dn_chain(cdd zi, int nstart, int nmax, cd* __res
 // that start is within a few Airy widths of |v| or |w|, and is only then
 // computed (two extra recurrences).  D_n(m2 x) and D_n(m2 y) are otherwise
 // not needed at all: pass 1 runs only the D_n(m1 x) recurrence.
+// One point of pass 2 (upward recurrences of psi, chi at v = m2 x, w = m2 y
+// and y, the shell combination, a_n, b_n and the efficiency sums), on rows
+// stored by pass 1 in the slot [DU | RV | RW | DDV | DDW | RY] (planes of
+// `plane` doubles; complex planes take two).  Every row's arithmetic is
+// branch-free: the upward and the ratio form of psi are both formed and one
+// is selected, and rows only some lanes need are loaded through the range
+// check (zeros elsewhere, which leave the results unchanged: U - 0 * psi = U,
+// dns + 0 = dns).  DD: the start-index corrections; STORE: write a_n, b_n.
+struct CoatedPt {
+  cd v, w;
+  double y;
+  cd m, m2;
+  int nmax, nsw_v, nsw_w, nsw_y;
+  bool ddv_on, ddw_on;
+};
+template <bool DD, bool STORE>
+__device__ __forceinline__ void coated_up(const CoatedPt& c, __amdgpu_buffer_rsrc_t rs,
+                                          uint32_t plane, int lane, int64_t off,
+                                          double* __restrict__ an, double* __restrict__ bn,
+                                          Props& pr) {
+  const cdd vinv = crcp_dd(c.v), winv = crcp_dd(c.w), minv = crcp_dd(c.m), m2inv = crcp_dd(c.m2);
+  const ddr yinv = rcp_dd(c.y);
+  const cd m = c.m, m2 = c.m2;
+  const int nmax = c.nmax, nsw_v = c.nsw_v, nsw_w = c.nsw_w, nsw_y = c.nsw_y;
+  cd pv_pp = ccos_(c.v), pv_p = csin_(c.v);   // psi_{-1}(v), psi_0(v)
+  cd cv_pp = -pv_p, cv_p = pv_pp;             // chi_{-1}(v), chi_0(v)
+  cd pw_pp = ccos_(c.w), pw_p = csin_(c.w);
+  cd cw_pp = -pw_p, cw_p = pw_pp;
+  double sn, cs;
+  sincos(c.y, &sn, &cs);
+  double py_pp = cs, py_p = sn, cy_pp = -sn, cy_p = cs;
+  // byte offsets of row 1 of each plane for this lane; a row adds 64 entries
+  const uint32_t b_du = 16u * (uint32_t)lane, b_rv = 16u * (plane + (uint32_t)lane),
+                 b_rw = 16u * (2u * plane + (uint32_t)lane),
+                 b_ddv = 16u * (3u * plane + (uint32_t)lane),
+                 b_ddw = 16u * (4u * plane + (uint32_t)lane),
+                 b_ry = 8u * (10u * plane + (uint32_t)lane);
+  // the stored rows of step n + 1 are loaded during step n
+  cd du_nx = ld_cd(rs, b_du, true), rv_nx = ld_cd(rs, b_rv, nsw_v < 1),
+     rw_nx = ld_cd(rs, b_rw, nsw_w < 1);
+  double ry_nx = ld_d(rs, b_ry, nsw_y < 1);
+#pragma unroll 2
+  for (int n = 1; n <= nmax; ++n) {
+    const uint32_t r = (uint32_t)(n - 1) * 64u;
+    const cd du = du_nx, rv = rv_nx, rw = rw_nx;
+    const double ry = ry_nx;
+    const bool more = n < nmax;
+    du_nx = ld_cd(rs, b_du + 16u * (r + 64u), more);
+    rv_nx = ld_cd(rs, b_rv + 16u * (r + 64u), more && n >= nsw_v);
+    rw_nx = ld_cd(rs, b_rw + 16u * (r + 64u), more && n >= nsw_w);
+    ry_nx = ld_d(rs, b_ry + 8u * (r + 64u), more && n >= nsw_y);
+    const double fn = (double)(2 * n - 1), dn = (double)n;
+    const cd fv = mul_dd(fn, vinv), fw = mul_dd(fn, winv);
+    const cd pv_up = fv * pv_p - pv_pp, pv_rt = pv_p * rv;
+    const cd pv = (n <= nsw_v) ? pv_up : pv_rt;
+    const cd chv = fv * cv_p - cv_pp;
+    const cd pw_up = fw * pw_p - pw_pp, pw_rt = pw_p * rw;
+    const cd pw = (n <= nsw_w) ? pw_up : pw_rt;
+    const cd chw = fw * cw_p - cw_pp;
+    const double f = mul_dd(fn, yinv);
+    const double py_up = fma(f, py_p, -py_pp), py_rt = py_p * ry;
+    const double py = (n <= nsw_y) ? py_up : py_rt;
+    const double chy = fma(f, cy_p, -cy_pp);
+    // derivatives psi'_n = psi_{n-1} - (n/z) psi_n
+    const cd dpv = pv_p - mul_dd(dn, vinv) * pv;
+    const cd nw = mul_dd(dn, winv);
+    const cd dpw = pw_p - nw * pw, dcw = cw_p - nw * chw;
+    const cd t = du * pv;
+    cd U = m * t - dpv, V = mul_dd(t, minv) - dpv;
+    if constexpr (DD) {
+      const cd cc = ld_cd(rs, b_ddv + 16u * r, c.ddv_on) * pv;
+      U = U - cc;
+      V = V - cc;
+    }
+    const cd A = pw * chv - chw * pv, B = dpw * chv - dcw * pv;
+    cd dns = cdiv(dpw + U * B, pw + U * A);
+    cd gns = cdiv(dpw + V * B, pw + V * A);
+    if constexpr (DD) {
+      const cd cc = ld_cd(rs, b_ddw + 16u * r, c.ddw_on);
+      dns = dns + cc;
+      gns = gns + cc;
+    }
+    const double nrat = mul_dd(dn, yinv);
+    const cd a1 = mul_dd(dns, m2inv) + nrat;
+    const cd b1 = m2 * gns + nrat;
+    const cd gsy = cd{py, -chy}, gs1y = cd{py_p, -cy_p};
+    const cd a = cdiv(a1 * py - py_p, gsy * a1 - gs1y);
+    const cd b = cdiv(b1 * py - py_p, gsy * b1 - gs1y);
+    pr.add(n, a, b);
+    if constexpr (STORE) {
+      an[2 * (off + n - 1)] = a.re;
+      an[2 * (off + n - 1) + 1] = a.im;
+      bn[2 * (off + n - 1)] = b.re;
+      bn[2 * (off + n - 1) + 1] = b.im;
+    }
+    pv_pp = pv_p; pv_p = pv; cv_pp = cv_p; cv_p = chv;
+    pw_pp = pw_p; pw_p = pw; cw_pp = cw_p; cw_p = chw;
+    py_pp = py_p; py_p = py; cy_pp = cy_p; cy_p = chy;
+  }
+}
+
 template <int MINW>
 __global__ __launch_bounds__(64, MINW) void k_mie_coated_stable(
     const int64_t* __restrict__ order, const int64_t* __restrict__ range, int64_t n_all,
@@ -914,70 +1038,20 @@ __global__ __launch_bounds__(64, MINW) void k_mie_coated_stable(
       dn_chain(wi, nmx, nmax, DDW, lane, false);
       dn_chain(wi, nmx + 16 + 8 * (int)ceil(cbrt(aw)), nmax, DDW, lane, true);
     }
-    // ---- pass 2: upward Riccati-Bessel recurrences and the combination
-    const cdd vinv = crcp_dd(v), winv = crcp_dd(w), minv = crcp_dd(m), m2inv = crcp_dd(m2);
-    const ddr yinv = rcp_dd(yy);
-    cd pv_pp = ccos_(v), pv_p = csin_(v);   // psi_{-1}(v), psi_0(v)
-    cd cv_pp = -pv_p, cv_p = pv_pp;         // chi_{-1}(v), chi_0(v)
-    cd pw_pp = ccos_(w), pw_p = csin_(w);
-    cd cw_pp = -pw_p, cw_p = pw_pp;
-    double sn, cs;
-    sincos(yy, &sn, &cs);
-    double py_pp = cs, py_p = sn, cy_pp = -sn, cy_p = cs;
+    // ---- pass 2: upward Riccati-Bessel recurrences and the combination,
+    // without divergent branches (coated_up); the start-index corrections
+    // are compiled in only when a lane of the wavefront needs them
     Props pr;
     pr.init();
-    const bool store = coef_off != nullptr;
-    const int64_t off = store ? coef_off[p] : 0;
-    cd du_nx = DU[lane];
-#pragma unroll 2
-    for (int n = 1; n <= nmax; ++n) {
-      const size_t o = (size_t)(n - 1) * 64 + lane;
-      const cd du = du_nx;
-      if (n < nmax) du_nx = DU[o + 64];
-      const double fn = (double)(2 * n - 1), dn = (double)n;
-      const cd fv = mul_dd(fn, vinv), fw = mul_dd(fn, winv);
-      const cd pv = (n <= nsw_v) ? fv * pv_p - pv_pp : pv_p * RV[o];
-      const cd chv = fv * cv_p - cv_pp;
-      const cd pw = (n <= nsw_w) ? fw * pw_p - pw_pp : pw_p * RW[o];
-      const cd chw = fw * cw_p - cw_pp;
-      const double f = mul_dd(fn, yinv);
-      const double py = (n <= nsw_y) ? fma(f, py_p, -py_pp) : py_p * RY[o];
-      const double chy = fma(f, cy_p, -cy_pp);
-      // derivatives psi'_n = psi_{n-1} - (n/z) psi_n
-      const cd dpv = pv_p - mul_dd(dn, vinv) * pv;
-      const cd nw = mul_dd(dn, winv);
-      const cd dpw = pw_p - nw * pw, dcw = cw_p - nw * chw;
-      const cd t = du * pv;
-      cd U = m * t - dpv, V = mul_dd(t, minv) - dpv;
-      if (ddv_on) {
-        const cd c = DDV[o] * pv;
-        U = U - c;
-        V = V - c;
-      }
-      const cd A = pw * chv - chw * pv, B = dpw * chv - dcw * pv;
-      cd dns = cdiv(dpw + U * B, pw + U * A);
-      cd gns = cdiv(dpw + V * B, pw + V * A);
-      if (ddw_on) {
-        const cd c = DDW[o];
-        dns = dns + c;
-        gns = gns + c;
-      }
-      const double nrat = mul_dd(dn, yinv);
-      const cd a1 = mul_dd(dns, m2inv) + nrat;
-      const cd b1 = m2 * gns + nrat;
-      const cd gsy = cd{py, -chy}, gs1y = cd{py_p, -cy_p};
-      const cd a = cdiv(a1 * py - py_p, gsy * a1 - gs1y);
-      const cd b = cdiv(b1 * py - py_p, gsy * b1 - gs1y);
-      pr.add(n, a, b);
-      if (store) {
-        an[2 * (off + n - 1)] = a.re;
-        an[2 * (off + n - 1) + 1] = a.im;
-        bn[2 * (off + n - 1)] = b.re;
-        bn[2 * (off + n - 1) + 1] = b.im;
-      }
-      pv_pp = pv_p; pv_p = pv; cv_pp = cv_p; cv_p = chv;
-      pw_pp = pw_p; pw_p = pw; cw_pp = cw_p; cw_p = chw;
-      py_pp = py_p; py_p = py; cy_pp = cy_p; cy_p = chy;
+    const int64_t off = coef_off ? coef_off[p] : 0;
+    const CoatedPt cp{v, w, yy, m, m2, nmax, nsw_v, nsw_w, nsw_y, ddv_on, ddw_on};
+    const __amdgpu_buffer_rsrc_t rs = slot_rsrc(slot, (uint32_t)(plane * 11 * sizeof(double)));
+    if (__any(ddv_on || ddw_on)) {
+      if (coef_off) coated_up<true, true>(cp, rs, (uint32_t)plane, lane, off, an, bn, pr);
+      else coated_up<true, false>(cp, rs, (uint32_t)plane, lane, off, an, bn, pr);
+    } else {
+      if (coef_off) coated_up<false, true>(cp, rs, (uint32_t)plane, lane, off, an, bn, pr);
+      else coated_up<false, false>(cp, rs, (uint32_t)plane, lane, off, an, bn, pr);
     }
     if (props) pr.finish(yy, p, n_all, props);
   }

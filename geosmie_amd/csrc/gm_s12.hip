@@ -313,23 +313,23 @@ __global__ __launch_bounds__(64 * kS12LdsWaves) void k_s12_f64_lds(
     }
 }
 
-// ---- S1/S2 -> Mueller -> weighted point sums, fused (FP64) ------------------
+// ---- S1/S2 -> Mueller -> weighted point sums, fused --------------------------
 //
 // integratePSD integrates each Mueller element of rawMie against a row of PSD
 // weights (dointegration.py:1044-1050, 1164-1166: np.dot(s_ij.T, psd)).  When
 // every weight row covers a contiguous range of points and few rows share a
 // point (the runoptics passes: one block of points per (rh, refractive index),
 // one row per mode), the Mueller planes need not exist in memory: each
-// wavefront forms S1/S2 of 8 points x 64 angles on the matrix cores as
+// wavefront forms S1/S2 of 8-point tiles x 64 angles on the matrix cores as
 // k_s12_f64 does, turns them into s11, s12, s33, s34 in registers and adds
-// weight x element into per-row sums.  A block walks one chunk of chunk_pts
-// consecutive points (whole 8-point tiles) of one row group (rows with the
-// same range), its four wavefronts taking 64 angles each; the sums go to a
-// partial buffer [chunk][RM][4][n_ang] that k_s12_rows_reduce adds up per row
-// in chunk order (deterministic).  Memory traffic drops from 2 x 32 B per
-// (point, angle) to the partials (8 / RM times less with one tile per chunk).
-// Chunks of one tile keep the work units as small as k_s12_f64's, so the
-// long K loops of large size parameters do not end up in a few long waves.
+// weight x element into per-row sums.  A block takes one chunk (one or two
+// whole 8-point tiles) of one row group (rows with the same range), its four
+// wavefronts taking 64 angles each; the sums go to a partial buffer
+// [chunk][RM][4][n_ang] that k_s12_rows_reduce adds up per row in chunk
+// order (deterministic).  Memory traffic drops from 2 x 32 B per (point,
+// angle) to the partials (8 / RM times less per tile).  Chunks of one or two
+// tiles keep the work units as small as k_s12_f64's, so the long K loops of
+// large size parameters do not end up in a few long waves.
 constexpr int kRowsPerCall = 64;   // rows per call (range arrays passed by value)
 constexpr int kRowsWaves = 4;      // wavefronts (64 angles each) per block
 
@@ -1105,14 +1105,17 @@ int gm_s12_contract_rows(gm_ctx* ctx, int64_t n, const int64_t* coef_off, const 
   for (int r = 0; r < n_rows; ++r)
     if (count_host[r] > 0) tiles += (count_host[r] + 7) / 8;
   const int64_t ablocks = (n_ang + 64 * gm::kRowsWaves - 1) / (64 * gm::kRowsWaves);
-  // FP64: one 8-point tile per chunk, two when that still leaves >= 16k
-  // wavefronts; reduced precision: the chunk is the kernel's MT tiles (two
-  // when that leaves >= 8k wavefronts, so each table word feeds both)
-  const bool wide = (tiles / 2) * ablocks * gm::kRowsWaves >= 16384;
-  const int lp_mt = ctx->tune_s12_lp_tiles
-                        ? ctx->tune_s12_lp_tiles
-                        : ((tiles / 2) * ablocks * gm::kRowsWaves >= 8192 ? 2 : 1);
-  const int chunk_pts = precision == GM_PREC_FP64 ? (wide ? 16 : 8) : 8 * lp_mt;
+  // tiles per chunk: FP64 walks them one after the other in one wavefront
+  // (two when that still leaves >= 16k wavefronts: fewer, larger work units
+  // and half the partial sums); the FP32 / split-BF16 kernel multiplies them
+  // together, sharing each table fetch (two when that leaves >= 8k wavefronts)
+  const int row_tiles =
+      ctx->tune_s12_row_tiles
+          ? ctx->tune_s12_row_tiles
+          : ((tiles / 2) * ablocks * gm::kRowsWaves >= (precision == GM_PREC_FP64 ? 16384 : 8192)
+                 ? 2
+                 : 1);
+  const int chunk_pts = 8 * row_tiles;
   gm::RowGroups g;
   g.n_groups = 0;
   g.chunk0[0] = 0;
@@ -1187,10 +1190,10 @@ int gm_s12_contract_rows(gm_ctx* ctx, int64_t n, const int64_t* coef_off, const 
   else if (RM == 2) GM_ROWS_LP(P_, M_, 2); \
   else GM_ROWS_LP(P_, M_, 4)
     if (precision == GM_PREC_FP32) {
-      if (lp_mt == 2) { GM_ROWS_LP_RM(GM_PREC_FP32, 2); }
+      if (row_tiles == 2) { GM_ROWS_LP_RM(GM_PREC_FP32, 2); }
       else { GM_ROWS_LP_RM(GM_PREC_FP32, 1); }
     } else {
-      if (lp_mt == 2) { GM_ROWS_LP_RM(GM_PREC_BF16X3, 2); }
+      if (row_tiles == 2) { GM_ROWS_LP_RM(GM_PREC_BF16X3, 2); }
       else { GM_ROWS_LP_RM(GM_PREC_BF16X3, 1); }
     }
 #undef GM_ROWS_LP_RM

@@ -73,13 +73,13 @@ def test_fused_rows_match_unfused(gpu, layout, precision):
     W = rng.uniform(0, 1, (len(ranges), n))
     from geosmie_amd import autotune
     gots = []
-    for tiles in ((0,) if precision == 0 else (1, 2)):    # 8-point tiles per wavefront
+    for tiles in (1, 2):    # 8-point tiles per wavefront
         print("fused rows", layout, "precision", precision, "tiles", tiles, flush=True)
-        autotune.set_knob(_lib.ctx(), "s12_lp_tiles", tiles)
+        autotune.set_knob(_lib.ctx(), "s12_row_tiles", tiles)
         try:
             gots.append(_fused(dm, ang, cap, pi_t, tau_t, starts, counts, W, precision))
         finally:
-            autotune.set_knob(_lib.ctx(), "s12_lp_tiles", 0)
+            autotune.set_knob(_lib.ctx(), "s12_row_tiles", 0)
     mu = dm.mueller(pi_t, tau_t, precision)
     ref = t.empty(len(ranges) * 4 * ang.size, dtype=t.float64, device=_lib.device())
     st = (ctypes.c_int64 * len(ranges))(*starts.tolist())

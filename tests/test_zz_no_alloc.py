@@ -109,12 +109,18 @@ def test_noalloc_status_bits_and_limits(gpu):
     assert rc == 0
     torch.cuda.synchronize()
     assert int(d_status.item()) == 2
-    # n_max(1e6) rows x 64 lanes x 88 bytes is far beyond the static array
-    rc = L.gm_mie_batch_noalloc(h, 2, ptr(d_x), ptr(d_y), ptr(d_eps), None, ptr(d_eps2), 1e6,
+    # n_max(1e5) rows x 64 lanes x 88 bytes is far beyond the static array
+    rc = L.gm_mie_batch_noalloc(h, 2, ptr(d_x), ptr(d_y), ptr(d_eps), None, ptr(d_eps2), 1e5,
                                 None, None, None, ptr(d_props), None, _lib.GM_MIE_SORT,
                                 ptr(d_status), s)
     assert rc == -1
     assert b"static scratch holds" in L.gm_last_error()
+    # and n_max(1e6) beyond what one wavefront's slot may span at all
+    rc = L.gm_mie_batch_noalloc(h, 2, ptr(d_x), ptr(d_y), ptr(d_eps), None, ptr(d_eps2), 1e6,
+                                None, None, None, ptr(d_props), None, _lib.GM_MIE_SORT,
+                                ptr(d_status), s)
+    assert rc == -1
+    assert b"too large" in L.gm_last_error()
     rc = L.gm_mie_batch_noalloc(h, 2, ptr(d_x), ptr(d_y), ptr(d_eps), None, ptr(d_eps2), 3.0,
                                 None, None, None, ptr(d_props), None, _lib.GM_MIE_SORT, None, s)
     assert rc == -1
