@@ -25,9 +25,6 @@
 
 namespace gm {
 
-constexpr int kLutPsdChunk = 16;  // PSDs per thread (accumulators in VGPRs)
-constexpr int kLutStage = 64;     // radii per LDS stage of the weights
-constexpr int kLutCols = 128;     // columns (threads) per block
 constexpr int kLutFeat = 4;       // r^2, qext r^2, qsca r^2, g qsca r^2
 constexpr int kLutMaxMom = 64;
 
@@ -50,52 +47,88 @@ __global__ void k_lut_points(int64_t n_slab, int n_r, const double* __restrict__
 }
 
 // sums[s][k][c], c < n_ang: sum_p w[k][p] P11[s][p][c];  c = n_ang + f: the
-// four area-weighted features.  Block = (slab, 128 columns, 16 PSDs); the
-// weights of 64 radii x 16 PSDs are staged in LDS and read as broadcasts.
-__global__ __launch_bounds__(kLutCols) void k_lut_contract(
+// four area-weighted features.  Per slab this is the product
+//   [PSD rows x n_r radii] x [n_r x (n_ang + 4) columns]
+// on the FP64 matrix cores (v_mfma_f64_16x16x4_f64: A[i][k] from lane
+// i + 16k, B[k][j] from lane j + 16k, D[g + 4r][j] in lane j + 16g element r).
+// One wavefront per (slab, 64 columns, 48 PSD rows): three 16-row tiles x four
+// 16-column tiles, walking the radii four at a time; every plane element is
+// read once, the weights (shared by the slabs) come from the cache, and each
+// sum is formed in one fixed order.
+typedef double double4_ __attribute__((ext_vector_type(4)));
+constexpr int kLutRowTiles = 3;   // 48 PSD rows per wavefront
+__global__ __launch_bounds__(64) void k_lut_contract(
     int64_t n_slab, int n_r, int n_ang, int n_psd, const double* __restrict__ w,
     int64_t w_slab_stride, const double* __restrict__ x, const double* __restrict__ lam,
     const double* __restrict__ props, const double* __restrict__ p11, double* __restrict__ sums) {
-  __shared__ double ws[kLutPsdChunk][kLutStage];
   const int ncol = n_ang + kLutFeat;
-  const int ncb = (ncol + kLutCols - 1) / kLutCols;
+  const int ncb = (ncol + 63) / 64;
   const int64_t s = blockIdx.x / ncb;
-  const int c = (int)(blockIdx.x - s * ncb) * kLutCols + threadIdx.x;
-  const int k0 = blockIdx.y * kLutPsdChunk;
-  const int kn = min(kLutPsdChunk, n_psd - k0);
+  const int c0 = (int)(blockIdx.x - s * ncb) * 64;
+  const int k0 = blockIdx.y * 16 * kLutRowTiles;
+  const int lane = threadIdx.x;
+  const int i = lane & 15, kq = lane >> 4;
   const int64_t n_all = n_slab * n_r;
-  const double* wb = w + s * w_slab_stride + (size_t)k0 * n_r;
+  const double* wb = w + s * w_slab_stride;
   const double rscale = lam[s] / (2.0 * 3.14159265358979323846);
-  double acc[kLutPsdChunk];
+  double4_ acc[kLutRowTiles][4];
 #pragma unroll
-  for (int k = 0; k < kLutPsdChunk; ++k) acc[k] = 0.0;
-  for (int pb = 0; pb < n_r; pb += kLutStage) {
-    const int pn = min(kLutStage, n_r - pb);
-    __syncthreads();
-    for (int t = threadIdx.x; t < kLutPsdChunk * kLutStage; t += kLutCols) {
-      const int kk = t / kLutStage, pp = t - kk * kLutStage;
-      ws[kk][pp] = (kk < kn && pp < pn) ? wb[(size_t)kk * n_r + pb + pp] : 0.0;
+  for (int mt = 0; mt < kLutRowTiles; ++mt)
+#pragma unroll
+    for (int t = 0; t < 4; ++t) acc[mt][t] = (double4_){0.0, 0.0, 0.0, 0.0};
+  // operands of radius p = p0 + kq: the weight of PSD k0 + 16 mt + i and the
+  // column c0 + 16 t + i (a P11 value or, past n_ang, a feature of point p)
+  auto load = [&](int p0, double* a, double* bv) {
+    const int p = p0 + kq;
+    const bool pok = p < n_r;
+#pragma unroll
+    for (int mt = 0; mt < kLutRowTiles; ++mt) {
+      const int k = k0 + 16 * mt + i;
+      a[mt] = (pok && k < n_psd) ? wb[(size_t)k * n_r + p] : 0.0;
     }
-    __syncthreads();
-    if (c < ncol) {
-      for (int pp = 0; pp < pn; ++pp) {
-        const int64_t i = s * n_r + pb + pp;
-        double v;
-        if (c < n_ang) {
-          v = p11[i * n_ang + c];
-        } else {
-          const double r = x[i] * rscale, r2 = r * r;
-          const int f = c - n_ang;
-          const double qext = props[i], qsca = props[n_all + i], g = props[4 * n_all + i];
-          v = f == 0 ? r2 : (f == 1 ? qext * r2 : (f == 2 ? qsca * r2 : g * qsca * r2));
-        }
+    const int64_t pt = s * n_r + p;
 #pragma unroll
-        for (int k = 0; k < kLutPsdChunk; ++k) acc[k] = fma(ws[k][pp], v, acc[k]);
+    for (int t = 0; t < 4; ++t) {
+      const int c = c0 + 16 * t + i;
+      double v = 0.0;
+      if (pok && c < n_ang) {
+        v = p11[pt * n_ang + c];
+      } else if (pok && c < ncol) {
+        const double r = x[pt] * rscale, r2 = r * r;
+        const int f = c - n_ang;
+        const double qext = props[pt], qsca = props[n_all + pt], g = props[4 * n_all + pt];
+        v = f == 0 ? r2 : (f == 1 ? qext * r2 : (f == 2 ? qsca * r2 : g * qsca * r2));
+      }
+      bv[t] = v;
+    }
+  };
+  double a_n[kLutRowTiles], b_n[4];
+  load(0, a_n, b_n);
+  for (int p0 = 0; p0 < n_r; p0 += 4) {
+    double a_c[kLutRowTiles], b_c[4];
+#pragma unroll
+    for (int mt = 0; mt < kLutRowTiles; ++mt) a_c[mt] = a_n[mt];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) b_c[t] = b_n[t];
+    if (p0 + 4 < n_r) load(p0 + 4, a_n, b_n);
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+#pragma unroll
+      for (int mt = 0; mt < kLutRowTiles; ++mt)
+        acc[mt][t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a_c[mt], b_c[t], acc[mt][t], 0, 0, 0);
+  }
+  // lane (i, kq) element r of acc[mt][t]: PSD k0 + 16 mt + kq + 4 r, column c0 + 16 t + i
+#pragma unroll
+  for (int mt = 0; mt < kLutRowTiles; ++mt)
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      const int c = c0 + 16 * t + i;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int k = k0 + 16 * mt + kq + 4 * r;
+        if (k < n_psd && c < ncol) sums[((size_t)s * n_psd + k) * ncol + c] = acc[mt][t][r];
       }
     }
-  }
-  if (c >= ncol) return;
-  for (int k = 0; k < kn; ++k) sums[((size_t)s * n_psd + k0 + k) * ncol + c] = acc[k];
 }
 
 // One wavefront per entry: lane l forms alpha1_l from the GSF table G1[a][l].
@@ -261,7 +294,7 @@ int gm_lut_bulk(gm_ctx* ctx, int64_t n_slab, int n_r, int n_ang, int n_psd, cons
   for (int a = 1; a < n_ang; ++a)
     if (!(ang_deg_host[a] > ang_deg_host[a - 1]))
       return gm_fail(GM_EINVAL, "gm_lut_bulk: angles must increase");
-  const int64_t ncb = (n_ang + gm::kLutFeat + gm::kLutCols - 1) / gm::kLutCols;
+  const int64_t ncb = (n_ang + gm::kLutFeat + 63) / 64;
   const int64_t n_ent = n_slab * n_psd;
   if (n_slab * ncb > 0x7fffffffLL || n_ent > 0x7fffffffLL)
     return gm_fail(GM_EINVAL, "gm_lut_bulk: too many slabs for one call");
@@ -277,8 +310,8 @@ int gm_lut_bulk(gm_ctx* ctx, int64_t n_slab, int n_r, int n_ang, int n_psd, cons
   ctx->mark_begin(gm::kKLut, s, &slot);
   hipLaunchKernelGGL(gm::k_lut_contract,
                      dim3((unsigned)(n_slab * ncb),
-                          (unsigned)((n_psd + gm::kLutPsdChunk - 1) / gm::kLutPsdChunk)),
-                     dim3(gm::kLutCols), 0, s, n_slab, n_r, n_ang, n_psd, w,
+                          (unsigned)((n_psd + 16 * gm::kLutRowTiles - 1) / (16 * gm::kLutRowTiles))),
+                     dim3(64), 0, s, n_slab, n_r, n_ang, n_psd, w,
                      w_per_slab ? (int64_t)n_psd * n_r : (int64_t)0, x, lam, props, p11, sums);
   hipLaunchKernelGGL(gm::k_lut_finish, dim3((unsigned)n_ent), dim3(64), 0, s, n_ang, ng, n_mom,
                      (const double*)ctx->gsf_tab.ptr, sums, out);
