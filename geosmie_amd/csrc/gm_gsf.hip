@@ -18,6 +18,7 @@
 #include <math.h>
 #include <string.h>
 
+#include <algorithm>
 #include <vector>
 
 #include "gm_complex.h"
@@ -133,45 +134,90 @@ static void build_tables(int ng, int n_ang, const double* ang_deg, std::vector<d
   }
 }
 
-constexpr int kGsfWaves = 4;   // wavefronts splitting one entry's angle sum
-
-// Block (entry e, 64 coefficients l0..l0+63); wavefront w sums angles
-// a = w, w + 4, ...; the four partial sums are added in fixed order in LDS.
-// F[e][6][n_ang] in table order p11, p12, p33, p34, p22, p44; raw[e][6][ng]
-// receives the unnormalised coefficients in the same order (pmom(bin,
-// wavelength, rh, p, m), convertncdf.py:76, 381-394).
-__global__ __launch_bounds__(256) void k_gsf_expand(int64_t n_ent, int n_ang, int ng,
-                                                    const double* __restrict__ G,
-                                                    const double* __restrict__ F,
-                                                    double* __restrict__ raw) {
-  __shared__ double part[kGsfWaves][6][64];
-  const int64_t e = blockIdx.x;
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int l = blockIdx.y * 64 + lane;
+// The six angle sums of every entry as products on the FP64 matrix cores:
+//   [entries x angles] x G_t[angles x l]
+// for the rows F11, F44 (t = P00), F22 + F33 (P22), F22 - F33 (P2-2) and F12,
+// F34 (P02).  F[e][6][n_ang] in table order p11, p12, p33, p34, p22, p44.  A
+// wavefront takes 16 entries x 16 coefficients over one chunk of angles, four
+// angles per v_mfma_f64_16x16x4_f64 (A[i][k] from lane i + 16k, B[k][j] from
+// lane j + 16k, D[g + 4r][j] in lane j + 16g element r); the chunks split the
+// angle sum so that a few entries still fill the GPU, and k_gsf_reduce adds
+// them up in chunk order (deterministic).
+//   part[chunk][e][c][l], c: 0 F11 P00, 1 (F22+F33) P22, 2 (F22-F33) P2-2,
+//   3 F44 P00, 4 F12 P02, 5 F34 P02
+typedef double double4_ __attribute__((ext_vector_type(4)));
+__global__ __launch_bounds__(64) void k_gsf_expand(int64_t n_ent, int n_ang, int ng,
+                                                   int chunk_ang, const double* __restrict__ G,
+                                                   const double* __restrict__ F,
+                                                   double* __restrict__ part) {
+  const int lane = threadIdx.x, i = lane & 15, kq = lane >> 4;
+  const int64_t e0 = (int64_t)blockIdx.x * 16;
+  const int l0 = blockIdx.y * 16;
+  const int ch = blockIdx.z;
+  const int a_lo = ch * chunk_ang, a_hi = min(n_ang, a_lo + chunk_ang);
   const size_t plane = (size_t)n_ang * ng;
-  const double* Fe = F + (size_t)e * 6 * n_ang;
-  double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-  if (l < ng) {
-    for (int a = wv; a < n_ang; a += kGsfWaves) {
-      const double g1 = G[(size_t)a * ng + l], g2 = G[plane + (size_t)a * ng + l];
-      const double g3 = G[2 * plane + (size_t)a * ng + l], g4 = G[3 * plane + (size_t)a * ng + l];
-      const double f11 = Fe[a], f12 = Fe[n_ang + a], f33 = Fe[2 * n_ang + a];
-      const double f34 = Fe[3 * n_ang + a], f22 = Fe[4 * n_ang + a], f44 = Fe[5 * n_ang + a];
-      acc[0] = fma(f11, g1, acc[0]);          // alpha1 <- F11 P00
-      acc[1] = fma(f22 + f33, g2, acc[1]);    // A2 <- (F22+F33) P22
-      acc[2] = fma(f22 - f33, g3, acc[2]);    // A3 <- (F22-F33) P2-2
-      acc[3] = fma(f44, g1, acc[3]);          // alpha4 <- F44 P00
-      acc[4] = fma(f12, g4, acc[4]);          // beta1 <- F12 P02
-      acc[5] = fma(f34, g4, acc[5]);          // beta2 <- F34 P02
+  const int64_t ea = e0 + i;
+  const bool e_ok = ea < n_ent;
+  const double* Fe = F + (size_t)(e_ok ? ea : 0) * 6 * n_ang;
+  const int lb = l0 + i;
+  const bool l_ok = lb < ng;
+  double4_ acc[6];
+#pragma unroll
+  for (int c = 0; c < 6; ++c) acc[c] = (double4_){0.0, 0.0, 0.0, 0.0};
+  for (int a0 = a_lo; a0 < a_hi; a0 += 4) {
+    const int a = a0 + kq;
+    const bool ok = a < a_hi;
+    double f11 = 0.0, f12 = 0.0, f33 = 0.0, f34 = 0.0, f22 = 0.0, f44 = 0.0;
+    if (ok && e_ok) {
+      f11 = Fe[a];
+      f12 = Fe[n_ang + a];
+      f33 = Fe[2 * n_ang + a];
+      f34 = Fe[3 * n_ang + a];
+      f22 = Fe[4 * n_ang + a];
+      f44 = Fe[5 * n_ang + a];
     }
+    double g1 = 0.0, g2 = 0.0, g3 = 0.0, g4 = 0.0;
+    if (ok && l_ok) {
+      const size_t o = (size_t)a * ng + lb;
+      g1 = G[o];
+      g2 = G[plane + o];
+      g3 = G[2 * plane + o];
+      g4 = G[3 * plane + o];
+    }
+    acc[0] = __builtin_amdgcn_mfma_f64_16x16x4f64(f11, g1, acc[0], 0, 0, 0);
+    acc[1] = __builtin_amdgcn_mfma_f64_16x16x4f64(f22 + f33, g2, acc[1], 0, 0, 0);
+    acc[2] = __builtin_amdgcn_mfma_f64_16x16x4f64(f22 - f33, g3, acc[2], 0, 0, 0);
+    acc[3] = __builtin_amdgcn_mfma_f64_16x16x4f64(f44, g1, acc[3], 0, 0, 0);
+    acc[4] = __builtin_amdgcn_mfma_f64_16x16x4f64(f12, g4, acc[4], 0, 0, 0);
+    acc[5] = __builtin_amdgcn_mfma_f64_16x16x4f64(f34, g4, acc[5], 0, 0, 0);
   }
+  // lane (i, kq) element r: entry e0 + kq + 4 r, coefficient l0 + i
 #pragma unroll
-  for (int f = 0; f < 6; ++f) part[wv][f][lane] = acc[f];
-  __syncthreads();
-  if (wv != 0 || l >= ng) return;
+  for (int c = 0; c < 6; ++c)
 #pragma unroll
-  for (int f = 0; f < 6; ++f)
-    acc[f] = ((part[0][f][lane] + part[1][f][lane]) + part[2][f][lane]) + part[3][f][lane];
+    for (int r = 0; r < 4; ++r) {
+      const int64_t e = e0 + kq + 4 * r;
+      if (e < n_ent && l_ok) part[(((size_t)ch * n_ent + e) * 6 + c) * ng + lb] = acc[c][r];
+    }
+}
+
+// raw[e][6][l] from the chunk partials, added in chunk order, then alpha /
+// beta by the (l + 1/2) factors (spher_expan.f:335-347), in the table order
+// p11 <- alpha1, p12 <- beta1, p33 <- alpha3, p34 <- beta2, p22 <- alpha2,
+// p44 <- alpha4 (pmom(bin, wavelength, rh, p, m), convertncdf.py:76, 381-394).
+__global__ __launch_bounds__(256) void k_gsf_reduce(int64_t n_ent, int ng, int n_chunks,
+                                                    const double* __restrict__ part,
+                                                    double* __restrict__ raw) {
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= n_ent * ng) return;
+  const int64_t e = idx / ng;
+  const int l = (int)(idx - e * ng);
+  double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  for (int ch = 0; ch < n_chunks; ++ch) {
+    const double* pc = part + (((size_t)ch * n_ent + e) * 6) * ng + l;
+#pragma unroll
+    for (int c = 0; c < 6; ++c) acc[c] += pc[(size_t)c * ng];
+  }
   const double cl = (double)l + 0.5;
   double* o = raw + (size_t)e * 6 * ng;
   const double a2 = acc[1] * cl * 0.5, a3 = acc[2] * cl * 0.5;
@@ -323,15 +369,28 @@ int gm_gsf_expand(gm_ctx* ctx, int64_t n_entries, int n_ang, const double* ang_d
   hipStream_t s = (hipStream_t)stream;
   int rc = gm::gsf_tables(ctx, ng, n_ang, ang_deg_host, s);
   if (rc) return rc;
-  size_t raw_bytes = sizeof(double) * (size_t)n_entries * 6 * ng;
-  rc = ctx->reserve(ctx->gemm_tmp, raw_bytes);
+  // angle chunks (>= 32 angles each) until ~2048 wavefronts are in flight
+  const int64_t tiles = ((n_entries + 15) / 16) * ((ng + 15) / 16);
+  int n_chunks = (int)std::max<int64_t>(
+      1, std::min<int64_t>((2048 + tiles - 1) / tiles, (n_ang + 31) / 32));
+  const int chunk_ang = ((n_ang + n_chunks - 1) / n_chunks + 3) / 4 * 4;
+  n_chunks = (n_ang + chunk_ang - 1) / chunk_ang;
+  if ((n_entries + 15) / 16 > 0x7fffffffLL || (ng + 15) / 16 > 65535)
+    return gm_fail(-1, "gm_gsf_expand: too many entries or coefficients for one call");
+  const size_t raw_bytes = sizeof(double) * (size_t)n_entries * 6 * ng;
+  rc = ctx->reserve(ctx->gemm_tmp, raw_bytes * (1 + (size_t)n_chunks));
   if (rc) return rc;
   double* raw = (double*)ctx->gemm_tmp.ptr;
+  double* part = raw + (size_t)n_entries * 6 * ng;
   size_t slot;
   ctx->mark_begin(gm::kKGsf, s, &slot);
-  hipLaunchKernelGGL(gm::k_gsf_expand, dim3((unsigned)n_entries, (unsigned)((ng + 63) / 64)),
-                     dim3(64 * gm::kGsfWaves), 0, s, n_entries, n_ang, ng,
-                     (const double*)ctx->gsf_tab.ptr, F, raw);
+  hipLaunchKernelGGL(gm::k_gsf_expand,
+                     dim3((unsigned)((n_entries + 15) / 16), (unsigned)((ng + 15) / 16),
+                          (unsigned)n_chunks),
+                     dim3(64), 0, s, n_entries, n_ang, ng, chunk_ang,
+                     (const double*)ctx->gsf_tab.ptr, F, part);
+  hipLaunchKernelGGL(gm::k_gsf_reduce, dim3((unsigned)((n_entries * ng + 255) / 256)), dim3(256),
+                     0, s, n_entries, ng, n_chunks, (const double*)part, raw);
   hipLaunchKernelGGL(gm::k_gsf_normalize, dim3((unsigned)n_entries), dim3(256), 0, s, ng, raw, out,
                      quantize);
   ctx->mark_end(slot, s);
