@@ -114,6 +114,10 @@ def lib():
             if not os.path.exists(LIB_PATH):
                 raise GMError("libgeosmie_hip.so not built (run python -m geosmie_amd.build): %s"
                               % LIB_PATH)
+            # torch first: its HIP runtime is then the one this library binds
+            # (same soname), so the process holds one runtime, not two that
+            # both claim the GPU (the second one then finds no device)
+            import torch  # noqa: F401
             L = ctypes.CDLL(LIB_PATH)
             missing = [name for name in SIGNATURES if not hasattr(L, name)]
             if missing:

@@ -38,7 +38,7 @@ KNOBS = {"mie_grid_mult": (0, (1, 2, 4)),
          "s12_layout": (6, (0, 1, 2)),
          "mie_ckpt": (7, (4, 8, 0)),
          "mie_lanes": (8, (0, 1, 16, 32, 64)),
-         "s12_row_tiles": (10, (0, 1, 2)),
+         "s12_row_tiles": (10, (0, 1, 2, 4)),
          # a precision setting rather than a shape; set explicitly, never tuned for speed
          "mlp_weight_terms": (3, (1, 2)),
          # a behaviour setting: the integratePSD quirks reproduced (bit 0: the aliased

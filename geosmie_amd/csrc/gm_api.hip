@@ -142,7 +142,11 @@ struct MieGrids {
 // four per SIMD), otherwise P lanes per point, more for longer series.
 int mie_lanes(const gm_ctx* ctx, int64_t n, int rows_cap) {
   if (ctx->tune_mie_lanes) return ctx->tune_mie_lanes;
+  // one point per lane once the batch fills the GPU: four wavefronts per
+  // SIMD, or one for short series (n_max <= 64), where the cooperative
+  // kernels' scans cost more than the serial rows they split
   if (n >= (int64_t)64 * ctx->n_cu * 4 * 4) return 1;
+  if (rows_cap <= 64 && n >= (int64_t)64 * ctx->n_cu * 4) return 1;
   return rows_cap <= 300 ? 16 : rows_cap <= 1200 ? 32 : 64;
 }
 
@@ -426,8 +430,8 @@ int gm_ctx_tune(gm_ctx* ctx, int param, int value) {
       ctx->tune_mie_lanes = value;
       return GM_OK;
     case GM_TUNE_S12_ROW_TILES:
-      if (value < 0 || value > 2)
-        return gm_fail(GM_EINVAL, "gm_ctx_tune: S1/S2 tiles per wavefront must be 0, 1 or 2");
+      if (value != 0 && value != 1 && value != 2 && value != 4)
+        return gm_fail(GM_EINVAL, "gm_ctx_tune: S1/S2 tiles per work unit must be 0, 1, 2 or 4");
       ctx->tune_s12_row_tiles = value;
       return GM_OK;
     case GM_TUNE_PSD_QUIRKS:

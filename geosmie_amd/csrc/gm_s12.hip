@@ -1106,15 +1106,17 @@ int gm_s12_contract_rows(gm_ctx* ctx, int64_t n, const int64_t* coef_off, const 
     if (count_host[r] > 0) tiles += (count_host[r] + 7) / 8;
   const int64_t ablocks = (n_ang + 64 * gm::kRowsWaves - 1) / (64 * gm::kRowsWaves);
   // tiles per chunk: FP64 walks them one after the other in one wavefront
-  // (two when that still leaves >= 16k wavefronts: fewer, larger work units
-  // and half the partial sums); the FP32 / split-BF16 kernel multiplies them
-  // together, sharing each table fetch (two when that leaves >= 8k wavefronts)
+  // (two for short series when that still leaves >= 16k wavefronts: fewer
+  // work units and half the partial sums where the K loop is short; long
+  // series run faster one tile per wavefront); the FP32 / split-BF16 kernel
+  // multiplies them together, sharing each table fetch (two when that leaves
+  // >= 8k wavefronts)
+  const int64_t waves2 = (tiles / 2) * ablocks * gm::kRowsWaves;
   const int row_tiles =
       ctx->tune_s12_row_tiles
           ? ctx->tune_s12_row_tiles
-          : ((tiles / 2) * ablocks * gm::kRowsWaves >= (precision == GM_PREC_FP64 ? 16384 : 8192)
-                 ? 2
-                 : 1);
+          : (precision == GM_PREC_FP64 ? (nmax_cap <= 256 && waves2 >= 16384 ? 2 : 1)
+                                       : (waves2 >= 8192 ? 2 : 1));
   const int chunk_pts = 8 * row_tiles;
   gm::RowGroups g;
   g.n_groups = 0;
@@ -1190,10 +1192,12 @@ int gm_s12_contract_rows(gm_ctx* ctx, int64_t n, const int64_t* coef_off, const 
   else if (RM == 2) GM_ROWS_LP(P_, M_, 2); \
   else GM_ROWS_LP(P_, M_, 4)
     if (precision == GM_PREC_FP32) {
-      if (row_tiles == 2) { GM_ROWS_LP_RM(GM_PREC_FP32, 2); }
+      if (row_tiles == 4) { GM_ROWS_LP_RM(GM_PREC_FP32, 4); }
+      else if (row_tiles == 2) { GM_ROWS_LP_RM(GM_PREC_FP32, 2); }
       else { GM_ROWS_LP_RM(GM_PREC_FP32, 1); }
     } else {
-      if (row_tiles == 2) { GM_ROWS_LP_RM(GM_PREC_BF16X3, 2); }
+      if (row_tiles == 4) { GM_ROWS_LP_RM(GM_PREC_BF16X3, 4); }
+      else if (row_tiles == 2) { GM_ROWS_LP_RM(GM_PREC_BF16X3, 2); }
       else { GM_ROWS_LP_RM(GM_PREC_BF16X3, 1); }
     }
 #undef GM_ROWS_LP_RM

@@ -109,7 +109,7 @@ enum gm_tune_param {
   GM_TUNE_PSD_QUIRKS = 9,       /* a behaviour setting, not tuned for speed: the integratePSD
                                    quirks gm_integrate_psd[_rows] reproduce (GM_PSD_QUIRK_* bits;
                                    default GM_PSD_QUIRK_ALIAS) */
-  GM_TUNE_S12_ROW_TILES = 10    /* gm_s12_contract_rows: 8-point tiles per work unit (1, 2): at
+  GM_TUNE_S12_ROW_TILES = 10    /* gm_s12_contract_rows: 8-point tiles per work unit (1, 2, 4): at
                                    FP64 one after the other, at FP32 / split-BF16 multiplied
                                    together, sharing each angle-table fetch; 0 (default) = by
                                    batch size.  Results agree to rounding */

@@ -73,7 +73,7 @@ def test_fused_rows_match_unfused(gpu, layout, precision):
     W = rng.uniform(0, 1, (len(ranges), n))
     from geosmie_amd import autotune
     gots = []
-    for tiles in (1, 2):    # 8-point tiles per wavefront
+    for tiles in (1, 2, 4):    # 8-point tiles per work unit
         print("fused rows", layout, "precision", precision, "tiles", tiles, flush=True)
         autotune.set_knob(_lib.ctx(), "s12_row_tiles", tiles)
         try:

@@ -40,6 +40,23 @@ def test_nc4_library_exports_every_declared_symbol():
     assert not [n for n in names if not hasattr(L, n)]
 
 
+def test_one_hip_runtime_per_process():
+    """Loading the library first (before any torch import) still leaves one HIP
+    runtime in the process: torch's, which the library's libamdhip64 binds to.
+    With two, the second runtime finds no GPU."""
+    import subprocess
+    import sys
+    code = ("import re, sys; sys.path.insert(0, %r)\n"
+            "from geosmie_amd import _lib\n"
+            "_lib.lib()\n"
+            "import torch\n"
+            "maps = open('/proc/self/maps').read()\n"
+            "print(len(set(re.findall(r'/\\S*libamdhip64\\S*', maps))))\n" % ROOT)
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    assert r.stdout.strip().splitlines()[-1] == "1"
+
+
 def test_host_only_entry_points():
     from geosmie_amd import _lib
     L = _lib.lib()
