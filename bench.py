@@ -608,8 +608,10 @@ def main():
                              _lib.stream()), "gm_mie_batch")
 
     from geosmie_amd import autotune
-    tuned = autotune.tune(h, step, knobs=("mie_ckpt", "mie_waves", "mie_grid_mult"), reps=2,
-                          warmup=1)
+    # the grid multiple before and after the checkpoint interval: the best
+    # multiple differs between the two coated kernels
+    tuned = autotune.tune(h, step, knobs=("mie_grid_mult", "mie_ckpt", "mie_grid_mult",
+                                          "mie_waves"), reps=2, warmup=1)
     ckpt = autotune.get(h, "mie_ckpt")
     kernel_name = "k_mie_coated_ckpt<1,%d>" % ckpt if ckpt else "k_mie_coated_stable"
     traffic, traffic_src = measured_traffic(kernel_name)
