@@ -1060,6 +1060,140 @@ __global__ __launch_bounds__(64, MINW) void k_mie_coated_stable(
 // ---------------------------------------------------------------------------
 // Coated sphere, real or weakly absorbing shell, checkpointed downward chains
 // ---------------------------------------------------------------------------
+// Pass 2 of k_mie_coated_ckpt for one point: per block, reload the
+// checkpoint (the next one in flight), regenerate the block's K rows into LDS
+// with exactly pass 1's operations, then the K upward steps.  The upward
+// steps are branch-free as in coated_up (both psi forms formed, one
+// selected; the LDS rows of an inactive chain are read and ignored), the
+// start-index corrections come through range-checked loads and are compiled
+// in only when a lane of the wavefront needs them (DD); STORE writes a_n, b_n.
+struct CkptPt {
+  cd v, w, m, m2;
+  double y;
+  cdd ui, vi, wi, minv, m2inv;
+  ddr yi;
+  int nmax, nmx, sv, sw, sy, nsw_v, nsw_w, nsw_y, nblk;
+  bool ddv_on, ddw_on;
+};
+template <int K, bool DD, bool STORE>
+__device__ __forceinline__ Props ckpt_pass2(const CkptPt& c, double (*rows)[7][64],
+                                            const double* __restrict__ CK,
+                                            __amdgpu_buffer_rsrc_t rs, uint32_t b_ddv,
+                                            uint32_t b_ddw, int lane, int64_t off,
+                                            double* __restrict__ an, double* __restrict__ bn) {
+  cd pv_pp = ccos_(c.v), pv_p = csin_(c.v);   // psi_{-1}(v), psi_0(v)
+  cd cv_pp = -pv_p, cv_p = pv_pp;             // chi_{-1}(v), chi_0(v)
+  cd pw_pp = ccos_(c.w), pw_p = csin_(c.w);
+  cd cw_pp = -pw_p, cw_p = pw_pp;
+  double sn, cs;
+  sincos(c.y, &sn, &cs);
+  double py_pp = cs, py_p = sn, cy_pp = -sn, cy_p = cs;
+  const cd m = c.m, m2 = c.m2;
+  Props pr;
+  pr.init();
+  double ck[7], ckn[7];
+#pragma unroll
+  for (int q = 0; q < 7; ++q) ckn[q] = CK[q * 64 + lane];
+  for (int b = 0; b < c.nblk; ++b) {
+#pragma unroll
+    for (int q = 0; q < 7; ++q) ck[q] = ckn[q];
+    if (b + 1 < c.nblk) {
+#pragma unroll
+      for (int q = 0; q < 7; ++q) ckn[q] = CK[((size_t)(b + 1) * 7 + q) * 64 + lane];
+    }
+    {
+      cd du = cd{ck[0], ck[1]}, dv = cd{ck[2], ck[3]}, dw = cd{ck[4], ck[5]};
+      double dy = ck[6];
+#pragma unroll
+      for (int kk = K - 1; kk >= 0; --kk) {
+        const int k = b * K + kk + 1;
+        if (k <= c.nmx - 1) {
+          const cd r = mul_dd((double)(k + 1), c.ui);
+          du = r - crcp_fast(du + r);
+        }
+        rows[kk][0][lane] = du.re;
+        rows[kk][1][lane] = du.im;
+        if (k <= c.sv && k > c.nsw_v) {
+          const cd r = mul_dd((double)k, c.vi);
+          const cd it = crcp_fast(dv + r);
+          rows[kk][2][lane] = it.re;
+          rows[kk][3][lane] = it.im;
+          dv = r - it;
+        }
+        if (k <= c.sw && k > c.nsw_w) {
+          const cd r = mul_dd((double)k, c.wi);
+          const cd it = crcp_fast(dw + r);
+          rows[kk][4][lane] = it.re;
+          rows[kk][5][lane] = it.im;
+          dw = r - it;
+        }
+        if (k <= c.sy && k > c.nsw_y) {
+          const double r = mul_dd((double)k, c.yi);
+          const double it = rcp_nr(dy + r);
+          rows[kk][6][lane] = it;
+          dy = r - it;
+        }
+      }
+    }
+    const int kend = min(K, c.nmax - b * K);
+    for (int kk = 0; kk < kend; ++kk) {
+      const int n = b * K + kk + 1;
+      const uint32_t r = (uint32_t)(n - 1) * 64u;
+      const cd du = cd{rows[kk][0][lane], rows[kk][1][lane]};
+      const cd rv = cd{rows[kk][2][lane], rows[kk][3][lane]};
+      const cd rw = cd{rows[kk][4][lane], rows[kk][5][lane]};
+      const double ry = rows[kk][6][lane];
+      const double fn = (double)(2 * n - 1), dn = (double)n;
+      const cd fv = mul_dd(fn, c.vi), fw = mul_dd(fn, c.wi);
+      const cd pv_up = fv * pv_p - pv_pp, pv_rt = pv_p * rv;
+      const cd pv = (n <= c.nsw_v) ? pv_up : pv_rt;
+      const cd chv = fv * cv_p - cv_pp;
+      const cd pw_up = fw * pw_p - pw_pp, pw_rt = pw_p * rw;
+      const cd pw = (n <= c.nsw_w) ? pw_up : pw_rt;
+      const cd chw = fw * cw_p - cw_pp;
+      const double f = mul_dd(fn, c.yi);
+      const double py_up = fma(f, py_p, -py_pp), py_rt = py_p * ry;
+      const double py = (n <= c.nsw_y) ? py_up : py_rt;
+      const double chy = fma(f, cy_p, -cy_pp);
+      const cd dpv = pv_p - mul_dd(dn, c.vi) * pv;
+      const cd nw = mul_dd(dn, c.wi);
+      const cd dpw = pw_p - nw * pw, dcw = cw_p - nw * chw;
+      const cd t = du * pv;
+      cd U = m * t - dpv, V = mul_dd(t, c.minv) - dpv;
+      if constexpr (DD) {
+        const cd cc = ld_cd(rs, b_ddv + 16u * r, c.ddv_on) * pv;
+        U = U - cc;
+        V = V - cc;
+      }
+      const cd A = pw * chv - chw * pv, B = dpw * chv - dcw * pv;
+      cd dns = cdiv(dpw + U * B, pw + U * A);
+      cd gns = cdiv(dpw + V * B, pw + V * A);
+      if constexpr (DD) {
+        const cd cc = ld_cd(rs, b_ddw + 16u * r, c.ddw_on);
+        dns = dns + cc;
+        gns = gns + cc;
+      }
+      const double nrat = mul_dd(dn, c.yi);
+      const cd a1 = mul_dd(dns, c.m2inv) + nrat;
+      const cd b1 = m2 * gns + nrat;
+      const cd gsy = cd{py, -chy}, gs1y = cd{py_p, -cy_p};
+      const cd a = cdiv(a1 * py - py_p, gsy * a1 - gs1y);
+      const cd bb = cdiv(b1 * py - py_p, gsy * b1 - gs1y);
+      pr.add(n, a, bb);
+      if constexpr (STORE) {
+        an[2 * (off + n - 1)] = a.re;
+        an[2 * (off + n - 1) + 1] = a.im;
+        bn[2 * (off + n - 1)] = bb.re;
+        bn[2 * (off + n - 1) + 1] = bb.im;
+      }
+      pv_pp = pv_p; pv_p = pv; cv_pp = cv_p; cv_p = chv;
+      pw_pp = pw_p; pw_p = pw; cw_pp = cw_p; cw_p = chw;
+      py_pp = py_p; py_p = py; cy_pp = cy_p; cy_p = chy;
+    }
+  }
+  return pr;
+}
+
 // The arithmetic of k_mie_coated_stable, with a different memory schedule.
 // Pass 1 runs the four downward chains (D_n(u) and the psi ratios of v, w,
 // y) over every row but stores only their states every K rows: checkpoint b
@@ -1089,6 +1223,9 @@ __global__ __launch_bounds__(64, MINW) void k_mie_coated_ckpt(
   double* CK = slot;                                          // [b][7][lane]
   cd* DDV = reinterpret_cast<cd*>(slot + (size_t)7 * bcap * 64);
   cd* DDW = DDV + (size_t)rows_cap * 64;
+  // the start-correction planes [DDV | DDW] through a range-checked resource
+  const __amdgpu_buffer_rsrc_t rs_dd =
+      slot_rsrc(reinterpret_cast<double*>(DDV), (uint32_t)((size_t)rows_cap * 64 * 4 * sizeof(double)));
 
   for (int64_t base = begin + (int64_t)blockIdx.x * 64; base < end;
        base += (int64_t)gridDim.x * 64) {
@@ -1157,114 +1294,21 @@ __global__ __launch_bounds__(64, MINW) void k_mie_coated_ckpt(
       dn_chain(wi, nmx + 16 + 8 * (int)ceil(cbrt(aw)), nmax, DDW, lane, true);
     }
     // ---- pass 2: per block, regenerate K rows into LDS, then K upward steps
-    const cdd vinv = vi, winv = wi, minv = crcp_dd(m), m2inv = crcp_dd(m2);
-    const ddr yinv = yi;
-    cd pv_pp = ccos_(v), pv_p = csin_(v);   // psi_{-1}(v), psi_0(v)
-    cd cv_pp = -pv_p, cv_p = pv_pp;         // chi_{-1}(v), chi_0(v)
-    cd pw_pp = ccos_(w), pw_p = csin_(w);
-    cd cw_pp = -pw_p, cw_p = pw_pp;
-    double sn, cs;
-    sincos(yy, &sn, &cs);
-    double py_pp = cs, py_p = sn, cy_pp = -sn, cy_p = cs;
+    const CkptPt cp{v,   w,     m,     m2,    yy,    ui,   vi,    wi,    crcp_dd(m),
+                    crcp_dd(m2), yi, nmax, nmx, sv, sw, sy, nsw_v, nsw_w, nsw_y,
+                    nblk, ddv_on, ddw_on};
+    const int64_t off = coef_off ? coef_off[p] : 0;
+    const uint32_t b_ddv = 16u * (uint32_t)lane, b_ddw = 16u * ((uint32_t)rows_cap * 64u + lane);
     Props pr;
-    pr.init();
-    const bool store = coef_off != nullptr;
-    const int64_t off = store ? coef_off[p] : 0;
-    double ck[7], ckn[7];
-#pragma unroll
-    for (int q = 0; q < 7; ++q) ckn[q] = CK[q * 64 + lane];
-    for (int b = 0; b < nblk; ++b) {
-#pragma unroll
-      for (int q = 0; q < 7; ++q) ck[q] = ckn[q];
-      if (b + 1 < nblk) {
-#pragma unroll
-        for (int q = 0; q < 7; ++q) ckn[q] = CK[((size_t)(b + 1) * 7 + q) * 64 + lane];
-      }
-      {
-        cd du = cd{ck[0], ck[1]}, dv = cd{ck[2], ck[3]}, dw = cd{ck[4], ck[5]};
-        double dy = ck[6];
-#pragma unroll
-        for (int kk = K - 1; kk >= 0; --kk) {
-          const int k = b * K + kk + 1;
-          if (k <= nmx - 1) {
-            const cd r = mul_dd((double)(k + 1), ui);
-            du = r - crcp_fast(du + r);
-          }
-          rows[kk][0][lane] = du.re;
-          rows[kk][1][lane] = du.im;
-          if (k <= sv && k > nsw_v) {
-            const cd r = mul_dd((double)k, vi);
-            const cd it = crcp_fast(dv + r);
-            rows[kk][2][lane] = it.re;
-            rows[kk][3][lane] = it.im;
-            dv = r - it;
-          }
-          if (k <= sw && k > nsw_w) {
-            const cd r = mul_dd((double)k, wi);
-            const cd it = crcp_fast(dw + r);
-            rows[kk][4][lane] = it.re;
-            rows[kk][5][lane] = it.im;
-            dw = r - it;
-          }
-          if (k <= sy && k > nsw_y) {
-            const double r = mul_dd((double)k, yi);
-            const double it = rcp_nr(dy + r);
-            rows[kk][6][lane] = it;
-            dy = r - it;
-          }
-        }
-      }
-      const int kend = min(K, nmax - b * K);
-      for (int kk = 0; kk < kend; ++kk) {
-        const int n = b * K + kk + 1;
-        const size_t o = (size_t)(n - 1) * 64 + lane;
-        const cd du = cd{rows[kk][0][lane], rows[kk][1][lane]};
-        const double fn = (double)(2 * n - 1), dn = (double)n;
-        const cd fv = mul_dd(fn, vinv), fw = mul_dd(fn, winv);
-        const cd pv = (n <= nsw_v) ? fv * pv_p - pv_pp
-                                   : pv_p * cd{rows[kk][2][lane], rows[kk][3][lane]};
-        const cd chv = fv * cv_p - cv_pp;
-        const cd pw = (n <= nsw_w) ? fw * pw_p - pw_pp
-                                   : pw_p * cd{rows[kk][4][lane], rows[kk][5][lane]};
-        const cd chw = fw * cw_p - cw_pp;
-        const double f = mul_dd(fn, yinv);
-        const double py = (n <= nsw_y) ? fma(f, py_p, -py_pp) : py_p * rows[kk][6][lane];
-        const double chy = fma(f, cy_p, -cy_pp);
-        const cd dpv = pv_p - mul_dd(dn, vinv) * pv;
-        const cd nw = mul_dd(dn, winv);
-        const cd dpw = pw_p - nw * pw, dcw = cw_p - nw * chw;
-        const cd t = du * pv;
-        cd U = m * t - dpv, V = mul_dd(t, minv) - dpv;
-        if (ddv_on) {
-          const cd c = DDV[o] * pv;
-          U = U - c;
-          V = V - c;
-        }
-        const cd A = pw * chv - chw * pv, B = dpw * chv - dcw * pv;
-        cd dns = cdiv(dpw + U * B, pw + U * A);
-        cd gns = cdiv(dpw + V * B, pw + V * A);
-        if (ddw_on) {
-          const cd c = DDW[o];
-          dns = dns + c;
-          gns = gns + c;
-        }
-        const double nrat = mul_dd(dn, yinv);
-        const cd a1 = mul_dd(dns, m2inv) + nrat;
-        const cd b1 = m2 * gns + nrat;
-        const cd gsy = cd{py, -chy}, gs1y = cd{py_p, -cy_p};
-        const cd a = cdiv(a1 * py - py_p, gsy * a1 - gs1y);
-        const cd bb = cdiv(b1 * py - py_p, gsy * b1 - gs1y);
-        pr.add(n, a, bb);
-        if (store) {
-          an[2 * (off + n - 1)] = a.re;
-          an[2 * (off + n - 1) + 1] = a.im;
-          bn[2 * (off + n - 1)] = bb.re;
-          bn[2 * (off + n - 1) + 1] = bb.im;
-        }
-        pv_pp = pv_p; pv_p = pv; cv_pp = cv_p; cv_p = chv;
-        pw_pp = pw_p; pw_p = pw; cw_pp = cw_p; cw_p = chw;
-        py_pp = py_p; py_p = py; cy_pp = cy_p; cy_p = chy;
-      }
+    if (__any(ddv_on || ddw_on)) {
+      pr = coef_off ? ckpt_pass2<K, true, true>(cp, rows, CK, rs_dd, b_ddv, b_ddw, lane, off, an, bn)
+                    : ckpt_pass2<K, true, false>(cp, rows, CK, rs_dd, b_ddv, b_ddw, lane, off, an,
+                                                 bn);
+    } else {
+      pr = coef_off ? ckpt_pass2<K, false, true>(cp, rows, CK, rs_dd, b_ddv, b_ddw, lane, off, an,
+                                                 bn)
+                    : ckpt_pass2<K, false, false>(cp, rows, CK, rs_dd, b_ddv, b_ddw, lane, off, an,
+                                                  bn);
     }
     if (props) pr.finish(yy, p, n_all, props);
   }
