@@ -281,6 +281,30 @@ struct Props {
     qb = cd{0.0, 0.0};
     ap = bp = cd{0.0, 0.0};
   }
+  // Orders visited one after another from n = 1: the asymmetry weights
+  // c1(n) = (n-1)(n+1)/n = n - 1/n and c2(n) = (2n+1)/(n(n+1)) = 1/n + 1/(n+1)
+  // from one reciprocal per order, 1/(n+1) carried into the next (rn holds 1/n;
+  // init_seq sets it for n = 1).
+  double rn;
+  __device__ __forceinline__ void init_seq() {
+    init();
+    rn = 1.0;
+  }
+  __device__ __forceinline__ void add_seq(int n, cd a, cd b) {
+    const double dn = (double)n;
+    const double cn = 2.0 * dn + 1.0;
+    const double rn1 = rcp_nr(dn + 1.0);
+    sext = fma(cn, a.re + b.re, sext);
+    ssca = fma(cn, norm2(a) + norm2(b), ssca);
+    const double sg = (n & 1) ? -cn : cn;
+    qb = qb + (a - b) * sg;
+    // at n = 1, c1 = 0 and a_0 = b_0 = 0 leave asy1 unchanged (no branch)
+    asy1 = fma(dn - rn, ap.re * a.re + ap.im * a.im + bp.re * b.re + bp.im * b.im, asy1);
+    asy2 = fma(rn + rn1, a.re * b.re + a.im * b.im, asy2);
+    rn = rn1;
+    ap = a;
+    bp = b;
+  }
   __device__ __forceinline__ void add(int n, cd a, cd b) {
     double dn = (double)n;
     double cn = 2.0 * dn + 1.0;
@@ -863,7 +887,7 @@ __device__ __forceinline__ void coated_up(const CoatedPt& c, __amdgpu_buffer_rsr
                                           uint32_t plane, int lane, int64_t off,
                                           double* __restrict__ an, double* __restrict__ bn,
                                           Props& pr) {
-  const cdd vinv = crcp_dd(c.v), winv = crcp_dd(c.w), minv = crcp_dd(c.m), m2inv = crcp_dd(c.m2);
+  const cdd vinv = crcp_dd(c.v), winv = crcp_dd(c.w), minv = crcp_dd(c.m);
   const ddr yinv = rcp_dd(c.y);
   const cd m = c.m, m2 = c.m2;
   const int nmax = c.nmax, nsw_v = c.nsw_v, nsw_w = c.nsw_w, nsw_y = c.nsw_y;
@@ -918,20 +942,20 @@ __device__ __forceinline__ void coated_up(const CoatedPt& c, __amdgpu_buffer_rsr
       V = V - cc;
     }
     const cd A = pw * chv - chw * pv, B = dpw * chv - dcw * pv;
-    cd dns = cdiv(dpw + U * B, pw + U * A);
-    cd gns = cdiv(dpw + V * B, pw + V * A);
+    // dns = Nd / Dd and gns = Ng / Dg (plus dD_w: N += dD_w D)
+    cd Nd = dpw + U * B, Ng = dpw + V * B;
+    const cd Dd = pw + U * A, Dg = pw + V * A;
     if constexpr (DD) {
       const cd cc = ld_cd(rs, b_ddw + 16u * r, c.ddw_on);
-      dns = dns + cc;
-      gns = gns + cc;
+      Nd = Nd + cc * Dd;
+      Ng = Ng + cc * Dg;
     }
     const double nrat = mul_dd(dn, yinv);
-    const cd a1 = mul_dd(dns, m2inv) + nrat;
-    const cd b1 = m2 * gns + nrat;
-    const cd gsy = cd{py, -chy}, gs1y = cd{py_p, -cy_p};
-    const cd a = cdiv(a1 * py - py_p, gsy * a1 - gs1y);
-    const cd b = cdiv(b1 * py - py_p, gsy * b1 - gs1y);
-    pr.add(n, a, b);
+    const cd xi = cd{py, -chy}, xi1 = cd{py_p, -cy_p};
+    const cd Ma = m2 * Dd, Pa = Nd + Ma * nrat, Pb = m2 * Ng + Dg * nrat;
+    const cd a = cdiv(Pa * py - Ma * py_p, Pa * xi - Ma * xi1);
+    const cd b = cdiv(Pb * py - Dg * py_p, Pb * xi - Dg * xi1);
+    pr.add_seq(n, a, b);
     if constexpr (STORE) {
       an[2 * (off + n - 1)] = a.re;
       an[2 * (off + n - 1) + 1] = a.im;
@@ -1042,7 +1066,7 @@ __global__ __launch_bounds__(64, MINW) void k_mie_coated_stable(
     // without divergent branches (coated_up); the start-index corrections
     // are compiled in only when a lane of the wavefront needs them
     Props pr;
-    pr.init();
+    pr.init_seq();
     const int64_t off = coef_off ? coef_off[p] : 0;
     const CoatedPt cp{v, w, yy, m, m2, nmax, nsw_v, nsw_w, nsw_y, ddv_on, ddw_on};
     const __amdgpu_buffer_rsrc_t rs = slot_rsrc(slot, (uint32_t)(plane * 11 * sizeof(double)));
@@ -1070,7 +1094,7 @@ __global__ __launch_bounds__(64, MINW) void k_mie_coated_stable(
 struct CkptPt {
   cd v, w, m, m2;
   double y;
-  cdd ui, vi, wi, minv, m2inv;
+  cdd ui, vi, wi, minv;
   ddr yi;
   int nmax, nmx, sv, sw, sy, nsw_v, nsw_w, nsw_y, nblk;
   bool ddv_on, ddw_on;
@@ -1090,7 +1114,7 @@ __device__ __forceinline__ Props ckpt_pass2(const CkptPt& c, double (*rows)[7][6
   double py_pp = cs, py_p = sn, cy_pp = -sn, cy_p = cs;
   const cd m = c.m, m2 = c.m2;
   Props pr;
-  pr.init();
+  pr.init_seq();
   double ck[7], ckn[7];
 #pragma unroll
   for (int q = 0; q < 7; ++q) ckn[q] = CK[q * 64 + lane];
@@ -1166,20 +1190,19 @@ __device__ __forceinline__ Props ckpt_pass2(const CkptPt& c, double (*rows)[7][6
         V = V - cc;
       }
       const cd A = pw * chv - chw * pv, B = dpw * chv - dcw * pv;
-      cd dns = cdiv(dpw + U * B, pw + U * A);
-      cd gns = cdiv(dpw + V * B, pw + V * A);
+      cd Nd = dpw + U * B, Ng = dpw + V * B;
+      const cd Dd = pw + U * A, Dg = pw + V * A;
       if constexpr (DD) {
         const cd cc = ld_cd(rs, b_ddw + 16u * r, c.ddw_on);
-        dns = dns + cc;
-        gns = gns + cc;
+        Nd = Nd + cc * Dd;
+        Ng = Ng + cc * Dg;
       }
       const double nrat = mul_dd(dn, c.yi);
-      const cd a1 = mul_dd(dns, c.m2inv) + nrat;
-      const cd b1 = m2 * gns + nrat;
-      const cd gsy = cd{py, -chy}, gs1y = cd{py_p, -cy_p};
-      const cd a = cdiv(a1 * py - py_p, gsy * a1 - gs1y);
-      const cd bb = cdiv(b1 * py - py_p, gsy * b1 - gs1y);
-      pr.add(n, a, bb);
+      const cd xi = cd{py, -chy}, xi1 = cd{py_p, -cy_p};
+      const cd Ma = m2 * Dd, Pa = Nd + Ma * nrat, Pb = m2 * Ng + Dg * nrat;
+      const cd a = cdiv(Pa * py - Ma * py_p, Pa * xi - Ma * xi1);
+      const cd bb = cdiv(Pb * py - Dg * py_p, Pb * xi - Dg * xi1);
+      pr.add_seq(n, a, bb);
       if constexpr (STORE) {
         an[2 * (off + n - 1)] = a.re;
         an[2 * (off + n - 1) + 1] = a.im;
@@ -1295,7 +1318,7 @@ __global__ __launch_bounds__(64, MINW) void k_mie_coated_ckpt(
     }
     // ---- pass 2: per block, regenerate K rows into LDS, then K upward steps
     const CkptPt cp{v,   w,     m,     m2,    yy,    ui,   vi,    wi,    crcp_dd(m),
-                    crcp_dd(m2), yi, nmax, nmx, sv, sw, sy, nsw_v, nsw_w, nsw_y,
+                    yi, nmax, nmx, sv, sw, sy, nsw_v, nsw_w, nsw_y,
                     nblk, ddv_on, ddw_on};
     const int64_t off = coef_off ? coef_off[p] : 0;
     const uint32_t b_ddv = 16u * (uint32_t)lane, b_ddw = 16u * ((uint32_t)rows_cap * 64u + lane);
