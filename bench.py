@@ -179,21 +179,43 @@ def cpu_chain_baseline(target_s=8.0):
                       % (len(times), dt, float(np.median(times)), cores)}
 
 
+def warm(fn, calls, seconds=0.1):
+    """Runs fn at least `calls` times and for at least `seconds`: the GPU
+    raises its clocks over some tens of ms of sustained load, so a timed
+    region that follows an idle stretch (a graph capture, host work) is
+    preceded by this much work."""
+    import torch
+    t0 = time.perf_counter()
+    k = 0
+    while k < calls or time.perf_counter() - t0 < seconds:
+        fn()
+        k += 1
+        if k % 8 == 0:
+            torch.cuda.synchronize()
+
+
 def time_pipeline(pipe, steps, warmup):
-    """Eager ms/step with per-kernel HIP-event times, then ms/step replaying
-    the step captured as a HIP graph."""
+    """ms/step replaying the step captured as a HIP graph (after warmup
+    replays), then, right after, eager ms/step with per-kernel HIP-event
+    times."""
     import torch
     from geosmie_amd import _lib
     L, h = _lib.lib(), pipe.ctx
-    for _ in range(warmup):
+    for _ in range(max(warmup, 1)):
         pipe.step()
+    pipe.capture()
+    warm(pipe.replay, warmup)
     torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    for _ in range(steps):
+        pipe.replay()
+    torch.cuda.synchronize()
+    t3 = time.perf_counter()
     L.gm_ctx_set_timing(h, 1)
     ms = ctypes.c_double(0)
     cnt = ctypes.c_int64(0)
     for k in range(9):
         L.gm_ctx_kernel_time(h, k, ctypes.byref(ms), ctypes.byref(cnt))
-    torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(steps):
         pipe.step()
@@ -206,14 +228,6 @@ def time_pipeline(pipe, steps, warmup):
         if cnt.value:
             kt[name] = ms.value / cnt.value
     L.gm_ctx_set_timing(h, 0)
-    pipe.capture()
-    pipe.replay()
-    torch.cuda.synchronize()
-    t2 = time.perf_counter()
-    for _ in range(steps):
-        pipe.replay()
-    torch.cuda.synchronize()
-    t3 = time.perf_counter()
     return (t1 - t0) * 1e3 / steps, kt, (t3 - t2) * 1e3 / steps
 
 
@@ -233,6 +247,7 @@ def _secondary_homog(steps, warmup, autotune, RunopticsChain, cpu=True):
     pipe = RunopticsChain(x, m, ang, w, lam, [(i * 2000, 2000) for i in range(10)])
     # the FP64 step fuses S1/S2 with the row integrals: only the Mie kernel's
     # register budget is a knob of it
+    warm(pipe.step, 1)
     tuned = autotune.tune(pipe.ctx, pipe.step, knobs=("mie_lanes", "mie_waves"), reps=3)
     ms_eager, kt, ms = time_pipeline(pipe, steps, warmup)
     s12_flop = 16.0 * pipe.sum_nmax * ang.size
@@ -282,6 +297,7 @@ def _secondary_largex(steps, warmup, autotune, RunopticsChain):
         w[i, i * 5000:(i + 1) * 5000] = p / p.sum()
     pipe = RunopticsChain(x, m, ang, w, lam_row, [(i * 5000, 5000) for i in range(8)],
                           precision=0, gsf_ng=300)
+    warm(pipe.step, 1)
     tuned3 = autotune.tune(pipe.ctx, pipe.step, knobs=("mie_lanes", "mie_waves"), reps=2)
     s12_flop = 16.0 * pipe.sum_nmax * ang.size
     modes = []
@@ -557,8 +573,8 @@ def coated_conventions(x, y, m1, m2):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calib", action="store_true",
                     help="also run 3 device copies of 512 MiB (PMC byte-counter calibration)")
@@ -608,6 +624,10 @@ def main():
                              _lib.stream()), "gm_mie_batch")
 
     from geosmie_amd import autotune
+    # the GPU raises its clocks over some tens of ms of sustained load: run
+    # the step for a while before the candidates are timed
+    for _ in range(10):
+        step()
     # the grid multiple before and after the checkpoint interval: the best
     # multiple differs between the two coated kernels
     tuned = autotune.tune(h, step, knobs=("mie_grid_mult", "mie_ckpt", "mie_grid_mult",
@@ -615,28 +635,12 @@ def main():
     ckpt = autotune.get(h, "mie_ckpt")
     kernel_name = "k_mie_coated_ckpt<1,%d>" % ckpt if ckpt else "k_mie_coated_stable"
     traffic, traffic_src = measured_traffic(kernel_name)
-    for _ in range(args.warmup):
-        step()
     torch.cuda.synchronize()
     flags = ctypes.c_int32(0)
     check(L.gm_ctx_check(h, _lib.stream(), ctypes.byref(flags)), "gm_ctx_check")
     assert flags.value == 0, flags.value
-    # eager pass with the library's own HIP-event pairs around the coated
-    # kernel, on the stream it is launched on
-    L.gm_ctx_set_timing(h, 1)
-    ms_k = ctypes.c_double(0)
-    cnt = ctypes.c_int64(0)
-    L.gm_ctx_kernel_time(h, 1, ctypes.byref(ms_k), ctypes.byref(cnt))  # drop stale pairs
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        step()
-    torch.cuda.synchronize()
-    ms_eager = (time.perf_counter() - t0) * 1e3 / args.steps
-    check(L.gm_ctx_kernel_time(h, 1, ctypes.byref(ms_k), ctypes.byref(cnt)), "kernel_time")
-    L.gm_ctx_set_timing(h, 0)
-    kern_ms = ms_k.value / max(cnt.value, 1)
-    # the timed region: the step captured once as a HIP graph, replayed
+    # the step captured once as a HIP graph (one eager run on a side stream
+    # first, as graph capture requires)
     side = torch.cuda.Stream()
     side.wait_stream(torch.cuda.current_stream())
     with torch.cuda.stream(side):
@@ -646,7 +650,10 @@ def main():
     graph = torch.cuda.CUDAGraph()
     with torch.cuda.graph(graph):
         step()
-    graph.replay()
+    # warmup replays, then the timed region: exactly `steps` replays between
+    # two synchronisations
+    for _ in range(args.warmup):
+        graph.replay()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
@@ -654,6 +661,21 @@ def main():
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     ms_step = (t1 - t0) * 1e3 / args.steps
+    # right after it, with the clocks where the timed region left them: an
+    # eager pass with the library's own HIP-event pairs around the coated
+    # kernel, on the stream it is launched on (the kernel's launch duration)
+    L.gm_ctx_set_timing(h, 1)
+    ms_k = ctypes.c_double(0)
+    cnt = ctypes.c_int64(0)
+    L.gm_ctx_kernel_time(h, 1, ctypes.byref(ms_k), ctypes.byref(cnt))  # drop stale pairs
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    ms_eager = (time.perf_counter() - t0) * 1e3 / args.steps
+    check(L.gm_ctx_kernel_time(h, 1, ctypes.byref(ms_k), ctypes.byref(cnt)), "kernel_time")
+    L.gm_ctx_set_timing(h, 0)
+    kern_ms = ms_k.value / max(cnt.value, 1)
     props = d_props.view(6, n)
     finite = bool(torch.isfinite(props).all().item())
     frac = {k: v / (kern_ms * 1e-3) / 1e12 / PEAK_FP64_TFLOPS for k, v in conv.items()}
