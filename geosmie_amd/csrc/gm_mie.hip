@@ -1160,6 +1160,8 @@ __device__ __forceinline__ Props ckpt_pass2(const CkptPt& c, double (*rows)[7][6
       }
     }
     const int kend = min(K, c.nmax - b * K);
+    // unrolled by two: the psi/chi state rotates through registers without moves
+#pragma unroll 2
     for (int kk = 0; kk < kend; ++kk) {
       const int n = b * K + kk + 1;
       const uint32_t r = (uint32_t)(n - 1) * 64u;
