@@ -1115,17 +1115,21 @@ __device__ __forceinline__ Props ckpt_pass2(const CkptPt& c, double (*rows)[7][6
   const cd m = c.m, m2 = c.m2;
   Props pr;
   pr.init_seq();
+  // block 0's rows are in LDS already (pass 1 leaves them there); block b > 0
+  // is regenerated from checkpoint b, the next checkpoint in flight meanwhile
   double ck[7], ckn[7];
+  if (c.nblk > 1) {
 #pragma unroll
-  for (int q = 0; q < 7; ++q) ckn[q] = CK[q * 64 + lane];
+    for (int q = 0; q < 7; ++q) ckn[q] = CK[(7 + q) * 64 + lane];
+  }
   for (int b = 0; b < c.nblk; ++b) {
+    if (b > 0) {
 #pragma unroll
-    for (int q = 0; q < 7; ++q) ck[q] = ckn[q];
-    if (b + 1 < c.nblk) {
+      for (int q = 0; q < 7; ++q) ck[q] = ckn[q];
+      if (b + 1 < c.nblk) {
 #pragma unroll
-      for (int q = 0; q < 7; ++q) ckn[q] = CK[((size_t)(b + 1) * 7 + q) * 64 + lane];
-    }
-    {
+        for (int q = 0; q < 7; ++q) ckn[q] = CK[((size_t)(b + 1) * 7 + q) * 64 + lane];
+      }
       cd du = cd{ck[0], ck[1]}, dv = cd{ck[2], ck[3]}, dw = cd{ck[4], ck[5]};
       double dy = ck[6];
 #pragma unroll
@@ -1222,11 +1226,13 @@ __device__ __forceinline__ Props ckpt_pass2(const CkptPt& c, double (*rows)[7][6
 // The arithmetic of k_mie_coated_stable, with a different memory schedule.
 // Pass 1 runs the four downward chains (D_n(u) and the psi ratios of v, w,
 // y) over every row but stores only their states every K rows: checkpoint b
-// is the state after row bK+K+1, from which rows bK+K .. bK+1 follow.  Pass 2
-// walks the blocks upward: it reloads block b's checkpoint (the next one is
-// already in flight), regenerates the block's K rows into LDS with exactly
-// the operations of pass 1 (so every row is bit-identical to the value pass 1
-// computed), and consumes them in the K upward steps.  Scratch traffic per
+// is the state after row bK+K+1, from which rows bK+K .. bK+1 follow; rows
+// K .. 1, the last pass 1 forms, it writes to LDS directly (no checkpoint 0).
+// Pass 2 walks the blocks upward: block 0 from those LDS rows, block b > 0 by
+// reloading checkpoint b (the next one is already in flight) and
+// regenerating the block's K rows into LDS with exactly the operations of
+// pass 1 (so every row is bit-identical to the value pass 1 computed); the K
+// upward steps consume them.  Scratch traffic per
 // point drops from up to 7 doubles per row written and read to 7 doubles per
 // K rows, and the upward pass no longer waits on scratch rows.
 // Slot layout: checkpoints [b][7][lane] (du.re, du.im, dv.re, dv.im, dw.re,
@@ -1284,7 +1290,7 @@ __global__ __launch_bounds__(64, MINW) void k_mie_coated_ckpt(
       const int top = max(max(nmx - 1, sv), max(sw, sy));
       cd du = cd{0.0, 0.0}, dv = cd{0.0, 0.0}, dw = cd{0.0, 0.0};
       double dy = 0.0;
-      for (int k = top; k >= 1; --k) {
+      for (int k = top; k > K; --k) {
         if (k <= nmx - 1) {   // D_k(u) = (k+1)/u - 1/(D_{k+1} + (k+1)/u), D_nmx = 0
           const cd r = mul_dd((double)(k + 1), ui);
           du = r - crcp_fast(du + r);
@@ -1302,10 +1308,40 @@ __global__ __launch_bounds__(64, MINW) void k_mie_coated_ckpt(
           dy = r - rcp_nr(dy + r);
         }
         const int b = (k - 1) / K - 1;      // state after row bK+K+1
-        if ((k - 1) % K == 0 && b >= 0 && b < nblk) {
+        if ((k - 1) % K == 0 && b >= 1 && b < nblk) {
           double* c = CK + (size_t)b * 7 * 64 + lane;
           c[0] = du.re; c[64] = du.im; c[128] = dv.re; c[192] = dv.im;
           c[256] = dw.re; c[320] = dw.im; c[384] = dy;
+        }
+      }
+      // rows K..1 (pass 2's first block) go to LDS as pass 2 would regenerate
+      // them, so block 0 needs no checkpoint and no regeneration
+      for (int k = min(top, K); k >= 1; --k) {
+        if (k <= nmx - 1) {
+          const cd r = mul_dd((double)(k + 1), ui);
+          du = r - crcp_fast(du + r);
+        }
+        rows[k - 1][0][lane] = du.re;
+        rows[k - 1][1][lane] = du.im;
+        if (k <= sv && k > nsw_v) {
+          const cd r = mul_dd((double)k, vi);
+          const cd it = crcp_fast(dv + r);
+          rows[k - 1][2][lane] = it.re;
+          rows[k - 1][3][lane] = it.im;
+          dv = r - it;
+        }
+        if (k <= sw && k > nsw_w) {
+          const cd r = mul_dd((double)k, wi);
+          const cd it = crcp_fast(dw + r);
+          rows[k - 1][4][lane] = it.re;
+          rows[k - 1][5][lane] = it.im;
+          dw = r - it;
+        }
+        if (k <= sy && k > nsw_y) {
+          const double r = mul_dd((double)k, yi);
+          const double it = rcp_nr(dy + r);
+          rows[k - 1][6][lane] = it;
+          dy = r - it;
         }
       }
     }
@@ -1318,7 +1354,8 @@ __global__ __launch_bounds__(64, MINW) void k_mie_coated_ckpt(
       dn_chain(wi, nmx, nmax, DDW, lane, false);
       dn_chain(wi, nmx + 16 + 8 * (int)ceil(cbrt(aw)), nmax, DDW, lane, true);
     }
-    // ---- pass 2: per block, regenerate K rows into LDS, then K upward steps
+    // ---- pass 2: per block, its K rows in LDS (block 0 left by pass 1, the
+    // others regenerated), then K upward steps
     const CkptPt cp{v,   w,     m,     m2,    yy,    ui,   vi,    wi,    crcp_dd(m),
                     yi, nmax, nmx, sv, sw, sy, nsw_v, nsw_w, nsw_y,
                     nblk, ddv_on, ddw_on};
