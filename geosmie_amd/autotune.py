@@ -19,6 +19,9 @@ Knobs (include/geosmie.h, enum gm_tune_param):
                  regenerated per block in LDS, or every row kept in scratch (0)
   mie_lanes      homogeneous kernel: lanes per point (1, or 16/32/64 with the serial
                  recurrences joined by parallel scans; 0 = chosen by batch)
+  s12_row_tiles  fused S1/S2 row sums: 8-point tiles per work unit (0 = by batch)
+  s12_share      split-BF16 fused S1/S2: angle-table tiles shared by four work units
+                 through LDS (1) or fetched per wavefront (0)
 (mlp_weight_terms, the BF16 terms per network weight, is a precision
 setting reached through the same call; it is not tuned for speed.)
 """
@@ -39,6 +42,7 @@ KNOBS = {"mie_grid_mult": (0, (1, 2, 4)),
          "mie_ckpt": (7, (4, 8, 0)),
          "mie_lanes": (8, (0, 1, 16, 32, 64)),
          "s12_row_tiles": (10, (0, 1, 2, 4)),
+         "s12_share": (11, (1, 0)),
          # a precision setting rather than a shape; set explicitly, never tuned for speed
          "mlp_weight_terms": (3, (1, 2)),
          # a behaviour setting: the integratePSD quirks reproduced (bit 0: the aliased

@@ -434,6 +434,11 @@ int gm_ctx_tune(gm_ctx* ctx, int param, int value) {
         return gm_fail(GM_EINVAL, "gm_ctx_tune: S1/S2 tiles per work unit must be 0, 1, 2 or 4");
       ctx->tune_s12_row_tiles = value;
       return GM_OK;
+    case GM_TUNE_S12_SHARE:
+      if (value != 0 && value != 1)
+        return gm_fail(GM_EINVAL, "gm_ctx_tune: S1/S2 table sharing must be 0 or 1");
+      ctx->tune_s12_share = value;
+      return GM_OK;
     case GM_TUNE_PSD_QUIRKS:
       if (value & ~(int)GM_PSD_QUIRK_ALIAS)
         return gm_fail(GM_EINVAL, "gm_ctx_tune: unknown integratePSD quirk bits %d", value);
@@ -463,6 +468,7 @@ int gm_ctx_get_tune(gm_ctx* ctx, int param, int* value) {
     case GM_TUNE_MIE_LANES: *value = ctx->tune_mie_lanes; return GM_OK;
     case GM_TUNE_PSD_QUIRKS: *value = ctx->psd_quirks; return GM_OK;
     case GM_TUNE_S12_ROW_TILES: *value = ctx->tune_s12_row_tiles; return GM_OK;
+    case GM_TUNE_S12_SHARE: *value = ctx->tune_s12_share; return GM_OK;
     default: return gm_fail(GM_EINVAL, "gm_ctx_get_tune: unknown parameter %d", param);
   }
 }

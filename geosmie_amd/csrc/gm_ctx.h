@@ -39,6 +39,7 @@ struct gm_ctx {
   int tune_mie_ckpt = 4;        // coated kernel: rows per checkpoint (4, 8) or 0 = every row stored
   int tune_mie_lanes = 0;       // homogeneous kernel: lanes per point (1, 16, 32, 64; 0 = by batch)
   int psd_quirks = 1;           // integratePSD quirks reproduced (GM_PSD_QUIRK_* bits)
+  int tune_s12_share = 1;        // split-BF16 fused S1/S2: angle-table tiles shared through LDS
   int tune_s12_row_tiles = 0;    // reduced-precision fused S1/S2: 8-point tiles per wave (0 = auto)
   Buf status;                                    // int32 flags + scan scratch
   Buf order, key, hist, scratch, scan_tmp;

@@ -756,6 +756,66 @@ __global__ void k_s12_split_coef(int64_t n_coef, const double* __restrict__ an,
   out[c] = o;
 }
 
+// Epilogue of the reduced-precision row kernels: D[4 kq + r][i] of tile mt
+// holds, in elements (0, 1), point p0 + 8 mt + 2 kq (re, im) and in (2, 3)
+// the next; Mueller elements in FP64, the row sums over the chunk's points in
+// point order, then over the lane groups, into the chunk's partials.
+template <int MT, int RM>
+__device__ __forceinline__ void rows_lp_epilogue(const float4_ (&accp)[MT][4],
+                                                 const float4_ (&accm)[MT][4], int lane,
+                                                 int64_t chunk, int j0, int n_ang, int64_t p0,
+                                                 int64_t cend, int nr, const double* wr,
+                                                 int64_t ldw, double* __restrict__ part) {
+  const int i = lane & 15;
+  const int kq = lane >> 4;
+  double* dst = part + (size_t)chunk * RM * 4 * n_ang;
+  double wv[MT][2][RM];
+#pragma unroll
+  for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int64_t p = p0 + 8 * mt + 2 * kq + h;
+#pragma unroll
+      for (int r = 0; r < RM; ++r)
+        wv[mt][h][r] = (r < nr && p < cend) ? wr[(size_t)r * ldw + p] : 0.0;
+    }
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    double el[MT][2][4];
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const cd sp = cd{(double)accp[mt][t][2 * h], (double)accp[mt][t][2 * h + 1]};
+        const cd sm = cd{(double)accm[mt][t][2 * h], (double)accm[mt][t][2 * h + 1]};
+        const cd s1 = (sp + sm) * 0.5, s2 = (sp - sm) * 0.5;
+        const double a1 = norm2(s1), a2 = norm2(s2);
+        const cd c = s1 * conj(s2);
+        el[mt][h][0] = 0.5 * (a1 + a2);
+        el[mt][h][1] = 0.5 * (a2 - a1);
+        el[mt][h][2] = c.re;
+        el[mt][h][3] = c.im;
+      }
+    const int j = j0 + t * 16 + i;
+#pragma unroll
+    for (int r = 0; r < RM; ++r) {
+      double mine = 0.0;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        double v = 0.0;
+#pragma unroll
+        for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+          for (int h = 0; h < 2; ++h) v = fma(wv[mt][h][r], el[mt][h][e], v);
+        v += __shfl_xor(v, 16);
+        v += __shfl_xor(v, 32);
+        if (e == kq) mine = v;
+      }
+      if (r < nr && j < n_ang) dst[((size_t)r * 4 + kq) * n_ang + j] = mine;
+    }
+  }
+}
+
 // S1/S2 of 8 MT points x 64 angles per wavefront on the FP32 or split-BF16
 // matrix cores (operands as in k_s12_f32 / k_s12_bf16x3, coefficients from
 // k_s12_split_coef), then the Mueller elements and their weighted row sums
@@ -904,55 +964,167 @@ __global__ __launch_bounds__(64 * kRowsWaves) void k_s12_rows_lp(
       }
     }
   }
-  // D[4 kq + r][i] of tile mt: elements (0, 1) = point p0 + 8 mt + 2 kq
-  // (re, im), (2, 3) = the next; the row sums over the chunk's points in
-  // point order, then over the lane groups
-  double* dst = part + (size_t)chunk * RM * 4 * n_ang;
-  double wv[MT][2][RM];
+  rows_lp_epilogue<MT, RM>(accp, accm, lane, chunk, j0, n_ang, p0, cend, nr, wr, ldw, part);
+}
+
+// Split-BF16 S1/S2 fused with the row sums, kShChunks chunks per block
+// sharing each K-step's angle-table tile through LDS.  Wavefront w of block
+// (bx, by) takes chunk kShChunks bx + w (8 MT points, as k_s12_rows_lp) and
+// the 64 angles from 64 by.  The block's threads stage the next tile of kShK
+// table rows (4 planes x 32 K x 64 angles, 16 KB) into the other half of a
+// double buffer while the wavefronts issue the current tile's MFMAs, so each
+// table word crosses the L2 once per kShChunks chunks and no wavefront holds
+// table words of a later step in registers beyond its own staging share.
+// Used with one 8-point tile per chunk, where it is the faster layout (the
+// register-prefetching k_s12_rows_lp wins with two).  The block's K loop runs to the
+// largest series length of its chunks; a wavefront whose own chunk is done
+// still stages its share of the tiles.  Products, sums and the partial
+// layout are those of k_s12_rows_lp (bit for bit: same operands, same
+// accumulation order).
+constexpr int kShChunks = 4;
+constexpr int kShK = 32;   // K rows per staged tile (one MFMA step per barrier)
+template <int MT, int RM>
+__global__ __launch_bounds__(64 * kShChunks) void k_s12_rows_bf16s(
+    RowGroups g, int64_t chunks, const int64_t* __restrict__ off,
+    const int32_t* __restrict__ nmax, const uint4* __restrict__ coef, int n_ang, int apad,
+    int kpad, const void* __restrict__ tab, int nmax_cap, const double* __restrict__ w,
+    int64_t ldw, double* __restrict__ part) {
+  // [buffer][plane][K-octet][angle], 2 x 16 KB
+  __shared__ uint4 st[2][4][kShK / 8][64];
+  __shared__ int s_kmax[kShChunks];
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int64_t chunk = (int64_t)blockIdx.x * kShChunks + wave;
+  const bool live = chunk < chunks;
+  const int j0 = blockIdx.y * 64;
+  const int i = lane & 15;
+  const int kq = lane >> 4;
+  const int comp = i & 1;
+  int gi = 0;
+  int64_t p0 = 0, cend = 0;
+  int nr = 0;
+  const double* wr = w;
+  int nm_a[MT];
+  int64_t oa[MT];
+  int kmax = 0;
+  if (live) {
+    while (gi + 1 < g.n_groups && g.chunk0[gi + 1] <= chunk) ++gi;
+    p0 = g.start[gi] + (chunk - g.chunk0[gi]) * 8 * MT;
+    cend = min(p0 + 8 * MT, g.start[gi] + g.count[gi]);
+    nr = g.nrow[gi];
+    wr = w + (size_t)g.row0[gi] * ldw;
+    for (int q = 0; q < 8 * MT; ++q)
+      if (p0 + q < cend) kmax = max(kmax, nmax[p0 + q]);
+    kmax = (min(kmax, nmax_cap) + kLpK - 1) / kLpK * kLpK;   // <= kpad
+  }
+#pragma unroll
+  for (int mt = 0; mt < MT; ++mt) {
+    const int64_t pa = p0 + 8 * mt + (i >> 1);
+    const bool ok = live && pa < cend;
+    nm_a[mt] = ok ? nmax[pa] : 0;
+    oa[mt] = ok ? off[pa] : 0;
+  }
+  if (lane == 0) s_kmax[wave] = kmax;
+  __syncthreads();
+  int kmax_b = 0;
+#pragma unroll
+  for (int q = 0; q < kShChunks; ++q) kmax_b = max(kmax_b, s_kmax[q]);
+  float4_ accp[MT][4], accm[MT][4];
 #pragma unroll
   for (int mt = 0; mt < MT; ++mt)
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      const int64_t p = p0 + 8 * mt + 2 * kq + h;
-#pragma unroll
-      for (int r = 0; r < RM; ++r)
-        wv[mt][h][r] = (r < nr && p < cend) ? wr[(size_t)r * ldw + p] : 0.0;
+    for (int t = 0; t < 4; ++t) {
+      accp[mt][t] = (float4_){0.f, 0.f, 0.f, 0.f};
+      accm[mt][t] = (float4_){0.f, 0.f, 0.f, 0.f};
     }
-#pragma unroll
-  for (int t = 0; t < 4; ++t) {
-    double el[MT][2][4];
+  const uint2* cb = (const uint2*)coef;
+  const uint4* tb = (const uint4*)tab;   // 8 BF16 (one K-octet) per uint4
+  const size_t plane4 = (size_t)kpad * apad / 8;
+  // this thread's staging words: K-octet tid / 64 of the tile, angle tid % 64,
+  // every plane (kpad is a multiple of kShK and apad of 64: the tile lies
+  // inside the zero-padded table)
+  const int so8 = tid >> 6, sa = tid & 63;
+  const size_t so = (size_t)so8 * apad + j0 + sa;
+  uint4 g0, g1, g2, g3;   // one word per plane
+#define GM_STAGE_LOAD(k0_)                                       \
+  {                                                              \
+    const size_t o_ = (size_t)((k0_) >> 3) * apad + so;          \
+    g0 = tb[o_];                                                 \
+    g1 = tb[plane4 + o_];                                        \
+    g2 = tb[2 * plane4 + o_];                                    \
+    g3 = tb[3 * plane4 + o_];                                    \
+  }
+#define GM_STAGE_STORE(b_)                                       \
+  {                                                              \
+    st[b_][0][so8][sa] = g0;                                     \
+    st[b_][1][so8][sa] = g1;                                     \
+    st[b_][2][so8][sa] = g2;                                     \
+    st[b_][3][so8][sa] = g3;                                     \
+  }
+  uint2 cn[MT][8];
+  auto load_coef = [&](int k0) {
+    const int kb = k0 + 8 * kq;
 #pragma unroll
     for (int mt = 0; mt < MT; ++mt)
 #pragma unroll
-      for (int h = 0; h < 2; ++h) {
-        const cd sp = cd{(double)accp[mt][t][2 * h], (double)accp[mt][t][2 * h + 1]};
-        const cd sm = cd{(double)accm[mt][t][2 * h], (double)accm[mt][t][2 * h + 1]};
-        const cd s1 = (sp + sm) * 0.5, s2 = (sp - sm) * 0.5;
-        const double a1 = norm2(s1), a2 = norm2(s2);
-        const cd c = s1 * conj(s2);
-        el[mt][h][0] = 0.5 * (a1 + a2);
-        el[mt][h][1] = 0.5 * (a2 - a1);
-        el[mt][h][2] = c.re;
-        el[mt][h][3] = c.im;
-      }
-    const int j = j0 + t * 16 + i;
+      for (int t = 0; t < 8; ++t)
+        cn[mt][t] = kb + t < nm_a[mt] ? cb[2 * (oa[mt] + kb + t) + comp] : make_uint2(0u, 0u);
+  };
+  if (kmax_b > 0) {
+    GM_STAGE_LOAD(0);
+    GM_STAGE_STORE(0);
+    load_coef(0);
+  }
+  __syncthreads();
+  int buf = 0;
+  for (int k0 = 0; k0 < kmax_b; k0 += kShK) {
+    const bool more = k0 + kShK < kmax_b;
+    if (more) GM_STAGE_LOAD(k0 + kShK);
 #pragma unroll
-    for (int r = 0; r < RM; ++r) {
-      double mine = 0.0;
-#pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        double v = 0.0;
+    for (int sub = 0; sub < kShK / kLpK; ++sub) {
+      const int ks = k0 + sub * kLpK;
+      if (ks < kmax) {   // this wavefront's own chunk still has terms
+        union { bf16x8_ v; unsigned u[4]; } uh[MT], ul[MT], vh[MT], vl[MT];
 #pragma unroll
         for (int mt = 0; mt < MT; ++mt)
 #pragma unroll
-          for (int h = 0; h < 2; ++h) v = fma(wv[mt][h][r], el[mt][h][e], v);
-        v += __shfl_xor(v, 16);
-        v += __shfl_xor(v, 32);
-        if (e == kq) mine = v;
+          for (int t = 0; t < 4; ++t) {
+            const uint2 a = cn[mt][2 * t], b = cn[mt][2 * t + 1];
+            uh[mt].u[t] = (a.x & 0xffffu) | (b.x << 16);
+            ul[mt].u[t] = (a.x >> 16) | (b.x & 0xffff0000u);
+            vh[mt].u[t] = (a.y & 0xffffu) | (b.y << 16);
+            vl[mt].u[t] = (a.y >> 16) | (b.y & 0xffff0000u);
+          }
+        if (ks + kLpK < kmax) load_coef(ks + kLpK);
+        const int ko = 4 * sub + kq;
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+          union { uint4 q; bf16x8_ v; } ph, pl, mh, ml;
+          ph.q = st[buf][0][ko][16 * t + i];
+          pl.q = st[buf][1][ko][16 * t + i];
+          mh.q = st[buf][2][ko][16 * t + i];
+          ml.q = st[buf][3][ko][16 * t + i];
+#pragma unroll
+          for (int mt = 0; mt < MT; ++mt) {
+            accp[mt][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(uh[mt].v, ph.v, accp[mt][t], 0, 0, 0);
+            accp[mt][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(uh[mt].v, pl.v, accp[mt][t], 0, 0, 0);
+            accp[mt][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ul[mt].v, ph.v, accp[mt][t], 0, 0, 0);
+            accm[mt][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(vh[mt].v, mh.v, accm[mt][t], 0, 0, 0);
+            accm[mt][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(vh[mt].v, ml.v, accm[mt][t], 0, 0, 0);
+            accm[mt][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(vl[mt].v, mh.v, accm[mt][t], 0, 0, 0);
+          }
+        }
       }
-      if (r < nr && j < n_ang) dst[((size_t)r * 4 + kq) * n_ang + j] = mine;
     }
+    // the other buffer was last read in the previous tile, before the barrier
+    if (more) GM_STAGE_STORE(buf ^ 1);
+    __syncthreads();
+    buf ^= 1;
   }
+#undef GM_STAGE_STORE
+#undef GM_STAGE_LOAD
+  if (live)
+    rows_lp_epilogue<MT, RM>(accp, accm, lane, chunk, j0, n_ang, p0, cend, nr, wr, ldw, part);
 }
 
 }  // namespace gm
@@ -1195,6 +1367,20 @@ int gm_s12_contract_rows(gm_ctx* ctx, int64_t n, const int64_t* coef_off, const 
       if (row_tiles == 4) { GM_ROWS_LP_RM(GM_PREC_FP32, 4); }
       else if (row_tiles == 2) { GM_ROWS_LP_RM(GM_PREC_FP32, 2); }
       else { GM_ROWS_LP_RM(GM_PREC_FP32, 1); }
+    } else if (ctx->tune_s12_share && row_tiles == 1) {
+      // kShChunks work units per block share the angle-table tiles (LDS)
+      const dim3 sgrid((unsigned)((chunks + gm::kShChunks - 1) / gm::kShChunks),
+                       (unsigned)((n_ang + 63) / 64));
+#define GM_ROWS_SH(M_, R_)                                                                     \
+  hipLaunchKernelGGL((gm::k_s12_rows_bf16s<M_, R_>), sgrid, dim3(64 * gm::kShChunks), 0, s, g, \
+                     chunks, coef_off, nmax, cf, n_ang, apad, kpad, tb, nmax_cap, w, n, part)
+#define GM_ROWS_SH_RM(M_)        \
+  if (RM == 1) GM_ROWS_SH(M_, 1); \
+  else if (RM == 2) GM_ROWS_SH(M_, 2); \
+  else GM_ROWS_SH(M_, 4)
+      GM_ROWS_SH_RM(1);
+#undef GM_ROWS_SH_RM
+#undef GM_ROWS_SH
     } else {
       if (row_tiles == 4) { GM_ROWS_LP_RM(GM_PREC_BF16X3, 4); }
       else if (row_tiles == 2) { GM_ROWS_LP_RM(GM_PREC_BF16X3, 2); }

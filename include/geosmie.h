@@ -109,10 +109,14 @@ enum gm_tune_param {
   GM_TUNE_PSD_QUIRKS = 9,       /* a behaviour setting, not tuned for speed: the integratePSD
                                    quirks gm_integrate_psd[_rows] reproduce (GM_PSD_QUIRK_* bits;
                                    default GM_PSD_QUIRK_ALIAS) */
-  GM_TUNE_S12_ROW_TILES = 10    /* gm_s12_contract_rows: 8-point tiles per work unit (1, 2, 4): at
+  GM_TUNE_S12_ROW_TILES = 10,   /* gm_s12_contract_rows: 8-point tiles per work unit (1, 2, 4): at
                                    FP64 one after the other, at FP32 / split-BF16 multiplied
                                    together, sharing each angle-table fetch; 0 (default) = by
                                    batch size.  Results agree to rounding */
+  GM_TUNE_S12_SHARE = 11        /* gm_s12_contract_rows at split-BF16 with one tile per work
+                                   unit: 1 (default) = four work units per block share each
+                                   K-step's angle-table tile through LDS; 0 = each wavefront
+                                   fetches its own.  Identical results */
 };
 int gm_ctx_tune(gm_ctx* ctx, int param, int value);
 int gm_ctx_get_tune(gm_ctx* ctx, int param, int* value);

@@ -97,6 +97,36 @@ def test_fused_rows_match_unfused(gpu, layout, precision):
             assert np.abs(got[r] - want).max() <= 1e-12 * scale, (layout, r)
 
 
+def test_shared_table_tiles_bitwise(gpu):
+    """Split-BF16 fused rows, one 8-point tile per work unit, with the
+    angle-table tiles shared by four work units through LDS against each
+    wavefront fetching its own: the same
+    operands in the same order, so the results are identical bit for bit.
+    The ranges give a work-unit count that is not a multiple of four (idle
+    wavefronts in the last block), blocks whose units have different series
+    lengths, and a partial last angle group (181 angles)."""
+    from geosmie_amd import _lib, autotune
+    n = 3000
+    dm, ang, cap, pi_t, tau_t, rng = _batch(n, 11)
+    ranges = [(0, 700), (700, 1), (701, 799), (701, 799), (1500, 1499), (2999, 1), (5, 37)]
+    starts = np.array([a for a, _ in ranges])
+    counts = np.array([c for _, c in ranges])
+    W = rng.uniform(0, 1, (len(ranges), n))
+    h = _lib.ctx()
+    got = {}
+    try:
+        autotune.set_knob(h, "s12_row_tiles", 1)
+        for share in (1, 0):
+            autotune.set_knob(h, "s12_share", share)
+            print("split-BF16 rows, one tile per unit, share", share, flush=True)
+            got[share] = _fused(dm, ang, cap, pi_t, tau_t, starts, counts, W, 2)
+    finally:
+        autotune.set_knob(h, "s12_share", 1)
+        autotune.set_knob(h, "s12_row_tiles", 0)
+    assert np.isfinite(got[1]).all()
+    assert np.array_equal(got[1], got[0])
+
+
 def test_fused_rows_rejects_bad_ranges(gpu):
     dm, ang, cap, pi_t, tau_t, rng = _batch(200, 3)
     W = np.ones((1, 200))
