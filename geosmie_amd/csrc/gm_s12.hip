@@ -909,7 +909,7 @@ __global__ __launch_bounds__(64 * kRowsWaves) void k_s12_rows_lp(
     const uint4* tb = (const uint4*)tab;   // 8 BF16 (one K-octet) per uint4
     const size_t plane4 = (size_t)kpad * apad / 8;
     // next step's operands: each 16-angle sub-tile's table words are reloaded
-    // as soon as that sub-tile's products are issued
+    // as soon as that sub-tile's products are issued, into the same registers
     uint4 tn[4][4];
     uint2 cn[MT][8];
     auto load_coef = [&](int k0) {
@@ -918,20 +918,19 @@ __global__ __launch_bounds__(64 * kRowsWaves) void k_s12_rows_lp(
       for (int mt = 0; mt < MT; ++mt)
 #pragma unroll
         for (int t = 0; t < 8; ++t)
-          cn[mt][t] = kb + t < nm_a[mt] ? cb[2 * (oa[mt] + kb + t) + comp] : make_uint2(0u, 0u);
+          cn[mt][t] =
+              kb + t < nm_a[mt] ? cb[2 * (oa[mt] + kb + t) + comp] : make_uint2(0u, 0u);
     };
     auto load_tab = [&](int k0, int t) {
       const size_t o4 = (size_t)((k0 + 8 * kq) >> 3) * apad + j0 + i + 16 * t;
 #pragma unroll
       for (int pl = 0; pl < 4; ++pl) tn[t][pl] = tb[pl * plane4 + o4];
     };
-    if (kmax > 0) {
-      load_coef(0);
-#pragma unroll
-      for (int t = 0; t < 4; ++t) load_tab(0, t);
-    }
-    for (int k0 = 0; k0 < kmax; k0 += kLpK) {
-      union { bf16x8_ v; unsigned u[4]; } uh[MT], ul[MT], vh[MT], vl[MT];
+    // the operands of the current step, unpacked from the coefficient words
+    // at the end of the step before (the words themselves live only
+    // within a step: no copies of them cross the loop)
+    union { bf16x8_ v; unsigned u[4]; } uh[MT], ul[MT], vh[MT], vl[MT];
+    auto unpack = [&]() {
 #pragma unroll
       for (int mt = 0; mt < MT; ++mt)
 #pragma unroll
@@ -942,26 +941,36 @@ __global__ __launch_bounds__(64 * kRowsWaves) void k_s12_rows_lp(
           vh[mt].u[t] = (a.y & 0xffffu) | (b.y << 16);
           vl[mt].u[t] = (a.y >> 16) | (b.y & 0xffff0000u);
         }
+    };
+    if (kmax > 0) {
+      load_coef(0);
+#pragma unroll
+      for (int t = 0; t < 4; ++t) load_tab(0, t);
+      unpack();
+    }
+    for (int k0 = 0; k0 < kmax; k0 += kLpK) {
       const bool more = k0 + kLpK < kmax;
       if (more) load_coef(k0 + kLpK);
 #pragma unroll
       for (int t = 0; t < 4; ++t) {
-        union { uint4 q; bf16x8_ v; } ph, pl, mh, ml;
-        ph.q = tn[t][0];
-        pl.q = tn[t][1];
-        mh.q = tn[t][2];
-        ml.q = tn[t][3];
-        if (more) load_tab(k0 + kLpK, t);
+        const bf16x8_ ph = __builtin_bit_cast(bf16x8_, tn[t][0]);
+        const bf16x8_ pl = __builtin_bit_cast(bf16x8_, tn[t][1]);
+        const bf16x8_ mh = __builtin_bit_cast(bf16x8_, tn[t][2]);
+        const bf16x8_ ml = __builtin_bit_cast(bf16x8_, tn[t][3]);
 #pragma unroll
         for (int mt = 0; mt < MT; ++mt) {
-          accp[mt][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(uh[mt].v, ph.v, accp[mt][t], 0, 0, 0);
-          accp[mt][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(uh[mt].v, pl.v, accp[mt][t], 0, 0, 0);
-          accp[mt][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ul[mt].v, ph.v, accp[mt][t], 0, 0, 0);
-          accm[mt][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(vh[mt].v, mh.v, accm[mt][t], 0, 0, 0);
-          accm[mt][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(vh[mt].v, ml.v, accm[mt][t], 0, 0, 0);
-          accm[mt][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(vl[mt].v, mh.v, accm[mt][t], 0, 0, 0);
+          accp[mt][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(uh[mt].v, ph, accp[mt][t], 0, 0, 0);
+          accp[mt][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(uh[mt].v, pl, accp[mt][t], 0, 0, 0);
+          accp[mt][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ul[mt].v, ph, accp[mt][t], 0, 0, 0);
+          accm[mt][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(vh[mt].v, mh, accm[mt][t], 0, 0, 0);
+          accm[mt][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(vh[mt].v, ml, accm[mt][t], 0, 0, 0);
+          accm[mt][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(vl[mt].v, mh, accm[mt][t], 0, 0, 0);
         }
+        // the sub-tile's words for the next step go into the registers its
+        // products have just read (no copies of the buffer)
+        if (more) load_tab(k0 + kLpK, t);
       }
+      if (more) unpack();
     }
   }
   rows_lp_epilogue<MT, RM>(accp, accm, lane, chunk, j0, n_ang, p0, cend, nr, wr, ldw, part);
