@@ -36,8 +36,8 @@ PEAK_FP64_TFLOPS = 78.6  # MI355X FP64 (vector = matrix on gfx950)
 
 
 TRAFFIC_FILES_REL = {
-    "k_mie_coated_stable": ("profiles/k_mie_coated_stable__coated8.4M__pmc_fetch_write.txt",
-                            "traffic_bytes_per_launch:"),
+    "k_mie_coated_stable": ("profiles/k_mie_coated_ckpt__coated8.4M__pmc_fetch_write.txt",
+                            "traffic_bytes_per_launch_k0:"),
     "k_mie_coated_ckpt<1,4>": ("profiles/k_mie_coated_ckpt__coated8.4M__pmc_fetch_write.txt",
                                "traffic_bytes_per_launch_k4:"),
     "k_mie_coated_ckpt<1,8>": ("profiles/k_mie_coated_ckpt__coated8.4M__pmc_fetch_write.txt",
