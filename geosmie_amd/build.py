@@ -23,9 +23,15 @@ NC4_LIB = os.path.join(HERE, "libgeosmie_nc4.so")
 NO_CONTRACT = {"gm_band.hip", "gm_grasp.hip"}
 
 # -packed-fp32-ops: no packed FP32 instructions from the compiler (some of their
-# forms have given wrong results on these GPUs without an error)
+# forms have given wrong results on these GPUs without an error).
+# -amdgpu-mfma-vgpr-form: matrix-instruction accumulators in ordinary VGPRs
+# (gfx950 reads and writes them there): without it the compiler keeps a
+# K loop's accumulators in VGPRs and copies them to and from AGPRs around
+# every step's MFMAs (128 extra instructions per step of the FP64 S1/S2
+# kernel), and AGPR allocations cost occupancy.
 FLAGS = ["-O3", "-std=c++17", "--offload-arch=gfx950", "-fPIC", "-shared",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+         "-mllvm", "-amdgpu-mfma-vgpr-form=1",
          "-ffp-contract=fast", "-Wno-unused-result", "-I" + INCLUDE, "-I" + CSRC]
 
 
