@@ -45,13 +45,18 @@ TRAFFIC_FILES_REL = {
 }
 
 
-def measured_traffic(kernel):
-    """(HBM bytes per launch, source file) of the coated kernel variant from
-    the committed PMC summary (FETCH_SIZE and WRITE_SIZE passes,
-    copy-calibrated), or (None, None)."""
-    if kernel not in TRAFFIC_FILES_REL:
+ISSUED_FLOP_FILES_REL = {
+    "k_mie_coated_ckpt<1,4>": ("profiles/k_mie_coated_ckpt__coated8.4M__pmc_issue.txt",
+                               "issued_fp64_flop_per_launch_k4:"),
+}
+
+
+def profile_figure(table, kernel):
+    """(figure, source file) for the kernel from a committed profile
+    summary's `key: value` line, or (None, None)."""
+    if kernel not in table:
         return None, None
-    rel, key = TRAFFIC_FILES_REL[kernel]
+    rel, key = table[kernel]
     try:
         with open(os.path.join(ROOT, rel)) as f:
             for line in f:
@@ -60,6 +65,13 @@ def measured_traffic(kernel):
     except OSError:
         pass
     return None, None
+
+
+def measured_traffic(kernel):
+    """(HBM bytes per launch, source file) of the coated kernel variant from
+    the committed PMC summary (FETCH_SIZE and WRITE_SIZE passes,
+    copy-calibrated), or (None, None)."""
+    return profile_figure(TRAFFIC_FILES_REL, kernel)
 
 
 def coated_workload():
@@ -679,6 +691,9 @@ def main():
     props = d_props.view(6, n)
     finite = bool(torch.isfinite(props).all().item())
     frac = {k: v / (kern_ms * 1e-3) / 1e12 / PEAK_FP64_TFLOPS for k, v in conv.items()}
+    # the FP64 work the kernel issues, from its committed PMC instruction counts
+    issued, issued_src = profile_figure(ISSUED_FLOP_FILES_REL, kernel_name)
+    frac_issued = issued / (kern_ms * 1e-3) / 1e12 / PEAK_FP64_TFLOPS if issued else None
     achieved = conv["survey"] / (kern_ms * 1e-3) / 1e12
     out = {
         "metric": METRIC,
@@ -711,7 +726,9 @@ def main():
                                         "recurrences, 40 per coefficient pair, 30 per "
                                         "efficiency term)",
                      "frac_other_conventions": {"single_chain_10nmx_70nmax": frac["single_chain"],
-                                                "full_term_30nmx_270nmax": frac["full_term"]},
+                                                "full_term_30nmx_270nmax": frac["full_term"],
+                                                "issued_fp64_pmc": frac_issued,
+                                                "issued_fp64_source": issued_src},
                      "traffic": traffic, "traffic_source": traffic_src,
                      "kernel_ms": kern_ms, "flop_per_launch": conv["survey"],
                      "algorithmic_bytes_per_launch": 104.0 * n},
