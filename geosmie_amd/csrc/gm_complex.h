@@ -56,6 +56,21 @@ __device__ __forceinline__ cd cdiv(cd a, cd b) {
 }
 __device__ __forceinline__ cd cdiv(double a, cd b) { return cdiv(cd{a, 0.0}, b); }
 
+// a / b and c / d sharing one reciprocal: each divisor scaled by its own
+// power of two as in cdiv, then 1 / (|b'|^2 |d'|^2) (the product lies in
+// [1/16, 4)) split into 1/|b'|^2 = |d'|^2 r and 1/|d'|^2 = |b'|^2 r.
+__device__ __forceinline__ void cdiv2(cd a, cd b, cd c, cd d, cd& q1, cd& q2) {
+  const int e1 = __builtin_amdgcn_frexp_exp(fmax(fabs(b.re), fabs(b.im)));
+  const int e2 = __builtin_amdgcn_frexp_exp(fmax(fabs(d.re), fabs(d.im)));
+  const double br = ldexp(b.re, -e1), bi = ldexp(b.im, -e1);
+  const double dr = ldexp(d.re, -e2), di = ldexp(d.im, -e2);
+  const double nb = fma(br, br, bi * bi), nd = fma(dr, dr, di * di);
+  const double r = rcp_nr(nb * nd);
+  const double ib = nd * r, id = nb * r;
+  q1 = cd{ldexp(fma(a.re, br, a.im * bi) * ib, -e1), ldexp(fma(a.im, br, -(a.re * bi)) * ib, -e1)};
+  q2 = cd{ldexp(fma(c.re, dr, c.im * di) * id, -e2), ldexp(fma(c.im, dr, -(c.re * di)) * id, -e2)};
+}
+
 // 1 / t for the log-derivative recurrence.  |t| stays far from the FP64
 // overflow/underflow range there (t = D_n + n/z), so no scaling is needed.
 __device__ __forceinline__ cd crcp_fast(cd t) {
@@ -128,4 +143,19 @@ __device__ __forceinline__ cd ccos_(cd z) {
   double s, c;
   sincos(z.re, &s, &c);
   return cd{c * cosh(z.im), -s * sinh(z.im)};
+}
+
+// sin z and cos z together: one sincos of Re z, and sinh, cosh of Im z from
+// one expm1: with m = e^|t| - 1, sinh |t| = (m + m/(m+1))/2 and
+// cosh t = (m + 1 + 1/(m+1))/2, free of cancellation for small |t|.
+__device__ __forceinline__ void csincos_(cd z, cd& sn, cd& cs) {
+  double s, c;
+  sincos(z.re, &s, &c);
+  const double m = expm1(fabs(z.im));
+  const double ie = 1.0 / (m + 1.0);
+  const double mi = m < 1e300 ? m * ie : 1.0;   // m / (m + 1), also where e^|t| overflows
+  const double sh = copysign(0.5 * (m + mi), z.im);
+  const double ch = 0.5 * ((m + 1.0) + ie);
+  sn = cd{s * ch, c * sh};
+  cs = cd{c * ch, -s * sh};
 }

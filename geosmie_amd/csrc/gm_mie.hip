@@ -891,9 +891,11 @@ __device__ __forceinline__ void coated_up(const CoatedPt& c, __amdgpu_buffer_rsr
   const ddr yinv = rcp_dd(c.y);
   const cd m = c.m, m2 = c.m2;
   const int nmax = c.nmax, nsw_v = c.nsw_v, nsw_w = c.nsw_w, nsw_y = c.nsw_y;
-  cd pv_pp = ccos_(c.v), pv_p = csin_(c.v);   // psi_{-1}(v), psi_0(v)
+  cd pv_p, pv_pp;   // psi_0(v) = sin v, psi_{-1}(v) = cos v
+  csincos_(c.v, pv_p, pv_pp);
   cd cv_pp = -pv_p, cv_p = pv_pp;             // chi_{-1}(v), chi_0(v)
-  cd pw_pp = ccos_(c.w), pw_p = csin_(c.w);
+  cd pw_p, pw_pp;
+  csincos_(c.w, pw_p, pw_pp);
   cd cw_pp = -pw_p, cw_p = pw_pp;
   double sn, cs;
   sincos(c.y, &sn, &cs);
@@ -953,8 +955,8 @@ __device__ __forceinline__ void coated_up(const CoatedPt& c, __amdgpu_buffer_rsr
     const double nrat = mul_dd(dn, yinv);
     const cd xi = cd{py, -chy}, xi1 = cd{py_p, -cy_p};
     const cd Ma = m2 * Dd, Pa = Nd + Ma * nrat, Pb = m2 * Ng + Dg * nrat;
-    const cd a = cdiv(Pa * py - Ma * py_p, Pa * xi - Ma * xi1);
-    const cd b = cdiv(Pb * py - Dg * py_p, Pb * xi - Dg * xi1);
+    cd a, b;
+    cdiv2(Pa * py - Ma * py_p, Pa * xi - Ma * xi1, Pb * py - Dg * py_p, Pb * xi - Dg * xi1, a, b);
     pr.add_seq(n, a, b);
     if constexpr (STORE) {
       an[2 * (off + n - 1)] = a.re;
@@ -1105,9 +1107,11 @@ __device__ __forceinline__ Props ckpt_pass2(const CkptPt& c, double (*rows)[7][6
                                             __amdgpu_buffer_rsrc_t rs, uint32_t b_ddv,
                                             uint32_t b_ddw, int lane, int64_t off,
                                             double* __restrict__ an, double* __restrict__ bn) {
-  cd pv_pp = ccos_(c.v), pv_p = csin_(c.v);   // psi_{-1}(v), psi_0(v)
+  cd pv_p, pv_pp;   // psi_0(v) = sin v, psi_{-1}(v) = cos v
+  csincos_(c.v, pv_p, pv_pp);
   cd cv_pp = -pv_p, cv_p = pv_pp;             // chi_{-1}(v), chi_0(v)
-  cd pw_pp = ccos_(c.w), pw_p = csin_(c.w);
+  cd pw_p, pw_pp;
+  csincos_(c.w, pw_p, pw_pp);
   cd cw_pp = -pw_p, cw_p = pw_pp;
   double sn, cs;
   sincos(c.y, &sn, &cs);
@@ -1206,8 +1210,9 @@ __device__ __forceinline__ Props ckpt_pass2(const CkptPt& c, double (*rows)[7][6
       const double nrat = mul_dd(dn, c.yi);
       const cd xi = cd{py, -chy}, xi1 = cd{py_p, -cy_p};
       const cd Ma = m2 * Dd, Pa = Nd + Ma * nrat, Pb = m2 * Ng + Dg * nrat;
-      const cd a = cdiv(Pa * py - Ma * py_p, Pa * xi - Ma * xi1);
-      const cd bb = cdiv(Pb * py - Dg * py_p, Pb * xi - Dg * xi1);
+      cd a, bb;
+      cdiv2(Pa * py - Ma * py_p, Pa * xi - Ma * xi1, Pb * py - Dg * py_p, Pb * xi - Dg * xi1, a,
+            bb);
       pr.add_seq(n, a, bb);
       if constexpr (STORE) {
         an[2 * (off + n - 1)] = a.re;
