@@ -1086,13 +1086,14 @@ __global__ __launch_bounds__(64, MINW) void k_mie_coated_stable(
 // ---------------------------------------------------------------------------
 // Coated sphere, real or weakly absorbing shell, checkpointed downward chains
 // ---------------------------------------------------------------------------
-// Pass 2 of k_mie_coated_ckpt for one point: per block, reload the
-// checkpoint (the next one in flight), regenerate the block's K rows into LDS
-// with exactly pass 1's operations, then the K upward steps.  The upward
-// steps are branch-free as in coated_up (both psi forms formed, one
-// selected; the LDS rows of an inactive chain are read and ignored), the
-// start-index corrections come through range-checked loads and are compiled
-// in only when a lane of the wavefront needs them (DD); STORE writes a_n, b_n.
+// Pass 2 of k_mie_coated_ckpt for one point: per block, its K rows in LDS
+// (block 0 as pass 1 left them; block b > 0 regenerated from checkpoint b,
+// the next one in flight, with exactly pass 1's operations), then the K
+// upward steps, unrolled by two.  The upward steps are branch-free as in
+// coated_up (both psi forms formed, one selected; the LDS rows of an
+// inactive chain are read and ignored), the start-index corrections come
+// through range-checked loads and are compiled in only when a lane of the
+// wavefront needs them (DD); STORE writes a_n, b_n.
 struct CkptPt {
   cd v, w, m, m2;
   double y;
