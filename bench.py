@@ -376,7 +376,8 @@ def lut_surrogate(n_queries=1_000_000, n_sample=4096, seed=20261202, full=True):
            "unit": "bulk LUT entries/s"}
     sw = LutSweep(g["r"], g["rmode"], g["sigma"], precision="fp64")
     sub = (g["mr"][::13], g["mi"][::10], g["lam"], g["gf"][::5])
-    sw.sweep(*[a[:1] for a in sub])                # warm: tables, workspaces
+    sw.sweep(*[a[:1] for a in sub])                # tables, workspaces
+    warm(lambda: sw.sweep(*sub), 1)                # and the clocks
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     tab = sw.sweep(*sub)
@@ -430,15 +431,23 @@ def lut_surrogate(n_queries=1_000_000, n_sample=4096, seed=20261202, full=True):
     t_net = time.perf_counter() - t0
     f = flags.cpu().numpy()
     n_fb = int(np.sum(f != 0))
-    # plain GPU Mie on a random sample: per-query rate, fallback cost and the error judge
+    # plain GPU Mie on a random sample: per-query rate (the fallback's own
+    # precision, FP32 S1/S2, and FP64 S1/S2), fallback cost, and the error
+    # judge (FP64)
     pick = r.choice(n, size=min(n_sample, n), replace=False)
-    sur.solve_plain(q[pick[:64]])
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    plain = sur.solve_plain(q[pick])
-    torch.cuda.synchronize()
-    t_plain = time.perf_counter() - t0
-    per_query = t_plain / pick.size
+    per_q = {}
+    plain = None
+    for prec in (sur.fallback_precision, "fp64"):
+        sur.solve_plain(q[pick[:64]], precision=prec)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        res = sur.solve_plain(q[pick], precision=prec)
+        torch.cuda.synchronize()
+        per_q[prec] = (time.perf_counter() - t0) / pick.size
+        if prec == "fp64":
+            plain = res
+    t_plain = per_q[sur.fallback_precision] * pick.size
+    per_query = per_q[sur.fallback_precision]
     served = f[pick] == 0
     net = rows[torch.from_numpy(pick).to(rows.device)].cpu().numpy()
     err = rel_errors(net, plain.cpu().numpy())
@@ -463,6 +472,8 @@ def lut_surrogate(n_queries=1_000_000, n_sample=4096, seed=20261202, full=True):
         "queries": n, "served_by_network": n - n_fb, "fallback": n_fb,
         "network_s": t_net, "network_entries_per_s": n / t_net,
         "plain_per_query_entries_per_s": 1.0 / per_query,
+        "plain_per_query_precision": sur.fallback_precision + " S1/S2 (the fallback's)",
+        "plain_per_query_fp64_entries_per_s": 1.0 / per_q["fp64"],
         "value": n / t_all,
         "value_note": "measured end to end on all %d queries: network, gate and the plain-Mie "
                       "re-solve of the %d flagged queries" % (n, nf_all),
@@ -474,6 +485,7 @@ def lut_surrogate(n_queries=1_000_000, n_sample=4096, seed=20261202, full=True):
                                 % pick.size},
         "sample_end_to_end": {"queries": int(pick.size), "fallback": int(nf_s), "s": t_e2e,
                               "plain_s": t_plain, "speedup_vs_plain": t_plain / t_e2e},
+        "served_err_judge": "plain GPU Mie, FP64 S1/S2",
         "served_err_vs_plain": None if not served.any() else {
             "qext_max": float(err[served, 0].max()), "ssa_max": float(err[served, 1].max()),
             "g_max": float(err[served, 2].max()),
