@@ -341,8 +341,11 @@ struct RowGroups {
 };
 
 // RM = most rows of one group this instance accumulates (registers: 16 RM
-// doubles per lane for the sums).
-template <int RM>
+// doubles per lane for the sums).  MT = 8-point tiles multiplied together:
+// each angle-table word a lane loads feeds the products of all MT tiles (the
+// table loads, not the matrix cores, bound this kernel at MT = 1); a chunk
+// of chunk_pts points runs as consecutive groups of MT tiles.
+template <int RM, int MT>
 __global__ __launch_bounds__(64 * kRowsWaves) void k_s12_rows_f64(
     RowGroups g, int chunk_pts, const int64_t* __restrict__ off, const int32_t* __restrict__ nmax,
     const double* __restrict__ an, const double* __restrict__ bn, int n_ang,
@@ -368,29 +371,39 @@ __global__ __launch_bounds__(64 * kRowsWaves) void k_s12_rows_f64(
     for (int t = 0; t < 4; ++t)
 #pragma unroll
       for (int e = 0; e < 4; ++e) sum[r][t][e] = 0.0;
-  for (int64_t p0 = cstart; p0 < cend; p0 += 8) {
-    const int64_t pa = p0 + 2 * (i & 3) + (i >> 3);
-    const bool ok = pa < cend;
-    const int nm_a = ok ? nmax[pa] : 0;
-    const int64_t oa = ok ? off[pa] : 0;
+  for (int64_t p0 = cstart; p0 < cend; p0 += 8 * MT) {
+    int nm_a[MT];
+    int64_t oa[MT];
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt) {
+      const int64_t pa = p0 + 8 * mt + 2 * (i & 3) + (i >> 3);
+      const bool ok = pa < cend;
+      nm_a[mt] = ok ? nmax[pa] : 0;
+      oa[mt] = ok ? off[pa] : 0;
+    }
     int kmax = 0;
-    for (int q = 0; q < 8; ++q)
+    for (int q = 0; q < 8 * MT; ++q)
       if (p0 + q < cend) kmax = max(kmax, nmax[p0 + q]);
     kmax = min(kmax, nmax_cap);
-    double4_ accp[4], accm[4];
+    double4_ accp[MT][4], accm[MT][4];
 #pragma unroll
-    for (int t = 0; t < 4; ++t) {
-      accp[t] = (double4_){0.0, 0.0, 0.0, 0.0};
-      accm[t] = (double4_){0.0, 0.0, 0.0, 0.0};
-    }
-    double ar_n = 0.0, br_n = 0.0, pp_n[4], tt_n[4];
+    for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        accp[mt][t] = (double4_){0.0, 0.0, 0.0, 0.0};
+        accm[mt][t] = (double4_){0.0, 0.0, 0.0, 0.0};
+      }
+    double ar_n[MT], br_n[MT], pp_n[4], tt_n[4];
     auto load_step = [&](int k0) {
       const int kk = k0 + kq;
-      ar_n = 0.0;
-      br_n = 0.0;
-      if (kk < nm_a) {
-        ar_n = an[2 * (oa + kk) + comp];
-        br_n = bn[2 * (oa + kk) + comp];
+#pragma unroll
+      for (int mt = 0; mt < MT; ++mt) {
+        ar_n[mt] = 0.0;
+        br_n[mt] = 0.0;
+        if (kk < nm_a[mt]) {
+          ar_n[mt] = an[2 * (oa[mt] + kk) + comp];
+          br_n[mt] = bn[2 * (oa[mt] + kk) + comp];
+        }
       }
       const bool krow = kk < kmax;
 #pragma unroll
@@ -406,8 +419,12 @@ __global__ __launch_bounds__(64 * kRowsWaves) void k_s12_rows_f64(
     };
     if (kmax > 0) load_step(0);
     for (int k0 = 0; k0 < kmax; k0 += 4) {
-      const double ua = ar_n + br_n, va = ar_n - br_n;
-      double P[4], M[4];
+      double ua[MT], va[MT], P[4], M[4];
+#pragma unroll
+      for (int mt = 0; mt < MT; ++mt) {
+        ua[mt] = ar_n[mt] + br_n[mt];
+        va[mt] = ar_n[mt] - br_n[mt];
+      }
 #pragma unroll
       for (int t = 0; t < 4; ++t) {
         P[t] = pp_n[t] + tt_n[t];
@@ -415,32 +432,38 @@ __global__ __launch_bounds__(64 * kRowsWaves) void k_s12_rows_f64(
       }
       if (k0 + 4 < kmax) load_step(k0 + 4);
 #pragma unroll
-      for (int t = 0; t < 4; ++t) {
-        accp[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(ua, P[t], accp[t], 0, 0, 0);
-        accm[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(va, M[t], accm[t], 0, 0, 0);
-      }
+      for (int t = 0; t < 4; ++t)
+#pragma unroll
+        for (int mt = 0; mt < MT; ++mt) {
+          accp[mt][t] = __builtin_amdgcn_mfma_f64_16x16x4f64(ua[mt], P[t], accp[mt][t], 0, 0, 0);
+          accm[mt][t] = __builtin_amdgcn_mfma_f64_16x16x4f64(va[mt], M[t], accm[mt][t], 0, 0, 0);
+        }
     }
-    // lane (i, kq) holds angle j0 + 16 t + i of points p0 + 2 kq + h (h = 0, 1)
+    // lane (i, kq) holds angle j0 + 16 t + i of points p0 + 8 mt + 2 kq + h
+    // (h = 0, 1)
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      const int64_t p = p0 + 2 * kq + h;
-      double wv[RM];
+    for (int mt = 0; mt < MT; ++mt)
 #pragma unroll
-      for (int r = 0; r < RM; ++r) wv[r] = (r < nr && p < cend) ? wr[(size_t)r * ldw + p] : 0.0;
-#pragma unroll
-      for (int t = 0; t < 4; ++t) {
-        const cd sp = cd{accp[t][2 * h], accp[t][2 * h + 1]};
-        const cd sm = cd{accm[t][2 * h], accm[t][2 * h + 1]};
-        const cd s1 = (sp + sm) * 0.5, s2 = (sp - sm) * 0.5;
-        const double a1 = norm2(s1), a2 = norm2(s2);
-        const cd c = s1 * conj(s2);
-        const double el[4] = {0.5 * (a1 + a2), 0.5 * (a2 - a1), c.re, c.im};
+      for (int h = 0; h < 2; ++h) {
+        const int64_t p = p0 + 8 * mt + 2 * kq + h;
+        double wv[RM];
 #pragma unroll
         for (int r = 0; r < RM; ++r)
+          wv[r] = (r < nr && p < cend) ? wr[(size_t)r * ldw + p] : 0.0;
 #pragma unroll
-          for (int e = 0; e < 4; ++e) sum[r][t][e] = fma(wv[r], el[e], sum[r][t][e]);
+        for (int t = 0; t < 4; ++t) {
+          const cd sp = cd{accp[mt][t][2 * h], accp[mt][t][2 * h + 1]};
+          const cd sm = cd{accm[mt][t][2 * h], accm[mt][t][2 * h + 1]};
+          const cd s1 = (sp + sm) * 0.5, s2 = (sp - sm) * 0.5;
+          const double a1 = norm2(s1), a2 = norm2(s2);
+          const cd c = s1 * conj(s2);
+          const double el[4] = {0.5 * (a1 + a2), 0.5 * (a2 - a1), c.re, c.im};
+#pragma unroll
+          for (int r = 0; r < RM; ++r)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) sum[r][t][e] = fma(wv[r], el[e], sum[r][t][e]);
+        }
       }
-    }
   }
   // add the four point pairs (lanes i + 16 kq) and write; lane group kq
   // writes element kq
@@ -1286,18 +1309,26 @@ int gm_s12_contract_rows(gm_ctx* ctx, int64_t n, const int64_t* coef_off, const 
   for (int r = 0; r < n_rows; ++r)
     if (count_host[r] > 0) tiles += (count_host[r] + 7) / 8;
   const int64_t ablocks = (n_ang + 64 * gm::kRowsWaves - 1) / (64 * gm::kRowsWaves);
-  // tiles per chunk: FP64 walks them one after the other in one wavefront
-  // (two for short series when that still leaves >= 16k wavefronts: fewer
-  // work units and half the partial sums where the K loop is short; long
-  // series run faster one tile per wavefront); the FP32 / split-BF16 kernel
-  // multiplies them together, sharing each table fetch (two when that leaves
-  // >= 8k wavefronts)
+  // most rows of one group (consecutive rows with the same range, at most 4)
+  int rm_pre = 1;
+  for (int r = 1, run = 1; r < n_rows; ++r) {
+    run = (start_host[r] == start_host[r - 1] && count_host[r] == count_host[r - 1] && run < 4)
+              ? run + 1 : 1;
+    rm_pre = max(rm_pre, run);
+  }
+  // tiles per chunk.  The kernels multiply two tiles together, sharing each
+  // angle-table fetch (FP64 with one row sum per group; otherwise FP64 walks
+  // the tiles one after the other, which pays for short series only: fewer
+  // work units and half the partial sums where the K loop is short).  Two
+  // when that still leaves >= 16k wavefronts at FP64, >= 8k at FP32 and
+  // split-BF16.
   const int64_t waves2 = (tiles / 2) * ablocks * gm::kRowsWaves;
   const int row_tiles =
       ctx->tune_s12_row_tiles
           ? ctx->tune_s12_row_tiles
-          : (precision == GM_PREC_FP64 ? (nmax_cap <= 256 && waves2 >= 16384 ? 2 : 1)
-                                       : (waves2 >= 8192 ? 2 : 1));
+          : (precision == GM_PREC_FP64
+                 ? ((rm_pre == 1 || nmax_cap <= 256) && waves2 >= 16384 ? 2 : 1)
+                 : (waves2 >= 8192 ? 2 : 1));
   const int chunk_pts = 8 * row_tiles;
   gm::RowGroups g;
   g.n_groups = 0;
@@ -1340,15 +1371,20 @@ int gm_s12_contract_rows(gm_ctx* ctx, int64_t n, const int64_t* coef_off, const 
   size_t slot;
   if (precision == GM_PREC_FP64) {
     ctx->mark_begin(gm::kKS12, s, &slot);
-    if (RM == 1)
-      hipLaunchKernelGGL(gm::k_s12_rows_f64<1>, grid, block, 0, s, g, chunk_pts, coef_off, nmax,
-                         an, bn, n_ang, pi_t, tau_t, nmax_cap, w, n, part);
-    else if (RM == 2)
-      hipLaunchKernelGGL(gm::k_s12_rows_f64<2>, grid, block, 0, s, g, chunk_pts, coef_off, nmax,
-                         an, bn, n_ang, pi_t, tau_t, nmax_cap, w, n, part);
-    else
-      hipLaunchKernelGGL(gm::k_s12_rows_f64<4>, grid, block, 0, s, g, chunk_pts, coef_off, nmax,
-                         an, bn, n_ang, pi_t, tau_t, nmax_cap, w, n, part);
+    // two tiles multiplied together where a chunk has two and one row sum per
+    // group (one table load feeds both tiles; with more row sums two tiles
+    // leave one wavefront per SIMD)
+#define GM_ROWS_F64(R_, M_)                                                                    \
+  hipLaunchKernelGGL((gm::k_s12_rows_f64<R_, M_>), grid, block, 0, s, g, chunk_pts, coef_off, \
+                     nmax, an, bn, n_ang, pi_t, tau_t, nmax_cap, w, n, part)
+    if (RM == 1) {
+      if (row_tiles >= 2) GM_ROWS_F64(1, 2); else GM_ROWS_F64(1, 1);
+    } else if (RM == 2) {
+      GM_ROWS_F64(2, 1);
+    } else {
+      GM_ROWS_F64(4, 1);
+    }
+#undef GM_ROWS_F64
     ctx->mark_end(slot, s);
   } else {
     // P, M tables of this precision (padded to whole K-steps and 256 angles)
