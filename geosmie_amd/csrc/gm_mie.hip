@@ -1122,19 +1122,32 @@ __device__ __forceinline__ Props ckpt_pass2(const CkptPt& c, double (*rows)[7][6
   pr.init_seq();
   // block 0's rows are in LDS already (pass 1 leaves them there); block b > 0
   // is regenerated from checkpoint b, the next checkpoint in flight meanwhile
-  double ck[7], ckn[7];
-  if (c.nblk > 1) {
-#pragma unroll
-    for (int q = 0; q < 7; ++q) ckn[q] = CK[(7 + q) * 64 + lane];
-  }
+  // checkpoint b holds a ratio chain's state only where block b uses it
+  // (its top row bK + K above the chain's nsw); the other words are left
+  // unloaded (keeping the earlier block's), and the regeneration never uses
+  // them
+  double ck[7], ckn[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  auto load_ck = [&](int b) {
+    const double* src = CK + (size_t)b * 7 * 64 + lane;
+    const int top = b * K + K;
+    ckn[0] = src[0];
+    ckn[1] = src[64];
+    if (top > c.nsw_v) {
+      ckn[2] = src[128];
+      ckn[3] = src[192];
+    }
+    if (top > c.nsw_w) {
+      ckn[4] = src[256];
+      ckn[5] = src[320];
+    }
+    if (top > c.nsw_y) ckn[6] = src[384];
+  };
+  if (c.nblk > 1) load_ck(1);
   for (int b = 0; b < c.nblk; ++b) {
     if (b > 0) {
 #pragma unroll
       for (int q = 0; q < 7; ++q) ck[q] = ckn[q];
-      if (b + 1 < c.nblk) {
-#pragma unroll
-        for (int q = 0; q < 7; ++q) ckn[q] = CK[((size_t)(b + 1) * 7 + q) * 64 + lane];
-      }
+      if (b + 1 < c.nblk) load_ck(b + 1);
       cd du = cd{ck[0], ck[1]}, dv = cd{ck[2], ck[3]}, dw = cd{ck[4], ck[5]};
       double dy = ck[6];
 #pragma unroll
@@ -1315,9 +1328,20 @@ __global__ __launch_bounds__(64, MINW) void k_mie_coated_ckpt(
         }
         const int b = (k - 1) / K - 1;      // state after row bK+K+1
         if ((k - 1) % K == 0 && b >= 1 && b < nblk) {
+          // a ratio chain's state only where block b has a row that uses it
+          // (a row above the chain's nsw); pass 2 reads it only there
           double* c = CK + (size_t)b * 7 * 64 + lane;
-          c[0] = du.re; c[64] = du.im; c[128] = dv.re; c[192] = dv.im;
-          c[256] = dw.re; c[320] = dw.im; c[384] = dy;
+          c[0] = du.re;
+          c[64] = du.im;
+          if (k - 1 > nsw_v) {
+            c[128] = dv.re;
+            c[192] = dv.im;
+          }
+          if (k - 1 > nsw_w) {
+            c[256] = dw.re;
+            c[320] = dw.im;
+          }
+          if (k - 1 > nsw_y) c[384] = dy;
         }
       }
       // rows K..1 (pass 2's first block) go to LDS as pass 2 would regenerate
