@@ -1094,14 +1094,21 @@ __global__ __launch_bounds__(64, MINW) void k_mie_coated_stable(
 // inactive chain are read and ignored), the start-index corrections come
 // through range-checked loads and are compiled in only when a lane of the
 // wavefront needs them (DD); STORE writes a_n, b_n.
+// nsw_v, nsw_w, nsw_y: the last row of the upward psi form; a chain with no
+// ratio rows (|z| >= n_max) carries kNoRatio there, so "row k uses the
+// ratio chain" is k > nsw alone.  Pass 2 never reaches a row above
+// nblk K <= n_max + K - 1, which lies below nmx - 1 (nmx >= n_max + 16) and
+// below every active chain's start (>= n_max + 18): the regeneration needs
+// neither bound.
 struct CkptPt {
   cd v, w, m, m2;
   double y;
   cdd ui, vi, wi, minv;
   ddr yi;
-  int nmax, nmx, sv, sw, sy, nsw_v, nsw_w, nsw_y, nblk;
+  int nmax, nsw_v, nsw_w, nsw_y, nblk;
   bool ddv_on, ddw_on;
 };
+constexpr int kNoRatio = 1 << 30;
 template <int K, bool DD, bool STORE>
 __device__ __forceinline__ Props ckpt_pass2(const CkptPt& c, double (*rows)[7][64],
                                             const double* __restrict__ CK,
@@ -1153,27 +1160,27 @@ __device__ __forceinline__ Props ckpt_pass2(const CkptPt& c, double (*rows)[7][6
 #pragma unroll
       for (int kk = K - 1; kk >= 0; --kk) {
         const int k = b * K + kk + 1;
-        if (k <= c.nmx - 1) {
+        {
           const cd r = mul_dd((double)(k + 1), c.ui);
           du = r - crcp_fast(du + r);
         }
         rows[kk][0][lane] = du.re;
         rows[kk][1][lane] = du.im;
-        if (k <= c.sv && k > c.nsw_v) {
+        if (k > c.nsw_v) {
           const cd r = mul_dd((double)k, c.vi);
           const cd it = crcp_fast(dv + r);
           rows[kk][2][lane] = it.re;
           rows[kk][3][lane] = it.im;
           dv = r - it;
         }
-        if (k <= c.sw && k > c.nsw_w) {
+        if (k > c.nsw_w) {
           const cd r = mul_dd((double)k, c.wi);
           const cd it = crcp_fast(dw + r);
           rows[kk][4][lane] = it.re;
           rows[kk][5][lane] = it.im;
           dw = r - it;
         }
-        if (k <= c.sy && k > c.nsw_y) {
+        if (k > c.nsw_y) {
           const double r = mul_dd((double)k, c.yi);
           const double it = rcp_nr(dy + r);
           rows[kk][6][lane] = it;
@@ -1265,6 +1272,9 @@ __global__ __launch_bounds__(64, MINW) void k_mie_coated_ckpt(
     const int64_t* __restrict__ coef_off, double* __restrict__ an, double* __restrict__ bn,
     double* __restrict__ props, int32_t* __restrict__ status, double* __restrict__ scratch,
     int rows_cap, int dn_extra) {
+  // rows above n_max in the last block stay below nmx - 1 and every ratio
+  // chain's start (CkptPt)
+  static_assert(K >= 1 && K <= 12, "checkpoint interval");
   __shared__ double rows[K][7][64];
   const int lane = threadIdx.x;
   const int64_t begin = range[2], end = range[3];
@@ -1297,10 +1307,14 @@ __global__ __launch_bounds__(64, MINW) void k_mie_coated_ckpt(
     const int nmx =
         nmx_rule(nmax, mxv) + (dn_extra < 0 ? 16 + 10 * (int)ceil(cbrt(mxv)) : dn_extra);
     const double av = cabs_(v), aw = cabs_(w);
-    const int nsw_v = min((int)av, nmax), nsw_w = min((int)aw, nmax), nsw_y = min((int)yy, nmax);
+    // ratio rows (nsw, s] of each chain; kNoRatio where |z| >= n_max (no ratio
+    // rows: the upward form serves every n <= n_max)
+    const int nsw_v = (int)av < nmax ? (int)av : kNoRatio;
+    const int nsw_w = (int)aw < nmax ? (int)aw : kNoRatio;
+    const int nsw_y = (int)yy < nmax ? (int)yy : kNoRatio;
     const cdd ui = crcp_dd(u), vi = crcp_dd(v), wi = crcp_dd(w);
     const ddr yi = rcp_dd(yy);
-    const int sv = nsw_v < nmax ? ratio_start(nmax, av) : 0;   // ratio rows (nsw, s]
+    const int sv = nsw_v < nmax ? ratio_start(nmax, av) : 0;
     const int sw = nsw_w < nmax ? ratio_start(nmax, aw) : 0;
     const int sy = nsw_y < nmax ? ratio_start(nmax, yy) : 0;
     const int nblk = (nmax + K - 1) / K;
@@ -1347,27 +1361,27 @@ __global__ __launch_bounds__(64, MINW) void k_mie_coated_ckpt(
       // rows K..1 (pass 2's first block) go to LDS as pass 2 would regenerate
       // them, so block 0 needs no checkpoint and no regeneration
       for (int k = min(top, K); k >= 1; --k) {
-        if (k <= nmx - 1) {
+        {   // k <= K < nmx - 1, and k <= s for every active chain
           const cd r = mul_dd((double)(k + 1), ui);
           du = r - crcp_fast(du + r);
         }
         rows[k - 1][0][lane] = du.re;
         rows[k - 1][1][lane] = du.im;
-        if (k <= sv && k > nsw_v) {
+        if (k > nsw_v) {
           const cd r = mul_dd((double)k, vi);
           const cd it = crcp_fast(dv + r);
           rows[k - 1][2][lane] = it.re;
           rows[k - 1][3][lane] = it.im;
           dv = r - it;
         }
-        if (k <= sw && k > nsw_w) {
+        if (k > nsw_w) {
           const cd r = mul_dd((double)k, wi);
           const cd it = crcp_fast(dw + r);
           rows[k - 1][4][lane] = it.re;
           rows[k - 1][5][lane] = it.im;
           dw = r - it;
         }
-        if (k <= sy && k > nsw_y) {
+        if (k > nsw_y) {
           const double r = mul_dd((double)k, yi);
           const double it = rcp_nr(dy + r);
           rows[k - 1][6][lane] = it;
@@ -1386,9 +1400,8 @@ __global__ __launch_bounds__(64, MINW) void k_mie_coated_ckpt(
     }
     // ---- pass 2: per block, its K rows in LDS (block 0 left by pass 1, the
     // others regenerated), then K upward steps
-    const CkptPt cp{v,   w,     m,     m2,    yy,    ui,   vi,    wi,    crcp_dd(m),
-                    yi, nmax, nmx, sv, sw, sy, nsw_v, nsw_w, nsw_y,
-                    nblk, ddv_on, ddw_on};
+    const CkptPt cp{v, w, m, m2, yy, ui, vi, wi, crcp_dd(m), yi,
+                    nmax, nsw_v, nsw_w, nsw_y, nblk, ddv_on, ddw_on};
     const int64_t off = coef_off ? coef_off[p] : 0;
     const uint32_t b_ddv = 16u * (uint32_t)lane, b_ddw = 16u * ((uint32_t)rows_cap * 64u + lane);
     Props pr;
