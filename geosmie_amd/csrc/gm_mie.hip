@@ -1323,7 +1323,16 @@ __global__ __launch_bounds__(64, MINW) void k_mie_coated_ckpt(
       const int top = max(max(nmx - 1, sv), max(sw, sy));
       cd du = cd{0.0, 0.0}, dv = cd{0.0, 0.0}, dw = cd{0.0, 0.0};
       double dy = 0.0;
-      for (int k = top; k > K; --k) {
+      // rows above every ratio chain's start and above the highest
+      // checkpoint (row nblk K + 1) advance D_n(u) alone; such rows exist
+      // only when top = nmx - 1
+      const int lo = max(max(sv, sw), max(sy, nblk * K + 1));
+      int k = top;
+      for (; k > lo; --k) {
+        const cd r = mul_dd((double)(k + 1), ui);
+        du = r - crcp_fast(du + r);
+      }
+      for (; k > K; --k) {
         if (k <= nmx - 1) {   // D_k(u) = (k+1)/u - 1/(D_{k+1} + (k+1)/u), D_nmx = 0
           const cd r = mul_dd((double)(k + 1), ui);
           du = r - crcp_fast(du + r);
