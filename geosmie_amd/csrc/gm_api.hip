@@ -150,6 +150,13 @@ int mie_lanes(const gm_ctx* ctx, int64_t n, int rows_cap) {
   return rows_cap <= 300 ? 16 : rows_cap <= 1200 ? 32 : 64;
 }
 
+// The lane-per-point homogeneous kernel in its checkpointed form: with the
+// coated kernel's checkpoint knob at 4 or 8 (both use K = 4 here: the
+// homogeneous rows are 3 doubles, and its step's registers leave room for
+// more resident wavefronts than 8-row blocks of LDS would) and no forced
+// register budget (that variant is built on the full-row kernel).
+bool homog_ckpt(const gm_ctx* ctx) { return ctx->tune_mie_ckpt != 0 && ctx->tune_mie_waves == 0; }
+
 MieGrids mie_grids(const gm_ctx* ctx, int64_t n, int rows_cap, size_t budget) {
   const int64_t waves = (n + 63) / 64;
   auto grid_for = [&](int occ, int row_doubles) {
@@ -162,7 +169,14 @@ MieGrids mie_grids(const gm_ctx* ctx, int64_t n, int rows_cap, size_t budget) {
   const int var = ctx->tune_mie_waves == 4 ? 2 : ctx->tune_mie_waves == 3 ? 1 : 0;
   MieGrids g;
   g.lanes = mie_lanes(ctx, n, rows_cap);
-  if (g.lanes == 1) {
+  if (g.lanes == 1 && homog_ckpt(ctx)) {
+    // a checkpointed slot: 3 doubles per lane per K rows
+    const size_t slot_bytes = (size_t)64 * gm::homog_ckpt_lane_doubles(rows_cap, 4) * sizeof(double);
+    int64_t gg = std::min<int64_t>(waves, (int64_t)ctx->n_cu * std::max(ctx->occ_homog_ckpt, 1) *
+                                              std::max(ctx->tune_mie_grid_mult, 1));
+    gg = std::min<int64_t>(gg, std::max<int64_t>(1, (int64_t)(budget / slot_bytes)));
+    g.homog = std::make_pair(gg, (size_t)gg * slot_bytes);
+  } else if (g.lanes == 1) {
     g.homog = grid_for(ctx->occ_homog[var], gm::kHomogRowDoubles);
   } else {
     g.lcap = (rows_cap + g.lanes - 1) / g.lanes;
@@ -257,6 +271,7 @@ int mie_batch_run(gm_ctx* ctx, int64_t n, const double* x, const double* y, cons
   size_t slot;
   ctx->mark_begin(gm::kKMieHomog, s, &slot);
   auto homog = var == 2 ? gm::k_mie_homog<4> : gm::k_mie_homog<1>;
+  if (homog_ckpt(ctx)) homog = gm::k_mie_homog_ckpt<1, 4>;
   int cap_arg = rows_cap;
   const int64_t* hrange = range;
   if (g.lanes > 1) {
@@ -329,6 +344,9 @@ int gm_ctx_create(int device, gm_ctx** out) {
     c->occ_homog[0] = c->occ_homog[1] = occ;   // the unconstrained homog kernel allows 3 already
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, gm::k_mie_homog<4>, 64, 0) == hipSuccess)
     c->occ_homog[2] = occ;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, gm::k_mie_homog_ckpt<1, 4>, 64, 0) ==
+      hipSuccess)
+    c->occ_homog_ckpt = occ;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, gm::k_mie_coated_stable<1>, 64, 0) ==
       hipSuccess)
     c->occ_stable[0] = occ;

@@ -34,6 +34,12 @@ __global__ void k_mie_homog(const int64_t* order, const int64_t* range, int64_t 
                             const double* eps2, const int64_t* coef_off, double* an, double* bn,
                             double* props, int32_t* status, double* scratch, int rows_cap,
                             int dn_extra);
+template <int MINW, int K>
+__global__ void k_mie_homog_ckpt(const int64_t* order, const int64_t* range, int64_t n_all,
+                                 const double* x, const double* y, const double* eps,
+                                 const double* mu, const double* eps2, const int64_t* coef_off,
+                                 double* an, double* bn, double* props, int32_t* status,
+                                 double* scratch, int rows_cap, int dn_extra);
 template <int MINW>
 __global__ void k_mie_coated_stable(const int64_t* order, const int64_t* range, int64_t n_all,
                                     const double* x, const double* y, const double* eps,
@@ -66,6 +72,10 @@ constexpr int kCoatedRowDoubles = 11;
 // two complex start-correction planes.
 inline int64_t coated_ckpt_lane_doubles(int rows_cap, int k) {
   return 7 * (int64_t)((rows_cap + k - 1) / k) + 4 * (int64_t)rows_cap;
+}
+// Doubles per lane of a checkpointed homogeneous slot: 3 per K-row block.
+inline int64_t homog_ckpt_lane_doubles(int rows_cap, int k) {
+  return 3 * (int64_t)((rows_cap + k - 1) / k);
 }
 
 }  // namespace gm

@@ -40,6 +40,9 @@ __device__ __forceinline__ int nmax_rule(double s) {
 __device__ __forceinline__ int nmx_rule(int nmax, double zabs) {
   return (int)rint(fmax((double)nmax, zabs) + 16.0);
 }
+// Switch row of a psi chain with no ratio rows (|z| >= n_max): above every
+// row, so "row k uses the ratio chain" is k > nsw alone.
+constexpr int kNoRatio = 1 << 30;
 // Start of the downward psi-ratio chains: rows n <= nmax agree to < 1e-15
 // with any later start from nmax + 8 + 4 ceil(|z|^(1/3)) on (checked over
 // |z| = 0.01..2000 and the refractive indices of the workloads); the margin
@@ -444,6 +447,186 @@ __global__ __launch_bounds__(64, MINW) void k_mie_homog(
       psi_p = psi;
       chi_pp = chi_p;
       chi_p = chi;
+    }
+    if (props) pr.finish(s, p, n_all, props);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Homogeneous sphere, checkpointed downward chains
+// ---------------------------------------------------------------------------
+// The arithmetic of k_mie_homog with the memory schedule of
+// k_mie_coated_ckpt: pass 1 runs D_n(mx) and the psi ratio chain but stores
+// only their states every K rows (checkpoint b: the state after row bK+K+1;
+// the ratio state only where block b has rows above the switch row), and
+// rows K..1 straight to LDS; pass 2 walks the blocks upward, regenerating
+// block b > 0 into LDS from checkpoint b (the next one in flight) with
+// exactly pass 1's operations, so every row, and every result, is
+// bit-identical to k_mie_homog's.  A lane-per-point step has little
+// arithmetic to hide a scratch row's load behind (~115 FP64 instructions per
+// row against ~24 B read and written per row and lane), so the full-row form
+// is bound by its vector-memory instructions; here per-row traffic is one
+// checkpoint per K rows and the upward steps read LDS.
+// Slot layout: checkpoints [b][3][lane] (D.re, D.im, ratio state).
+template <int MINW, int K>
+__global__ __launch_bounds__(64, MINW) void k_mie_homog_ckpt(
+    const int64_t* __restrict__ order, const int64_t* __restrict__ range, int64_t n_all,
+    const double* __restrict__ x, const double* __restrict__ y, const double* __restrict__ eps,
+    const double* __restrict__ mu, const double* __restrict__ eps2,
+    const int64_t* __restrict__ coef_off, double* __restrict__ an, double* __restrict__ bn,
+    double* __restrict__ props, int32_t* __restrict__ status, double* __restrict__ scratch,
+    int rows_cap, int dn_extra) {
+  // rows above n_max in the last block stay below nmx - 1 (nmx >= n_max + 16)
+  // and below the ratio chain's start (>= n_max + 18)
+  static_assert(K >= 1 && K <= 12, "checkpoint interval");
+  __shared__ double rows[K][3][64];
+  const int lane = threadIdx.x;
+  const int64_t begin = range[0], end = range[1];
+  const int bcap = (rows_cap + K - 1) / K;
+  double* CK = scratch + (size_t)blockIdx.x * (size_t)64 * 3 * bcap;   // [b][3][lane]
+
+  for (int64_t base = begin + (int64_t)blockIdx.x * 64; base < end;
+       base += (int64_t)gridDim.x * 64) {
+    const int64_t idx = base + lane;
+    if (idx >= end) continue;
+    const int64_t p = order[idx];
+    PointDesc d = describe(p, x, y, eps, mu, eps2);
+    if (d.cls != 0) continue;
+    const double s = d.size;
+    const int nmax = nmax_rule(s);
+    if (nmax > rows_cap) {
+      atomicOr(status, 1);
+      continue;
+    }
+    const cd z = csqrt_(d.eps * d.mu) * s;
+    const cd m = csqrt_(cdiv(d.eps, d.mu));
+    const cdd minv = crcp_dd(m);
+    const double zabs = cabs_(z);
+    const int nmx =
+        nmx_rule(nmax, zabs) + (dn_extra < 0 ? 16 + 10 * (int)ceil(cbrt(zabs)) : dn_extra);
+    const cdd zinv = crcp_dd(z);
+    const ddr xinv = rcp_dd(s);
+    // ratio rows (nsw, sr]; kNoRatio where x >= n_max (no ratio rows: the
+    // upward form serves every n <= n_max)
+    const int nsw = (int)s < nmax ? (int)s : kNoRatio;
+    const int sr = nsw < nmax ? ratio_start(nmax, s) : 0;
+    const int nblk = (nmax + K - 1) / K;
+    // ---- pass 1: both downward chains, checkpointed every K rows
+    {
+      const int top = max(nmx - 1, sr);
+      cd dd = cd{0.0, 0.0};
+      double dr = 0.0;
+      // rows above the ratio chain's start and the highest checkpoint (row
+      // nblk K + 1) advance D_n alone; they exist only when top = nmx - 1
+      const int lo = max(sr, nblk * K + 1);
+      int k = top;
+      for (; k > lo; --k) {
+        const cd r = mul_dd((double)(k + 1), zinv);
+        dd = r - crcp_fast(dd + r);
+      }
+      for (; k > K; --k) {
+        if (k <= nmx - 1) {   // D_k = (k+1)/z - 1/(D_{k+1} + (k+1)/z), D_nmx = 0
+          const cd r = mul_dd((double)(k + 1), zinv);
+          dd = r - crcp_fast(dd + r);
+        }
+        if (k <= sr && k > nsw) {   // keeps psi_k / psi_{k-1} = 1/t
+          const double r = mul_dd((double)k, xinv);
+          dr = r - rcp_nr(dr + r);
+        }
+        const int b = (k - 1) / K - 1;      // state after row bK+K+1
+        if ((k - 1) % K == 0 && b >= 1 && b < nblk) {
+          double* c = CK + (size_t)b * 3 * 64 + lane;
+          c[0] = dd.re;
+          c[64] = dd.im;
+          if (k - 1 > nsw) c[128] = dr;
+        }
+      }
+      // rows K..1 (pass 2's first block) to LDS; k <= K < nmx - 1, and k <= sr
+      // when the ratio chain is active
+      for (int kk = min(top, K); kk >= 1; --kk) {
+        {
+          const cd r = mul_dd((double)(kk + 1), zinv);
+          dd = r - crcp_fast(dd + r);
+        }
+        rows[kk - 1][0][lane] = dd.re;
+        rows[kk - 1][1][lane] = dd.im;
+        if (kk > nsw) {
+          const double r = mul_dd((double)kk, xinv);
+          const double it = rcp_nr(dr + r);
+          rows[kk - 1][2][lane] = it;
+          dr = r - it;
+        }
+      }
+    }
+    // ---- pass 2: per block, its K rows in LDS, then K upward steps
+    double sn, cs;
+    sincos(s, &sn, &cs);
+    double psi_pp = cs, psi_p = sn;   // psi_{-1}, psi_0
+    double chi_pp = -sn, chi_p = cs;  // chi_{-1}, chi_0
+    Props pr;
+    pr.init();
+    const bool store = coef_off != nullptr;
+    const int64_t off = store ? coef_off[p] : 0;
+    double ck[3], ckn[3] = {0.0, 0.0, 0.0};
+    auto load_ck = [&](int b) {
+      const double* src = CK + (size_t)b * 3 * 64 + lane;
+      ckn[0] = src[0];
+      ckn[1] = src[64];
+      if (b * K + K > nsw) ckn[2] = src[128];
+    };
+    if (nblk > 1) load_ck(1);
+    for (int b = 0; b < nblk; ++b) {
+      if (b > 0) {
+#pragma unroll
+        for (int q = 0; q < 3; ++q) ck[q] = ckn[q];
+        if (b + 1 < nblk) load_ck(b + 1);
+        cd dd = cd{ck[0], ck[1]};
+        double dr = ck[2];
+#pragma unroll
+        for (int kk = K - 1; kk >= 0; --kk) {
+          const int k = b * K + kk + 1;
+          {
+            const cd r = mul_dd((double)(k + 1), zinv);
+            dd = r - crcp_fast(dd + r);
+          }
+          rows[kk][0][lane] = dd.re;
+          rows[kk][1][lane] = dd.im;
+          if (k > nsw) {
+            const double r = mul_dd((double)k, xinv);
+            const double it = rcp_nr(dr + r);
+            rows[kk][2][lane] = it;
+            dr = r - it;
+          }
+        }
+      }
+      const int kend = min(K, nmax - b * K);
+#pragma unroll 2
+      for (int kk = 0; kk < kend; ++kk) {
+        const int n = b * K + kk + 1;
+        const cd D = cd{rows[kk][0][lane], rows[kk][1][lane]};
+        const double R = rows[kk][2][lane];
+        double f = mul_dd((double)(2 * n - 1), xinv);
+        double psi = (n <= nsw) ? fma(f, psi_p, -psi_pp) : psi_p * R;
+        double chi = fma(f, chi_p, -chi_pp);
+        double nx = mul_dd((double)n, xinv);
+        cd da = mul_dd(D, minv) + nx;
+        cd db = D * m + nx;
+        cd xi = cd{psi, -chi};
+        cd xi1 = cd{psi_p, -chi_p};
+        cd a = cdiv(da * psi - psi_p, da * xi - xi1);
+        cd bb = cdiv(db * psi - psi_p, db * xi - xi1);
+        pr.add(n, a, bb);
+        if (store) {
+          an[2 * (off + n - 1)] = a.re;
+          an[2 * (off + n - 1) + 1] = a.im;
+          bn[2 * (off + n - 1)] = bb.re;
+          bn[2 * (off + n - 1) + 1] = bb.im;
+        }
+        psi_pp = psi_p;
+        psi_p = psi;
+        chi_pp = chi_p;
+        chi_p = chi;
+      }
     }
     if (props) pr.finish(s, p, n_all, props);
   }
@@ -1108,7 +1291,6 @@ struct CkptPt {
   int nmax, nsw_v, nsw_w, nsw_y, nblk;
   bool ddv_on, ddw_on;
 };
-constexpr int kNoRatio = 1 << 30;
 template <int K, bool DD, bool STORE>
 __device__ __forceinline__ Props ckpt_pass2(const CkptPt& c, double (*rows)[7][64],
                                             const double* __restrict__ CK,
@@ -1611,6 +1793,7 @@ __global__ __launch_bounds__(64) void k_mie_coated_literal(
       double*, int, int
 template __global__ void k_mie_homog<1>(GM_MIE_ARGS);
 template __global__ void k_mie_homog<4>(GM_MIE_ARGS);
+template __global__ void k_mie_homog_ckpt<1, 4>(GM_MIE_ARGS);
 template __global__ void k_mie_coated_stable<1>(GM_MIE_ARGS);
 template __global__ void k_mie_coated_stable<3>(GM_MIE_ARGS);
 template __global__ void k_mie_coated_stable<4>(GM_MIE_ARGS);

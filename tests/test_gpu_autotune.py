@@ -122,6 +122,42 @@ def test_coated_checkpoint_variants_agree(gpu):
         print("checkpoint every %d rows: bitwise identical to the full-row kernel: %s" % (ck, same))
 
 
+def test_homog_checkpoint_agrees(gpu):
+    """The checkpointed lane-per-point homogeneous kernel (D_n and psi-ratio
+    states kept every 4 rows, blocks regenerated in LDS) against the kernel
+    that keeps every row in scratch: the regenerated rows repeat pass 1's
+    operations from the same states and the upward step is the same source,
+    so the results agree to the last bits (asserted at 1e-12, the bitwise
+    outcome printed); sizes from the Rayleigh limit to x = 3400 (n_max
+    3455), weakly to strongly absorbing indices, coefficients stored."""
+    from geosmie_amd import _lib, autotune
+    from geosmie_amd.mie import mie_batch
+    h = _lib.ctx()
+    r = np.random.default_rng(21)
+    x = np.concatenate([np.logspace(-3, np.log10(3400.0), 2500), [0.5, 1.0, 4.0, 5.0, 8.0]])
+    m = (r.uniform(1.01, 1.9, x.size) + 1j * 10 ** r.uniform(-8, 0.3, x.size))
+    res = {}
+    try:
+        autotune.set_knob(h, "mie_lanes", 1)
+        for ck in (0, 4):
+            autotune.set_knob(h, "mie_ckpt", ck)
+            res[ck] = mie_batch(x, m=m, coeffs=True)
+    finally:
+        autotune.set_knob(h, "mie_ckpt", 4)
+        autotune.set_knob(h, "mie_lanes", 0)
+    same = True
+    for k in ("qext", "qsca", "asy", "qb"):
+        a, b = res[0][k], res[4][k]
+        same &= bool(np.array_equal(a, b))
+        assert np.all(np.abs(a - b) <= 1e-12 * np.abs(a)), k
+    for key in ("an", "bn"):
+        for a, b in zip(res[0][key], res[4][key]):
+            assert a.shape == b.shape
+            same &= bool(np.array_equal(a, b))
+            assert np.all(np.abs(a - b) <= 1e-12 * np.abs(a).max()), key
+    print("homogeneous, checkpoint every 4 rows: bitwise identical to the full-row kernel: %s" % same)
+
+
 def test_homog_lane_variants_agree(gpu):
     """The homogeneous kernel with one point per lane and with 16, 32 or 64
     lanes per point (segments joined by parallel scans) agree to rounding
