@@ -28,11 +28,7 @@ struct gm_ctx {
   int occ_coop = 8;             // the homogeneous kernel with P lanes per point
   int occ_homog_ckpt = 8;       // the checkpointed homogeneous kernel (K = 4)
 
-  // bytes of recurrence scratch: a one-point-per-lane slot holds n_max rows
-  // per lane (5.3 MB per wavefront at x ~ 3400), so this bounds the
-  // persistent grid of long-series batches (16 GiB: ~3200 wavefronts there,
-  // 3 per SIMD; a small share of the 288 GB of HBM)
-  size_t scratch_budget = (size_t)16 << 30;
+  size_t scratch_budget = (size_t)3 << 30;       // bytes of recurrence scratch
   // launch shapes chosen by gm_ctx_tune (an autotuner's knobs)
   int tune_mie_grid_mult = 1;   // persistent Mie grid = multiple of resident wavefronts
   int tune_s12_waves = 4;       // wavefronts (64 angles each) per S1/S2 block
