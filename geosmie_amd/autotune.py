@@ -16,7 +16,9 @@ Knobs (include/geosmie.h, enum gm_tune_param):
   s12_layout     FP64 S1/S2 blocks split angles (0) or points (1) among their waves,
                  or share LDS-staged angle tables among them (2)
   mie_ckpt       coated kernel: downward-chain states kept every 4 or 8 rows and
-                 regenerated per block in LDS, or every row kept in scratch (0)
+                 regenerated per block in LDS, or every row kept in scratch (0);
+                 non-zero also selects the checkpointed lane-per-point
+                 homogeneous kernel (every 4 rows)
   mie_lanes      homogeneous kernel: lanes per point (1, or 16/32/64 with the serial
                  recurrences joined by parallel scans; 0 = chosen by batch)
   s12_row_tiles  fused S1/S2 row sums: 8-point tiles per work unit (0 = by batch)

@@ -100,7 +100,10 @@ enum gm_tune_param {
   GM_TUNE_MIE_CKPT = 7          /* coated kernel (real / weakly absorbing shells): the downward
                                    chains' states kept every 4 or 8 rows and each block of rows
                                    regenerated in LDS before use (4, the default), or 0 = every
-                                   row kept in scratch; results agree to rounding */,
+                                   row kept in scratch; results agree to rounding.  Non-zero also
+                                   selects the checkpointed one-point-per-lane homogeneous kernel
+                                   (every 4 rows, unless GM_TUNE_MIE_WAVES forces a register
+                                   budget; bit-identical results) */,
   GM_TUNE_MIE_LANES = 8         /* homogeneous kernel: lanes per point.  1 = one point per lane
                                    (large batches); 16, 32, 64 = each point's rows split over that
                                    many lanes, its serial recurrences joined by parallel scans
