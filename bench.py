@@ -760,6 +760,7 @@ def main():
                                         args.steps)]
         del d_x, d_y, d_eps, d_eps2, d_props
         torch.cuda.empty_cache()
+        sec.append(coated_host_buffers(x, y, m1, m2))
         sec += secondary(cpu=not args.no_cpu)
         out["secondary"] = sec
         out["lut"] = lut_surrogate(full=not args.no_full_lut)
@@ -798,6 +799,29 @@ def coated_with_coefficients(L, h, n, d_x, d_y, d_eps, d_eps2, size_max, sum_nma
     return {"name": "coated_with_coefficients", "points": n, "ms_per_step": ms,
             "value": n / (ms * 1e-3), "unit": "points/s", "mode": "eager",
             "coefficient_bytes_written": 32.0 * sum_nmax, "coefficients_finite": ok}
+
+
+def coated_host_buffers(x, y, m1, m2, reps=3):
+    """The headline batch through the Python drop-in with host arrays
+    (geosmie_amd.mie.mie_batch: numpy in, numpy out): the PCIe-inclusive rate
+    a host-buffer caller sees (inputs copied to the device, the six
+    efficiencies and n_max copied back), beside the device-resident headline."""
+    import torch
+    from geosmie_amd.mie import mie_batch
+    mie_batch(x, m=m1, y=y, m2=m2)
+    torch.cuda.synchronize()
+    ts = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        r = mie_batch(x, m=m1, y=y, m2=m2)
+        ts.append(time.perf_counter() - t0)
+    best = min(ts)
+    n = x.size
+    return {"name": "coated_host_buffers", "points": n, "ms_per_call": best * 1e3,
+            "ms_per_call_all": [t * 1e3 for t in ts], "value": n / best, "unit": "points/s",
+            "pcie_bytes": 48.0 * n + 52.0 * n, "outputs_finite": bool(np.isfinite(r["qext"]).all()),
+            "note": "x, y, eps, eps2 to the device (48 B/point), six efficiencies and n_max back "
+                    "(52 B/point), plus the host-side numpy work of the drop-in; best of %d" % reps}
 
 
 if __name__ == "__main__":
