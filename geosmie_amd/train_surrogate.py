@@ -28,6 +28,9 @@ def main(argv=None):
     ap.add_argument("--stride", type=int, nargs=4, default=(1, 1, 1, 1),
                     help="subsampling of the m_r, m_i, lambda, gf axes")
     ap.add_argument("--val-slabs", type=int, default=30000)
+    ap.add_argument("--safety", type=float, default=1.0,
+                    help="a gate cell is trusted when every validation entry in it is within "
+                         "bound / safety")
     ap.add_argument("--cells", type=int, nargs=6, default=None,
                     help="gate cells along ln r_mode, sigma_g, m_r, ln m_i, ln lambda, f_w")
     ap.add_argument("--cw-head", type=float, default=1000.0,
@@ -45,13 +48,14 @@ def main(argv=None):
     if args.regate:
         sur = S.Surrogate.load(args.regate)
         rep = S.regate(sur, n_val_slabs=args.val_slabs, seed=args.seed + 11, cells=args.cells,
-                       log=lambda s: print(s, flush=True))
+                       safety=args.safety, log=lambda s: print(s, flush=True))
         sur.save(args.out)
         print(json.dumps(rep), flush=True)
         return
     sur, rep = S.train(train_stride=tuple(args.stride), n_val_slabs=args.val_slabs,
                        steps=args.steps, batch=args.batch, lr=args.lr, lr_final=args.lr_final,
                        cw_head=args.cw_head, seed=args.seed, precision=args.precision,
+                       safety=args.safety,
                        n_aug_psd=args.aug_psd, weight_terms=args.weight_terms,
                        init=S.Surrogate.load(args.init) if args.init else None,
                        log=lambda s: print(s, flush=True))
