@@ -116,3 +116,32 @@ def test_very_large_size_parameter(gpu, lanes):
         scale = max(np.abs(o1).max(), np.abs(o2).max())
         assert np.abs(s1[i] - o1).max() <= 1e-11 * scale, i
         assert np.abs(s2[i] - o2).max() <= 1e-11 * scale, i
+
+
+@pytest.mark.parametrize("ckpt", [4, 0])
+def test_coated_large_shells(gpu, ckpt):
+    """Coated spheres out to y = 1000 (5x the coated workload's largest
+    shell), real and weakly absorbing shells, strongly and weakly absorbing
+    cores, with the checkpointed kernel (4) and the one that keeps every row
+    (0), against the C restatement's cancellation-free form (its literal form
+    agrees with it to 5e-11 on these points): Qext, Qsca 1e-9 relative, g
+    1e-9 absolute."""
+    from oracle import cmie
+    from geosmie_amd import _lib, autotune
+    from geosmie_amd.mie import mie_batch
+    h = _lib.ctx()
+    r = np.random.default_rng(8)
+    y = np.exp(np.linspace(np.log(200.0), np.log(1000.0), 60))
+    x = y * r.uniform(0.3, 0.9, y.size)
+    m1 = np.where(r.uniform(size=y.size) < 0.5, 1.95 + 0.79j, 1.5 + 0.01j)
+    m2 = np.where(r.uniform(size=y.size) < 0.5, 1.33 + 0j, 1.45 + 1e-4j)
+    try:
+        autotune.set_knob(h, "mie_ckpt", ckpt)
+        got = mie_batch(x, m=m1, y=y, m2=m2)
+    finally:
+        autotune.set_knob(h, "mie_ckpt", 4)
+    c = cmie.batch(x, m1, y, m2, threads=8, stable=True)
+    for j, k in enumerate(("qext", "qsca")):
+        rel = np.abs(got[k] - c[j]) / np.abs(c[j])
+        assert rel.max() < 1e-9, (k, rel.max(), y[rel.argmax()])
+    assert np.abs(got["asy"] - c[4]).max() < 1e-9
