@@ -550,7 +550,7 @@ int gm_mie_batch(gm_ctx* ctx, int64_t n, const double* x, const double* y, const
   const int nb = gm::kClasses * gm::kBuckets;
   if ((rc = ctx->reserve(ctx->order, sizeof(int64_t) * n))) return rc;
   if ((rc = ctx->reserve(ctx->key, sizeof(int32_t) * n))) return rc;
-  if ((rc = ctx->reserve(ctx->hist, sizeof(int64_t) * (2 * nb + 2 * gm::kClasses + 4)))) return rc;
+  if ((rc = ctx->reserve(ctx->hist, sizeof(int64_t) * (2 * nb + gm::kQueueSlot + 1)))) return rc;
   MieWs ws;
   ws.order = (int64_t*)ctx->order.ptr;
   ws.key = (int32_t*)ctx->key.ptr;
@@ -588,7 +588,7 @@ int gm_mie_batch_noalloc(gm_ctx* ctx, int64_t n, const double* x, const double* 
   // layout of the static array: plan histogram/cursors/ranges, sort keys,
   // sorted order, then the recurrence slots in what is left
   const int nb = gm::kClasses * gm::kBuckets;
-  const size_t hist_bytes = sizeof(int64_t) * (2 * nb + 2 * gm::kClasses + 4);
+  const size_t hist_bytes = sizeof(int64_t) * (2 * nb + gm::kQueueSlot + 1);
   auto up = [](size_t v) { return (v + 255) / 256 * 256; };
   const size_t plan_bytes = up(hist_bytes) + up(sizeof(int32_t) * (size_t)n) +
                             up(sizeof(int64_t) * (size_t)n);

@@ -27,6 +27,10 @@ __global__ void k_plan_identity(int64_t n, int64_t* order, int64_t* range, int64
 // homogeneous points of cost below this (bucketed) run 4 lanes per point: then
 // 3 n_max + 16 <= cost < 160, so n_max < 48 and at most 12 rows per lane
 constexpr int kSmallCost = 160;
+// Plan workspace after the histogram and cursors: range[2 kClasses], the
+// homogeneous split ab[4], then one work-queue counter; kQueueSlot is the
+// counter's offset from range.
+constexpr int kQueueSlot = 2 * kClasses + 4;
 constexpr int kSmallLcap = 12;
 template <int MINW>
 __global__ void k_mie_homog(const int64_t* order, const int64_t* range, int64_t n_all,

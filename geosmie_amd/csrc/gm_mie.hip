@@ -218,6 +218,7 @@ __global__ __launch_bounds__(256) void k_plan_scan(const int64_t* __restrict__ h
     ab[1] = split;
     ab[2] = split;
     ab[3] = range[1];
+    ab[4] = 0;   // the coated class's work queue (kQueueSlot)
   }
 }
 
@@ -268,6 +269,7 @@ __global__ void k_plan_identity(int64_t n, int64_t* __restrict__ order,
       ab[1] = n;
       ab[2] = n;
       ab[3] = n;
+      ab[4] = 0;   // the coated class's work queue (kQueueSlot)
     }
   }
 }
@@ -1462,6 +1464,11 @@ __global__ __launch_bounds__(64, MINW) void k_mie_coated_ckpt(
   const int64_t begin = range[2], end = range[3];
   const int bcap = (rows_cap + K - 1) / K;
   double* slot = scratch + (size_t)blockIdx.x * (size_t)64 * (7 * (size_t)bcap + 4 * rows_cap);
+  // groups of 64 sorted points handed out in order (costliest first) by a
+  // counter the plan zeroes: a wavefront that finishes early takes the next
+  unsigned long long* queue =
+      reinterpret_cast<unsigned long long*>(const_cast<int64_t*>(range) + kQueueSlot);
+  const int64_t ngroups = (end - begin + 63) / 64;
   double* CK = slot;                                          // [b][7][lane]
   cd* DDV = reinterpret_cast<cd*>(slot + (size_t)7 * bcap * 64);
   cd* DDW = DDV + (size_t)rows_cap * 64;
@@ -1469,8 +1476,12 @@ __global__ __launch_bounds__(64, MINW) void k_mie_coated_ckpt(
   const __amdgpu_buffer_rsrc_t rs_dd =
       slot_rsrc(reinterpret_cast<double*>(DDV), (uint32_t)((size_t)rows_cap * 64 * 4 * sizeof(double)));
 
-  for (int64_t base = begin + (int64_t)blockIdx.x * 64; base < end;
-       base += (int64_t)gridDim.x * 64) {
+  for (;;) {
+    int64_t grp = 0;
+    if (lane == 0) grp = (int64_t)atomicAdd(queue, 1ull);
+    grp = __shfl(grp, 0);
+    if (grp >= ngroups) break;
+    const int64_t base = begin + grp * 64;
     const int64_t idx = base + lane;
     if (idx >= end) continue;
     const int64_t p = order[idx];
