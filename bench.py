@@ -819,9 +819,10 @@ def coated_host_buffers(x, y, m1, m2, reps=3):
     n = x.size
     return {"name": "coated_host_buffers", "points": n, "ms_per_call": best * 1e3,
             "ms_per_call_all": [t * 1e3 for t in ts], "value": n / best, "unit": "points/s",
-            "pcie_bytes": 48.0 * n + 52.0 * n, "outputs_finite": bool(np.isfinite(r["qext"]).all()),
-            "note": "x, y, eps, eps2 to the device (48 B/point), six efficiencies and n_max back "
-                    "(52 B/point), plus the host-side numpy work of the drop-in; best of %d" % reps}
+            "pcie_bytes": 32.0 * n + 52.0 * n, "outputs_finite": bool(np.isfinite(r["qext"]).all()),
+            "note": "x, y, eps2 to the device (32 B/point; the core's single index is filled on "
+                    "the device), six efficiencies and n_max back (52 B/point), plus the "
+                    "host-side numpy work of the drop-in; best of %d" % reps}
 
 
 if __name__ == "__main__":

@@ -22,10 +22,20 @@ def _batch(eps, x, mu=None, y=None, eps2=None, want_coeffs=True, flags=_lib.GM_M
     size_max = float(size.max()) if n else 0.0
     d_x = _lib.to_dev(x)
     d_y = _lib.to_dev(size) if y is not None else None
-    d_eps = _lib.cplx_dev(np.broadcast_to(np.asarray(eps, dtype=np.complex128), (n,)))
-    d_mu = None if mu is None else _lib.cplx_dev(np.broadcast_to(np.asarray(mu, dtype=np.complex128), (n,)))
-    d_eps2 = None if eps2 is None else _lib.cplx_dev(np.broadcast_to(np.asarray(eps2, dtype=np.complex128), (n,)))
     dev = _lib.device()
+
+    def per_point(v):
+        # a scalar is filled on the device (no n-element host array or copy)
+        v = np.asarray(v, dtype=np.complex128)
+        if v.ndim == 0:
+            d = t.empty((n, 2), dtype=t.float64, device=dev)
+            d[:, 0] = float(v.real)
+            d[:, 1] = float(v.imag)
+            return d.view(-1)
+        return _lib.cplx_dev(np.broadcast_to(v, (n,)))
+    d_eps = per_point(eps)
+    d_mu = None if mu is None else per_point(mu)
+    d_eps2 = None if eps2 is None else per_point(eps2)
     d_props = t.empty(6 * n, dtype=t.float64, device=dev)
     d_nmax = t.empty(n, dtype=t.int32, device=dev)
     L = _lib.lib()
