@@ -1443,8 +1443,9 @@ __device__ __forceinline__ Props ckpt_pass2(const CkptPt& c, double (*rows)[7][6
 // regenerating the block's K rows into LDS with exactly the operations of
 // pass 1 (so every row is bit-identical to the value pass 1 computed); the K
 // upward steps consume them.  Scratch traffic per
-// point drops from up to 7 doubles per row written and read to 7 doubles per
-// K rows, and the upward pass no longer waits on scratch rows.
+// point drops from up to 7 doubles per row written and read to at most 7
+// doubles per K rows (a ratio chain's state only for the blocks that use
+// it), and the upward pass no longer waits on scratch rows.
 // Slot layout: checkpoints [b][7][lane] (du.re, du.im, dv.re, dv.im, dw.re,
 // dw.im, dy), then the start-correction planes DDV, DDW [row][lane] (complex),
 // written only when the reference's start is not converged for v or w.
