@@ -1,4 +1,5 @@
-"""The runoptics chain captured as a HIP graph replays to the eager result.
+"""The runoptics chain and a coated Mie batch captured as HIP graphs replay to
+the eager result.
 
 Replays are interleaved with unrelated eager work on the default context and
 checked bitwise each time against one eager pass."""
@@ -48,3 +49,47 @@ def test_graph_replay_matches_eager(gpu, precision):
         ch.replay()
         t.cuda.synchronize()
         _same(eager, ch.results())
+
+
+def test_coated_batch_graph_replays(gpu):
+    """A coated gm_mie_batch (plan kernels, then the checkpointed kernel,
+    whose work-queue counter the plan's scan kernel resets) captured as a HIP
+    graph, as bench.py times it: three replays, each after unrelated coated
+    work on the default context and with the output clobbered, reproduce the
+    eager step bit for bit."""
+    t = __import__("torch")
+    from geosmie_amd import _lib
+    from geosmie_amd._lib import check, ptr
+    from geosmie_amd.mie import mie_batch as mb
+    r = np.random.default_rng(6)
+    y = np.exp(r.uniform(np.log(0.05), np.log(150.0), 20000))
+    x = y * r.uniform(0.3, 0.9, y.size)
+    n = y.size
+    d_x, d_y = _lib.to_dev(x), _lib.to_dev(y)
+    d_eps = _lib.cplx_dev(np.full(n, (1.95 + 0.79j) ** 2))
+    d_eps2 = _lib.cplx_dev(np.where(r.uniform(size=n) < 0.5, 1.45 + 0j, 1.33 + 1e-4j) ** 2)
+    d_props = t.empty(6 * n, dtype=t.float64, device="cuda")
+    L = _lib.lib()
+    h = _lib.private_ctx()
+
+    def step():
+        check(L.gm_mie_batch(h, n, ptr(d_x), ptr(d_y), ptr(d_eps), None, ptr(d_eps2),
+                             float(y.max()), None, None, None, ptr(d_props), None,
+                             _lib.GM_MIE_SORT, _lib.stream()), "gm_mie_batch")
+    side = t.cuda.Stream()
+    side.wait_stream(t.cuda.current_stream())
+    with t.cuda.stream(side):
+        step()
+    t.cuda.current_stream().wait_stream(side)
+    t.cuda.synchronize()
+    eager = d_props.cpu().numpy().copy()
+    assert np.isfinite(eager).all()
+    graph = t.cuda.CUDAGraph()
+    with t.cuda.graph(graph):
+        step()
+    for _ in range(3):
+        mb(np.linspace(0.1, 50.0, 333), m=1.6 + 0.2j, y=np.linspace(0.2, 60.0, 333), m2=1.4 + 0j)
+        d_props.fill_(-1.0)
+        graph.replay()
+        t.cuda.synchronize()
+        assert np.array_equal(d_props.cpu().numpy(), eager)
