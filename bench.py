@@ -40,6 +40,8 @@ TRAFFIC_FILES_REL = {
                             "traffic_bytes_per_launch_k0:"),
     "k_mie_coated_ckpt<1,4>": ("profiles/k_mie_coated_ckpt__coated8.4M__pmc_fetch_write.txt",
                                "traffic_bytes_per_launch_k4:"),
+    "k_mie_coated_ckpt<1,5>": ("profiles/k_mie_coated_ckpt__coated8.4M__pmc_fetch_write.txt",
+                               "traffic_bytes_per_launch_k5:"),
     "k_mie_coated_ckpt<1,8>": ("profiles/k_mie_coated_ckpt__coated8.4M__pmc_fetch_write.txt",
                                "traffic_bytes_per_launch_k8:"),
 }
@@ -48,6 +50,8 @@ TRAFFIC_FILES_REL = {
 ISSUED_FLOP_FILES_REL = {
     "k_mie_coated_ckpt<1,4>": ("profiles/k_mie_coated_ckpt__coated8.4M__pmc_issue.txt",
                                "issued_fp64_flop_per_launch_k4:"),
+    "k_mie_coated_ckpt<1,5>": ("profiles/k_mie_coated_ckpt__coated8.4M__pmc_issue.txt",
+                               "issued_fp64_flop_per_launch_k5:"),
 }
 
 

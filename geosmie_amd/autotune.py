@@ -15,7 +15,7 @@ Knobs (include/geosmie.h, enum gm_tune_param):
                  3 or 4 resident wavefronts per SIMD)
   s12_layout     FP64 S1/S2 blocks split angles (0) or points (1) among their waves,
                  or share LDS-staged angle tables among them (2)
-  mie_ckpt       coated kernel: downward-chain states kept every 4 or 8 rows and
+  mie_ckpt       coated kernel: downward-chain states kept every 4, 5 or 8 rows and
                  regenerated per block in LDS, or every row kept in scratch (0);
                  non-zero also selects the checkpointed lane-per-point
                  homogeneous kernel (every 4 rows)
@@ -41,7 +41,7 @@ KNOBS = {"mie_grid_mult": (0, (1, 2, 4)),
          "s12_points": (4, (8, 16)),
          "mie_waves": (5, (0, 3, 4)),
          "s12_layout": (6, (0, 1, 2)),
-         "mie_ckpt": (7, (4, 8, 0)),
+         "mie_ckpt": (7, (4, 5, 8, 0)),
          "mie_lanes": (8, (0, 1, 16, 32, 64)),
          "s12_row_tiles": (10, (0, 1, 2, 4)),
          "s12_share": (11, (1, 0)),

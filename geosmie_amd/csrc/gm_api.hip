@@ -202,7 +202,9 @@ MieGrids mie_grids(const gm_ctx* ctx, int64_t n, int rows_cap, size_t budget) {
     // beyond the start-correction ones)
     const size_t slot_bytes =
         (size_t)64 * gm::coated_ckpt_lane_doubles(rows_cap, ck) * sizeof(double);
-    const int occ = ck == 8 ? ctx->occ_ckpt8 : ctx->occ_ckpt4[var ? 1 : 0];
+    const int occ = ck == 8   ? ctx->occ_ckpt8
+                    : ck == 5 ? ctx->occ_ckpt5
+                              : ctx->occ_ckpt4[var ? 1 : 0];
     int64_t gg = std::min<int64_t>(
         waves, (int64_t)ctx->n_cu * std::max(occ, 1) * std::max(ctx->tune_mie_grid_mult, 1));
     gg = std::min<int64_t>(gg, std::max<int64_t>(1, (int64_t)(budget / slot_bytes)));
@@ -298,6 +300,8 @@ int mie_batch_run(gm_ctx* ctx, int64_t n, const double* x, const double* y, cons
                              : gm::k_mie_coated_stable<1>;
     if (ctx->tune_mie_ckpt == 4)
       stable = var ? gm::k_mie_coated_ckpt<3, 4> : gm::k_mie_coated_ckpt<1, 4>;
+    else if (ctx->tune_mie_ckpt == 5)
+      stable = gm::k_mie_coated_ckpt<1, 5>;
     else if (ctx->tune_mie_ckpt == 8)
       stable = gm::k_mie_coated_ckpt<1, 8>;
     hipLaunchKernelGGL(stable, dim3((unsigned)g.stable.first), dim3(64), 0, s, order, range, n, x,
@@ -365,6 +369,9 @@ int gm_ctx_create(int device, gm_ctx** out) {
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, gm::k_mie_coated_ckpt<3, 4>, 64, 0) ==
       hipSuccess)
     c->occ_ckpt4[1] = occ;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, gm::k_mie_coated_ckpt<1, 5>, 64, 0) ==
+      hipSuccess)
+    c->occ_ckpt5 = occ;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, gm::k_mie_coated_ckpt<1, 8>, 64, 0) ==
       hipSuccess)
     c->occ_ckpt8 = occ;
@@ -438,8 +445,8 @@ int gm_ctx_tune(gm_ctx* ctx, int param, int value) {
       ctx->tune_mie_waves = value;
       return GM_OK;
     case GM_TUNE_MIE_CKPT:
-      if (value != 0 && value != 4 && value != 8)
-        return gm_fail(GM_EINVAL, "gm_ctx_tune: rows per checkpoint must be 0, 4 or 8");
+      if (value != 0 && value != 4 && value != 5 && value != 8)
+        return gm_fail(GM_EINVAL, "gm_ctx_tune: rows per checkpoint must be 0, 4, 5 or 8");
       ctx->tune_mie_ckpt = value;
       return GM_OK;
     case GM_TUNE_MIE_LANES:

@@ -24,7 +24,7 @@ struct gm_ctx {
   int occ_homog[3] = {8, 8, 8}, occ_stable[3] = {8, 8, 8}, occ_literal = 4;
   // resident wavefronts per CU of the checkpointed coated kernel: K = 4
   // (unconstrained, >= 3 waves per SIMD) and K = 8
-  int occ_ckpt4[2] = {8, 8}, occ_ckpt8 = 5;
+  int occ_ckpt4[2] = {8, 8}, occ_ckpt5 = 8, occ_ckpt8 = 5;
   int occ_coop = 8;             // the homogeneous kernel with P lanes per point
   int occ_homog_ckpt = 8;       // the checkpointed homogeneous kernel (K = 4)
 

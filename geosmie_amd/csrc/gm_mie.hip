@@ -1811,6 +1811,7 @@ template __global__ void k_mie_coated_stable<3>(GM_MIE_ARGS);
 template __global__ void k_mie_coated_stable<4>(GM_MIE_ARGS);
 template __global__ void k_mie_coated_ckpt<1, 4>(GM_MIE_ARGS);
 template __global__ void k_mie_coated_ckpt<3, 4>(GM_MIE_ARGS);
+template __global__ void k_mie_coated_ckpt<1, 5>(GM_MIE_ARGS);
 template __global__ void k_mie_coated_ckpt<1, 8>(GM_MIE_ARGS);
 template __global__ void k_mie_homog_coop<4>(GM_MIE_ARGS);
 template __global__ void k_mie_homog_coop<16>(GM_MIE_ARGS);

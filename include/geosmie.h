@@ -98,7 +98,7 @@ enum gm_tune_param {
                                    of the same 64 angles (angle-table rows shared in cache),
                                    2 = as 1 with the angle tables staged through LDS */,
   GM_TUNE_MIE_CKPT = 7          /* coated kernel (real / weakly absorbing shells): the downward
-                                   chains' states kept every 4 or 8 rows and each block of rows
+                                   chains' states kept every 4, 5 or 8 rows and each block of rows
                                    regenerated in LDS before use (4, the default), or 0 = every
                                    row kept in scratch; results agree to rounding.  Non-zero also
                                    selects the checkpointed one-point-per-lane homogeneous kernel

@@ -89,7 +89,7 @@ def test_tune_picks_a_candidate(gpu):
 
 
 def test_coated_checkpoint_variants_agree(gpu):
-    """The checkpointed coated kernel (chain states kept every 4 or 8 rows,
+    """The checkpointed coated kernel (chain states kept every 4, 5 or 8 rows,
     each block of rows regenerated in LDS) against the kernel that keeps every
     row in scratch: the regenerated rows repeat pass 1's operations from the
     same states, so the results agree to the last bits (asserted at 1e-12, and
@@ -105,12 +105,12 @@ def test_coated_checkpoint_variants_agree(gpu):
     m2 = np.where(m1.real > 1.5, 1.45 + 0j, 1.5 + 1e-3j)
     res = {}
     try:
-        for ck in (0, 4, 8):
+        for ck in (0, 4, 5, 8):
             autotune.set_knob(h, "mie_ckpt", ck)
             res[ck] = mie_batch(x, m=m1, y=y, m2=m2, coeffs=True)
     finally:
         autotune.set_knob(h, "mie_ckpt", 4)
-    for ck in (4, 8):
+    for ck in (4, 5, 8):
         same = True
         for k in ("qext", "qsca", "asy", "qb"):
             a, b = res[0][k], res[ck][k]
