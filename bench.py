@@ -658,8 +658,11 @@ def main():
         step()
     # the grid multiple before and after the checkpoint interval: the best
     # multiple differs between the two coated kernels
+    # candidates within 1% of the fastest count as equal (the two-call times
+    # vary by about that much); the earlier-listed value is kept: for the
+    # checkpoint interval, 5 rows before 4 (the same speed, 16% fewer bytes)
     tuned = autotune.tune(h, step, knobs=("mie_grid_mult", "mie_ckpt", "mie_grid_mult",
-                                          "mie_waves"), reps=2, warmup=1)
+                                          "mie_waves"), reps=2, warmup=1, tie=0.01)
     ckpt = autotune.get(h, "mie_ckpt")
     kernel_name = "k_mie_coated_ckpt<1,%d>" % ckpt if ckpt else "k_mie_coated_stable"
     traffic, traffic_src = measured_traffic(kernel_name)

@@ -41,7 +41,7 @@ KNOBS = {"mie_grid_mult": (0, (1, 2, 4)),
          "s12_points": (4, (8, 16)),
          "mie_waves": (5, (0, 3, 4)),
          "s12_layout": (6, (0, 1, 2)),
-         "mie_ckpt": (7, (4, 5, 8, 0)),
+         "mie_ckpt": (7, (5, 4, 8, 0)),
          "mie_lanes": (8, (0, 1, 16, 32, 64)),
          "s12_row_tiles": (10, (0, 1, 2, 4)),
          "s12_share": (11, (1, 0)),
@@ -97,12 +97,14 @@ def store_cache(path, entry_key, choices):
 
 
 def tune(ctx, workload, knobs=("mie_grid_mult", "s12_points", "s12_waves", "psd_chunk"), reps=3,
-         warmup=1, candidates=None, cache_key=None, cache=None):
+         warmup=1, candidates=None, cache_key=None, cache=None, tie=0.0):
     """Returns {knob: {"best": value, "ms": {value: ms per pass}}}; the context
     keeps the best values.  With cache_key (a name for the workload), choices
     already in the cache file for this device and workload are applied without
     timing (reported with "cached": True), and new ones are saved there; cache
-    is the file (default cache_path())."""
+    is the file (default cache_path()).  tie: the first candidate, in list
+    order, whose time is within this fraction of the fastest is kept (times
+    that agree to within the timing noise go to the earlier-listed value)."""
     t = _lib.torch()
     out = {}
     path = cache or cache_path()
@@ -128,7 +130,8 @@ def tune(ctx, workload, knobs=("mie_grid_mult", "s12_points", "s12_waves", "psd_
                 workload()
             t.cuda.synchronize()
             ms[v] = (time.perf_counter() - t0) * 1e3 / reps
-        best = min(ms, key=ms.get)
+        fastest = min(ms.values())
+        best = next(v for v in cands if ms[v] <= fastest * (1.0 + tie))
         set_knob(ctx, name, best)
         out[name] = {"best": best, "ms": ms}
     if entry is not None and knobs:
